@@ -1,0 +1,251 @@
+"""Deterministic definitions of the small LEE cases used as golden fixtures.
+
+Every array is built point by point with Python's `math` module (glibc libm)
+so that the same inputs come out bit-for-bit on any x86-64 host; the cases
+are small enough for that to take well under a second each.
+
+A case is a dict with:
+  name, ni, nj, x, y (float64 (nj, ni)), mean (5, nj, ni) [s, u, v, w, p],
+  ic (5, nj, ni), bc (BC-file lines), deck overrides, nsteps, and the
+  solver settings the deck carries.
+"""
+import math
+
+import numpy as np
+
+GAMMA = 1.4
+LN2 = math.log(2.0)
+
+
+def _grid(ni, nj, fx, fy):
+    x = np.empty((nj, ni))
+    y = np.empty((nj, ni))
+    for j in range(nj):
+        for i in range(ni):
+            x[j, i] = fx(i, j)
+            y[j, i] = fy(i, j)
+    return x, y
+
+
+def _field(x, y, f):
+    nj, ni = x.shape
+    out = np.empty((nj, ni))
+    for j in range(nj):
+        for i in range(ni):
+            out[j, i] = f(x[j, i], y[j, i])
+    return out
+
+
+def _pulse(amp, x0, y0, halfwidth):
+    k = LN2 / (halfwidth * halfwidth)
+    return lambda x, y: amp * math.exp(-k * ((x - x0) ** 2 + (y - y0) ** 2))
+
+
+def _uniform_mean(x, u=0.0, v=0.0):
+    nj, ni = x.shape
+    m = np.zeros((5, nj, ni))
+    m[1] = u
+    m[2] = v
+    m[4] = 1.0 / GAMMA
+    return m
+
+
+def _sponges(width, sides=("+xi", "-xi", "+eta", "-eta")):
+    w = width - 1
+    spec = {
+        "+xi": ("0", str(w), "0", "-0", "0", "0"),
+        "-xi": (f"-{w}", "-0", "0", "-0", "0", "0"),
+        "+eta": ("0", "-0", "0", str(w), "0", "0"),
+        "-eta": ("0", "-0", f"-{w}", "-0", "0", "0"),
+    }
+    return [(0, "sponge_freund_ambient", s) + spec[s] for s in sides]
+
+
+BASE_SCHEME = dict(fd_order=4, filter_order=8, filter_blend=1.0, rk=4,
+                   sponge_order=2, sponge_constant=5.0)
+
+
+def case_c1():
+    """201x201 on [-50,50]^2, quiescent, Gaussian pulse (1e-3, half-width
+    3 dx) at the origin, four 20-point sponges, RK4 CFL 0.5, 200 steps."""
+    ni = nj = 201
+    x, y = _grid(ni, nj, lambda i, j: -50.0 + 0.5 * i, lambda i, j: -50.0 + 0.5 * j)
+    ic = np.zeros((5, nj, ni))
+    ic[4] = _field(x, y, _pulse(1e-3, 0.0, 0.0, 1.5))
+    return dict(name="c1", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x), ic=ic,
+                bc=_sponges(20), mode="FIX_CFL", cfl=0.5, dt=None, nsteps=200,
+                simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
+
+
+def _wavy(ni, nj, lx, ly, amp_frac):
+    """x = xi + a sin(2 pi eta / L), y = eta + a sin(2 pi xi / L) on a box
+    [-lx/2, lx/2] x [-ly/2, ly/2] (the C2 mapping, scaled)."""
+    L = max(lx, ly)
+    a = amp_frac * L
+    dxi = lx / (ni - 1)
+    deta = ly / (nj - 1)
+
+    def xi(i):
+        return -0.5 * lx + dxi * i
+
+    def eta(j):
+        return -0.5 * ly + deta * j
+
+    return _grid(ni, nj,
+                 lambda i, j: xi(i) + a * math.sin(2.0 * math.pi * eta(j) / L),
+                 lambda i, j: eta(j) + a * math.sin(2.0 * math.pi * xi(i) / L))
+
+
+def case_curv():
+    """97x81 wavy curvilinear grid, uniform mean flow M = 0.5 in x, off-centre
+    pulse, 12-point sponges, RK4 CFL 0.5, 60 steps."""
+    ni, nj = 97, 81
+    x, y = _wavy(ni, nj, 48.0, 40.0, 0.05)
+    ic = np.zeros((5, nj, ni))
+    ic[4] = _field(x, y, _pulse(1e-3, -4.0, 2.0, 1.5))
+    return dict(name="curv", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x, u=0.5), ic=ic,
+                bc=_sponges(12), mode="FIX_CFL", cfl=0.5, dt=None, nsteps=60,
+                simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
+
+
+def case_filt():
+    """One step with DT = 1e-300 on a curvilinear grid: the RK update adds
+    nothing representable, so the output is the filter applied to the IC."""
+    ni, nj = 61, 53
+    x, y = _wavy(ni, nj, 30.0, 26.0, 0.05)
+    ic = np.zeros((5, nj, ni))
+    ic[0] = _field(x, y, _pulse(2e-4, 3.0, -2.0, 2.0))
+    ic[1] = _field(x, y, _pulse(5e-4, -2.0, 1.0, 1.0))
+    ic[2] = _field(x, y, _pulse(-4e-4, 1.0, 3.0, 1.2))
+    ic[4] = _field(x, y, lambda a, b: 1e-3 * math.exp(-LN2 * (a * a + b * b) / 0.8 ** 2)
+                   + 2e-4 * math.sin(0.9 * a) * math.cos(1.3 * b))
+    return dict(name="filt", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x, u=0.2, v=-0.1), ic=ic,
+                bc=[], mode="FIX_DT", cfl=None, dt=1e-300, nsteps=1,
+                simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
+
+
+def case_shear():
+    """81x65 Cartesian box with a non-uniform mean (shear layer in u, entropy
+    and pressure bumps) so that every mean-gradient term and the aux density
+    enter the RHS; pulse plus entropy spot IC; RK4 CFL 0.5, 40 steps."""
+    ni, nj = 81, 65
+    x, y = _grid(ni, nj, lambda i, j: -10.0 + 0.25 * i, lambda i, j: -8.0 + 0.25 * j)
+    mean = np.zeros((5, nj, ni))
+    mean[0] = _field(x, y, lambda a, b: 0.1 * math.exp(-(b * b) / 4.0))
+    mean[1] = _field(x, y, lambda a, b: 0.25 * (1.0 + math.tanh(b / 1.5)))
+    mean[2] = _field(x, y, lambda a, b: 0.02 * math.sin(0.3 * a))
+    mean[4] = _field(x, y, lambda a, b: (1.0 / GAMMA) * (1.0 + 0.05 * math.exp(-(a * a + b * b) / 16.0)))
+    ic = np.zeros((5, nj, ni))
+    ic[0] = _field(x, y, _pulse(1e-4, 2.0, -1.0, 1.0))
+    ic[4] = _field(x, y, _pulse(1e-3, -2.0, 0.5, 1.0))
+    return dict(name="shear", ni=ni, nj=nj, x=x, y=y, mean=mean, ic=ic,
+                bc=_sponges(10), mode="FIX_CFL", cfl=0.5, dt=None, nsteps=40,
+                simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
+
+
+def case_cylsrc():
+    """The reference's two-cylinder scattering source (a time-dependent,
+    ramped pressure source at the origin) on an 81x81 box, FIX_DT 0.01,
+    40 steps: pins the time at which sources are evaluated."""
+    ni = nj = 81
+    x, y = _grid(ni, nj, lambda i, j: -2.0 + 0.05 * i, lambda i, j: -2.0 + 0.05 * j)
+    ic = np.zeros((5, nj, ni))
+    return dict(name="cylsrc", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x), ic=ic,
+                bc=_sponges(10), mode="FIX_DT", cfl=None, dt=0.01, nsteps=40,
+                simulation="CASE_SCATTERING_TWOCYLINDER", **BASE_SCHEME)
+
+
+def case_wall():
+    """Wavy channel with wall_slip_kinematic on both eta faces, uniform mean
+    flow 0.3, pulse IC, sponges on both xi faces; RK4 CFL 0.5, 60 steps."""
+    ni, nj = 121, 41
+    lx = 30.0
+
+    def xf(i, j):
+        return -15.0 + 0.25 * i
+
+    def yf(i, j):
+        return -5.0 + 0.25 * j + 0.6 * math.sin(2.0 * math.pi * xf(i, j) / lx)
+
+    x, y = _grid(ni, nj, xf, yf)
+    ic = np.zeros((5, nj, ni))
+    ic[4] = _field(x, y, _pulse(1e-3, -3.0, 0.0, 1.0))
+    bc = [(0, "wall_slip_kinematic", "+eta", "0", "-0", "0", "0", "0", "0"),
+          (0, "wall_slip_kinematic", "-eta", "0", "-0", "-0", "-0", "0", "0")]
+    bc += _sponges(12, sides=("+xi", "-xi"))
+    return dict(name="wall", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x, u=0.3), ic=ic,
+                bc=bc, mode="FIX_CFL", cfl=0.5, dt=None, nsteps=60,
+                simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
+
+
+def case_variants():
+    """Scheme variants on a curvilinear grid with a mean flow: RK2, 4th-order
+    filter blended at 0.7, FIX_CFL 0.4, 30 steps.  (The reference's 2nd-order
+    FD is not usable: its closure table is one entry long but two entries are
+    written, spatial_discretization.cpp:52,70-71, and runs diverge.)"""
+    ni, nj = 73, 65
+    x, y = _wavy(ni, nj, 36.0, 32.0, 0.04)
+    ic = np.zeros((5, nj, ni))
+    ic[4] = _field(x, y, _pulse(1e-3, 2.0, -1.0, 1.5))
+    ic[1] = _field(x, y, _pulse(3e-4, -1.0, 2.0, 1.2))
+    scheme = dict(BASE_SCHEME, filter_order=4, filter_blend=0.7, rk=2)
+    return dict(name="variants", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x, u=0.3, v=0.1), ic=ic,
+                bc=_sponges(8), mode="FIX_CFL", cfl=0.4, dt=None, nsteps=30,
+                simulation="CASE_TANNA_TPN49", **scheme)
+
+
+def case_jet():
+    """The reference's 2-D jet harmonic source (a cos(omega t) pressure
+    source) on a 97x73 box with 10-point sponges, FIX_DT 0.05, 40 steps."""
+    ni, nj = 97, 73
+    x, y = _grid(ni, nj, lambda i, j: -6.0 + 0.125 * i, lambda i, j: -4.5 + 0.125 * j)
+    ic = np.zeros((5, nj, ni))
+    return dict(name="jet", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x, u=0.2), ic=ic,
+                bc=_sponges(10), mode="FIX_DT", cfl=None, dt=0.05, nsteps=40,
+                simulation="CASE_2DJET_WITH_A_HARMONIC_SOURCE", **BASE_SCHEME)
+
+
+ALL = {f.__name__[5:]: f for f in (case_c1, case_curv, case_filt, case_shear, case_cylsrc, case_wall,
+                                    case_variants, case_jet)}
+
+
+def deck_items(case, grid_file="grid.xyz", mean_file="mean.q", bc_file="bc.in"):
+    """Deck lines for the reference solver (keys of src/core/input.cpp:99-284)."""
+    items = [
+        ("DIMENSION", "2"),
+        ("AXISYMMETRY", "FALSE"),
+        ("PHYSICAL_MODEL", "LEE"),
+        ("FLUID_MODEL", "IDEAL_GAS_CALORIC"),
+        ("SIMULATION", case["simulation"]),
+        ("GAMMA_SPECIFICHEAT", repr(GAMMA)),
+        ("FIX_DT_OR_CFL", case["mode"]),
+    ]
+    if case["mode"] == "FIX_CFL":
+        items.append(("CFL", repr(case["cfl"])))
+    else:
+        items.append(("DT", repr(case["dt"])))
+    items += [
+        ("TEMPORAL_SCHEME", "RUNGE_KUTTA4" if case["rk"] == 4 else "RUNGE_KUTTA2"),
+        ("NUM_TOTAL_TIMESTEP", str(case["nsteps"])),
+        ("REPORT_TIMESTEP", str(case["nsteps"])),
+        ("SPATIAL_SCHEME", f"STANDARD_CENTRAL, ORDER_ACCURACY = {case['fd_order']}"),
+    ]
+    if case["filter_order"]:
+        items.append(("SPATIAL_FILTER",
+                      f"STANDARD_CENTRAL, ORDER_ACCURACY = {case['filter_order']}, BLEND = {case['filter_blend']!r}"))
+    else:
+        items.append(("SPATIAL_FILTER", "NONE"))
+    items += [
+        ("BUFFER_ZONE", f"POLYNOMIAL_ORDER = {case['sponge_order']}, CONSTANT = {case['sponge_constant']!r}"),
+        ("METRIC_SCHEME", "THOMAS_LOMBARD"),
+        ("OVERSET", "NONE"),
+        ("DOMAIN_DECOMPOSITION", "DECOMP_1D"),
+        ("FILE_FORMAT", "PLOT3D"),
+        ("GRID_FILE", grid_file),
+        ("BASESTATE_FILE", mean_file),
+        ("BC_FILE", bc_file),
+        ("SOLUTION_WRITING_TIME", "1.0E+06"),
+        ("HARMONIC_WAVE", "NONE"),
+        ("INTERPOLATE_SOLUTION", "NONE"),
+    ]
+    return items

@@ -1,0 +1,98 @@
+"""Generate the golden fixtures by running the reference solver binary.
+
+Run from the repository root, in the build container only (the reference
+sources and the MPICH it links exist only there):
+
+    make -f oracle/ref.mk
+    python tests/golden/make_golden.py [case ...]
+
+For each case of tests/golden/cases.py this writes the reference's input
+files (deck, PLOT3D grid / base state / restart IC, BC file) into a scratch
+directory, runs `oracle/_ref/camille ic.q` there, and stores the outputs the
+parity tests need as tests/golden/<case>.npz:
+
+  sol      final solution (5, nj, ni) as written by the reference
+  tau      its PLOT3D tau header {0, 1, step, time}
+  met      metrics file camille.met.q (11, nj, ni), when requested
+  dts      dt printed at each report step (text, 6 digits; informative only)
+
+Only data (inputs are regenerated from cases.py, outputs stored) is kept.
+"""
+import glob
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+
+import cases  # noqa: E402
+import refio  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "camille")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+KEEP_METRICS = {"curv", "filt", "wall"}
+
+
+def run_case(case, nranks=1):
+    with tempfile.TemporaryDirectory(prefix=f"golden_{case['name']}_") as d:
+        x, y = case["x"], case["y"]
+        z = np.zeros_like(x)
+        refio.write_grid(os.path.join(d, "grid.xyz"), [(x[None], y[None], z[None])])
+        refio.write_function(os.path.join(d, "mean.q"), [case["mean"][:, None]])
+        refio.write_solution(os.path.join(d, "ic.q"), [case["ic"][:, None]], 0, 0.0)
+        refio.write_bc(os.path.join(d, "bc.in"), case["bc"])
+        refio.write_deck(os.path.join(d, "camille.in"), cases.deck_items(case))
+        env = dict(os.environ)
+        env["LD_LIBRARY_PATH"] = "/usr/lib/x86_64-linux-gnu:/opt/conda/lib"
+        cmd = [MPIEXEC, "-n", str(nranks), REF_BIN, "ic.q"]
+        p = subprocess.run(cmd, cwd=d, env=env, capture_output=True, text=True, timeout=3600)
+        out = p.stdout
+        if ">>> " in out and "graceful" in out.lower():
+            raise RuntimeError(out[-2000:])
+        files = sorted(glob.glob(os.path.join(d, "camille.[0-9]*.q")))
+        if len(files) < 2:
+            raise RuntimeError(f"reference wrote no final solution:\n{out[-3000:]}\n{p.stderr[-2000:]}")
+        sols, taus = refio.read_solution(files[-1])
+        res = dict(sol=sols[0][:, 0], tau=taus[0])
+        if case["name"] in KEEP_METRICS:
+            res["met"] = refio.read_function(os.path.join(d, "camille.met.q"))[0][:, 0]
+        dts = [ln for ln in out.splitlines() if ln.startswith("Time step:")]
+        res["log"] = "\n".join(dts)
+        return res
+
+
+def main(names):
+    names = names or list(cases.ALL)
+    manifest = {}
+    for name in names:
+        case = cases.ALL[name]()
+        res = run_case(case)
+        path = os.path.join(HERE, f"{name}.npz")
+        arrs = dict(tau=res["tau"])
+        sol = res["sol"]
+        # keep only the variables that are not identically zero
+        nz = [v for v in range(5) if np.any(sol[v] != 0.0)]
+        arrs["vars"] = np.array(nz, dtype=np.int32)
+        arrs["sol"] = sol[nz]
+        if "met" in res:
+            arrs["met"] = res["met"]
+        np.savez_compressed(path, **arrs)
+        manifest[name] = dict(nsteps=case["nsteps"], ni=case["ni"], nj=case["nj"], mode=case["mode"],
+                              cfl=case["cfl"], dt=case["dt"], simulation=case["simulation"],
+                              time=float(res["tau"][3]), step=int(res["tau"][2]), nonzero_vars=nz,
+                              report=res["log"])
+        print(name, "time", res["tau"][3], "vars", nz, "->", os.path.getsize(path), "bytes", flush=True)
+    mpath = os.path.join(HERE, "manifest.json")
+    old = json.load(open(mpath)) if os.path.exists(mpath) else {}
+    old.update(manifest)
+    json.dump(old, open(mpath, "w"), indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,101 @@
+"""Writers/readers for the reference solver's on-disk formats (test support).
+
+Only used to build inputs for, and read outputs of, the reference binary when
+golden fixtures are generated, and by tests that check the host-side readers.
+All binary files are raw native-endian int32/float64 with no record markers,
+i fastest (reference: src/io/plot3d.cpp:443-541 grid, :715-833 solution,
+:881-989 function; BC file src/io/io.cpp:957-1369).
+"""
+import numpy as np
+
+
+def write_grid(path, blocks):
+    """blocks: list of (x, y, z) arrays shaped (nk, nj, ni)."""
+    with open(path, "wb") as f:
+        np.array([len(blocks)], dtype=np.int32).tofile(f)
+        for x, _, _ in blocks:
+            nk, nj, ni = x.shape
+            np.array([ni, nj, nk], dtype=np.int32).tofile(f)
+        for x, y, z in blocks:
+            for c in (x, y, z):
+                np.ascontiguousarray(c, dtype=np.float64).tofile(f)
+
+
+def write_function(path, blocks):
+    """blocks: list of arrays shaped (nvar, nk, nj, ni)."""
+    with open(path, "wb") as f:
+        np.array([len(blocks)], dtype=np.int32).tofile(f)
+        for q in blocks:
+            nv, nk, nj, ni = q.shape
+            np.array([ni, nj, nk, nv], dtype=np.int32).tofile(f)
+        for q in blocks:
+            np.ascontiguousarray(q, dtype=np.float64).tofile(f)
+
+
+def write_solution(path, blocks, step=0, time=0.0):
+    """blocks: list of arrays shaped (5, nk, nj, ni); tau = {0, 1, step, time}."""
+    with open(path, "wb") as f:
+        np.array([len(blocks)], dtype=np.int32).tofile(f)
+        for q in blocks:
+            _, nk, nj, ni = q.shape
+            np.array([ni, nj, nk], dtype=np.int32).tofile(f)
+        for q in blocks:
+            np.array([0.0, 1.0, float(step), float(time)], dtype=np.float64).tofile(f)
+            np.ascontiguousarray(q, dtype=np.float64).tofile(f)
+
+
+def read_solution(path):
+    """Returns (list of (5, nk, nj, ni) arrays, list of tau arrays)."""
+    raw = open(path, "rb").read()
+    nb = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    dims = np.frombuffer(raw, np.int32, 3 * nb, 4).reshape(nb, 3)
+    off = 4 + 12 * nb
+    out, taus = [], []
+    for ni, nj, nk in dims:
+        taus.append(np.frombuffer(raw, np.float64, 4, off).copy())
+        off += 32
+        n = int(ni) * int(nj) * int(nk)
+        out.append(np.frombuffer(raw, np.float64, 5 * n, off).reshape(5, nk, nj, ni).copy())
+        off += 40 * n
+    return out, taus
+
+
+def read_function(path):
+    raw = open(path, "rb").read()
+    nb = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    dims = np.frombuffer(raw, np.int32, 4 * nb, 4).reshape(nb, 4)
+    off = 4 + 16 * nb
+    out = []
+    for ni, nj, nk, nv in dims:
+        n = int(ni) * int(nj) * int(nk) * int(nv)
+        out.append(np.frombuffer(raw, np.float64, n, off).reshape(nv, nk, nj, ni).copy())
+        off += 8 * n
+    return out
+
+
+def write_bc(path, lines):
+    """lines: iterable of (iblock, type, side, is, ie, js, je, ks, ke) with the
+    index tokens already formatted as strings (e.g. '-0', '19')."""
+    with open(path, "w") as f:
+        for ln in lines:
+            f.write(" ".join(str(t) for t in ln) + "\n")
+
+
+def write_deck(path, items):
+    """items: list of (name, value-string) in order."""
+    with open(path, "w") as f:
+        for k, v in items:
+            f.write(f"{k} = {v}\n")
+
+
+def sponge_lines(iblock, width, sides=("+xi", "-xi", "+eta", "-eta")):
+    """Freund ambient sponges of `width` points on the requested faces of a 2-D
+    block, in the BC-file index convention ('-k' means n-1-k)."""
+    w = width - 1
+    spec = {
+        "+xi": ("0", str(w), "0", "-0", "0", "0"),
+        "-xi": (f"-{w}", "-0", "0", "-0", "0", "0"),
+        "+eta": ("0", "-0", "0", str(w), "0", "0"),
+        "-eta": ("0", "-0", f"-{w}", "-0", "0", "0"),
+    }
+    return [(iblock, "sponge_freund_ambient", s) + spec[s] for s in sides]

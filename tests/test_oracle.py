@@ -1,0 +1,207 @@
+"""Pins the CPU oracle: bitwise against fixtures written by the reference
+binary, plus closed forms for what the fixtures leave open.  CPU only."""
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from oracle.pyoracle import Oracle, lib
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+
+
+def _golden(name):
+    return np.load(os.path.join(GOLD, f"{name}.npz"))
+
+
+def _run_case(name):
+    c = cases.ALL[name]()
+    o = Oracle.from_case(c)
+    o.run(c["nsteps"])
+    return c, o
+
+
+@pytest.mark.parametrize("name", sorted(cases.ALL))
+def test_golden_bitwise(name):
+    """The oracle reproduces the reference binary's output bit for bit."""
+    c, o = _run_case(name)
+    g = _golden(name)
+    sol = o.solution()[:, 0]
+    assert o.time == float(g["tau"][3])
+    assert o.time_step == int(g["tau"][2]) == c["nsteps"]
+    for k, v in enumerate(g["vars"]):
+        assert np.array_equal(sol[v], g["sol"][k]), f"var {v} differs"
+    for v in range(5):
+        if v not in g["vars"]:
+            assert np.all(sol[v] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["curv", "filt", "wall"])
+def test_metrics_match_reference_file(name):
+    """metrics[l][i] and J equal the reference's camille.met.q."""
+    c = cases.ALL[name]()
+    o = Oracle.from_case(c)
+    met, _, jac = o.metrics()
+    ref = _golden(name)["met"]
+    for r in range(3):
+        for cc in range(3):
+            assert np.array_equal(met[r, cc, 0], ref[3 * r + cc]), (r, cc)
+    assert np.array_equal(jac[0], ref[9])
+
+
+@pytest.mark.parametrize("plant,expect", [
+    (1, "curv"),    # |U m| through fabs instead of the reference's int abs
+    (2, "cylsrc"),  # source evaluated at t_n instead of t_{n+1}
+    (3, "shear"),   # aux density refreshed after the filter
+    (4, "curv"),    # metrics transposed in the chain rule
+    (5, "shear"),   # mean-gradient terms dropped
+    (6, "filt"),    # filter sweeps in the wrong order
+])
+def test_planted_mistakes_are_caught(plant, expect):
+    L = lib()
+    L.orc_set_plant.argtypes = [ctypes.c_int]
+    L.orc_set_plant(plant)
+    try:
+        _, o = _run_case(expect)
+        g = _golden(expect)
+        sol = o.solution()[:, 0]
+        same = all(np.array_equal(sol[v], g["sol"][k]) for k, v in enumerate(g["vars"]))
+        assert not same
+    finally:
+        L.orc_set_plant(0)
+
+
+def _box(ni, nj, dx=1.0, mean_u=0.0, scheme=None, bc=(), mode="FIX_DT", value=0.1, ic=None):
+    x = np.empty((nj, ni))
+    y = np.empty((nj, ni))
+    for j in range(nj):
+        for i in range(ni):
+            x[j, i] = dx * i
+            y[j, i] = dx * j
+    mean = np.zeros((5, nj, ni))
+    mean[1] = mean_u
+    mean[4] = 1.0 / cases.GAMMA
+    ic = np.zeros((5, nj, ni)) if ic is None else ic
+    sch = dict(cases.BASE_SCHEME) if scheme is None else scheme
+    return Oracle([dict(x=x, y=y, mean=mean, ic=ic)], sch, bc, None, mode, value), x, y
+
+
+def test_fd_polynomial_exactness():
+    """Interior OA4 rows differentiate degree <= 4 exactly; closure rows
+    degree <= 2 (unit spacing, so d/dxi of i^k is k i^(k-1))."""
+    o, x, y = _box(24, 9)
+    i = x.copy()
+    for k in range(5):
+        f = i ** k
+        d = o.apply_dir(f[None], 0, 0)[0]
+        exact = k * i ** (k - 1) if k > 0 else np.zeros_like(i)
+        err = np.abs(d - exact) / max(1.0, np.max(np.abs(exact)))
+        assert np.max(err[:, 2:-2]) < 1e-13, k
+        if k <= 2:
+            assert np.max(err) < 1e-13, k
+        else:
+            assert np.max(err[:, :2]) > 1e-6  # closures are only 2nd order
+
+
+def test_filter_properties():
+    """OA8 filter: constants pass (rows sum to 1), the interior removes the
+    odd-even mode exactly, rows 0 and 1 are the identity."""
+    o, x, y = _box(32, 12)
+    one = np.ones((1,) + x.shape)
+    f = o.apply_dir(one, 1, 0)
+    assert np.max(np.abs(f - 1.0)) < 4e-16
+    ii = x.astype(int)
+    nyq = ((-1.0) ** ii)[None]
+    g = o.apply_dir(nyq, 1, 0)[0]
+    assert np.max(np.abs(g[:, 4:-4])) < 1e-15
+    r = np.random.default_rng(0).standard_normal(x.shape)[None]
+    h = o.apply_dir(r, 1, 0)[0]
+    for col in (0, 1, -1, -2):
+        assert np.array_equal(h[:, col], r[0][:, col])
+
+
+def test_sponge_strength_closed_form():
+    """On a Cartesian grid, sigma = C ((W-1-i)/(W-1))^2 across a +xi zone;
+    overlapping zones add."""
+    W, C = 9, 5.0
+    bc = [(0, "sponge_freund_ambient", "+xi", "0", str(W - 1), "0", "-0", "0", "0"),
+          (0, "sponge_freund_ambient", "+eta", "0", "-0", "0", str(W - 1), "0", "0")]
+    o, x, y = _box(30, 25, dx=0.5, bc=bc)
+    s = o.sigma()[0]
+    ii = np.arange(30)
+    jj = np.arange(25)
+    sx = np.where(ii < W, C * ((W - 1 - ii) / (W - 1)) ** 2, 0.0)
+    sy = np.where(jj < W, C * ((W - 1 - jj) / (W - 1)) ** 2, 0.0)
+    exact = sx[None, :] + sy[:, None]
+    assert np.max(np.abs(s - exact)) < 1e-13
+
+
+def test_rk4_amplification_in_a_sponge():
+    """With u_bar = 0 and s_bar = 0 the entropy equation reduces to
+    ds'/dt = -sigma s' at every point, so one RK4 step multiplies s' by the
+    stability polynomial of z = -sigma dt (filter off)."""
+    W = 12
+    bc = [(0, "sponge_freund_ambient", "+xi", "0", str(W - 1), "0", "-0", "0", "0")]
+    sch = dict(cases.BASE_SCHEME, filter_order=0)
+    ni, nj = 40, 14
+    ic = np.zeros((5, nj, ni))
+    ic[0] = 1e-3
+    dt = 0.05
+    o, x, y = _box(ni, nj, dx=1.0, scheme=sch, bc=bc, mode="FIX_DT", value=dt, ic=ic)
+    sig = o.sigma()[0]
+    o.step()
+    s1 = o.solution()[0, 0]
+    z = -sig * dt
+    amp = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    assert np.max(np.abs(s1 / 1e-3 - amp)) < 1e-14
+
+
+def test_aux_closed_form():
+    """rho_bar = (gamma p_bar)^(1/gamma) e^(-s_bar); T_bar = gamma p_bar /((gamma-1) rho_bar)."""
+    o, x, y = _box(10, 8)
+    aux = o.aux()[:, 0]
+    assert np.allclose(aux[1], 1.0, rtol=0, atol=2e-16)
+    assert np.allclose(aux[3], 1.0 / (cases.GAMMA - 1.0), rtol=1e-15)
+
+
+def _hole_box(ring):
+    """Cartesian unit grid with a BLANKED rectangle, optionally surrounded by
+    a one-cell INTERPOLATED ring."""
+    ni, nj = 40, 30
+    o, x, y = _box(ni, nj)
+    ib = np.zeros((nj, ni), np.int32)
+    ib[10:20, 15:25] = 1
+    if ring:
+        ib[9:21, 14:26] = np.where(ib[9:21, 14:26] == 1, 1, -1)
+    o.set_iblank(ib[None])
+    return o, x, y, ib
+
+
+@pytest.mark.parametrize("ring", [True, False])
+def test_hole_closures(ring):
+    """d/dxi of a quadratic stays exact next to holes where closures are
+    inserted; hole-after-hole cells get 0; a fluid cell directly left of a
+    hole (no INTERPOLATED cell between) keeps the interior stencil, which
+    reads the hole (the reference's `last_iblank /= BLANKED` test)."""
+    o, x, y, ib = _hole_box(ring)
+    f = (x ** 2 + 3 * x)[None].copy()
+    f_holes = f.copy()
+    f_holes[0][ib == 1] = 0.0
+    d = o.apply_dir(f_holes, 0, 0)[0]
+    exact = 2 * x + 3
+    fluid = ib != 1
+    if ring:
+        assert np.max(np.abs(d[fluid] - exact[fluid])) < 1e-11
+    else:
+        # left of the hole: interior stencil reads zeros inside the hole
+        assert abs(d[12, 14] - exact[12, 14]) > 1.0
+        # right of the hole: closure inserted after a BLANKED cell
+        assert abs(d[12, 25] - exact[12, 25]) < 1e-11
+    inner = ib[12] == 1
+    first = np.argmax(inner)
+    assert np.all(d[12][first + 1:first + 10] == 0.0)
