@@ -1,0 +1,692 @@
+// C ABI of the LEE time advance (include/lee.h): handle lifecycle, the
+// reference's init sequence, step orchestration for the fused and plain
+// paths, and the unit entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lee_internal.h"
+
+using namespace lee;
+
+namespace {
+
+std::string g_create_err;
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      h->err = std::string("HIP error ") + hipGetErrorString(_e) + " at " #expr;       \
+      return LEE_EHIP;                                                                 \
+    }                                                                                  \
+  } while (0)
+
+int fail(lee_handle *h, int code, const std::string &msg) {
+  if (h) h->err = msg;
+  else g_create_err = msg;
+  return code;
+}
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return hipSuccess;
+  return hipMalloc((void **)p, sizeof(T) * count);
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+void free_block(Block &b) {
+  dfree(b.xyz);
+  dfree(b.met);
+  dfree(b.jac);
+  dfree(b.iblank);
+  dfree(b.code_fd);
+  dfree(b.code_flt);
+  dfree(b.mean);
+  dfree(b.mgrad);
+  dfree(b.rhobar);
+  dfree(b.rho64);
+  dfree(b.rho32);
+  for (int k = 0; k < 4; k++) {
+    dfree(b.Y64[k]);
+    dfree(b.Y32[k]);
+  }
+  dfree(b.dflux);
+  dfree(b.dsum);
+  dfree(b.func0);
+  dfree(b.func);
+  for (auto &z : b.zones) {
+    dfree(z.sigma64);
+    dfree(z.sigma32);
+  }
+  dfree(b.src.g);
+}
+
+bool is_fp32(const lee_handle *h) { return h->cfg.precision == LEE_FP32; }
+bool is_plain(const lee_handle *h) { return h->cfg.path == LEE_PATH_PLAIN; }
+
+// Buffer that receives a stage's boundary update in the fused path.
+int fused_target(const lee_handle *h, int stage) {
+  const int last = h->cfg.rk_stages - 1;
+  if (stage == last) return 1;
+  return stage % 2 == 0 ? 2 : 3;
+}
+
+int enqueue_fused_step(lee_handle *h) {
+  const int S = h->cfg.rk_stages, last = S - 1;
+  const double tn = h->time_sol, dt = h->dt;
+  const bool f32 = is_fp32(h);
+  for (int st = 0; st < S; st++) {
+    StageScalars sc{};
+    sc.h = dt;
+    sc.b = h->b_i[st];
+    sc.a = h->a_next[st];
+    source_factors(h, tn + h->c_i[st] * dt, sc.s1, sc.s2);
+    for (int ib = 0; ib < h->nblk; ib++) HIPCHK(fused_stage(h->stream, h, h->b[ib], st, sc));
+    const int tgt = fused_target(h, st);
+    for (int ib = 0; ib < h->nblk; ib++) {
+      Block &b = h->b[ib];
+      if (b.faces.empty()) continue;
+      const bool rho = st == last && !b.uniform_mean;
+      HIPCHK(apply_bcs(h->stream, h, b, b.Y64[tgt], b.Y32[tgt], f32, rho ? b.rho64 : nullptr,
+                       rho ? b.rho32 : nullptr));
+    }
+    HIPCHK(overset_interpolate(h->stream, h, f32, tgt, st == last));
+  }
+  for (int ib = 0; ib < h->nblk; ib++) {
+    Block &b = h->b[ib];
+    if (h->cfg.filter_order > 0) {
+      HIPCHK(fused_filter(h->stream, h, b, 1, 0));
+    } else {
+      // no filter: y_{n+1} becomes the state (holes are already at sol_ref)
+      std::swap(b.Y64[0], b.Y64[1]);
+      std::swap(b.Y32[0], b.Y32[1]);
+    }
+  }
+  return LEE_OK;
+}
+
+// temporal::time_advance with one kernel per reference operator (FP64).
+int enqueue_plain_step(lee_handle *h) {
+  const int S = h->cfg.rk_stages, last = S - 1;
+  const double tn = h->time_sol, dt = h->dt;
+  hipStream_t s = h->stream;
+  for (int ib = 0; ib < h->nblk; ib++) {
+    Block &b = h->b[ib];
+    const size_t bytes = sizeof(double) * NV * (size_t)b.n;
+    HIPCHK(hipMemcpyAsync(b.Y64[1], b.Y64[0], bytes, hipMemcpyDeviceToDevice, s));  // ynp1 = yn
+    HIPCHK(hipMemcpyAsync(b.Y64[2], b.Y64[0], bytes, hipMemcpyDeviceToDevice, s));  // interim = yn
+  }
+  for (int st = 0; st < S; st++) {
+    double s1, s2;
+    source_factors(h, tn + h->c_i[st] * dt, s1, s2);
+    for (int ib = 0; ib < h->nblk; ib++) {
+      Block &b = h->b[ib];
+      HIPCHK(plain_rhs(s, h, b, b.Y64[2], b.Y64[3], s1, s2, b.rho64));
+    }
+    for (int ib = 0; ib < h->nblk; ib++) {
+      Block &b = h->b[ib];
+      HIPCHK(plain_rk_update(s, b, b.Y64[3], b.Y64[1], b.Y64[2], b.Y64[0], dt, h->b_i[st], h->a_next[st],
+                             st == last));
+    }
+    const int tgt = st < last ? 2 : 1;
+    for (int ib = 0; ib < h->nblk; ib++) {
+      Block &b = h->b[ib];
+      if (!b.faces.empty()) HIPCHK(apply_bcs(s, h, b, b.Y64[tgt], nullptr, false, nullptr, nullptr));
+    }
+    HIPCHK(overset_interpolate(s, h, false, tgt, false));
+    for (int ib = 0; ib < h->nblk; ib++) {
+      Block &b = h->b[ib];
+      HIPCHK(aux_density(s, b, h->cfg.gamma, b.Y64[tgt], b.rho64));
+    }
+  }
+  for (int ib = 0; ib < h->nblk; ib++) {
+    Block &b = h->b[ib];
+    if (h->cfg.filter_order > 0) HIPCHK(plain_filter(s, h, b, b.Y64[1]));
+    HIPCHK(plain_iblank_reset(s, b, b.Y64[1]));
+    std::swap(b.Y64[0], b.Y64[1]);
+  }
+  return LEE_OK;
+}
+
+int enqueue_step(lee_handle *h) { return is_plain(h) ? enqueue_plain_step(h) : enqueue_fused_step(h); }
+
+// solver::solve step prologue (solver.cpp:41-75): CFL at the current dt,
+// dt rescaled under FIX_CFL, time_sol += dt.
+void advance_clock(lee_handle *h) {
+  h->time_step++;
+  const double cmax = cfl_max(h);
+  double dt = h->dt;
+  if (h->cfg.fix_cfl) {
+    const double ratio = h->cfg.cfl_or_dt / cmax;
+    dt *= ratio;
+    h->dt = dt;
+  }
+  h->time_sol += dt;
+}
+
+int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
+  if (!c) return fail(nullptr, LEE_EINVAL, "null config");
+  if (c->num_dim != 2) return fail(nullptr, LEE_EINVAL, "only DIMENSION = 2 is supported");
+  if (c->fd_order != 4)
+    return fail(nullptr, LEE_EINVAL, "SPATIAL_SCHEME STANDARD_CENTRAL ORDER_ACCURACY must be 4");
+  if (c->filter_order != 0 && c->filter_order != 4 && c->filter_order != 8)
+    return fail(nullptr, LEE_EINVAL, "filter order must be 0 (NONE), 4 or 8");
+  if (c->filter_blend < 0.0 || c->filter_blend > 1.0)
+    return fail(nullptr, LEE_EINVAL, "The filter-blending parameter is out-of-bound.");
+  if (c->rk_stages != 2 && c->rk_stages != 4) return fail(nullptr, LEE_EINVAL, "Unknown temporal scheme.");
+  if (c->precision != LEE_FP64 && c->precision != LEE_FP32) return fail(nullptr, LEE_EINVAL, "bad precision");
+  if (c->path != LEE_PATH_FUSED && c->path != LEE_PATH_PLAIN) return fail(nullptr, LEE_EINVAL, "bad path");
+  if (c->path == LEE_PATH_PLAIN && c->precision != LEE_FP64)
+    return fail(nullptr, LEE_EINVAL, "the plain path runs in FP64 only");
+  if (c->source_kind < LEE_SOURCE_NONE || c->source_kind > LEE_SOURCE_MONOPOLE)
+    return fail(nullptr, LEE_EINVAL, "unknown source");
+  if (c->source_kind == LEE_SOURCE_MONOPOLE && !(c->source_params[5] > 0.0))
+    return fail(nullptr, LEE_EINVAL, "monopole half-width must be positive");
+  if (nblocks < 1 || nblocks > MAX_BLOCKS) return fail(nullptr, LEE_EINVAL, "block count must be 1..16");
+  const int minlen = c->filter_order > 0 ? c->filter_order + 1 : 5;
+  for (int ib = 0; ib < nblocks; ib++) {
+    const lee_block &b = blocks[ib];
+    if (!b.xyz || !b.mean) return fail(nullptr, LEE_EINVAL, "block without coordinates or base state");
+    if (b.nk != 1) return fail(nullptr, LEE_EINVAL, "2-D blocks need nk = 1");
+    if (b.ni < minlen || b.nj < minlen) return fail(nullptr, LEE_EINVAL, "block too small for the stencils");
+    if ((long long)b.ni * b.nj > (1LL << 30)) return fail(nullptr, LEE_EINVAL, "block too large");
+    for (int q = 0; q < b.nboundary; q++) {
+      const lee_boundary &r = b.boundary[q];
+      const int nd[3] = {b.ni, b.nj, b.nk};
+      if (r.dir < 0 || r.dir > 1 || (r.end != 0 && r.end != 1))
+        return fail(nullptr, LEE_EINVAL, "boundary with a bad direction or side");
+      for (int d = 0; d < 3; d++)
+        if (r.is[d] < 0 || r.ie[d] >= nd[d] || r.is[d] > r.ie[d])
+          return fail(nullptr, LEE_EINVAL, "boundary box outside the block");
+      switch (r.type) {
+        case LEE_BC_DIRICHLET_ALLZERO:
+        case LEE_BC_NEUMANN:
+        case LEE_BC_WALL_SLIP_KINEMATIC:
+        case LEE_BC_CENTERLINE_CART_NORM2X:
+        case LEE_BC_CENTERLINE_CART_NORM2Y:
+        case LEE_BC_CENTERLINE_CART_NORM2Z:
+        case LEE_SPONGE_FREUND_AMBIENT:
+          break;
+        default:
+          return fail(nullptr, LEE_EINVAL, "unsupported boundary type " + std::to_string(r.type));
+      }
+    }
+  }
+  return LEE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lee_version(void) { return "camille_amd lee 0.1 (gfx950)"; }
+
+const char *lee_last_error(const lee_handle *h) { return h ? h->err.c_str() : g_create_err.c_str(); }
+
+void lee_destroy(lee_handle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (int ib = 0; ib < h->nblk; ib++) free_block(h->b[ib]);
+  dfree(h->ov.rb);
+  dfree(h->ov.rl);
+  dfree(h->ov.db);
+  dfree(h->ov.dl);
+  dfree(h->ov.w);
+  dfree(h->ov.coef);
+  dfree(h->ov.buf);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int lee_create(const lee_config *cfg, const lee_block *blocks, int nblocks, const lee_overset *ov,
+               lee_handle **out) {
+  if (!out) return fail(nullptr, LEE_EINVAL, "null output handle");
+  *out = nullptr;
+  int rc = validate(cfg, blocks, nblocks);
+  if (rc) return rc;
+  lee_handle *h = new (std::nothrow) lee_handle();
+  if (!h) return fail(nullptr, LEE_ENOMEM, "out of host memory");
+  h->cfg = *cfg;
+  h->nblk = nblocks;
+  rc = [&]() -> int {
+    HIPCHK(hipGetDevice(&h->device));
+    // a blocking stream: host-blocking copies order with the queued kernels
+    HIPCHK(hipStreamCreate(&h->stream));
+    HIPCHK(hipEventCreate(&h->ev0));
+    HIPCHK(hipEventCreate(&h->ev1));
+    make_fd_table(h->fd);
+    if (cfg->filter_order > 0) make_filter_table(h->flt, cfg->filter_order);
+    // Butcher tableau (RungeKutta::initialize, temporal:33-85)
+    if (cfg->rk_stages == 4) {
+      h->b_i[0] = 1.0 / 6.0; h->b_i[1] = 1.0 / 3.0; h->b_i[2] = 1.0 / 3.0; h->b_i[3] = 1.0 / 6.0;
+      h->c_i[0] = 0.0; h->c_i[1] = 1.0 / 2.0; h->c_i[2] = 1.0 / 2.0; h->c_i[3] = 1.0;
+      h->a_next[0] = 1.0 / 2.0; h->a_next[1] = 1.0 / 2.0; h->a_next[2] = 1.0;
+    } else {
+      h->b_i[0] = 1.0 / 2.0; h->b_i[1] = 1.0 / 2.0;
+      h->c_i[0] = 0.0; h->c_i[1] = 1.0;
+      h->a_next[0] = 1.0;
+    }
+    h->dt = cfg->fix_cfl ? 1e-6 : cfg->cfl_or_dt;  // input.cpp:130
+    hipStream_t s = h->stream;
+    // iblank from the overset tables: holes, then receivers
+    // (override_iblank_4holeCells / _4interpolatedCells, overset.cpp:683-790)
+    for (int ib = 0; ib < nblocks; ib++) {
+      Block &b = h->b[ib];
+      b.ni = blocks[ib].ni;
+      b.nj = blocks[ib].nj;
+      b.nk = 1;
+      b.n = b.ni * b.nj;
+      b.h_iblank.assign(b.n, (int8_t)NOTBLANKED);
+    }
+    if (ov && ov->nrecv > 0) {
+      for (int q = 0; q < ov->nhole; q++) {
+        const int bb = ov->hole_block[q], l = ov->hole_l[q];
+        if (bb < 0 || bb >= nblocks || l < 0 || l >= h->b[bb].n) return fail(h, LEE_EINVAL, "bad hole index");
+        h->b[bb].h_iblank[l] = (int8_t)BLANKED;
+      }
+      for (int r = 0; r < ov->nrecv; r++) {
+        const int bb = ov->recv_block[r], l = ov->recv_l[r];
+        if (bb < 0 || bb >= nblocks || l < 0 || l >= h->b[bb].n) return fail(h, LEE_EINVAL, "bad receiver index");
+        h->b[bb].h_iblank[l] = (int8_t)INTERPOLATED;
+      }
+    }
+    for (int ib = 0; ib < nblocks; ib++) {
+      Block &b = h->b[ib];
+      const lee_block &in = blocks[ib];
+      const int n = b.n;
+      b.has_holes = std::any_of(b.h_iblank.begin(), b.h_iblank.end(), [](int8_t v) { return v != 0; });
+      b.uniform_mean = true;
+      for (int v = 0; v < NV; v++) {
+        b.umean[v] = in.mean[(size_t)v * n];
+        for (int p = 1; p < n && b.uniform_mean; p++)
+          if (in.mean[(size_t)v * n + p] != b.umean[v]) b.uniform_mean = false;
+      }
+      HIPCHK(dalloc(&b.xyz, 3 * (size_t)n));
+      HIPCHK(dalloc(&b.met, 4 * (size_t)n));
+      HIPCHK(dalloc(&b.jac, (size_t)n));
+      HIPCHK(dalloc(&b.mean, NV * (size_t)n));
+      HIPCHK(dalloc(&b.mgrad, 2 * NV * (size_t)n));
+      HIPCHK(dalloc(&b.rhobar, (size_t)n));
+      HIPCHK(dalloc(&b.rho64, (size_t)n));
+      HIPCHK(dalloc(&b.dflux, (size_t)n));
+      HIPCHK(dalloc(&b.dsum, (size_t)n));
+      for (int k = 0; k < 4; k++) HIPCHK(dalloc(&b.Y64[k], NV * (size_t)n));
+      if (is_fp32(h)) {
+        HIPCHK(dalloc(&b.rho32, (size_t)n));
+        for (int k = 0; k < 4; k++) HIPCHK(dalloc(&b.Y32[k], NV * (size_t)n));
+      }
+      if (is_plain(h)) {
+        HIPCHK(dalloc(&b.func0, (size_t)n));
+        HIPCHK(dalloc(&b.func, (size_t)n));
+      }
+      for (int k = 0; k < 4; k++) HIPCHK(hipMemsetAsync(b.Y64[k], 0, sizeof(double) * NV * n, s));
+      if (is_fp32(h))
+        for (int k = 0; k < 4; k++) HIPCHK(hipMemsetAsync(b.Y32[k], 0, sizeof(float) * NV * n, s));
+      HIPCHK(hipMemcpyAsync(b.xyz, in.xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(b.mean, in.mean, sizeof(double) * NV * n, hipMemcpyHostToDevice, s));
+      // rho_bar = (gamma p_bar)^(1/gamma) e^(-s_bar)  (state.cpp:713-716), host libm
+      {
+        std::vector<double> rb(n);
+        const double g = cfg->gamma, gi = 1.0 / g;
+        for (int p = 0; p < n; p++)
+          rb[p] = std::pow(g * in.mean[(size_t)IP * n + p], gi) * std::exp(-in.mean[(size_t)IS * n + p]);
+        HIPCHK(hipMemcpy(b.rhobar, rb.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      }
+      // metrics before any hole (simulation.cpp:207)
+      HIPCHK(metrics2d(s, b, h->fd));
+      b.h_met.resize(4 * (size_t)n);
+      b.h_jac.resize(n);
+      HIPCHK(hipMemcpyAsync(b.h_met.data(), b.met, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(b.h_jac.data(), b.jac, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+      // operator codes around holes
+      if (b.has_holes) {
+        b.h_code_fd.resize(2 * (size_t)n);
+        std::string err;
+        for (int d = 0; d < 2; d++)
+          if (!plan_codes(b, h->fd, d, b.h_code_fd.data() + (size_t)d * n, err)) return fail(h, LEE_EHOLES, err);
+        HIPCHK(dalloc(&b.iblank, (size_t)n));
+        HIPCHK(dalloc(&b.code_fd, 2 * (size_t)n));
+        HIPCHK(hipMemcpy(b.iblank, b.h_iblank.data(), n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b.code_fd, b.h_code_fd.data(), 2 * (size_t)n, hipMemcpyHostToDevice));
+        if (cfg->filter_order > 0) {
+          b.h_code_flt.resize(2 * (size_t)n);
+          for (int d = 0; d < 2; d++)
+            if (!plan_codes(b, h->flt, d, b.h_code_flt.data() + (size_t)d * n, err))
+              return fail(h, LEE_EHOLES, err);
+          HIPCHK(dalloc(&b.code_flt, 2 * (size_t)n));
+          HIPCHK(hipMemcpy(b.code_flt, b.h_code_flt.data(), 2 * (size_t)n, hipMemcpyHostToDevice));
+        }
+      }
+      // mean gradients after the holes exist (simulation.cpp:240, spatial:2250-2271)
+      for (int v = 0; v < NV; v++)
+        for (int d = 0; d < 2; d++) {
+          HIPCHK(plain_take_derivative_xyz(s, b, h->fd, b.mean + (size_t)v * n, d));
+          HIPCHK(hipMemcpyAsync(b.mgrad + (size_t)(2 * v + d) * n, b.dflux, sizeof(double) * n,
+                                hipMemcpyDeviceToDevice, s));
+        }
+      // sponges and other boundaries (io.cpp:957-1369)
+      for (int q = 0; q < in.nboundary; q++) {
+        const lee_boundary &r = in.boundary[q];
+        if (r.type == LEE_SPONGE_FREUND_AMBIENT) {
+          SpongeZone z;
+          for (int d = 0; d < 3; d++) {
+            z.is[d] = r.is[d];
+            z.ie[d] = r.ie[d];
+          }
+          compute_sponge_zone(b, in.xyz, z, r.dir, r.end, cfg->sponge_order, cfg->sponge_constant);
+          const size_t cnt = z.host_sigma.size();
+          HIPCHK(dalloc(&z.sigma64, cnt));
+          HIPCHK(hipMemcpy(z.sigma64, z.host_sigma.data(), sizeof(double) * cnt, hipMemcpyHostToDevice));
+          if (is_fp32(h)) {
+            std::vector<float> f(cnt);
+            for (size_t k = 0; k < cnt; k++) f[k] = (float)z.host_sigma[k];
+            HIPCHK(dalloc(&z.sigma32, cnt));
+            HIPCHK(hipMemcpy(z.sigma32, f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice));
+          }
+          b.zones.push_back(std::move(z));
+        } else {
+          BcFace f;
+          f.type = r.type;
+          f.dir = r.dir;
+          f.end = r.end;
+          for (int d = 0; d < 3; d++) {
+            f.is[d] = r.is[d];
+            f.ie[d] = r.ie[d];
+          }
+          const int face = r.end == 0 ? r.is[r.dir] : r.ie[r.dir];
+          f.is[r.dir] = f.ie[r.dir] = face;
+          b.faces.push_back(f);
+        }
+      }
+      if ((int)b.zones.size() > MAX_ZONES) return fail(h, LEE_EINVAL, "more than 8 sponge zones in a block");
+      plan_source(h, b, in.xyz);
+      if (!b.src.host_g.empty()) {
+        HIPCHK(dalloc(&b.src.g, b.src.host_g.size()));
+        HIPCHK(hipMemcpy(b.src.g, b.src.host_g.data(), sizeof(double) * b.src.host_g.size(),
+                         hipMemcpyHostToDevice));
+      }
+      HIPCHK(hipStreamSynchronize(s));
+      plan_cfl(b, in.mean, cfg->gamma);
+    }
+    if (ov && ov->nrecv > 0) {
+      Overset &o = h->ov;
+      o.nrecv = ov->nrecv;
+      o.stride = ov->coef_stride;
+      o.h_rb.assign(ov->recv_block, ov->recv_block + o.nrecv);
+      o.h_rl.assign(ov->recv_l, ov->recv_l + o.nrecv);
+      o.h_db.assign(ov->donor_block, ov->donor_block + o.nrecv);
+      o.h_dl.assign(ov->donor_l, ov->donor_l + o.nrecv);
+      o.h_w.assign(ov->widths, ov->widths + 3 * nblocks);
+      o.h_coef.assign(ov->coef, ov->coef + (size_t)o.nrecv * o.stride);
+      for (int r = 0; r < o.nrecv; r++) {
+        const int db = o.h_db[r];
+        if (db < 0 || db >= nblocks) return fail(h, LEE_EINVAL, "bad donor block");
+        const int w1 = o.h_w[3 * db], w2 = o.h_w[3 * db + 1], w3 = o.h_w[3 * db + 2];
+        const int ns = w1 * w2 * w3;
+        if (w3 != 1 || ns > o.stride || ns < 1) return fail(h, LEE_EINVAL, "bad donor stencil width");
+        const Block &d = h->b[db];
+        const int l = o.h_dl[r], i = l % d.ni, j = l / d.ni;
+        if (l < 0 || i + w1 - 1 >= d.ni || j + w2 - 1 >= d.nj)
+          return fail(h, LEE_EINVAL, "donor stencil leaves its block");  // overset.cpp:536-537
+        // initialize_donor (overset.cpp:516-526): c *= 1 / sum(c)
+        double *c = o.h_coef.data() + (size_t)r * o.stride;
+        double sum = 0.0;
+        for (int q = 0; q < ns; q++) sum += c[q];
+        sum = 1.0 / sum;
+        for (int q = 0; q < ns; q++) c[q] *= sum;
+      }
+      HIPCHK(dalloc(&o.rb, o.nrecv));
+      HIPCHK(dalloc(&o.rl, o.nrecv));
+      HIPCHK(dalloc(&o.db, o.nrecv));
+      HIPCHK(dalloc(&o.dl, o.nrecv));
+      HIPCHK(dalloc(&o.w, 3 * (size_t)nblocks));
+      HIPCHK(dalloc(&o.coef, (size_t)o.nrecv * o.stride));
+      HIPCHK(dalloc(&o.buf, (size_t)o.nrecv * NV));
+      HIPCHK(hipMemcpy(o.rb, o.h_rb.data(), sizeof(int) * o.nrecv, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(o.rl, o.h_rl.data(), sizeof(int) * o.nrecv, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(o.db, o.h_db.data(), sizeof(int) * o.nrecv, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(o.dl, o.h_dl.data(), sizeof(int) * o.nrecv, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(o.w, o.h_w.data(), sizeof(int) * 3 * nblocks, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(o.coef, o.h_coef.data(), sizeof(double) * o.h_coef.size(), hipMemcpyHostToDevice));
+    }
+    return LEE_OK;
+  }();
+  if (rc) {
+    g_create_err = h->err;
+    lee_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return LEE_OK;
+}
+
+int lee_set_solution(lee_handle *h, int ib, const double *sol) {
+  if (!h || !sol || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  Block &b = h->b[ib];
+  const int n = b.n;
+  hipStream_t s = h->stream;
+  HIPCHK(hipStreamSynchronize(s));
+  b.carry_w = false;
+  for (int p = 0; p < n; p++)
+    if (sol[(size_t)IUZ * n + p] != 0.0) {
+      b.carry_w = true;
+      break;
+    }
+  HIPCHK(hipMemcpy(b.Y64[0], sol, sizeof(double) * NV * n, hipMemcpyHostToDevice));
+  // compute_dependent_variables after a restart (simulation.cpp:178-180)
+  HIPCHK(aux_density(s, b, h->cfg.gamma, b.Y64[0], b.rho64));
+  if (is_fp32(h)) {
+    HIPCHK(cast_state(s, b, NV, b.Y64[0], b.Y32[0], (size_t)NV * n));
+    HIPCHK(cast_state(s, b, 1, b.rho64, b.rho32, (size_t)n));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  h->have_solution = true;
+  return LEE_OK;
+}
+
+int lee_get_solution(lee_handle *h, int ib, double *sol) {
+  if (!h || !sol || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  Block &b = h->b[ib];
+  hipStream_t s = h->stream;
+  const size_t cnt = (size_t)NV * b.n;
+  if (is_fp32(h)) {
+    HIPCHK(cast_back(s, b, NV, b.Y32[0], b.Y64[3], cnt));
+    HIPCHK(hipMemcpyAsync(sol, b.Y64[3], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+  } else {
+    HIPCHK(hipMemcpyAsync(sol, b.Y64[0], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return LEE_OK;
+}
+
+int lee_set_time(lee_handle *h, int time_step, double time_sol, double dt) {
+  if (!h) return LEE_EINVAL;
+  h->time_step = time_step;
+  h->time_sol = time_sol;
+  h->dt = dt;
+  return LEE_OK;
+}
+
+int lee_get_time(const lee_handle *h, int *time_step, double *time_sol, double *dt) {
+  if (!h) return LEE_EINVAL;
+  if (time_step) *time_step = h->time_step;
+  if (time_sol) *time_sol = h->time_sol;
+  if (dt) *dt = h->dt;
+  return LEE_OK;
+}
+
+int lee_compute_cfl(lee_handle *h, double *cmax) {
+  if (!h || !cmax) return fail(h, LEE_EINVAL, "bad argument");
+  *cmax = cfl_max(h);
+  return LEE_OK;
+}
+
+int lee_time_advance(lee_handle *h) {
+  if (!h) return LEE_EINVAL;
+  if (!h->have_solution) return fail(h, LEE_ESTATE, "no solution set");
+  int rc = enqueue_step(h);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return LEE_OK;
+}
+
+int lee_solve(lee_handle *h, int nsteps) {
+  if (!h || nsteps < 0) return fail(h, LEE_EINVAL, "bad argument");
+  if (!h->have_solution) return fail(h, LEE_ESTATE, "no solution set");
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int k = 0; k < nsteps; k++) {
+    advance_clock(h);
+    int rc = enqueue_step(h);
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  return LEE_OK;
+}
+
+int lee_synchronize(lee_handle *h) {
+  if (!h) return LEE_EINVAL;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  return LEE_OK;
+}
+
+int lee_last_solve_ms(lee_handle *h, float *ms) {
+  if (!h || !ms) return LEE_EINVAL;
+  *ms = h->last_ms;
+  return LEE_OK;
+}
+
+int lee_compute_rhs(lee_handle *h, int ib, const double *y, double time, double *rhs) {
+  if (!h || !y || !rhs || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  Block &b = h->b[ib];
+  const size_t cnt = (size_t)NV * b.n;
+  hipStream_t s = h->stream;
+  double s1, s2;
+  source_factors(h, time, s1, s2);
+  HIPCHK(hipMemcpy(b.Y64[2], y, sizeof(double) * cnt, hipMemcpyHostToDevice));
+  if (is_plain(h)) {
+    HIPCHK(plain_rhs(s, h, b, b.Y64[2], b.Y64[3], s1, s2, b.dsum));
+    HIPCHK(hipMemcpyAsync(rhs, b.Y64[3], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+  } else {
+    StageScalars sc{1.0, 1.0, 0.0, s1, s2};
+    if (is_fp32(h)) {
+      HIPCHK(cast_state(s, b, NV, b.Y64[2], b.Y32[2], cnt));
+      HIPCHK(hipMemsetAsync(b.Y32[3], 0, sizeof(float) * cnt, s));
+      HIPCHK(fused_stage(s, h, b, -1, sc));
+      HIPCHK(cast_back(s, b, NV, b.Y32[3], b.Y64[3], cnt));
+    } else {
+      HIPCHK(hipMemsetAsync(b.Y64[3], 0, sizeof(double) * cnt, s));
+      HIPCHK(fused_stage(s, h, b, -1, sc));
+    }
+    HIPCHK(hipMemcpyAsync(rhs, b.Y64[3], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return LEE_OK;
+}
+
+int lee_apply_filter(lee_handle *h, int ib, double *y) {
+  if (!h || !y || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  if (h->cfg.filter_order == 0) return LEE_OK;
+  Block &b = h->b[ib];
+  const size_t cnt = (size_t)NV * b.n;
+  hipStream_t s = h->stream;
+  HIPCHK(hipMemcpy(b.Y64[1], y, sizeof(double) * cnt, hipMemcpyHostToDevice));
+  if (is_plain(h)) {
+    HIPCHK(plain_filter(s, h, b, b.Y64[1]));
+    HIPCHK(hipMemcpyAsync(y, b.Y64[1], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+  } else if (is_fp32(h)) {
+    HIPCHK(cast_state(s, b, NV, b.Y64[1], b.Y32[1], cnt));
+    HIPCHK(cast_state(s, b, NV, b.Y64[1], b.Y32[2], cnt));
+    HIPCHK(fused_filter(s, h, b, 1, 2));
+    HIPCHK(cast_back(s, b, NV, b.Y32[2], b.Y64[2], cnt));
+    HIPCHK(hipMemcpyAsync(y, b.Y64[2], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+  } else {
+    HIPCHK(hipMemcpyAsync(b.Y64[2], b.Y64[1], sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
+    HIPCHK(fused_filter(s, h, b, 1, 2));
+    HIPCHK(hipMemcpyAsync(y, b.Y64[2], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return LEE_OK;
+}
+
+int lee_take_derivative_xyz(lee_handle *h, int ib, const double *f, int dir_xyz, double *out) {
+  if (!h || !f || !out || ib < 0 || ib >= h->nblk || dir_xyz < 0 || dir_xyz > 1)
+    return fail(h, LEE_EINVAL, "bad argument");
+  Block &b = h->b[ib];
+  hipStream_t s = h->stream;
+  HIPCHK(hipMemcpy(b.dsum, f, sizeof(double) * b.n, hipMemcpyHostToDevice));
+  HIPCHK(plain_take_derivative_xyz(s, b, h->fd, b.dsum, dir_xyz));
+  HIPCHK(hipMemcpyAsync(out, b.dflux, sizeof(double) * b.n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return LEE_OK;
+}
+
+int lee_update_boundary(lee_handle *h, double *const *ys) {
+  if (!h || !ys) return fail(h, LEE_EINVAL, "bad argument");
+  hipStream_t s = h->stream;
+  for (int ib = 0; ib < h->nblk; ib++)
+    HIPCHK(hipMemcpy(h->b[ib].Y64[2], ys[ib], sizeof(double) * NV * h->b[ib].n, hipMemcpyHostToDevice));
+  for (int ib = 0; ib < h->nblk; ib++)
+    if (!h->b[ib].faces.empty()) HIPCHK(apply_bcs(s, h, h->b[ib], h->b[ib].Y64[2], nullptr, false, nullptr, nullptr));
+  HIPCHK(overset_interpolate(s, h, false, 2, false));
+  for (int ib = 0; ib < h->nblk; ib++)
+    HIPCHK(hipMemcpyAsync(ys[ib], h->b[ib].Y64[2], sizeof(double) * NV * h->b[ib].n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return LEE_OK;
+}
+
+int lee_get_metrics(lee_handle *h, int ib, double *metrics, double *jac) {
+  if (!h || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  Block &b = h->b[ib];
+  const size_t n = b.n;
+  if (metrics) {
+    // metrics[l][i] for l, i < 3; the 2-D block fills the upper 2x2
+    std::fill(metrics, metrics + 9 * n, 0.0);
+    const int map[4] = {0, 1, 3, 4};
+    for (int q = 0; q < 4; q++) std::copy(b.h_met.begin() + q * n, b.h_met.begin() + (q + 1) * n, metrics + map[q] * n);
+  }
+  if (jac) std::copy(b.h_jac.begin(), b.h_jac.end(), jac);
+  return LEE_OK;
+}
+
+int lee_get_iblank(lee_handle *h, int ib, int *iblank) {
+  if (!h || !iblank || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  for (int p = 0; p < h->b[ib].n; p++) iblank[p] = h->b[ib].h_iblank[p];
+  return LEE_OK;
+}
+
+int lee_get_mean_gradient(lee_handle *h, int ib, double *grad) {
+  if (!h || !grad || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  Block &b = h->b[ib];
+  HIPCHK(hipMemcpy(grad, b.mgrad, sizeof(double) * 2 * NV * b.n, hipMemcpyDeviceToHost));
+  return LEE_OK;
+}
+
+int lee_get_sponge(lee_handle *h, int ib, double *sigma) {
+  if (!h || !sigma || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  Block &b = h->b[ib];
+  std::fill(sigma, sigma + b.n, 0.0);
+  for (const auto &z : b.zones)
+    for (int k = 0; k < z.nz[2]; k++)
+      for (int j = 0; j < z.nz[1]; j++)
+        for (int i = 0; i < z.nz[0]; i++)
+          sigma[(size_t)(j + z.is[1]) * b.ni + (i + z.is[0])] += z.host_sigma[((size_t)k * z.nz[1] + j) * z.nz[0] + i];
+  return LEE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1100 @@
+// HIP kernels of the LEE time advance for gfx950 (CDNA4).
+//
+// Two families:
+//  * plain_*  -- one kernel per reference operator (take_derivative_xyz, each
+//    RHS term, sponge, source, RK update, filter sweep, blend, aux), FP64 with
+//    floating-point contraction off and the reference's operation order, so
+//    the GPU result equals the CPU reference bit for bit.  This is the
+//    drop-in FP64 path and the yardstick for the fused path.
+//  * fused_*  -- one LDS-tiled kernel per RK stage (both directional
+//    derivatives of the four active variables, chain rule with FP64 metrics,
+//    LEE right-hand side, source, sponge, RK update, aux density) and one
+//    LDS-tiled kernel per step for the two filter sweeps + blend + hole
+//    reset.  Templated on the bulk type (double or float).
+//
+// All global indices are bounds-checked against the block extents.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "lee_internal.h"
+
+namespace lee {
+
+namespace {
+
+constexpr int TPB = 256;
+
+__device__ __forceinline__ int default_code(int c, int len, int nb) {
+  if (c < nb) return CODE_LEFT + c;
+  if (c > len - 1 - nb) return CODE_LEFT + nb + (len - 1 - c);
+  return CODE_INTERIOR;
+}
+
+// ------------------------------------------------------------------ plain
+
+// One row of the reference operator at point p (coordinate c along a line of
+// length len, memory stride st), every coefficient included, left to right.
+__device__ double plain_row(const OpTable &op, const double *f, int p, int c, int len, int st, int code) {
+#pragma clang fp contract(off)
+  double acc = 0.0;
+  if (code == CODE_INTERIOR) {
+    for (int s = -op.half; s <= op.half; s++) {
+      const int q = c + s;
+      if (q >= 0 && q < len) acc += op.st[s + op.half] * f[p + s * st];
+    }
+  } else if (code == CODE_ZERO) {
+    acc = 0.0;
+  } else if (code < CODE_LEFT + op.nb) {
+    const int k = code - CODE_LEFT;
+    for (int s = 0; s < op.sb; s++) {
+      const int q = c - k + s;
+      if (q >= 0 && q < len) acc += op.bnd[k][s] * f[p + (s - k) * st];
+    }
+  } else {
+    const int k = code - CODE_LEFT - op.nb;
+    for (int s = 0; s < op.sb; s++) {
+      const int q = c + k - s;
+      if (q >= 0 && q < len) acc += op.sign_right * op.bnd[k][s] * f[p + (k - s) * st];
+    }
+  }
+  return acc;
+}
+
+__global__ void __launch_bounds__(TPB) k_plain_op_dir(const double *__restrict__ f, double *__restrict__ df,
+                                                      const int8_t *__restrict__ codes, int ni, int nj, int dir,
+                                                      OpTable op) {
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  const int n = ni * nj;
+  if (p >= n) return;
+  const int i = p % ni, j = p / ni;
+  const int c = dir == 0 ? i : j, len = dir == 0 ? ni : nj, st = dir == 0 ? 1 : ni;
+  const int code = codes ? (int)codes[p] : default_code(c, len, op.nb);
+  df[p] = plain_row(op, f, p, c, len, st, code);
+}
+
+// StandardCentral::take_derivative_xyz (spatial:1802-1826):
+// out = (0 + m[xi][d] d/dxi + m[eta][d] d/deta) * J
+__global__ void __launch_bounds__(TPB) k_plain_deriv_xyz(const double *__restrict__ f, double *__restrict__ out,
+                                                         const double *__restrict__ met,
+                                                         const double *__restrict__ jac,
+                                                         const int8_t *__restrict__ codes, int ni, int nj, int dxyz,
+                                                         OpTable op) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  const int n = ni * nj;
+  if (p >= n) return;
+  const int i = p % ni, j = p / ni;
+  const int c0 = codes ? (int)codes[p] : default_code(i, ni, op.nb);
+  const int c1 = codes ? (int)codes[n + p] : default_code(j, nj, op.nb);
+  const double d0 = plain_row(op, f, p, i, ni, 1, c0);
+  const double d1 = plain_row(op, f, p, j, nj, ni, c1);
+  double sum = 0.0;
+  sum += met[(size_t)(0 + dxyz) * n + p] * d0;
+  sum += met[(size_t)(2 + dxyz) * n + p] * d1;
+  out[p] = sum * jac[p];
+}
+
+// metrics::compute_ThomasLombard_2D + compute_Jacobian (metrics.cpp:107-129,
+// 255-328).  xx = {x_xi, x_eta, y_xi, y_eta}.
+__global__ void __launch_bounds__(TPB) k_metrics2d(const double *__restrict__ xx, double *__restrict__ met,
+                                                   double *__restrict__ jac, int n) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p >= n) return;
+  const double x_xi = xx[p], x_eta = xx[(size_t)n + p], y_xi = xx[(size_t)2 * n + p], y_eta = xx[(size_t)3 * n + p];
+  met[p] = y_eta;
+  met[(size_t)n + p] = -x_eta;
+  met[(size_t)2 * n + p] = -y_xi;
+  met[(size_t)3 * n + p] = x_xi;
+  const double inv = x_xi * y_eta - x_eta * y_xi;
+  jac[p] = 1.0 / inv;
+}
+
+__global__ void __launch_bounds__(TPB) k_fill(double *__restrict__ a, double v, size_t n) {
+  const size_t p = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (p < n) a[p] = v;
+}
+
+__global__ void __launch_bounds__(TPB) k_copy(const double *__restrict__ a, double *__restrict__ b, size_t n) {
+  const size_t p = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (p < n) b[p] = a[p];
+}
+
+// rhs -= a * b
+__global__ void __launch_bounds__(TPB) k_sub_ab(double *__restrict__ r, const double *__restrict__ a,
+                                                const double *__restrict__ b, int n) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p < n) r[p] -= a[p] * b[p];
+}
+
+// rhs -= (g * a) * b
+__global__ void __launch_bounds__(TPB) k_sub_gab(double *__restrict__ r, double g, const double *__restrict__ a,
+                                                 const double *__restrict__ b, int n) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p < n) r[p] -= g * a[p] * b[p];
+}
+
+// rhs -= a / b
+__global__ void __launch_bounds__(TPB) k_sub_div(double *__restrict__ r, const double *__restrict__ a,
+                                                 const double *__restrict__ b, int n) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p < n) r[p] -= a[p] / b[p];
+}
+
+// rhs -= (u' + rho'/rho_bar * u_bar) * grad   (spatial:2089-2092)
+__global__ void __launch_bounds__(TPB) k_sub_velgrad(double *__restrict__ r, const double *__restrict__ up,
+                                                     const double *__restrict__ rho,
+                                                     const double *__restrict__ rhob,
+                                                     const double *__restrict__ ub,
+                                                     const double *__restrict__ grad, int n) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p < n) r[p] -= (up[p] + rho[p] / rhob[p] * ub[p]) * grad[p];
+}
+
+// apply_physicalSource: rhs[P] += (s1 * g) * s2 over the source support
+__global__ void __launch_bounds__(TPB) k_plain_source(double *__restrict__ rp, const double *__restrict__ g,
+                                                      int ni, int i0, int i1, int j0, int j1, double s1,
+                                                      double s2) {
+#pragma clang fp contract(off)
+  const int bw = i1 - i0 + 1, bh = j1 - j0 + 1;
+  const int q = blockIdx.x * TPB + threadIdx.x;
+  if (q >= bw * bh) return;
+  const int i = i0 + q % bw, j = j0 + q / bw;
+  rp[(size_t)j * ni + i] += s1 * g[q] * s2;
+}
+
+// apply_bufferZone_Freund_ambient (source.cpp:174-201), one zone, all 5 vars
+__global__ void __launch_bounds__(TPB) k_plain_sponge(double *__restrict__ rhs, const double *__restrict__ y,
+                                                      const double *__restrict__ sig, int ni, int n, int i0,
+                                                      int nzi, int j0, int nzj) {
+#pragma clang fp contract(off)
+  const int q = blockIdx.x * TPB + threadIdx.x;
+  if (q >= nzi * nzj) return;
+  const int p = (j0 + q / nzi) * ni + (i0 + q % nzi);
+  const double s = sig[q];
+  for (int v = 0; v < NV; v++) {
+    const double relax = -s * (y[(size_t)v * n + p] - 0.0);
+    rhs[(size_t)v * n + p] += relax;
+  }
+}
+
+// RungeKutta::integrate update loop (temporal:208-230)
+__global__ void __launch_bounds__(TPB) k_plain_rk(double *__restrict__ rhs, double *__restrict__ ynext,
+                                                  double *__restrict__ interim, const double *__restrict__ ycur,
+                                                  const int8_t *__restrict__ iblank, int n, double h, double bi,
+                                                  double a, int last) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p >= n) return;
+  if (!iblank || iblank[p] != BLANKED) {
+    for (int v = 0; v < NV; v++) {
+      const size_t q = (size_t)v * n + p;
+      rhs[q] *= h;
+      ynext[q] += bi * rhs[q];
+      if (!last) interim[q] = ycur[q] + a * rhs[q];
+    }
+  } else {
+    for (int v = 0; v < NV; v++) {
+      const size_t q = (size_t)v * n + p;
+      rhs[q] = 0.0;
+      ynext[q] = 0.0;
+      interim[q] = 0.0;
+    }
+  }
+}
+
+// compute_auxiliary_variables_linear_Euler (state.cpp:701-736), rho' only:
+// rho' = (p' gamma^-1 p_bar^-1 - s') rho_bar
+__global__ void __launch_bounds__(TPB) k_aux_rho(const double *__restrict__ y, const double *__restrict__ mean,
+                                                 const double *__restrict__ rhob, double *__restrict__ rho,
+                                                 double gi, int n) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p >= n) return;
+  const double pbi = 1.0 / mean[(size_t)IP * n + p];
+  double r = y[(size_t)IP * n + p] * gi * pbi - y[(size_t)IS * n + p];
+  r *= rhob[p];
+  rho[p] = r;
+}
+
+// filter blend: y = (1 - blend) func0 + blend func
+__global__ void __launch_bounds__(TPB) k_blend(double *__restrict__ y, const double *__restrict__ f0,
+                                               const double *__restrict__ f, double rem, double blend, int n) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p < n) y[p] = rem * f0[p] + blend * f[p];
+}
+
+// temporal::iblank_correction (temporal:380-393): holes -> sol_ref = 0
+__global__ void __launch_bounds__(TPB) k_iblank_reset(double *__restrict__ y, const int8_t *__restrict__ iblank,
+                                                      int n) {
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p >= n || iblank[p] != BLANKED) return;
+  for (int v = 0; v < NV; v++) y[(size_t)v * n + p] = 0.0;
+}
+
+// ------------------------------------------------------ boundary conditions
+
+struct BcArgs {
+  int type, dir, end;
+  int i0, i1, j0, j1;   // face box (collapsed along dir)
+  int ni, nj;
+  double neu[3];        // closure row 0 of the FD operator
+  int sb;
+};
+
+// bc::apply_BC one face (boundary_condition.cpp:46-209) with bc_neumann
+// (:348-444), bc_wall_slip_kinematic (:448-494), dirichlet_allzero and the
+// Cartesian centerline conditions.  Arithmetic in FP64 for both precisions.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double *__restrict__ met,
+                                                 const double *__restrict__ jac,
+                                                 const int8_t *__restrict__ iblank, BcArgs a,
+                                                 T *__restrict__ rho, const double *__restrict__ mean,
+                                                 const double *__restrict__ rhob, double gi) {
+#pragma clang fp contract(off)
+  const int fw = a.i1 - a.i0 + 1, fh = a.j1 - a.j0 + 1;
+  const int q = blockIdx.x * TPB + threadIdx.x;
+  if (q >= fw * fh) return;
+  const int i = a.i0 + q % fw, j = a.j0 + q / fw;
+  const int n = a.ni * a.nj;
+  const int p = j * a.ni + i;
+  if (iblank && iblank[p] == BLANKED) return;
+  const int mf = a.dir == 0 ? 1 : a.ni;
+  const int c = a.dir == 0 ? i : j, len = a.dir == 0 ? a.ni : a.nj;
+  double data[NV];
+  for (int v = 0; v < NV; v++) data[v] = 1.0e9 * (double)(a.dir + 1);
+  if (a.type == LEE_BC_DIRICHLET_ALLZERO)
+    for (int v = 0; v < NV; v++) data[v] = 0.0;
+  const bool neumann = a.type == LEE_BC_NEUMANN || a.type == LEE_BC_WALL_SLIP_KINEMATIC ||
+                       (a.type >= LEE_BC_CENTERLINE_CART_NORM2X && a.type <= LEE_BC_CENTERLINE_CART_NORM2Z);
+  if (neumann) {
+    for (int v = 0; v < NV; v++) {
+      if (a.type == LEE_BC_WALL_SLIP_KINEMATIC && v >= IUX && v <= IUZ) continue;
+      double d = 0.0;
+      if (a.end == 0) {
+        for (int s = 1; s < a.sb; s++)
+          if (c + s < len) d -= a.neu[s] * (double)y[(size_t)v * n + p + mf * s];
+        d /= a.neu[0];
+      } else {
+        for (int s = 1; s < a.sb; s++)
+          if (c - s >= 0) d += a.neu[s] * (double)y[(size_t)v * n + p - mf * s];
+        d /= -a.neu[0];
+      }
+      data[v] = d;
+    }
+  }
+  if (a.type == LEE_BC_WALL_SLIP_KINEMATIC) {
+    const double m00 = met[p], m01 = met[(size_t)n + p], m10 = met[(size_t)2 * n + p], m11 = met[(size_t)3 * n + p];
+    const double u = (double)y[(size_t)IUX * n + p], v = (double)y[(size_t)IUY * n + p];
+    double k0 = 0.0, k1 = 0.0;
+    k0 += m00 * u;
+    k0 += m01 * v;
+    k1 += m10 * u;
+    k1 += m11 * v;
+    if (a.dir == 0) k0 = 0.0; else k1 = 0.0;
+    // metricsInverse (metrics.cpp:375-381): [X][XI] = eta_y, [X][ETA] = -xi_y,
+    // [Y][XI] = -eta_x, [Y][ETA] = xi_x
+    const double J = jac[p];
+    double ux = 0.0, uy = 0.0;
+    ux += m11 * k0;
+    ux += -m01 * k1;
+    ux *= J;
+    uy += -m10 * k0;
+    uy += m00 * k1;
+    uy *= J;
+    data[IUX] = ux;
+    data[IUY] = uy;
+    data[IUZ] = (double)y[(size_t)IUZ * n + p];
+  }
+  if (a.type >= LEE_BC_CENTERLINE_CART_NORM2X && a.type <= LEE_BC_CENTERLINE_CART_NORM2Z)
+    data[IUX + (a.type - LEE_BC_CENTERLINE_CART_NORM2X)] = 0.0;
+  for (int v = 0; v < NV; v++) y[(size_t)v * n + p] = (T)data[v];
+  if (rho) {
+    const double pbi = 1.0 / mean[(size_t)IP * n + p];
+    double r = data[IP] * gi * pbi - data[IS];
+    r *= rhob[p];
+    rho[p] = (T)r;
+  }
+}
+
+// ---------------------------------------------------------------- overset
+
+template <typename T>
+struct OvTables {
+  T *y[MAX_BLOCKS];
+  int ni[MAX_BLOCKS], nn[MAX_BLOCKS];
+  T *rho[MAX_BLOCKS];
+  const double *mean[MAX_BLOCKS];
+  const double *rhob[MAX_BLOCKS];
+};
+
+// compute_weighted_sum (overset.cpp:886-974): every receiver's weighted
+// donor sum into buf before any receiver is written.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_ov_gather(OvTables<T> t, const int *__restrict__ db,
+                                                   const int *__restrict__ dl, const int *__restrict__ w,
+                                                   const double *__restrict__ coef, int stride,
+                                                   double *__restrict__ buf, int nrecv) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * TPB + threadIdx.x;
+  if (r >= nrecv) return;
+  const int b = db[r];
+  const T *y = t.y[b];
+  const int ni = t.ni[b], n = t.nn[b];
+  const int w1 = w[3 * b], w2 = w[3 * b + 1];
+  const double *c = coef + (size_t)r * stride;
+  double acc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int s = 0;
+  for (int b2 = 0; b2 < w2; b2++)
+    for (int b1 = 0; b1 < w1; b1++) {
+      const int l1 = dl[r] + b1 + b2 * ni;
+      if (l1 >= 0 && l1 < n)
+        for (int v = 0; v < NV; v++) acc[v] += c[s] * (double)y[(size_t)v * n + l1];
+      s++;
+    }
+  for (int v = 0; v < NV; v++) buf[(size_t)r * NV + v] = acc[v];
+}
+
+// update_each_grid (overset.cpp:1241-1278); optionally refreshes rho'.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__restrict__ rb,
+                                                    const int *__restrict__ rl, const double *__restrict__ buf,
+                                                    int nrecv, int with_rho, double gi) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * TPB + threadIdx.x;
+  if (r >= nrecv) return;
+  const int b = rb[r];
+  const int n = t.nn[b], p = rl[r];
+  if (p < 0 || p >= n) return;
+  for (int v = 0; v < NV; v++) t.y[b][(size_t)v * n + p] = (T)buf[(size_t)r * NV + v];
+  if (with_rho && t.rho[b]) {
+    const double pbi = 1.0 / t.mean[b][(size_t)IP * n + p];
+    double q = buf[(size_t)r * NV + IP] * gi * pbi - buf[(size_t)r * NV + IS];
+    q *= t.rhob[b][p];
+    t.rho[b][p] = (T)q;
+  }
+}
+
+// ------------------------------------------------------------------ fused
+
+constexpr int ST_TX = 64, ST_TY = 16, ST_H = 2;   // stage tile and FD halo
+constexpr int FL_TX = 64, FL_TY = 32, FL_H = 4;   // filter tile and halo
+constexpr int STAGE_FIRST = 0, STAGE_MID = 1, STAGE_LAST = 2;
+
+template <typename T>
+struct CoefT {
+  T st[9];
+  T bnd[4][7];
+  T sign;
+  int half, nb, sb;
+  int lo[4], hi[4];   // first / one-past-last non-zero entry of each closure row
+};
+
+template <typename T>
+CoefT<T> make_coef(const OpTable &op) {
+  CoefT<T> c{};
+  c.half = op.half;
+  c.nb = op.nb;
+  c.sb = op.sb;
+  c.sign = (T)op.sign_right;
+  for (int s = 0; s < 2 * op.half + 1 && s < 9; s++) c.st[s] = (T)op.st[s];
+  for (int k = 0; k < op.nb && k < 4; k++) {
+    c.lo[k] = op.sb;
+    c.hi[k] = 0;
+    for (int s = 0; s < op.sb && s < 7; s++) {
+      c.bnd[k][s] = (T)op.bnd[k][s];
+      if (op.bnd[k][s] != 0.0) {
+        if (s < c.lo[k]) c.lo[k] = s;
+        c.hi[k] = s + 1;
+      }
+    }
+  }
+  return c;
+}
+
+// One operator row from LDS: `s` points at the point's element, `st` is the
+// LDS stride along the operator direction.  Zero closure entries are
+// skipped, so every read stays within the interior half width.
+template <typename T, int HALF>
+__device__ __forceinline__ T lds_row(const T *s, int st, int code, const CoefT<T> &c) {
+  if (code == CODE_INTERIOR) {
+    T acc = c.st[0] * s[-HALF * st];
+#pragma unroll
+    for (int q = 1; q <= 2 * HALF; q++) {
+      if (q == HALF) {
+        if (c.st[HALF] != (T)0) acc += c.st[HALF] * s[0];
+      } else {
+        acc += c.st[q] * s[(q - HALF) * st];
+      }
+    }
+    return acc;
+  }
+  if (code == CODE_ZERO) return (T)0;
+  T acc = (T)0;
+  if (code < CODE_LEFT + HALF) {
+    const int k = code - CODE_LEFT;
+    for (int q = c.lo[k]; q < c.hi[k]; q++)
+      if (c.bnd[k][q] != (T)0) acc += c.bnd[k][q] * s[(q - k) * st];
+  } else {
+    const int k = code - CODE_LEFT - HALF;
+    for (int q = c.lo[k]; q < c.hi[k]; q++)
+      if (c.bnd[k][q] != (T)0) acc += c.sign * c.bnd[k][q] * s[(k - q) * st];
+  }
+  return acc;
+}
+
+template <typename T>
+struct ZoneT {
+  int i0, i1, j0, j1, nzi;
+  const T *sig;
+};
+
+template <typename T>
+struct StageArgs {
+  int ni, nj, ntx, nty, ntiles, chunk;
+  const T *yin;        // stage input (5n, var-major)
+  const T *yn;         // y_n
+  T *acc;              // y_next accumulator (read-modify-write)
+  T *yout;             // next stage input
+  const double *met, *jac;
+  const int8_t *iblank, *code;  // holes only
+  const double *mean, *mgrad, *rhob;   // non-uniform mean only
+  const T *rho_in;     // stale rho' (first stage, non-uniform mean)
+  T *rho_out;          // rho' of the new y_{n+1} (last stage, non-uniform mean)
+  double u0, v0, gp0, rhob0, gamma, gi;
+  int nz;
+  ZoneT<T> z[MAX_ZONES];
+  int si0, si1, sj0, sj1;
+  const double *g;
+  double s1, s2;
+  T h, b, a;
+  CoefT<T> c;
+};
+
+// Maps the launch index to a tile so that each XCD (dispatch is round-robin
+// over the 8 XCDs) walks a contiguous band of tile rows and keeps the halo
+// rows of vertically adjacent tiles in its own L2.
+__device__ __forceinline__ int xcd_tile(int bid, int chunk) { return (bid & 7) * chunk + (bid >> 3); }
+
+template <typename T, bool MEANF, bool HOLES, int KIND>
+__global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
+  __shared__ T s[4][ST_TY + 2 * ST_H][ST_TX + 2 * ST_H];
+  const int tile = xcd_tile(blockIdx.x, a.chunk);
+  if (tile >= a.ntiles) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * ST_TY;
+  const int ni = a.ni, nj = a.nj, n = ni * nj;
+  const int VOFF[4] = {IS * n, IUX * n, IUY * n, IP * n};
+
+  for (int r = ty; r < ST_TY + 2 * ST_H; r += 4) {
+    const int gj = j0 - ST_H + r;
+    for (int cc = tx; cc < ST_TX + 2 * ST_H; cc += 64) {
+      const int gi = i0 - ST_H + cc;
+      T v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+      if (gj >= 0 && gj < nj && gi >= 0 && gi < ni) {
+        const size_t q = (size_t)gj * ni + gi;
+        v0 = a.yin[VOFF[0] + q];
+        v1 = a.yin[VOFF[1] + q];
+        v2 = a.yin[VOFF[2] + q];
+        v3 = a.yin[VOFF[3] + q];
+      }
+      s[0][r][cc] = v0;
+      s[1][r][cc] = v1;
+      s[2][r][cc] = v2;
+      s[3][r][cc] = v3;
+    }
+  }
+  __syncthreads();
+
+  constexpr int RS = ST_TX + 2 * ST_H;  // LDS row stride
+  for (int rr = ty; rr < ST_TY; rr += 4) {
+    const int gi = i0 + tx, gj = j0 + rr;
+    if (gi >= ni || gj >= nj) continue;
+    const int p = gj * ni + gi;
+    const int lj = rr + ST_H, li = tx + ST_H;
+    int cx, cy;
+    if (HOLES) {
+      cx = a.code[p];
+      cy = a.code[n + p];
+    } else {
+      cx = default_code(gi, ni, 2);
+      cy = default_code(gj, nj, 2);
+    }
+    T dx[4], dy[4], yc[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const T *sp = &s[v][lj][li];
+      yc[v] = sp[0];
+      dx[v] = lds_row<T, 2>(sp, 1, cx, a.c);
+      dy[v] = lds_row<T, 2>(sp, RS, cy, a.c);
+    }
+    // chain rule with FP64 metrics (spatial:1810-1820)
+    const double mxx = a.met[p], mxy = a.met[n + p], myx = a.met[2 * n + p], myy = a.met[3 * n + p];
+    const double J = a.jac[p];
+    T fx[4], fy[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      fx[v] = (T)((mxx * (double)dx[v] + myx * (double)dy[v]) * J);
+      fy[v] = (T)((mxy * (double)dx[v] + myy * (double)dy[v]) * J);
+    }
+    // LEE right-hand side (spatial:2053-2130), reference term order
+    const T sv = yc[0], uv = yc[1], vv = yc[2], pv = yc[3];
+    T rs, ru, rv, rp;
+    if (!MEANF) {
+      const T u0 = (T)a.u0, v0 = (T)a.v0, gp0 = (T)a.gp0;
+      rs = -(u0 * fx[0]);
+      rs -= v0 * fy[0];
+      ru = -(u0 * fx[1]);
+      ru -= v0 * fy[1];
+      rv = -(u0 * fx[2]);
+      rv -= v0 * fy[2];
+      if (sizeof(T) == 8) {
+        ru -= fx[3] / (T)a.rhob0;
+        rv -= fy[3] / (T)a.rhob0;
+      } else {
+        const T irb = (T)(1.0 / a.rhob0);
+        ru -= fx[3] * irb;
+        rv -= fy[3] * irb;
+      }
+      rp = -(u0 * fx[3]);
+      rp -= gp0 * fx[1];
+      rp -= v0 * fy[3];
+      rp -= gp0 * fy[2];
+    } else {
+      const T u0 = (T)a.mean[IUX * n + p], v0 = (T)a.mean[IUY * n + p];
+      const double pb = a.mean[IP * n + p];
+      const T rb = (T)a.rhob[p];
+      const T gsx = (T)a.mgrad[(2 * IS + 0) * n + p], gsy = (T)a.mgrad[(2 * IS + 1) * n + p];
+      const T gux = (T)a.mgrad[(2 * IUX + 0) * n + p], guy = (T)a.mgrad[(2 * IUX + 1) * n + p];
+      const T gvx = (T)a.mgrad[(2 * IUY + 0) * n + p], gvy = (T)a.mgrad[(2 * IUY + 1) * n + p];
+      const T gpx = (T)a.mgrad[(2 * IP + 0) * n + p], gpy = (T)a.mgrad[(2 * IP + 1) * n + p];
+      T rho;
+      if (KIND == STAGE_FIRST) {
+        rho = a.rho_in[p];
+      } else {
+        const double pbi = 1.0 / pb;
+        double q = (double)pv * a.gi * pbi - (double)sv;
+        q *= a.rhob[p];
+        rho = (T)q;
+      }
+      const T gp = (T)(a.gamma * pb);
+      const T g = (T)a.gamma;
+      const T ru_fac = rho / rb;
+      rs = -(u0 * fx[0]);
+      rs -= uv * gsx;
+      rs -= v0 * fy[0];
+      rs -= vv * gsy;
+      ru = -(u0 * fx[1]);
+      ru -= (uv + ru_fac * u0) * gux;
+      ru -= v0 * fy[1];
+      ru -= (vv + ru_fac * v0) * guy;
+      ru -= fx[3] / rb;
+      rv = -(u0 * fx[2]);
+      rv -= (uv + ru_fac * u0) * gvx;
+      rv -= v0 * fy[2];
+      rv -= (vv + ru_fac * v0) * gvy;
+      rv -= fy[3] / rb;
+      rp = -(u0 * fx[3]);
+      rp -= uv * gpx;
+      rp -= gp * fx[1];
+      rp -= g * pv * gux;
+      rp -= v0 * fy[3];
+      rp -= vv * gpy;
+      rp -= gp * fy[2];
+      rp -= g * pv * gvy;
+    }
+    // physical source (source.cpp:22-107), pressure equation
+    if (gi >= a.si0 && gi <= a.si1 && gj >= a.sj0 && gj <= a.sj1) {
+      const double gval = a.g[(gj - a.sj0) * (a.si1 - a.si0 + 1) + (gi - a.si0)];
+      rp += (T)(a.s1 * gval * a.s2);
+    }
+    // Freund sponges, zone by zone (source.cpp:136-201), sol_ref = 0
+#pragma unroll 1
+    for (int z = 0; z < a.nz; z++) {
+      const ZoneT<T> &zz = a.z[z];
+      if (gi >= zz.i0 && gi <= zz.i1 && gj >= zz.j0 && gj <= zz.j1) {
+        const T sg = zz.sig[(gj - zz.j0) * zz.nzi + (gi - zz.i0)];
+        rs += -sg * sv;
+        ru += -sg * uv;
+        rv += -sg * vv;
+        rp += -sg * pv;
+      }
+    }
+    // RK stage update (temporal:208-230)
+    T r[4] = {rs, ru, rv, rp};
+    const bool hole = HOLES && a.iblank[p] == BLANKED;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const size_t q = (size_t)VOFF[v] + p;
+      T accv, outv;
+      if (hole) {
+        accv = 0;
+        outv = 0;
+      } else {
+        const T rh = r[v] * a.h;
+        const T acc_in = KIND == STAGE_FIRST ? yc[v] : a.acc[q];
+        accv = acc_in + a.b * rh;
+        if (KIND != STAGE_LAST) {
+          const T yn = KIND == STAGE_FIRST ? yc[v] : a.yn[q];
+          outv = yn + a.a * rh;
+        }
+      }
+      a.acc[q] = accv;
+      if (KIND != STAGE_LAST) a.yout[q] = outv;
+      r[v] = accv;
+    }
+    if (MEANF && KIND == STAGE_LAST) {
+      const double pbi = 1.0 / a.mean[IP * n + p];
+      double q = (double)r[3] * a.gi * pbi - (double)r[0];
+      q *= a.rhob[p];
+      a.rho_out[p] = (T)q;
+    }
+  }
+}
+
+template <typename T>
+struct FilterArgs {
+  int ni, nj, ntx, nty, ntiles, chunk;
+  const T *src;      // unfiltered y (5n)
+  T *dst;            // filtered y (5n)
+  const int8_t *iblank, *code;  // holes only (filter codes, xi then eta)
+  int vars[NV];      // variable index per grid.y
+  T rem, blend;
+  CoefT<T> c;
+};
+
+// spatial::apply_filter (spatial:2191-2246) for one variable per grid.y:
+// xi sweep over the tile plus its eta halo, then the eta sweep, the blend
+// and temporal::iblank_correction.
+template <typename T, bool HOLES, int HALF>
+__global__ void __launch_bounds__(TPB) k_fused_filter(FilterArgs<T> a) {
+  __shared__ T in[FL_TY + 2 * FL_H][FL_TX + 2 * FL_H];
+  __shared__ T mid[FL_TY + 2 * FL_H][FL_TX];
+  const int tile = xcd_tile(blockIdx.x, a.chunk);
+  if (tile >= a.ntiles) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * FL_TY;
+  const int ni = a.ni, nj = a.nj, n = ni * nj;
+  const size_t voff = (size_t)a.vars[blockIdx.y] * n;
+  const T *src = a.src + voff;
+
+  for (int r = ty; r < FL_TY + 2 * FL_H; r += 4) {
+    const int gj = j0 - FL_H + r;
+    for (int cc = tx; cc < FL_TX + 2 * FL_H; cc += 64) {
+      const int gi = i0 - FL_H + cc;
+      T v = 0;
+      if (gj >= 0 && gj < nj && gi >= 0 && gi < ni) v = src[(size_t)gj * ni + gi];
+      in[r][cc] = v;
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < FL_TY + 2 * FL_H; r += 4) {
+    const int gj = j0 - FL_H + r, gi = i0 + tx;
+    T v = 0;
+    if (gj >= 0 && gj < nj && gi < ni) {
+      const int code = HOLES ? (int)a.code[gj * ni + gi] : default_code(gi, ni, HALF);
+      v = lds_row<T, HALF>(&in[r][tx + FL_H], 1, code, a.c);
+    }
+    mid[r][tx] = v;
+  }
+  __syncthreads();
+  for (int r = ty; r < FL_TY; r += 4) {
+    const int gj = j0 + r, gi = i0 + tx;
+    if (gj >= nj || gi >= ni) continue;
+    const int p = gj * ni + gi;
+    const int code = HOLES ? (int)a.code[n + p] : default_code(gj, nj, HALF);
+    const T f = lds_row<T, HALF>(&mid[r + FL_H][tx], FL_TX, code, a.c);
+    T out = a.rem * in[r + FL_H][tx + FL_H] + a.blend * f;
+    if (HOLES && a.iblank[p] == BLANKED) out = 0;
+    a.dst[voff + p] = out;
+  }
+}
+
+// Passive w' in 2-D: only the sponge acts on it (the velocity loop of
+// spatial:2077-2103 stops at num_dim), then the RK update.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_passive_w(StageArgs<T> a, int kind) {
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  const int ni = a.ni, nj = a.nj, n = ni * nj;
+  if (p >= n) return;
+  const int gi = p % ni, gj = p / ni;
+  const size_t q = (size_t)IUZ * n + p;
+  const T w = a.yin[q];
+  T r = 0;
+  for (int z = 0; z < a.nz; z++) {
+    const ZoneT<T> &zz = a.z[z];
+    if (gi >= zz.i0 && gi <= zz.i1 && gj >= zz.j0 && gj <= zz.j1)
+      r += -zz.sig[(gj - zz.j0) * zz.nzi + (gi - zz.i0)] * w;
+  }
+  if (a.iblank && a.iblank[p] == BLANKED) {
+    a.acc[q] = 0;
+    if (kind != STAGE_LAST) a.yout[q] = 0;
+    return;
+  }
+  const T rh = r * a.h;
+  const T acc_in = kind == STAGE_FIRST ? w : a.acc[q];
+  a.acc[q] = acc_in + a.b * rh;
+  if (kind != STAGE_LAST) {
+    const T yn = kind == STAGE_FIRST ? w : a.yn[q];
+    a.yout[q] = yn + a.a * rh;
+  }
+}
+
+template <typename S, typename D>
+__global__ void __launch_bounds__(TPB) k_cast(const S *__restrict__ a, D *__restrict__ b, size_t n) {
+  const size_t p = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (p < n) b[p] = (D)a[p];
+}
+
+inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+hipError_t plain_apply_op(hipStream_t s, const Block &b, const OpTable &op, const int8_t *codes, const double *f,
+                          double *df, int dir) {
+  hipLaunchKernelGGL(k_plain_op_dir, dim3(nblocks(b.n)), dim3(TPB), 0, s, f, df, codes, b.ni, b.nj, dir, op);
+  return hipGetLastError();
+}
+
+hipError_t metrics2d(hipStream_t s, Block &b, const OpTable &fd) {
+  // x_xi, x_eta, y_xi, y_eta by the FD operator with no holes (metrics are
+  // computed before overset marks any, simulation.cpp:207-217)
+  double *xx = nullptr;
+  hipError_t e = hipMalloc((void **)&xx, sizeof(double) * 4 * (size_t)b.n);
+  if (e != hipSuccess) return e;
+  for (int ph = 0; ph < 2; ph++)
+    for (int cp = 0; cp < 2; cp++) {
+      hipLaunchKernelGGL(k_plain_op_dir, dim3(nblocks(b.n)), dim3(TPB), 0, s, b.xyz + (size_t)ph * b.n,
+                         xx + (size_t)(2 * ph + cp) * b.n, (const int8_t *)nullptr, b.ni, b.nj, cp, fd);
+    }
+  hipLaunchKernelGGL(k_metrics2d, dim3(nblocks(b.n)), dim3(TPB), 0, s, xx, b.met, b.jac, b.n);
+  e = hipGetLastError();
+  hipError_t e2 = hipStreamSynchronize(s);
+  hipFree(xx);
+  return e != hipSuccess ? e : e2;
+}
+
+hipError_t plain_take_derivative_xyz(hipStream_t s, const Block &b, const OpTable &op, const double *f,
+                                     int dir_xyz) {
+  hipLaunchKernelGGL(k_plain_deriv_xyz, dim3(nblocks(b.n)), dim3(TPB), 0, s, f, b.dflux, b.met, b.jac,
+                     b.has_holes ? b.code_fd : nullptr, b.ni, b.nj, dir_xyz, op);
+  return hipGetLastError();
+}
+
+hipError_t aux_density(hipStream_t s, const Block &b, double gamma, const double *y, double *rho) {
+  hipLaunchKernelGGL(k_aux_rho, dim3(nblocks(b.n)), dim3(TPB), 0, s, y, b.mean, b.rhobar, rho, 1.0 / gamma, b.n);
+  return hipGetLastError();
+}
+
+// spatial::compute_RHS + compute_RHS_linearized_Euler + add_RHSSourceTerms
+// (spatial:1963-1990, 2053-2130; source.cpp:10-18), one kernel per array
+// operation of the reference.  rho: the aux density to use; anything other
+// than the block's carried array is first filled from y.
+hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double *y, double *rhs, double s1,
+                     double s2, double *rho) {
+  const int n = b.n;
+  const dim3 g(nblocks(n)), t(TPB);
+  const double gam = h->cfg.gamma;
+  auto Y = [&](int v) { return y + (size_t)v * n; };
+  auto R = [&](int v) { return rhs + (size_t)v * n; };
+  auto M = [&](int v) { return b.mean + (size_t)v * n; };
+  auto G = [&](int v, int d) { return b.mgrad + (size_t)(2 * v + d) * n; };
+  if (rho != b.rho64) {  // unit call: aux density of y itself, carried rho' untouched
+    hipError_t e = aux_density(s, b, gam, y, rho);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_fill, dim3(nblocks((size_t)NV * n)), t, 0, s, rhs, 0.0, (size_t)NV * n);
+  for (int d = 0; d < 2; d++) {
+    plain_take_derivative_xyz(s, b, h->fd, Y(IS), d);
+    hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IS), M(IUX + d), b.dflux, n);
+    hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IS), Y(IUX + d), G(IS, d), n);
+  }
+  for (int vel = 0; vel < 2; vel++) {
+    for (int d = 0; d < 2; d++) {
+      plain_take_derivative_xyz(s, b, h->fd, Y(IUX + vel), d);
+      hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IUX + vel), M(IUX + d), b.dflux, n);
+      hipLaunchKernelGGL(k_sub_velgrad, g, t, 0, s, R(IUX + vel), Y(IUX + d), rho, b.rhobar, M(IUX + d),
+                         G(IUX + vel, d), n);
+    }
+    plain_take_derivative_xyz(s, b, h->fd, Y(IP), vel);
+    hipLaunchKernelGGL(k_sub_div, g, t, 0, s, R(IUX + vel), b.dflux, b.rhobar, n);
+  }
+  for (int d = 0; d < 2; d++) {
+    plain_take_derivative_xyz(s, b, h->fd, Y(IP), d);
+    hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IP), M(IUX + d), b.dflux, n);
+    hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IP), Y(IUX + d), G(IP, d), n);
+    plain_take_derivative_xyz(s, b, h->fd, Y(IUX + d), d);
+    hipLaunchKernelGGL(k_sub_gab, g, t, 0, s, R(IP), gam, M(IP), b.dflux, n);
+    hipLaunchKernelGGL(k_sub_gab, g, t, 0, s, R(IP), gam, Y(IP), G(IUX + d, d), n);
+  }
+  if (b.src.i1 >= b.src.i0) {
+    const size_t cnt = (size_t)(b.src.i1 - b.src.i0 + 1) * (b.src.j1 - b.src.j0 + 1);
+    hipLaunchKernelGGL(k_plain_source, dim3(nblocks(cnt)), t, 0, s, R(IP), b.src.g, b.ni, b.src.i0, b.src.i1,
+                       b.src.j0, b.src.j1, s1, s2);
+  }
+  for (const auto &z : b.zones) {
+    const size_t cnt = (size_t)z.nz[0] * z.nz[1];
+    hipLaunchKernelGGL(k_plain_sponge, dim3(nblocks(cnt)), t, 0, s, rhs, y, z.sigma64, b.ni, n, z.is[0], z.nz[0],
+                       z.is[1], z.nz[1]);
+  }
+  return hipGetLastError();
+}
+
+hipError_t plain_rk_update(hipStream_t s, const Block &b, double *rhs, double *ynext, double *interim,
+                           const double *ycur, double hdt, double bi, double a, bool last) {
+  hipLaunchKernelGGL(k_plain_rk, dim3(nblocks(b.n)), dim3(TPB), 0, s, rhs, ynext, interim, ycur,
+                     b.has_holes ? b.iblank : nullptr, b.n, hdt, bi, a, last ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t plain_filter(hipStream_t s, lee_handle *h, const Block &b, double *y) {
+  const int n = b.n;
+  const dim3 g(nblocks(n)), t(TPB);
+  const double blend = h->cfg.filter_blend, rem = 1.0 - blend;
+  for (int v = 0; v < NV; v++) {
+    double *yv = y + (size_t)v * n;
+    hipLaunchKernelGGL(k_copy, g, t, 0, s, yv, b.func0, (size_t)n);
+    hipLaunchKernelGGL(k_copy, g, t, 0, s, b.func0, b.func, (size_t)n);
+    for (int d = 0; d < 2; d++) {
+      hipLaunchKernelGGL(k_plain_op_dir, g, t, 0, s, b.func, b.dflux,
+                         b.has_holes ? b.code_flt + (size_t)d * n : nullptr, b.ni, b.nj, d, h->flt);
+      hipLaunchKernelGGL(k_copy, g, t, 0, s, b.dflux, b.func, (size_t)n);
+    }
+    hipLaunchKernelGGL(k_blend, g, t, 0, s, yv, b.func0, b.func, rem, blend, n);
+  }
+  return hipGetLastError();
+}
+
+hipError_t plain_iblank_reset(hipStream_t s, const Block &b, double *y) {
+  if (!b.has_holes) return hipSuccess;
+  hipLaunchKernelGGL(k_iblank_reset, dim3(nblocks(b.n)), dim3(TPB), 0, s, y, b.iblank, b.n);
+  return hipGetLastError();
+}
+
+hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, float *y32, bool fp32,
+                     double *rho64, float *rho32) {
+  for (const auto &f : b.faces) {
+    BcArgs a{};
+    a.type = f.type;
+    a.dir = f.dir;
+    a.end = f.end;
+    a.i0 = f.is[0];
+    a.i1 = f.ie[0];
+    a.j0 = f.is[1];
+    a.j1 = f.ie[1];
+    a.ni = b.ni;
+    a.nj = b.nj;
+    a.sb = h->fd.sb;
+    for (int q = 0; q < 3; q++) a.neu[q] = h->fd.bnd[0][q];
+    const size_t cnt = (size_t)(a.i1 - a.i0 + 1) * (a.j1 - a.j0 + 1);
+    const int8_t *ib = b.has_holes ? b.iblank : nullptr;
+    if (fp32)
+      hipLaunchKernelGGL(k_bc_face<float>, dim3(nblocks(cnt)), dim3(TPB), 0, s, y32, b.met, b.jac, ib, a, rho32,
+                         b.mean, b.rhobar, 1.0 / h->cfg.gamma);
+    else
+      hipLaunchKernelGGL(k_bc_face<double>, dim3(nblocks(cnt)), dim3(TPB), 0, s, y64, b.met, b.jac, ib, a, rho64,
+                         b.mean, b.rhobar, 1.0 / h->cfg.gamma);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t overset_t(hipStream_t s, lee_handle *h, T *const *ys, T *const *rhos, bool with_rho) {
+  OvTables<T> t{};
+  for (int ib = 0; ib < h->nblk; ib++) {
+    t.y[ib] = ys[ib];
+    t.ni[ib] = h->b[ib].ni;
+    t.nn[ib] = h->b[ib].n;
+    t.rho[ib] = rhos ? rhos[ib] : nullptr;
+    t.mean[ib] = h->b[ib].mean;
+    t.rhob[ib] = h->b[ib].rhobar;
+  }
+  const Overset &o = h->ov;
+  hipLaunchKernelGGL(k_ov_gather<T>, dim3(nblocks(o.nrecv)), dim3(TPB), 0, s, t, o.db, o.dl, o.w, o.coef, o.stride,
+                     o.buf, o.nrecv);
+  hipLaunchKernelGGL(k_ov_scatter<T>, dim3(nblocks(o.nrecv)), dim3(TPB), 0, s, t, o.rb, o.rl, o.buf, o.nrecv,
+                     with_rho ? 1 : 0, 1.0 / h->cfg.gamma);
+  return hipGetLastError();
+}
+
+// buf_index selects the state buffer (Y[k]) of every block; buf_index < 0
+// means the plain path's arrays (-1 interim, -2 ynp1).
+hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_index, bool last_stage) {
+  if (h->ov.nrecv == 0) return hipSuccess;
+  if (fp32) {
+    float *ys[MAX_BLOCKS], *rh[MAX_BLOCKS];
+    bool any_rho = false;
+    for (int ib = 0; ib < h->nblk; ib++) {
+      ys[ib] = h->b[ib].Y32[buf_index];
+      rh[ib] = h->b[ib].uniform_mean ? nullptr : h->b[ib].rho32;
+      any_rho |= rh[ib] != nullptr;
+    }
+    return overset_t<float>(s, h, ys, rh, last_stage && any_rho);
+  }
+  double *ys[MAX_BLOCKS], *rh[MAX_BLOCKS];
+  bool any_rho = false;
+  for (int ib = 0; ib < h->nblk; ib++) {
+    if (buf_index >= 0) ys[ib] = h->b[ib].Y64[buf_index];
+    else ys[ib] = buf_index == -1 ? h->b[ib].interim : h->b[ib].ynp1;
+    rh[ib] = (buf_index >= 0 && !h->b[ib].uniform_mean) ? h->b[ib].rho64 : nullptr;
+    any_rho |= rh[ib] != nullptr;
+  }
+  return overset_t<double>(s, h, ys, rh, last_stage && any_rho);
+}
+
+template <typename T>
+static void fill_zones(StageArgs<T> &a, const Block &b) {
+  a.nz = 0;
+  for (const auto &z : b.zones) {
+    if (a.nz >= MAX_ZONES) break;
+    ZoneT<T> &q = a.z[a.nz++];
+    q.i0 = z.is[0];
+    q.i1 = z.ie[0];
+    q.j0 = z.is[1];
+    q.j1 = z.ie[1];
+    q.nzi = z.nz[0];
+    if constexpr (sizeof(T) == 8) q.sig = (const T *)z.sigma64;
+    else q.sig = (const T *)z.sigma32;
+  }
+}
+
+template <typename T>
+static hipError_t fused_stage_t(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc, T *const *Y,
+                                T *rho) {
+  const int last = h->cfg.rk_stages - 1;
+  // stage < 0: right-hand side only (lee_compute_rhs): y in Y[2], zeroed
+  // accumulator Y[3] with h = b = 1 receives the RHS, Y[1] is scratch
+  const int kind = stage <= 0 ? (stage == 0 ? STAGE_FIRST : STAGE_MID) : (stage == last ? STAGE_LAST : STAGE_MID);
+  StageArgs<T> a{};
+  a.ni = b.ni;
+  a.nj = b.nj;
+  a.ntx = (b.ni + ST_TX - 1) / ST_TX;
+  a.nty = (b.nj + ST_TY - 1) / ST_TY;
+  a.ntiles = a.ntx * a.nty;
+  a.chunk = (a.ntiles + 7) / 8;
+  // buffers: stage 0 reads Y0; odd stages write Y2 ... see header comment
+  const int in_idx = stage == 0 ? 0 : (stage % 2 == 1 ? 2 : 3);
+  const int out_idx = stage % 2 == 0 ? 2 : 3;
+  a.yin = Y[in_idx];
+  a.yn = Y[0];
+  a.acc = Y[1];
+  a.yout = Y[out_idx];
+  if (stage < 0) {
+    a.yin = Y[2];
+    a.yn = Y[2];
+    a.acc = Y[3];
+    a.yout = Y[1];
+  }
+  a.met = b.met;
+  a.jac = b.jac;
+  a.iblank = b.iblank;
+  a.code = b.code_fd;
+  a.mean = b.mean;
+  a.mgrad = b.mgrad;
+  a.rhob = b.rhobar;
+  a.rho_in = rho;
+  a.rho_out = rho;
+  a.u0 = b.umean[IUX];
+  a.v0 = b.umean[IUY];
+  a.gamma = h->cfg.gamma;
+  a.gi = 1.0 / h->cfg.gamma;
+  a.gp0 = h->cfg.gamma * b.umean[IP];
+  {
+    const double pbar = b.umean[IP], sbar = b.umean[IS];
+    a.rhob0 = std::pow(h->cfg.gamma * pbar, 1.0 / h->cfg.gamma) * std::exp(-sbar);
+  }
+  fill_zones(a, b);
+  a.si0 = b.src.i0;
+  a.si1 = b.src.i1;
+  a.sj0 = b.src.j0;
+  a.sj1 = b.src.j1;
+  a.g = b.src.g;
+  a.s1 = sc.s1;
+  a.s2 = sc.s2;
+  a.h = (T)sc.h;
+  a.b = (T)sc.b;
+  a.a = (T)sc.a;
+  a.c = make_coef<T>(h->fd);
+  const dim3 grid(a.chunk * 8), block(64, 4);
+  const bool mf = !b.uniform_mean, ho = b.has_holes;
+#define LEE_LAUNCH(MF, HO, K) hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K>), grid, block, 0, s, a)
+#define LEE_KINDS(MF, HO)                                      \
+  do {                                                         \
+    if (kind == STAGE_FIRST) LEE_LAUNCH(MF, HO, STAGE_FIRST);  \
+    else if (kind == STAGE_MID) LEE_LAUNCH(MF, HO, STAGE_MID); \
+    else LEE_LAUNCH(MF, HO, STAGE_LAST);                       \
+  } while (0)
+  if (mf && ho) LEE_KINDS(true, true);
+  else if (mf) LEE_KINDS(true, false);
+  else if (ho) LEE_KINDS(false, true);
+  else LEE_KINDS(false, false);
+#undef LEE_KINDS
+#undef LEE_LAUNCH
+  if (b.carry_w)
+    hipLaunchKernelGGL(k_passive_w<T>, dim3(nblocks(b.n)), dim3(TPB), 0, s, a, kind);
+  return hipGetLastError();
+}
+
+hipError_t fused_stage(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc) {
+  if (h->cfg.precision == LEE_FP32) return fused_stage_t<float>(s, h, b, stage, sc, b.Y32, b.rho32);
+  return fused_stage_t<double>(s, h, b, stage, sc, b.Y64, b.rho64);
+}
+
+template <typename T>
+static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *const *Y, int src, int dst) {
+  FilterArgs<T> a{};
+  a.ni = b.ni;
+  a.nj = b.nj;
+  a.ntx = (b.ni + FL_TX - 1) / FL_TX;
+  a.nty = (b.nj + FL_TY - 1) / FL_TY;
+  a.ntiles = a.ntx * a.nty;
+  a.chunk = (a.ntiles + 7) / 8;
+  a.src = Y[src];
+  a.dst = Y[dst];
+  a.iblank = b.iblank;
+  a.code = b.code_flt;
+  int nv = 0;
+  a.vars[nv++] = IS;
+  a.vars[nv++] = IUX;
+  a.vars[nv++] = IUY;
+  a.vars[nv++] = IP;
+  if (b.carry_w) a.vars[nv++] = IUZ;
+  a.blend = (T)h->cfg.filter_blend;
+  a.rem = (T)(1.0 - h->cfg.filter_blend);
+  a.c = make_coef<T>(h->flt);
+  const dim3 grid(a.chunk * 8, nv), block(64, 4);
+  const bool h8 = h->flt.half == 4;
+  if (b.has_holes) {
+    if (h8) hipLaunchKernelGGL((k_fused_filter<T, true, 4>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_fused_filter<T, true, 2>), grid, block, 0, s, a);
+  } else {
+    if (h8) hipLaunchKernelGGL((k_fused_filter<T, false, 4>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_fused_filter<T, false, 2>), grid, block, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t fused_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst) {
+  if (h->cfg.precision == LEE_FP32) return fused_filter_t<float>(s, h, b, b.Y32, src, dst);
+  return fused_filter_t<double>(s, h, b, b.Y64, src, dst);
+}
+
+hipError_t cast_state(hipStream_t s, const Block &, int, const double *src, float *dst, size_t n) {
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(nblocks(n)), dim3(TPB), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t cast_back(hipStream_t s, const Block &, int, const float *src, double *dst, size_t n) {
+  hipLaunchKernelGGL((k_cast<float, double>), dim3(nblocks(n)), dim3(TPB), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+}  // namespace lee

@@ -1,0 +1,396 @@
+"""ctypes binding of the C ABI in include/lee.h, plus a thin Python mirror of
+the reference's solver call sequence.
+
+The shared library camille_amd/liblee_amd.so is built ahead of time
+(python -m camille_amd.build); importing this module never compiles and
+loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblee_amd.so")
+
+FP64, FP32 = 0, 1
+PATH_FUSED, PATH_PLAIN = 0, 1
+
+# BC-file type names (src/io/io.cpp:1273-1320) -> src/core/param.h codes
+BC_TYPES = {
+    "dirichlet_allzero": 1,
+    "neumann": 100,
+    "wall_slip_kinematic": 201,
+    "centerline_cart_norm2x": 900,
+    "centerline_cart_norm2y": 901,
+    "centerline_cart_norm2z": 902,
+    "sponge_freund_ambient": 1000,
+}
+# SIMULATION key -> physical source
+SOURCES = {
+    "CASE_TANNA_TPN49": 0,
+    "CASE_KBK_COMBUSTOR": 0,
+    "CASE_SCATTERING_TWOCYLINDER": 1,
+    "CASE_2DJET_WITH_A_HARMONIC_SOURCE": 2,
+    "CASE_MONOPOLE": 3,
+}
+
+NV = 5
+
+
+class LeeError(RuntimeError):
+    pass
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("num_dim", ctypes.c_int),
+        ("gamma", ctypes.c_double),
+        ("fd_order", ctypes.c_int),
+        ("filter_order", ctypes.c_int),
+        ("filter_blend", ctypes.c_double),
+        ("rk_stages", ctypes.c_int),
+        ("fix_cfl", ctypes.c_int),
+        ("cfl_or_dt", ctypes.c_double),
+        ("sponge_order", ctypes.c_int),
+        ("sponge_constant", ctypes.c_double),
+        ("source_kind", ctypes.c_int),
+        ("source_params", ctypes.c_double * 8),
+        ("precision", ctypes.c_int),
+        ("path", ctypes.c_int),
+    ]
+
+
+class _Boundary(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("dir", ctypes.c_int), ("end", ctypes.c_int),
+                ("is_", ctypes.c_int * 3), ("ie", ctypes.c_int * 3)]
+
+
+class _Block(ctypes.Structure):
+    _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("nk", ctypes.c_int),
+                ("xyz", ctypes.POINTER(ctypes.c_double)), ("mean", ctypes.POINTER(ctypes.c_double)),
+                ("nboundary", ctypes.c_int), ("boundary", ctypes.POINTER(_Boundary))]
+
+
+class _Overset(ctypes.Structure):
+    _fields_ = [("nrecv", ctypes.c_int),
+                ("recv_block", ctypes.POINTER(ctypes.c_int)), ("recv_l", ctypes.POINTER(ctypes.c_int)),
+                ("donor_block", ctypes.POINTER(ctypes.c_int)), ("donor_l", ctypes.POINTER(ctypes.c_int)),
+                ("widths", ctypes.POINTER(ctypes.c_int)), ("coef", ctypes.POINTER(ctypes.c_double)),
+                ("coef_stride", ctypes.c_int), ("nhole", ctypes.c_int),
+                ("hole_block", ctypes.POINTER(ctypes.c_int)), ("hole_l", ctypes.POINTER(ctypes.c_int))]
+
+
+EXPORTS = [
+    "lee_create", "lee_destroy", "lee_last_error", "lee_set_solution", "lee_get_solution", "lee_set_time",
+    "lee_get_time", "lee_compute_cfl", "lee_time_advance", "lee_solve", "lee_synchronize", "lee_compute_rhs",
+    "lee_apply_filter", "lee_take_derivative_xyz", "lee_update_boundary", "lee_get_metrics", "lee_get_iblank",
+    "lee_get_mean_gradient", "lee_get_sponge", "lee_last_solve_ms", "lee_version",
+]
+
+_lib = None
+
+
+def library():
+    """Loads liblee_amd.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LeeError(f"{LIB_PATH} is missing: build it with `python -m camille_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.lee_create.argtypes = [ctypes.POINTER(_Config), ctypes.POINTER(_Block), i, ctypes.POINTER(_Overset),
+                             ctypes.POINTER(vp)]
+    L.lee_destroy.argtypes = [vp]
+    L.lee_destroy.restype = None
+    L.lee_last_error.argtypes = [vp]
+    L.lee_last_error.restype = ctypes.c_char_p
+    L.lee_version.restype = ctypes.c_char_p
+    L.lee_set_solution.argtypes = [vp, i, dp]
+    L.lee_get_solution.argtypes = [vp, i, dp]
+    L.lee_set_time.argtypes = [vp, i, d, d]
+    L.lee_get_time.argtypes = [vp, ip, dp, dp]
+    L.lee_compute_cfl.argtypes = [vp, dp]
+    L.lee_time_advance.argtypes = [vp]
+    L.lee_solve.argtypes = [vp, i]
+    L.lee_synchronize.argtypes = [vp]
+    L.lee_compute_rhs.argtypes = [vp, i, dp, d, dp]
+    L.lee_apply_filter.argtypes = [vp, i, dp]
+    L.lee_take_derivative_xyz.argtypes = [vp, i, dp, i, dp]
+    L.lee_update_boundary.argtypes = [vp, ctypes.POINTER(dp)]
+    L.lee_get_metrics.argtypes = [vp, i, dp, dp]
+    L.lee_get_iblank.argtypes = [vp, i, ip]
+    L.lee_get_mean_gradient.argtypes = [vp, i, dp]
+    L.lee_get_sponge.argtypes = [vp, i, dp]
+    L.lee_last_solve_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def parse_bc_line(tokens, dims):
+    """One line of the BC file (src/io/io.cpp:1158-1268): `iblock type +dir
+    is ie js je ks ke`; '+' is the low face, '-' the high one; an index
+    written '-k' means n-1-k.  dims = (ni, nj, nk) of the block.
+    Returns (iblock, type code, dir, end, is[3], ie[3])."""
+    if len(tokens) != 9:
+        raise LeeError("Number of entries in the boundary condition file is incorrect.")
+    blk = int(tokens[0])
+    name = str(tokens[1]).lower()
+    if name not in BC_TYPES:
+        raise LeeError(f"unsupported boundary type {tokens[1]}")
+    side = str(tokens[2])
+    if side[0] not in "+-":
+        raise LeeError("Unknown side for a boundary in the boundary condition file.")
+    end = 0 if side[0] == "+" else 1
+    dname = side[1:].lower()
+    if dname not in ("xi", "eta", "zeta"):
+        raise LeeError("Unknown direction for a boundary in the boundary condition file.")
+    direction = ("xi", "eta", "zeta").index(dname)
+    vals = []
+    for t, tok in enumerate(tokens[3:9]):
+        s = str(tok)
+        v = int(s[1:]) if s[0] in "+-" else int(s)
+        if s[0] == "-":
+            v = (dims[t // 2] - 1) - v
+        vals.append(v)
+    return blk, BC_TYPES[name], direction, end, [vals[0], vals[2], vals[4]], [vals[1], vals[3], vals[5]]
+
+
+def read_bc_file(path, dims_per_block):
+    """Reads a BC file: skips empty / whitespace / '#' lines (io.cpp:522-563)."""
+    out = []
+    with open(path) as f:
+        for line in f:
+            s = line.strip()
+            if not s or s.startswith("#"):
+                continue
+            toks = line.split()
+            blk = int(toks[0])
+            out.append(parse_bc_line(toks, dims_per_block[blk]))
+    return out
+
+
+class Solver:
+    """One lee_handle: a set of blocks advanced on the current HIP device.
+
+    blocks: list of dict(x, y, [z], mean (5, nj, ni)) with x, y shaped (nj, ni)
+    boundaries: list of parsed BC entries (parse_bc_line) or raw BC-file
+    token tuples.  overset: dict of receiver/donor/hole tables or None.
+    """
+
+    def __init__(self, blocks, *, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4,
+                 mode="FIX_CFL", value=0.5, sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49",
+                 source_params=(), precision=FP64, path=PATH_FUSED, boundaries=(), overset=None):
+        L = library()
+        self.L = L
+        cfg = _Config()
+        cfg.num_dim = 2
+        cfg.gamma = gamma
+        cfg.fd_order = fd_order
+        cfg.filter_order = filter_order
+        cfg.filter_blend = filter_blend
+        cfg.rk_stages = rk
+        cfg.fix_cfl = 1 if mode == "FIX_CFL" else 0
+        cfg.cfl_or_dt = value
+        cfg.sponge_order = sponge_order
+        cfg.sponge_constant = sponge_constant
+        cfg.source_kind = SOURCES[simulation]
+        for k, v in enumerate(source_params):
+            cfg.source_params[k] = v
+        cfg.precision = precision
+        cfg.path = path
+        self.cfg = cfg
+        self.shapes = []
+        self._keep = []
+        nb = len(blocks)
+        barr = (_Block * nb)()
+        dims = []
+        for ib, b in enumerate(blocks):
+            x = np.asarray(b["x"], np.float64)
+            nj, ni = x.shape
+            dims.append((ni, nj, 1))
+            self.shapes.append((nj, ni))
+        parsed = [bc if isinstance(bc[1], int) else parse_bc_line(bc, dims[int(bc[0])]) for bc in boundaries]
+        for ib, b in enumerate(blocks):
+            x = np.asarray(b["x"], np.float64)
+            nj, ni = x.shape
+            z = np.zeros_like(x) if b.get("z") is None else np.asarray(b["z"], np.float64)
+            xyz = np.ascontiguousarray(np.stack([x, np.asarray(b["y"], np.float64), z]))
+            mean = np.ascontiguousarray(np.asarray(b["mean"], np.float64).reshape(5, nj, ni))
+            mine = [p for p in parsed if p[0] == ib]
+            bar = (_Boundary * max(1, len(mine)))()
+            for k, (_, typ, dr, end, is3, ie3) in enumerate(mine):
+                bar[k].type, bar[k].dir, bar[k].end = typ, dr, end
+                for q in range(3):
+                    bar[k].is_[q] = is3[q]
+                    bar[k].ie[q] = ie3[q]
+            self._keep += [xyz, mean, bar]
+            barr[ib].ni, barr[ib].nj, barr[ib].nk = ni, nj, 1
+            barr[ib].xyz = _dp(xyz)
+            barr[ib].mean = _dp(mean)
+            barr[ib].nboundary = len(mine)
+            barr[ib].boundary = bar
+        ovp = None
+        if overset is not None:
+            ovp = self._overset_struct(overset)
+        h = ctypes.c_void_p()
+        rc = L.lee_create(ctypes.byref(cfg), barr, nb, ovp, ctypes.byref(h))
+        if rc != 0:
+            raise LeeError(f"lee_create failed ({rc}): {L.lee_last_error(None).decode()}")
+        self.h = h
+
+    def _overset_struct(self, ov):
+        o = _Overset()
+        arrs = {}
+        for k in ("recv_block", "recv_l", "donor_block", "donor_l", "widths", "hole_block", "hole_l"):
+            arrs[k] = np.ascontiguousarray(np.asarray(ov.get(k, []), np.int32).reshape(-1))
+        coef = np.ascontiguousarray(np.asarray(ov["coef"], np.float64))
+        self._keep += list(arrs.values()) + [coef]
+        o.nrecv = len(arrs["recv_block"])
+        o.recv_block, o.recv_l = _ip(arrs["recv_block"]), _ip(arrs["recv_l"])
+        o.donor_block, o.donor_l = _ip(arrs["donor_block"]), _ip(arrs["donor_l"])
+        o.widths = _ip(arrs["widths"])
+        o.coef = _dp(coef)
+        o.coef_stride = coef.shape[1]
+        o.nhole = len(arrs["hole_block"])
+        o.hole_block, o.hole_l = _ip(arrs["hole_block"]), _ip(arrs["hole_l"])
+        self._keep.append(o)
+        return ctypes.byref(o)
+
+    @classmethod
+    def from_case(cls, case, precision=FP64, path=PATH_FUSED):
+        value = case["cfl"] if case["mode"] == "FIX_CFL" else case["dt"]
+        s = cls([dict(x=case["x"], y=case["y"], mean=case["mean"])], fd_order=case["fd_order"],
+                filter_order=case["filter_order"], filter_blend=case["filter_blend"], rk=case["rk"],
+                mode=case["mode"], value=value, sponge_order=case["sponge_order"],
+                sponge_constant=case["sponge_constant"], simulation=case["simulation"],
+                source_params=case.get("source_params", ()), precision=precision, path=path,
+                boundaries=case["bc"])
+        s.set_solution(case["ic"])
+        return s
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LeeError(f"camille_amd error {rc}: {self.L.lee_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lee_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def n(self, ib=0):
+        nj, ni = self.shapes[ib]
+        return ni * nj
+
+    # state ---------------------------------------------------------------
+    def set_solution(self, sol, ib=0):
+        a = np.ascontiguousarray(np.asarray(sol, np.float64).reshape(-1))
+        if a.size != NV * self.n(ib):
+            raise LeeError("solution size mismatch")
+        self._check(self.L.lee_set_solution(self.h, ib, _dp(a)))
+
+    def solution(self, ib=0):
+        out = np.empty((NV,) + self.shapes[ib])
+        self._check(self.L.lee_get_solution(self.h, ib, _dp(out)))
+        return out
+
+    def set_time(self, step, time, dt):
+        self._check(self.L.lee_set_time(self.h, int(step), float(time), float(dt)))
+
+    def clock(self):
+        st, t, dt = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.L.lee_get_time(self.h, ctypes.byref(st), ctypes.byref(t), ctypes.byref(dt)))
+        return st.value, t.value, dt.value
+
+    @property
+    def time(self):
+        return self.clock()[1]
+
+    @property
+    def dt(self):
+        return self.clock()[2]
+
+    @property
+    def time_step(self):
+        return self.clock()[0]
+
+    # advance -------------------------------------------------------------
+    def compute_cfl(self):
+        c = ctypes.c_double()
+        self._check(self.L.lee_compute_cfl(self.h, ctypes.byref(c)))
+        return c.value
+
+    def time_advance(self):
+        self._check(self.L.lee_time_advance(self.h))
+
+    def solve(self, nsteps, sync=True):
+        self._check(self.L.lee_solve(self.h, int(nsteps)))
+        if sync:
+            self.synchronize()
+
+    def synchronize(self):
+        self._check(self.L.lee_synchronize(self.h))
+
+    def last_solve_ms(self):
+        ms = ctypes.c_float()
+        self._check(self.L.lee_last_solve_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    # unit entry points ---------------------------------------------------
+    def compute_rhs(self, y, time, ib=0):
+        y = np.ascontiguousarray(np.asarray(y, np.float64))
+        out = np.empty_like(y)
+        self._check(self.L.lee_compute_rhs(self.h, ib, _dp(y), float(time), _dp(out)))
+        return out
+
+    def apply_filter(self, y, ib=0):
+        y = np.array(y, np.float64, copy=True, order="C")
+        self._check(self.L.lee_apply_filter(self.h, ib, _dp(y)))
+        return y
+
+    def take_derivative_xyz(self, f, direction, ib=0):
+        f = np.ascontiguousarray(np.asarray(f, np.float64))
+        out = np.empty_like(f)
+        self._check(self.L.lee_take_derivative_xyz(self.h, ib, _dp(f), int(direction), _dp(out)))
+        return out
+
+    def update_boundary(self, ys):
+        arrs = [np.array(y, np.float64, copy=True, order="C") for y in ys]
+        ptrs = (ctypes.POINTER(ctypes.c_double) * len(arrs))(*[_dp(a) for a in arrs])
+        self._check(self.L.lee_update_boundary(self.h, ptrs))
+        return arrs
+
+    # init products -------------------------------------------------------
+    def metrics(self, ib=0):
+        n = self.n(ib)
+        met, jac = np.empty(9 * n), np.empty(n)
+        self._check(self.L.lee_get_metrics(self.h, ib, _dp(met), _dp(jac)))
+        return met.reshape((3, 3) + self.shapes[ib]), jac.reshape(self.shapes[ib])
+
+    def iblank(self, ib=0):
+        out = np.empty(self.shapes[ib], np.int32)
+        self._check(self.L.lee_get_iblank(self.h, ib, _ip(out)))
+        return out
+
+    def mean_gradient(self, ib=0):
+        out = np.empty((10,) + self.shapes[ib])
+        self._check(self.L.lee_get_mean_gradient(self.h, ib, _dp(out)))
+        return out
+
+    def sponge(self, ib=0):
+        out = np.empty(self.shapes[ib])
+        self._check(self.L.lee_get_sponge(self.h, ib, _dp(out)))
+        return out

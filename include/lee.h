@@ -1,0 +1,179 @@
+/*
+ * lee.h -- C ABI of camille_amd, the MI355X-native linearized-Euler (LEE)
+ * time advance.  Plain pointers and sizes only; every call returns an int
+ * status (0 = ok) and never exits the process.
+ *
+ * The reference has no plugin API or FFI: its seam is the C++ function
+ *   void temporal::time_advance(UserInput*, StructuredGrid*, StructuredBlock*, State*)
+ *   (src/solver/temporal_discretization.cpp:334), called once per step by
+ *   solver::solve (src/solver/solver.cpp:41-89) after compute_CFL and
+ *   `time_sol += dt`.
+ * Each entry point below names the reference function it replaces.
+ *
+ * Arrays passed in or out are host memory in the reference layout: one
+ * block = n = ni*nj*nk points, i fastest (Geometry::StructuredBlock::idx1D,
+ * src/geometry/geometry.h:131-136); a field set is variable-major, e.g. a
+ * solution is 5*n doubles ordered s', u', v', w', p' (src/core/param.h:64-71).
+ * The library copies what it needs; the caller keeps ownership.
+ *
+ * One handle = one set of blocks on the current HIP device, one stream;
+ * calls on one handle must not overlap.
+ */
+#ifndef CAMILLE_AMD_LEE_H
+#define CAMILLE_AMD_LEE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LEE_OK 0
+#define LEE_EINVAL 1      /* bad argument or unsupported option */
+#define LEE_EHIP 2        /* HIP runtime error */
+#define LEE_ENOMEM 3
+#define LEE_ESTATE 4      /* call out of order (e.g. before lee_set_solution) */
+#define LEE_EHOLES 5      /* too few fluid points between holes (spatial:1794) */
+
+/* precision of the advance */
+#define LEE_FP64 0        /* everything in FP64 */
+#define LEE_FP32 1        /* FP32 state/stencils/RK; FP64 metrics, J, weights, time */
+
+/* execution path */
+#define LEE_PATH_FUSED 0  /* fused stage + filter kernels (default) */
+#define LEE_PATH_PLAIN 1  /* one kernel per reference operator, FP64 only,
+                             bit-identical arithmetic order */
+
+/* boundary / buffer-zone types: src/core/param.h:104-125 numbering */
+#define LEE_BC_DIRICHLET_ALLZERO 1
+#define LEE_BC_NEUMANN 100
+#define LEE_BC_WALL_SLIP_KINEMATIC 201
+#define LEE_BC_CENTERLINE_CART_NORM2X 900
+#define LEE_BC_CENTERLINE_CART_NORM2Y 901
+#define LEE_BC_CENTERLINE_CART_NORM2Z 902
+#define LEE_SPONGE_FREUND_AMBIENT 1000
+
+/* physical sources (SIMULATION key; src/solver/source.cpp:22-107) */
+#define LEE_SOURCE_NONE 0         /* CASE_TANNA_TPN49 and other source-free cases */
+#define LEE_SOURCE_TWOCYLINDER 1  /* CASE_SCATTERING_TWOCYLINDER */
+#define LEE_SOURCE_JET2D 2        /* CASE_2DJET_WITH_A_HARMONIC_SOURCE */
+#define LEE_SOURCE_MONOPOLE 3     /* extension: A exp(-ln2 |x-x0|^2/w^2) sin(omega t);
+                                     params {A, omega, x0, y0, z0, w} */
+
+typedef struct lee_handle lee_handle;
+
+/* Scheme and run settings: the deck keys of src/core/input.cpp:99-284. */
+typedef struct {
+  int num_dim;             /* DIMENSION: 2 */
+  double gamma;            /* GAMMA_SPECIFICHEAT */
+  int fd_order;            /* SPATIAL_SCHEME STANDARD_CENTRAL ORDER_ACCURACY: 4 */
+  int filter_order;        /* SPATIAL_FILTER STANDARD_CENTRAL ORDER_ACCURACY: 0 (NONE), 4, 8 */
+  double filter_blend;     /* SPATIAL_FILTER BLEND */
+  int rk_stages;           /* TEMPORAL_SCHEME: 2 (RUNGE_KUTTA2) or 4 (RUNGE_KUTTA4) */
+  int fix_cfl;             /* FIX_DT_OR_CFL: 1 = FIX_CFL, 0 = FIX_DT */
+  double cfl_or_dt;        /* CFL or DT */
+  int sponge_order;        /* BUFFER_ZONE POLYNOMIAL_ORDER */
+  double sponge_constant;  /* BUFFER_ZONE CONSTANT */
+  int source_kind;         /* LEE_SOURCE_* */
+  double source_params[8];
+  int precision;           /* LEE_FP64 / LEE_FP32 */
+  int path;                /* LEE_PATH_FUSED / LEE_PATH_PLAIN */
+} lee_config;
+
+/* One BC-file region (src/io/io.cpp:1158-1268) with negative indices
+ * resolved: type, normal direction (0 xi, 1 eta, 2 zeta), end (0 = '+' low
+ * face, 1 = '-' high face) and the block-index box is..ie (inclusive). */
+typedef struct {
+  int type, dir, end;
+  int is[3], ie[3];
+} lee_boundary;
+
+/* One block: grid coordinates (3n: x, y, z) and base state (5n: s, u, v,
+ * w, p) as read from GRID_FILE and BASESTATE_FILE. */
+typedef struct {
+  int ni, nj, nk;
+  const double *xyz;
+  const double *mean;
+  int nboundary;
+  const lee_boundary *boundary;
+} lee_block;
+
+/* Explicit overset interpolation tables (the Overture file of
+ * src/io/overture.cpp:473-680, flattened): per receiver its block and 1-D
+ * index, its donor block and the 1-D index of the stencil's lower-left
+ * corner, and width[0]*width[1]*width[2] raw coefficients (w1 fastest) of
+ * the donor block's stencil; holes as (block, 1-D index). */
+typedef struct {
+  int nrecv;
+  const int *recv_block, *recv_l;
+  const int *donor_block, *donor_l;
+  const int *widths;       /* 3 per block */
+  const double *coef;      /* nrecv * coef_stride */
+  int coef_stride;
+  int nhole;
+  const int *hole_block, *hole_l;
+} lee_overset;
+
+/* simulation::initialize from the base state onwards (src/core/simulation.cpp:
+ * 164-240): uploads the blocks, computes metrics and Jacobians (before any
+ * hole, metrics.cpp:9-44), marks holes and receivers, computes sponge
+ * strengths (geometry.cpp:1028-1142) and mean gradients (spatial.cpp:2250-2271),
+ * and prepares the CFL evaluation.  `ov` may be NULL. */
+int lee_create(const lee_config *cfg, const lee_block *blocks, int nblocks, const lee_overset *ov,
+               lee_handle **out);
+void lee_destroy(lee_handle *h);
+
+/* Message of the last failed call on this handle (or of lee_create when h is NULL). */
+const char *lee_last_error(const lee_handle *h);
+
+/* Restart / IC: io::read_solution (plot3d.cpp:715-833) + compute_dependent_variables.
+ * sol = 5n doubles; sets time_step and time_sol like tau[2], tau[3]. */
+int lee_set_solution(lee_handle *h, int iblock, const double *sol);
+int lee_get_solution(lee_handle *h, int iblock, double *sol);
+int lee_set_time(lee_handle *h, int time_step, double time_sol, double dt);
+int lee_get_time(const lee_handle *h, int *time_step, double *time_sol, double *dt);
+
+/* temporal::compute_CFL (temporal_discretization.cpp:397-450) at the
+ * current dt, with the reference's integer truncation of |U m|. */
+int lee_compute_cfl(lee_handle *h, double *cfl_max);
+
+/* temporal::time_advance (temporal_discretization.cpp:334-376): one step
+ * from the current solution with the handle's dt, sources and boundary
+ * data evaluated at time_sol + c_i dt (time_sol already advanced). */
+int lee_time_advance(lee_handle *h);
+
+/* solver::solve (solver.cpp:41-89) for nsteps: per step compute_CFL, dt
+ * rescaled under FIX_CFL, time_sol += dt, time_advance.  Enqueued
+ * asynchronously; lee_synchronize waits. */
+int lee_solve(lee_handle *h, int nsteps);
+int lee_synchronize(lee_handle *h);
+
+/* Unit entry points on one block (host arrays, 5n doubles).
+ * spatial::compute_RHS (spatial:1963-1990) of y at `time`, with the aux
+ * density taken from y itself. */
+int lee_compute_rhs(lee_handle *h, int iblock, const double *y, double time, double *rhs);
+/* spatial::apply_filter (spatial:2191-2246), in place. */
+int lee_apply_filter(lee_handle *h, int iblock, double *y);
+/* StandardCentral::take_derivative_xyz (spatial:1802-1826) of one field. */
+int lee_take_derivative_xyz(lee_handle *h, int iblock, const double *f, int dir_xyz, double *out);
+/* spatial::update_boundary (spatial:2162-2187): BCs then overset
+ * interpolation, in place on all blocks (ys[b] = 5n doubles of block b). */
+int lee_update_boundary(lee_handle *h, double *const *ys);
+
+/* Init products, for checks: metrics[l][i] (9n), J (n), iblank (n),
+ * mean gradients (5*num_dim*n, ivar*num_dim+dir), summed sponge strength (n). */
+int lee_get_metrics(lee_handle *h, int iblock, double *metrics, double *jac);
+int lee_get_iblank(lee_handle *h, int iblock, int *iblank);
+int lee_get_mean_gradient(lee_handle *h, int iblock, double *grad);
+int lee_get_sponge(lee_handle *h, int iblock, double *sigma);
+
+/* Timing of the last lee_solve on the device stream (HIP events) and the
+ * number of stage-kernel launches it made; for benchmarks. */
+int lee_last_solve_ms(lee_handle *h, float *ms);
+
+/* Library identification: returns a static string (build info). */
+const char *lee_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CAMILLE_AMD_LEE_H */
