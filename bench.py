@@ -1,0 +1,217 @@
+"""Benchmark of the LEE time advance on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 2-D
+curvilinear single block 1025x1025, x = xi + 0.05 L sin(2 pi eta / L),
+y = eta + 0.05 L sin(2 pi xi / L) on [-50, 50]^2 (L = 100), uniform mean
+flow M = 0.5 in x, time-harmonic monopole (omega = 0.2 pi, Gaussian
+half-width 4 dx, amplitude 1e-3) at a position drawn by
+numpy.random.default_rng(2) in the central 60 %, 40-point Freund sponges on
+all faces, RK4 at CFL 0.5 with the 8th-order filter.  Synthetic inputs,
+built here; state resident in HBM for the timed region.
+
+A step is one call of the time advance (4 RK stages + filter); the metric
+counts grid-point RK-stage updates: 1025^2 * 4 per step.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+N1D = 1025
+
+
+def c2_inputs(n=N1D):
+    L = 100.0
+    d = 100.0 / (n - 1)
+    xi = -50.0 + d * np.arange(n)
+    X, E = np.meshgrid(xi, xi)
+    x = X + 0.05 * L * np.sin(2.0 * np.pi * E / L)
+    y = E + 0.05 * L * np.sin(2.0 * np.pi * X / L)
+    mean = np.zeros((5, n, n))
+    mean[1] = 0.5
+    mean[4] = 1.0 / 1.4
+    rng = np.random.default_rng(2)
+    xs, ys = rng.uniform(-30.0, 30.0, 2)
+    w = 40 - 1
+    bc = [(0, "sponge_freund_ambient", "+xi", "0", str(w), "0", "-0", "0", "0"),
+          (0, "sponge_freund_ambient", "-xi", f"-{w}", "-0", "0", "-0", "0", "0"),
+          (0, "sponge_freund_ambient", "+eta", "0", "-0", "0", str(w), "0", "0"),
+          (0, "sponge_freund_ambient", "-eta", "0", "-0", f"-{w}", "-0", "0", "0")]
+    src = (1e-3, 0.2 * math.pi, float(xs), float(ys), 0.0, 4.0 * d)
+    return dict(x=x, y=y, mean=mean, bc=bc, source=src, ic=np.zeros((5, n, n)), dx=d)
+
+
+def make_solver(inp, precision, path):
+    from camille_amd import Solver
+    s = Solver([dict(x=inp["x"], y=inp["y"], mean=inp["mean"])], simulation="CASE_MONOPOLE",
+               source_params=inp["source"], precision=precision, path=path, boundaries=inp["bc"],
+               mode="FIX_CFL", value=0.5)
+    s.set_solution(inp["ic"])
+    return s
+
+
+def timed(s, warmup, steps, kernel_timing=False):
+    """Wall time of `steps` steps after `warmup` untimed ones; then, when
+    kernel_timing, the same number of steps again with HIP events around
+    every kernel (the events themselves slow the stream down, so they stay
+    out of the throughput figure)."""
+    s.solve(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.solve(steps, sync=False)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    kt = None
+    if kernel_timing:
+        s.set_kernel_timing(True)
+        s.solve(steps)
+        kt = s.kernel_timing()
+        s.set_kernel_timing(False)
+    return wall, kt
+
+
+def cpu_baseline(inp, nsteps):
+    """The CPU oracle (FP64 restatement, 1 thread) on the same workload for a
+    bounded number of steps."""
+    from oracle.pyoracle import Oracle
+    scheme = dict(fd_order=4, filter_order=8, filter_blend=1.0, rk=4, sponge_order=2, sponge_constant=5.0)
+    o = Oracle([dict(x=inp["x"], y=inp["y"], mean=inp["mean"], ic=inp["ic"])], scheme, inp["bc"],
+               ("CASE_MONOPOLE", inp["source"]), "FIX_CFL", 0.5)
+    t0 = time.perf_counter()
+    o.run(nsteps)
+    wall = time.perf_counter() - t0
+    return wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--accuracy-steps", type=int, default=2000)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64", "plain"], default=None,
+                    help="time one path only (for profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (N = 1); larger N is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from camille_amd import FP32, FP64, PATH_FUSED, PATH_PLAIN
+    inp = c2_inputs()
+    npts = N1D * N1D
+    upd_per_step = npts * 4
+
+    res = {}
+    paths = [("fp32", FP32, PATH_FUSED, args.steps), ("fp64", FP64, PATH_FUSED, args.steps),
+             ("plain", FP64, PATH_PLAIN, max(10, args.steps // 20))]
+    if args.only:
+        paths = [p for p in paths if p[0] == args.only]
+    for label, prec, path, steps in paths:
+        print(f"[bench] timing {label}: {args.warmup} warmup + {steps} steps", file=sys.stderr, flush=True)
+        s = make_solver(inp, prec, path)
+        wall, kt = timed(s, args.warmup if label != "plain" else 2, steps, kernel_timing=(path == PATH_FUSED))
+        res[label] = dict(steps=steps, wall=wall, ups=upd_per_step * steps / wall, kt=kt)
+        s.close()
+        print(f"[bench] {label}: {res[label]['ups']:.3e} updates/s", file=sys.stderr, flush=True)
+    if args.only:
+        print(json.dumps({k: dict(ups=v["ups"], kt=v["kt"]) for k, v in res.items()}))
+        return
+
+    # accuracy leg: the full 2000-step configuration in FP32 and FP64 (fused)
+    # and FP64 plain (bit-identical arithmetic to the CPU reference)
+    print(f"[bench] accuracy leg: {args.accuracy_steps} steps x 3 paths", file=sys.stderr, flush=True)
+    finals = {}
+    for label, prec, path in [("fp32", FP32, PATH_FUSED), ("fp64", FP64, PATH_FUSED), ("plain", FP64, PATH_PLAIN)]:
+        s = make_solver(inp, prec, path)
+        s.solve(args.accuracy_steps)
+        finals[label] = s.solution()
+        clock = s.clock()
+        s.close()
+        print(f"[bench] accuracy {label} done (t = {clock[1]:.6f})", file=sys.stderr, flush=True)
+    p64 = finals["plain"][4]
+    pmax = float(np.max(np.abs(p64)))
+    err32 = float(np.max(np.abs(finals["fp32"][4] - p64)) / pmax)
+    err64 = float(np.max(np.abs(finals["fp64"][4] - p64)) / pmax)
+
+    # dominant kernel: the class with the most time in the FP32 timed region
+    kt = res["fp32"]["kt"]
+    stage_classes = ["stage_first", "stage_mid", "stage_last", "filter"]
+    dom = max(stage_classes, key=lambda c: kt[c][0])
+    avg_ms = kt[dom][0] / kt[dom][1]
+    b = 4  # FP32 bulk
+    V, D = 4, 2
+    per_point = {  # algorithmic bytes per point per launch: state arrays, FP64 metrics + J, sigma
+        "stage_first": 3 * V * b + 8 * (D * D + 1) + b,      # read y_n; write acc, next input
+        "stage_mid": 5 * V * b + 8 * (D * D + 1) + b,        # read in, y_n, acc; write acc, next input
+        "stage_last": 3 * V * b + 8 * (D * D + 1) + b,       # read in, acc; write acc
+        "filter": 2 * V * b,                                 # read y_{n+1}; write filtered
+    }[dom]
+    achieved = per_point * npts / (avg_ms * 1e-3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c2_fp32.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(dom)
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print(f"[bench] CPU oracle baseline: {args.cpu_steps} steps, 1 thread", file=sys.stderr, flush=True)
+        wall = cpu_baseline(inp, args.cpu_steps)
+        cpu = {"value": upd_per_step * args.cpu_steps / wall, "unit": "grid-point RK-stage updates/s",
+               "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} RK4 steps of the same 1025x1025 workload in FP64 by the oracle "
+                         f"(oracle/lee_oracle.c, 1 thread, {wall:.1f} s)"}
+
+    # algorithmic bytes per update of the whole step, FP32 bulk, uniform mean:
+    # 5 V b state + 8 (D^2 + 1) metrics + b sigma + V b / 2 filter
+    algo_step_bytes = 5 * 4 * 4 + 8 * 5 + 4 + 8
+    v = res["fp32"]["ups"]
+    out = {
+        "metric": "grid-point RK-stage updates/s + % HBM peak; FP32 vs FP64 at <=1e-5 rel p-error",
+        "value": v,
+        "unit": "grid-point RK-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["fp32"]["wall"] / res["fp32"]["steps"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 metrics/J/time)",
+        "data": "synthetic (grid, base state and source built in bench.py)",
+        "config": {"workload": "2-D curvilinear 1025x1025, M=0.5, monopole omega=0.2pi, RK4 CFL 0.5, "
+                               "8th-order filter, 40-pt sponges (C2)",
+                   "grid": [N1D, N1D], "steps_accuracy": args.accuracy_steps, "path": "fused"},
+        "hbm_frac_step": v * algo_step_bytes / HBM_PEAK,
+        "fp64_fused_updates_per_s": res["fp64"]["ups"],
+        "fp64_plain_updates_per_s": res["plain"]["ups"],
+        "speedup_fp32_vs_fp64_fused": v / res["fp64"]["ups"],
+        "speedup_fp32_vs_fp64_plain": v / res["plain"]["ups"],
+        "p_rel_err_fp32_vs_fp64": err32,
+        "p_rel_err_fp64_fused_vs_plain": err64,
+        "kernel_ms": {k: [round(t, 4), c] for k, (t, c) in kt.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes_per_point": per_point, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

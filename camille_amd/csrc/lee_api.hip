@@ -84,6 +84,18 @@ int fused_target(const lee_handle *h, int stage) {
   return stage % 2 == 0 ? 2 : 3;
 }
 
+// Records one event of the timing pool on the handle's stream.
+hipError_t mark(lee_handle *h) {
+  if (h->evused == h->evpool.size()) {
+    hipEvent_t e;
+    // device-scope fences only: the timed kernels never touch host memory
+    hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+    if (r != hipSuccess) return r;
+    h->evpool.push_back(e);
+  }
+  return hipEventRecord(h->evpool[h->evused++], h->stream);
+}
+
 int enqueue_fused_step(lee_handle *h) {
   const int S = h->cfg.rk_stages, last = S - 1;
   const double tn = h->time_sol, dt = h->dt;
@@ -94,8 +106,16 @@ int enqueue_fused_step(lee_handle *h) {
     sc.b = h->b_i[st];
     sc.a = h->a_next[st];
     source_factors(h, tn + h->c_i[st] * dt, sc.s1, sc.s2);
+    if (h->ktiming) HIPCHK(mark(h));
     for (int ib = 0; ib < h->nblk; ib++) HIPCHK(fused_stage(h->stream, h, h->b[ib], st, sc));
+    if (h->ktiming) {
+      HIPCHK(mark(h));
+      h->evclass.push_back(st == 0 ? 0 : (st == last ? 2 : 1));
+    }
     const int tgt = fused_target(h, st);
+    bool any_bnd = h->ov.nrecv > 0;
+    for (int ib = 0; ib < h->nblk; ib++) any_bnd |= !h->b[ib].faces.empty();
+    if (h->ktiming && any_bnd) HIPCHK(mark(h));
     for (int ib = 0; ib < h->nblk; ib++) {
       Block &b = h->b[ib];
       if (b.faces.empty()) continue;
@@ -104,11 +124,20 @@ int enqueue_fused_step(lee_handle *h) {
                        rho ? b.rho32 : nullptr));
     }
     HIPCHK(overset_interpolate(h->stream, h, f32, tgt, st == last));
+    if (h->ktiming && any_bnd) {
+      HIPCHK(mark(h));
+      h->evclass.push_back(4);
+    }
   }
   for (int ib = 0; ib < h->nblk; ib++) {
     Block &b = h->b[ib];
     if (h->cfg.filter_order > 0) {
+      if (h->ktiming) HIPCHK(mark(h));
       HIPCHK(fused_filter(h->stream, h, b, 1, 0));
+      if (h->ktiming) {
+        HIPCHK(mark(h));
+        h->evclass.push_back(3);
+      }
     } else {
       // no filter: y_{n+1} becomes the state (holes are already at sol_ref)
       std::swap(b.Y64[0], b.Y64[1]);
@@ -247,6 +276,7 @@ void lee_destroy(lee_handle *h) {
   dfree(h->ov.w);
   dfree(h->ov.coef);
   dfree(h->ov.buf);
+  for (auto e : h->evpool) (void)hipEventDestroy(e);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -562,6 +592,33 @@ int lee_synchronize(lee_handle *h) {
   if (!h) return LEE_EINVAL;
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  return LEE_OK;
+}
+
+int lee_set_kernel_timing(lee_handle *h, int on) {
+  if (!h) return LEE_EINVAL;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->ktiming = on != 0;
+  h->evused = 0;
+  h->evclass.clear();
+  return LEE_OK;
+}
+
+int lee_get_kernel_timing(lee_handle *h, double *ms, int *count) {
+  if (!h || !ms || !count) return fail(h, LEE_EINVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int c = 0; c < 5; c++) {
+    ms[c] = 0.0;
+    count[c] = 0;
+  }
+  for (size_t k = 0; k < h->evclass.size(); k++) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, h->evpool[2 * k], h->evpool[2 * k + 1]));
+    ms[h->evclass[k]] += t;
+    count[h->evclass[k]]++;
+  }
+  h->evused = 0;
+  h->evclass.clear();
   return LEE_OK;
 }
 

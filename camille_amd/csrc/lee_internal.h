@@ -120,6 +120,12 @@ struct lee_handle {
   float last_ms = 0.f;
   std::string err;
   int device = 0;
+  // optional per-kernel timing: HIP events around each fused launch,
+  // classes 0 first stage, 1 middle stages, 2 last stage, 3 filter, 4 boundary
+  bool ktiming = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  std::vector<int> evclass;  // one entry per (start, stop) pair
 };
 
 namespace lee {

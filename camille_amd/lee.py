@@ -86,6 +86,7 @@ EXPORTS = [
     "lee_get_time", "lee_compute_cfl", "lee_time_advance", "lee_solve", "lee_synchronize", "lee_compute_rhs",
     "lee_apply_filter", "lee_take_derivative_xyz", "lee_update_boundary", "lee_get_metrics", "lee_get_iblank",
     "lee_get_mean_gradient", "lee_get_sponge", "lee_last_solve_ms", "lee_version",
+    "lee_set_kernel_timing", "lee_get_kernel_timing",
 ]
 
 _lib = None
@@ -125,6 +126,8 @@ def library():
     L.lee_get_mean_gradient.argtypes = [vp, i, dp]
     L.lee_get_sponge.argtypes = [vp, i, dp]
     L.lee_last_solve_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.lee_set_kernel_timing.argtypes = [vp, i]
+    L.lee_get_kernel_timing.argtypes = [vp, dp, ip]
     _lib = L
     return L
 
@@ -348,6 +351,18 @@ class Solver:
         ms = ctypes.c_float()
         self._check(self.L.lee_last_solve_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    KERNEL_CLASSES = ("stage_first", "stage_mid", "stage_last", "filter", "boundary")
+
+    def set_kernel_timing(self, on):
+        self._check(self.L.lee_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_timing(self):
+        """{class: (total ms, launches)} since the last call."""
+        ms = np.zeros(5)
+        cnt = np.zeros(5, np.int32)
+        self._check(self.L.lee_get_kernel_timing(self.h, _dp(ms), _ip(cnt)))
+        return {k: (float(ms[c]), int(cnt[c])) for c, k in enumerate(self.KERNEL_CLASSES)}
 
     # unit entry points ---------------------------------------------------
     def compute_rhs(self, y, time, ib=0):

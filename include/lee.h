@@ -169,6 +169,14 @@ int lee_get_sponge(lee_handle *h, int iblock, double *sigma);
  * number of stage-kernel launches it made; for benchmarks. */
 int lee_last_solve_ms(lee_handle *h, float *ms);
 
+/* Per-kernel timing of the fused path for benchmarks (no reference
+ * counterpart): while enabled, HIP events bracket every launch on the
+ * handle's stream.  lee_get_kernel_timing sums the recorded durations since
+ * the last call into ms[5] / count[5] by class: 0 first RK stage, 1 middle
+ * stages, 2 last stage, 3 filter, 4 boundary conditions + overset. */
+int lee_set_kernel_timing(lee_handle *h, int on);
+int lee_get_kernel_timing(lee_handle *h, double *ms, int *count);
+
 /* Library identification: returns a static string (build info). */
 const char *lee_version(void);
 

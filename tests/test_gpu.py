@@ -1,0 +1,197 @@
+"""GPU parity of the HIP path (through the C ABI) against the reference's
+golden fixtures and the CPU oracle.
+
+Tolerances (relative to the field's max magnitude):
+  plain FP64 path   bit-identical (same operation order, no contraction)
+  fused FP64 path   1e-12
+  fused FP32 bulk   1e-5 (FP64 metrics, Jacobians and time)
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from camille_amd import FP32, FP64, PATH_FUSED, PATH_PLAIN, Solver
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL = {"fused64": 1e-12, "fused32": 1e-5}
+
+
+def _golden(name):
+    return np.load(os.path.join(GOLD, f"{name}.npz"))
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("name", sorted(cases.ALL))
+def test_plain_path_bit_identical_to_reference(name):
+    c = cases.ALL[name]()
+    g = _golden(name)
+    s = Solver.from_case(c, precision=FP64, path=PATH_PLAIN)
+    s.solve(c["nsteps"])
+    sol = s.solution()
+    print(f"case {name}: plain t = {s.time!r}", flush=True)
+    assert s.time == float(g["tau"][3])
+    for k, v in enumerate(g["vars"]):
+        assert np.array_equal(sol[v], g["sol"][k]), f"var {v}"
+    for v in range(5):
+        if v not in g["vars"]:
+            assert np.all(sol[v] == 0.0)
+
+
+@pytest.mark.parametrize("label,prec", [("fused64", FP64), ("fused32", FP32)])
+@pytest.mark.parametrize("name", sorted(cases.ALL))
+def test_fused_path_matches_reference(name, label, prec):
+    c = cases.ALL[name]()
+    g = _golden(name)
+    s = Solver.from_case(c, precision=prec, path=PATH_FUSED)
+    s.solve(c["nsteps"])
+    sol = s.solution()
+    assert s.time == float(g["tau"][3])  # dt and time are FP64 host arithmetic in every mode
+    errs = [_rel(sol[v], g["sol"][k]) for k, v in enumerate(g["vars"])]
+    print(f"case {name} {label}: rel errors {['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) <= TOL[label]
+
+
+def _smooth_state(c, seed=0):
+    x, y = c["x"], c["y"]
+    rng = np.random.default_rng(seed)
+    y5 = np.zeros((5,) + x.shape)
+    for v in (0, 1, 2, 4):
+        a, b, ph = rng.uniform(0.05, 0.4, 3)
+        y5[v] = 1e-3 * np.sin(a * x + ph) * np.cos(b * y - ph) * np.exp(-(x * x + y * y) / 400.0)
+    return y5
+
+
+@pytest.mark.parametrize("name", ["curv", "shear", "cylsrc", "wall"])
+def test_rhs_unit_parity(name):
+    c = cases.ALL[name]()
+    o = Oracle.from_case(c)
+    y5 = _smooth_state(c)
+    t = 0.37
+    ref = o.rhs(y5[:, None], t)[:, 0]
+    for path, prec, tol in ((PATH_PLAIN, FP64, 0.0), (PATH_FUSED, FP64, 1e-13), (PATH_FUSED, FP32, 1e-5)):
+        s = Solver.from_case(c, precision=prec, path=path)
+        r = s.compute_rhs(y5, t)
+        err = max(_rel(r[v], ref[v]) for v in (0, 1, 2, 4) if np.any(ref[v]))
+        print(f"rhs {name} path {path} prec {prec}: {err:.2e}", flush=True)
+        assert err <= tol
+        s.close()
+
+
+@pytest.mark.parametrize("name", ["filt", "variants"])
+def test_filter_unit_parity(name):
+    c = cases.ALL[name]()
+    o = Oracle.from_case(c)
+    y5 = _smooth_state(c, 1) + 1e-4 * np.random.default_rng(3).standard_normal((5,) + c["x"].shape)
+    y5[3] = 0.0
+    ref = o.filter(y5[:, None])[:, 0]
+    for path, prec, tol in ((PATH_PLAIN, FP64, 0.0), (PATH_FUSED, FP64, 1e-14), (PATH_FUSED, FP32, 1e-6)):
+        s = Solver.from_case(c, precision=prec, path=path)
+        f = s.apply_filter(y5)
+        err = max(_rel(f[v], ref[v]) for v in (0, 1, 2, 4))
+        print(f"filter {name} path {path} prec {prec}: {err:.2e}", flush=True)
+        assert err <= tol
+        s.close()
+
+
+@pytest.mark.parametrize("name", ["curv", "shear", "wall"])
+def test_init_products_bit_identical(name):
+    """Metrics, Jacobian, mean gradients, sponge strengths and the CFL
+    maximum equal the oracle's (and so the reference's) bit for bit."""
+    c = cases.ALL[name]()
+    o = Oracle.from_case(c)
+    s = Solver.from_case(c, precision=FP64, path=PATH_FUSED)
+    met, jac = s.metrics()
+    om, _, oj = o.metrics()
+    assert np.array_equal(met[:2, :2], om[:2, :2, 0])
+    assert np.array_equal(jac, oj[0])
+    assert np.array_equal(s.mean_gradient(), o.mean_gradient()[:, 0])
+    assert np.array_equal(s.sponge(), o.sigma()[0])
+    assert s.compute_cfl() == o.cfl()
+    s.close()
+
+
+def _c2like(n, cartesian):
+    L = 100.0
+    d = L / (n - 1)
+    xi = -50.0 + d * np.arange(n)
+    X, E = np.meshgrid(xi, xi)
+    if cartesian:
+        x, y = X, E
+    else:
+        x = X + 0.05 * L * np.sin(2 * np.pi * E / L)
+        y = E + 0.05 * L * np.sin(2 * np.pi * X / L)
+    return x, y, d
+
+
+def _sponges(w):
+    w -= 1
+    return [(0, "sponge_freund_ambient", "+xi", "0", str(w), "0", "-0", "0", "0"),
+            (0, "sponge_freund_ambient", "-xi", f"-{w}", "-0", "0", "-0", "0", "0"),
+            (0, "sponge_freund_ambient", "+eta", "0", "-0", "0", str(w), "0", "0"),
+            (0, "sponge_freund_ambient", "-eta", "0", "-0", f"-{w}", "-0", "0", "0")]
+
+
+def test_full_size_transpose_symmetry_plain():
+    """A centred pulse on a symmetric 1025^2 Cartesian grid stays exactly
+    symmetric under i <-> j (p = p^T, u = v^T) in the plain path, whose
+    x- and y-derivative sums differ only by operand order.  p_bar = 0.75 has
+    an exactly zero FD gradient, and no sponge or filter is used (zones add
+    in file order and the filter sweeps xi before eta, neither symmetric in
+    floating point), so no sum depends on the direction order; the pulse
+    stays far from the block edges in 30 steps."""
+    n = 1025
+    x, y, d = _c2like(n, True)
+    mean = np.zeros((5, n, n))
+    mean[4] = 0.75
+    ic = np.zeros((5, n, n))
+    ic[4] = 1e-3 * np.exp(-math.log(2) * (x * x + y * y) / (3 * d) ** 2)
+    s = Solver([dict(x=x, y=y, mean=mean)], boundaries=(), path=PATH_PLAIN, filter_order=0)
+    s.set_solution(ic)
+    s.solve(30)
+    q = s.solution()
+    assert np.array_equal(q[4], q[4].T)
+    assert np.array_equal(q[1], q[2].T)
+    assert np.max(np.abs(q[4])) > 1e-5
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-12), (FP32, 1e-5)])
+def test_full_size_fused_vs_plain(prec, tol):
+    """1025^2 curvilinear monopole case: fused path against the bit-exact
+    plain path after 100 steps."""
+    n = 1025
+    x, y, d = _c2like(n, False)
+    mean = np.zeros((5, n, n))
+    mean[1] = 0.5
+    mean[4] = 1 / 1.4
+    src = (1e-3, 0.2 * math.pi, 7.0, -11.0, 0.0, 4 * d)
+    out = {}
+    for path, pr in ((PATH_PLAIN, FP64), (PATH_FUSED, prec)):
+        s = Solver([dict(x=x, y=y, mean=mean)], boundaries=_sponges(40), path=path, precision=pr,
+                   simulation="CASE_MONOPOLE", source_params=src)
+        s.set_solution(np.zeros((5, n, n)))
+        s.solve(100)
+        out[path] = s.solution()
+        s.close()
+    err = _rel(out[PATH_FUSED][4], out[PATH_PLAIN][4])
+    print(f"1025^2 fused prec {prec} vs plain: p rel err {err:.2e}", flush=True)
+    assert err <= tol
+
+
+def test_handle_errors_are_reported():
+    c = cases.case_curv()
+    s = Solver.from_case(c)
+    with pytest.raises(Exception):
+        s.set_solution(np.zeros(7))
+    with pytest.raises(Exception):
+        s.compute_rhs(np.zeros((5,) + c["x"].shape), 0.0, ib=3)
+    s.close()
