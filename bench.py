@@ -123,11 +123,12 @@ def main():
         print(f"[bench] timing {label}: {args.warmup} warmup + {steps} steps", file=sys.stderr, flush=True)
         s = make_solver(inp, prec, path)
         wall, kt = timed(s, args.warmup if label != "plain" else 2, steps, kernel_timing=(path == PATH_FUSED))
-        res[label] = dict(steps=steps, wall=wall, ups=upd_per_step * steps / wall, kt=kt)
+        res[label] = dict(steps=steps, wall=wall, ups=upd_per_step * steps / wall, kt=kt,
+                          tiles=s.tile_shape() if path == PATH_FUSED else None)
         s.close()
         print(f"[bench] {label}: {res[label]['ups']:.3e} updates/s", file=sys.stderr, flush=True)
     if args.only:
-        print(json.dumps({k: dict(ups=v["ups"], kt=v["kt"]) for k, v in res.items()}))
+        print(json.dumps({k: dict(ups=v["ups"], kt=v["kt"], tiles=v["tiles"]) for k, v in res.items()}))
         return
 
     # accuracy leg: the full 2000-step configuration in FP32 and FP64 (fused)
@@ -205,6 +206,7 @@ def main():
         "p_rel_err_fp32_vs_fp64": err32,
         "p_rel_err_fp64_fused_vs_plain": err64,
         "kernel_ms": {k: [round(t, 4), c] for k, (t, c) in kt.items()},
+        "tile_shape": {"stage_rows_per_thread": res["fp32"]["tiles"][0], "filter_tile_rows": res["fp32"]["tiles"][1]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes_per_point": per_point, "avg_launch_ms": avg_ms},

@@ -192,6 +192,88 @@ int enqueue_plain_step(lee_handle *h) {
 
 int enqueue_step(lee_handle *h) { return is_plain(h) ? enqueue_plain_step(h) : enqueue_fused_step(h); }
 
+// Tile-shape autotuner of the fused kernels.  For each block, every stage
+// tile height (rows per thread R = 1, 2, 4) and filter tile height (8, 16,
+// 32) is timed with HIP events on the block itself, in buffers that carry no
+// state between steps (stage: input Y2, accumulator Y3, output Y1; filter:
+// Y1 -> Y2); the fastest is kept and cached per (ni, nj, precision, mean
+// kind, holes) for the life of the process.
+struct TuneKey {
+  int ni, nj, prec, meanf, holes;
+  bool operator==(const TuneKey &o) const {
+    return ni == o.ni && nj == o.nj && prec == o.prec && meanf == o.meanf && holes == o.holes;
+  }
+};
+std::vector<std::pair<TuneKey, std::pair<int, int>>> g_tune_cache;
+
+int autotune_block(lee_handle *h, Block &b) {
+  const TuneKey key{b.ni, b.nj, h->cfg.precision, b.uniform_mean ? 0 : 1, b.has_holes ? 1 : 0};
+  for (const auto &e : g_tune_cache)
+    if (e.first == key) {
+      b.tune_r = e.second.first;
+      b.tune_fty = e.second.second;
+      b.tuned = true;
+      return LEE_OK;
+    }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int reps = 10;
+  const StageScalars sc{1.0, 1.0, 0.0, 0.0, 0.0};
+  float best = 1e30f;
+  int best_r = b.tune_r;
+  for (int r : {1, 2, 4}) {
+    b.tune_r = r;
+    for (int k = 0; k < 2; k++) HIPCHK(fused_stage(h->stream, h, b, -1, sc));
+    HIPCHK(hipEventRecord(e0, h->stream));
+    for (int k = 0; k < reps; k++) HIPCHK(fused_stage(h->stream, h, b, -1, sc));
+    HIPCHK(hipEventRecord(e1, h->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_r = r;
+    }
+  }
+  b.tune_r = best_r;
+  best = 1e30f;
+  int best_f = b.tune_fty;
+  if (h->cfg.filter_order > 0) {
+    for (int ty : {4, 8, 16}) {
+      if (is_fp32(h) ? false : ty > 4) continue;  // FP64 tiles above 4 rows exceed 64 KB of LDS
+      b.tune_fty = ty;
+      for (int k = 0; k < 2; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
+      HIPCHK(hipEventRecord(e0, h->stream));
+      for (int k = 0; k < reps; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
+      HIPCHK(hipEventRecord(e1, h->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_f = ty;
+      }
+    }
+  }
+  b.tune_fty = best_f;
+  b.tuned = true;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g_tune_cache.push_back({key, {b.tune_r, b.tune_fty}});
+  return LEE_OK;
+}
+
+int autotune(lee_handle *h) {
+  if (is_plain(h)) return LEE_OK;
+  for (int ib = 0; ib < h->nblk; ib++) {
+    if (h->b[ib].tuned) continue;
+    int rc = autotune_block(h, h->b[ib]);
+    if (rc) return rc;
+  }
+  return LEE_OK;
+}
+
 // solver::solve step prologue (solver.cpp:41-75): CFL at the current dt,
 // dt rescaled under FIX_CFL, time_sol += dt.
 void advance_clock(lee_handle *h) {
@@ -578,6 +660,10 @@ int lee_time_advance(lee_handle *h) {
 int lee_solve(lee_handle *h, int nsteps) {
   if (!h || nsteps < 0) return fail(h, LEE_EINVAL, "bad argument");
   if (!h->have_solution) return fail(h, LEE_ESTATE, "no solution set");
+  if (nsteps > 0) {
+    int rc = autotune(h);
+    if (rc) return rc;
+  }
   HIPCHK(hipEventRecord(h->ev0, h->stream));
   for (int k = 0; k < nsteps; k++) {
     advance_clock(h);
@@ -592,6 +678,35 @@ int lee_synchronize(lee_handle *h) {
   if (!h) return LEE_EINVAL;
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  return LEE_OK;
+}
+
+int lee_autotune(lee_handle *h) {
+  if (!h) return LEE_EINVAL;
+  for (int ib = 0; ib < h->nblk; ib++) h->b[ib].tuned = false;
+  g_tune_cache.clear();
+  int rc = autotune(h);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return LEE_OK;
+}
+
+int lee_set_tile_shape(lee_handle *h, int ib, int stage_rows, int filter_rows) {
+  if (!h || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  if (stage_rows != 1 && stage_rows != 2 && stage_rows != 4) return fail(h, LEE_EINVAL, "stage rows must be 1, 2 or 4");
+  if (filter_rows != 4 && filter_rows != 8 && filter_rows != 16)
+    return fail(h, LEE_EINVAL, "filter tile height must be 4, 8 or 16");
+  if (!is_fp32(h) && filter_rows > 4) return fail(h, LEE_EINVAL, "FP64 filter tiles are 4 rows high");
+  h->b[ib].tune_r = stage_rows;
+  h->b[ib].tune_fty = filter_rows;
+  h->b[ib].tuned = true;
+  return LEE_OK;
+}
+
+int lee_get_tile_shape(lee_handle *h, int ib, int *stage_rows, int *filter_rows) {
+  if (!h || ib < 0 || ib >= h->nblk || !stage_rows || !filter_rows) return fail(h, LEE_EINVAL, "bad argument");
+  *stage_rows = h->b[ib].tune_r;
+  *filter_rows = h->b[ib].tune_fty;
   return LEE_OK;
 }
 

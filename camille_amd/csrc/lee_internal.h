@@ -77,6 +77,10 @@ struct Block {
   double *Y64[4] = {nullptr, nullptr, nullptr, nullptr};
   float *Y32[4] = {nullptr, nullptr, nullptr, nullptr};
   bool carry_w = false;        // 2-D with a non-zero w' (sponge + filter only)
+  // tile shape of the fused kernels: stage rows per thread (1, 2, 4; tile
+  // height 4 R) and filter tile height (4, 8, 16; FP64 4); set by the autotuner
+  int tune_r = 1, tune_fty = 4;
+  bool tuned = false;
   // plain-path scratch (FP64): dflux, sum, rhs (5n), func0, func, interim (5n), ynp1 (5n)
   double *dflux = nullptr, *dsum = nullptr, *rhs = nullptr, *func0 = nullptr, *func = nullptr;
   double *interim = nullptr, *ynp1 = nullptr, *sol = nullptr;

@@ -383,70 +383,50 @@ __global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__
 
 // ------------------------------------------------------------------ fused
 
-constexpr int ST_TX = 64, ST_TY = 16, ST_H = 2;   // stage tile and FD halo
-constexpr int FL_TX = 64, FL_TY = 32, FL_H = 4;   // filter tile and halo
+constexpr int ST_TX = 64, ST_H = 2;   // stage tile width and FD halo
+constexpr int FL_TX = 64, FL_H = 4;   // filter tile width and halo (height: autotuned)
 constexpr int STAGE_FIRST = 0, STAGE_MID = 1, STAGE_LAST = 2;
 
+// OA4 finite-difference rows of StandardCentral::initialize (spatial:64-105)
+// as compile-time constants: interior {1/12, -8/12, 0, 8/12, -1/12}; left
+// rows 0 {-3/2, 2, -1/2} (anchored at the point) and 1 {-1/2, 0, 1/2}
+// (anchored one point left); right rows mirrored with the sign flipped.
 template <typename T>
-struct CoefT {
-  T st[9];
-  T bnd[4][7];
-  T sign;
-  int half, nb, sb;
-  int lo[4], hi[4];   // first / one-past-last non-zero entry of each closure row
-};
-
-template <typename T>
-CoefT<T> make_coef(const OpTable &op) {
-  CoefT<T> c{};
-  c.half = op.half;
-  c.nb = op.nb;
-  c.sb = op.sb;
-  c.sign = (T)op.sign_right;
-  for (int s = 0; s < 2 * op.half + 1 && s < 9; s++) c.st[s] = (T)op.st[s];
-  for (int k = 0; k < op.nb && k < 4; k++) {
-    c.lo[k] = op.sb;
-    c.hi[k] = 0;
-    for (int s = 0; s < op.sb && s < 7; s++) {
-      c.bnd[k][s] = (T)op.bnd[k][s];
-      if (op.bnd[k][s] != 0.0) {
-        if (s < c.lo[k]) c.lo[k] = s;
-        c.hi[k] = s + 1;
-      }
+__device__ __forceinline__ T fd_row(const T *s, int st, int code) {
+  const T c1 = (T)(8.0 / 12.0), c2 = (T)(-1.0 / 12.0);
+  switch (code) {
+    case CODE_INTERIOR: {
+      T acc = -c2 * s[-2 * st];
+      acc += -c1 * s[-st];
+      acc += c1 * s[st];
+      acc += c2 * s[2 * st];
+      return acc;
+    }
+    case CODE_ZERO:
+      return (T)0;
+    case CODE_LEFT + 0: {
+      T acc = (T)(-3.0 / 2.0) * s[0];
+      acc += (T)(4.0 / 2.0) * s[st];
+      acc += (T)(-1.0 / 2.0) * s[2 * st];
+      return acc;
+    }
+    case CODE_LEFT + 1: {
+      T acc = (T)(-1.0 / 2.0) * s[-st];
+      acc += (T)(1.0 / 2.0) * s[st];
+      return acc;
+    }
+    case CODE_LEFT + 2: {
+      T acc = (T)(3.0 / 2.0) * s[0];
+      acc += (T)(-4.0 / 2.0) * s[-st];
+      acc += (T)(1.0 / 2.0) * s[-2 * st];
+      return acc;
+    }
+    default: {
+      T acc = (T)(1.0 / 2.0) * s[st];
+      acc += (T)(-1.0 / 2.0) * s[-st];
+      return acc;
     }
   }
-  return c;
-}
-
-// One operator row from LDS: `s` points at the point's element, `st` is the
-// LDS stride along the operator direction.  Zero closure entries are
-// skipped, so every read stays within the interior half width.
-template <typename T, int HALF>
-__device__ __forceinline__ T lds_row(const T *s, int st, int code, const CoefT<T> &c) {
-  if (code == CODE_INTERIOR) {
-    T acc = c.st[0] * s[-HALF * st];
-#pragma unroll
-    for (int q = 1; q <= 2 * HALF; q++) {
-      if (q == HALF) {
-        if (c.st[HALF] != (T)0) acc += c.st[HALF] * s[0];
-      } else {
-        acc += c.st[q] * s[(q - HALF) * st];
-      }
-    }
-    return acc;
-  }
-  if (code == CODE_ZERO) return (T)0;
-  T acc = (T)0;
-  if (code < CODE_LEFT + HALF) {
-    const int k = code - CODE_LEFT;
-    for (int q = c.lo[k]; q < c.hi[k]; q++)
-      if (c.bnd[k][q] != (T)0) acc += c.bnd[k][q] * s[(q - k) * st];
-  } else {
-    const int k = code - CODE_LEFT - HALF;
-    for (int q = c.lo[k]; q < c.hi[k]; q++)
-      if (c.bnd[k][q] != (T)0) acc += c.sign * c.bnd[k][q] * s[(k - q) * st];
-  }
-  return acc;
 }
 
 template <typename T>
@@ -474,7 +454,6 @@ struct StageArgs {
   const double *g;
   double s1, s2;
   T h, b, a;
-  CoefT<T> c;
 };
 
 // Maps the launch index to a tile so that each XCD (dispatch is round-robin
@@ -482,47 +461,85 @@ struct StageArgs {
 // rows of vertically adjacent tiles in its own L2.
 __device__ __forceinline__ int xcd_tile(int bid, int chunk) { return (bid & 7) * chunk + (bid >> 3); }
 
-template <typename T, bool MEANF, bool HOLES, int KIND>
+// One RK stage on a 64 x TY tile.  Every global operand of the thread (its
+// share of the LDS tile, the FP64 metrics and Jacobian, the accumulator and
+// y_n of its R points) is requested up front, so a launch that is a single
+// wave of tiles on the chip waits on HBM once instead of once per row.
+// R rows per thread (tile height 4 R) is chosen per block by the autotuner.
+template <typename T, bool MEANF, bool HOLES, int KIND, int R>
 __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
-  __shared__ T s[4][ST_TY + 2 * ST_H][ST_TX + 2 * ST_H];
+  constexpr int TY = 4 * R;
+  constexpr int RS = ST_TX + 2 * ST_H, NR = TY + 2 * ST_H;  // NR = 4 (R + 1)
+  __shared__ T s[4][NR][RS];
   const int tile = xcd_tile(blockIdx.x, a.chunk);
   if (tile >= a.ntiles) return;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * ST_TY;
+  const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
   const int VOFF[4] = {IS * n, IUX * n, IUY * n, IP * n};
 
-  for (int r = ty; r < ST_TY + 2 * ST_H; r += 4) {
-    const int gj = j0 - ST_H + r;
-    for (int cc = tx; cc < ST_TX + 2 * ST_H; cc += 64) {
-      const int gi = i0 - ST_H + cc;
-      T v0 = 0, v1 = 0, v2 = 0, v3 = 0;
-      if (gj >= 0 && gj < nj && gi >= 0 && gi < ni) {
-        const size_t q = (size_t)gj * ni + gi;
-        v0 = a.yin[VOFF[0] + q];
-        v1 = a.yin[VOFF[1] + q];
-        v2 = a.yin[VOFF[2] + q];
-        v3 = a.yin[VOFF[3] + q];
-      }
-      s[0][r][cc] = v0;
-      s[1][r][cc] = v1;
-      s[2][r][cc] = v2;
-      s[3][r][cc] = v3;
+  // a tile whose halo lies inside the block needs no bounds checks and has
+  // interior stencils only
+  const bool inner = !HOLES && i0 >= ST_H && i0 + ST_TX + ST_H <= ni && j0 >= ST_H && j0 + TY + ST_H <= nj;
+  // (1) tile values: rows ty + 4k (k = 0..R), columns tx and, for tx < 4, 64 + tx
+  T ta[4][R + 1], tb[4][R + 1];
+  const bool xtra = tx < 2 * ST_H;
+#pragma unroll
+  for (int k = 0; k <= R; k++) {
+    const int gj = j0 - ST_H + ty + 4 * k;
+    const bool rowok = inner || (gj >= 0 && gj < nj);
+    const int gia = i0 - ST_H + tx, gib = i0 - ST_H + ST_TX + tx;
+    const bool oka = inner || (rowok && gia >= 0 && gia < ni), okb = xtra && (inner || (rowok && gib < ni));
+    const int qa = gj * ni + gia, qb = gj * ni + gib;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      ta[v][k] = oka ? a.yin[VOFF[v] + qa] : (T)0;
+      tb[v][k] = okb ? a.yin[VOFF[v] + qb] : (T)0;
+    }
+  }
+  // (2) per-point operands of rows ty + 4k (k < R)
+  const int gi = i0 + tx;
+  double pm[R][4], pj[R];
+  T pacc[R][4], pyn[R][4];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int gj = j0 + ty + 4 * k;
+    const bool ok = inner || (gi < ni && gj < nj);
+    const int p = ok ? gj * ni + gi : 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) pm[k][q] = ok ? a.met[(size_t)q * n + p] : 0.0;
+    pj[k] = ok ? a.jac[p] : 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      pacc[k][v] = (KIND != STAGE_FIRST && ok) ? a.acc[VOFF[v] + p] : (T)0;
+      pyn[k][v] = (KIND == STAGE_MID && ok) ? a.yn[VOFF[v] + p] : (T)0;
+    }
+  }
+  // (3) tile to LDS
+#pragma unroll
+  for (int k = 0; k <= R; k++) {
+    const int r = ty + 4 * k;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      s[v][r][tx] = ta[v][k];
+      if (xtra) s[v][r][ST_TX + tx] = tb[v][k];
     }
   }
   __syncthreads();
 
-  constexpr int RS = ST_TX + 2 * ST_H;  // LDS row stride
-  for (int rr = ty; rr < ST_TY; rr += 4) {
-    const int gi = i0 + tx, gj = j0 + rr;
-    if (gi >= ni || gj >= nj) continue;
+  // (4) the R points
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int rr = ty + 4 * k;
+    const int gj = j0 + rr;
+    if (!inner && (gi >= ni || gj >= nj)) continue;
     const int p = gj * ni + gi;
     const int lj = rr + ST_H, li = tx + ST_H;
-    int cx, cy;
+    int cx = CODE_INTERIOR, cy = CODE_INTERIOR;
     if (HOLES) {
       cx = a.code[p];
       cy = a.code[n + p];
-    } else {
+    } else if (!inner) {
       cx = default_code(gi, ni, 2);
       cy = default_code(gj, nj, 2);
     }
@@ -531,12 +548,12 @@ __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
     for (int v = 0; v < 4; v++) {
       const T *sp = &s[v][lj][li];
       yc[v] = sp[0];
-      dx[v] = lds_row<T, 2>(sp, 1, cx, a.c);
-      dy[v] = lds_row<T, 2>(sp, RS, cy, a.c);
+      dx[v] = fd_row<T>(sp, 1, cx);
+      dy[v] = fd_row<T>(sp, RS, cy);
     }
     // chain rule with FP64 metrics (spatial:1810-1820)
-    const double mxx = a.met[p], mxy = a.met[n + p], myx = a.met[2 * n + p], myy = a.met[3 * n + p];
-    const double J = a.jac[p];
+    const double mxx = pm[k][0], mxy = pm[k][1], myx = pm[k][2], myy = pm[k][3];
+    const double J = pj[k];
     T fx[4], fy[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -632,16 +649,13 @@ __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const size_t q = (size_t)VOFF[v] + p;
-      T accv, outv;
-      if (hole) {
-        accv = 0;
-        outv = 0;
-      } else {
+      T accv = 0, outv = 0;
+      if (!hole) {
         const T rh = r[v] * a.h;
-        const T acc_in = KIND == STAGE_FIRST ? yc[v] : a.acc[q];
+        const T acc_in = KIND == STAGE_FIRST ? yc[v] : pacc[k][v];
         accv = acc_in + a.b * rh;
         if (KIND != STAGE_LAST) {
-          const T yn = KIND == STAGE_FIRST ? yc[v] : a.yn[q];
+          const T yn = KIND == STAGE_FIRST ? yc[v] : pyn[k][v];
           outv = yn + a.a * rh;
         }
       }
@@ -664,55 +678,157 @@ struct FilterArgs {
   const T *src;      // unfiltered y (5n)
   T *dst;            // filtered y (5n)
   const int8_t *iblank, *code;  // holes only (filter codes, xi then eta)
-  int vars[NV];      // variable index per grid.y
+  int vars[4];       // variables of this launch
   T rem, blend;
-  CoefT<T> c;
 };
 
-// spatial::apply_filter (spatial:2191-2246) for one variable per grid.y:
-// xi sweep over the tile plus its eta halo, then the eta sweep, the blend
-// and temporal::iblank_correction.
-template <typename T, bool HOLES, int HALF>
+// Filter rows with the reference's coefficients as compile-time constants
+// (StandardCentralFilter::initialize, spatial:113-224; the standard filter
+// always runs at strength 1, input.cpp:50-53, so the tables are fixed).
+// ORDER 8: interior 1 - 35/128 at the centre and 7/32, -7/64, 1/32, -1/256
+// off it; rows 0, 1 identity, row 2 Lele's 4th-order and row 3 his
+// 6th-order centred filter.  ORDER 4: Lele's 4th-order filter, rows 0, 1
+// identity.  Right rows are the mirror images (sign +1).  `s` points at the
+// point, `st` is the stride along the sweep.
+template <typename T, int ORDER>
+__device__ __forceinline__ T filter_row(const T *s, int st, int code) {
+  if (ORDER == 8) {
+    if (code == CODE_INTERIOR) {
+      const T c0 = (T)(93.0 / 128.0), c1 = (T)(7.0 / 32.0), c2 = (T)(-7.0 / 64.0), c3 = (T)(1.0 / 32.0),
+              c4 = (T)(-1.0 / 256.0);
+      T acc = c4 * s[-4 * st];
+      acc += c3 * s[-3 * st];
+      acc += c2 * s[-2 * st];
+      acc += c1 * s[-st];
+      acc += c0 * s[0];
+      acc += c1 * s[st];
+      acc += c2 * s[2 * st];
+      acc += c3 * s[3 * st];
+      acc += c4 * s[4 * st];
+      return acc;
+    }
+    if (code == CODE_ZERO) return (T)0;
+    const int k = code - CODE_LEFT;
+    const int row = k < 4 ? k : k - 4;
+    if (row <= 1) return s[0];
+    if (row == 2) {
+      const T a = (T)((-1.0 / 8.0) / 2.0), b = (T)((1.0 / 2.0) / 2.0), c = (T)(5.0 / 8.0);
+      T acc = a * s[-2 * st];
+      acc += b * s[-st];
+      acc += c * s[0];
+      acc += b * s[st];
+      acc += a * s[2 * st];
+      return acc;
+    }
+    const T a = (T)((1.0 / 32.0) / 2.0), b = (T)((-3.0 / 16.0) / 2.0), c = (T)((15.0 / 32.0) / 2.0),
+            d = (T)(11.0 / 16.0);
+    T acc = a * s[-3 * st];
+    acc += b * s[-2 * st];
+    acc += c * s[-st];
+    acc += d * s[0];
+    acc += c * s[st];
+    acc += b * s[2 * st];
+    acc += a * s[3 * st];
+    return acc;
+  } else {
+    if (code == CODE_INTERIOR) {
+      const T a = (T)((-1.0 / 8.0) / 2.0), b = (T)((1.0 / 2.0) / 2.0), c = (T)(5.0 / 8.0);
+      T acc = a * s[-2 * st];
+      acc += b * s[-st];
+      acc += c * s[0];
+      acc += b * s[st];
+      acc += a * s[2 * st];
+      return acc;
+    }
+    if (code == CODE_ZERO) return (T)0;
+    return s[0];
+  }
+}
+
+// spatial::apply_filter (spatial:2191-2246) on a 64 x TY tile for NVAR
+// variables at once: every global operand is requested up front; the xi
+// sweep of all variables over the tile plus its eta halo goes to LDS, then
+// the eta sweep, the blend and temporal::iblank_correction; two barriers
+// per tile.  Tiles whose halo stays inside the block skip all edge logic.
+template <typename T, bool HOLES, int ORDER, int NVAR, int TY>
 __global__ void __launch_bounds__(TPB) k_fused_filter(FilterArgs<T> a) {
-  __shared__ T in[FL_TY + 2 * FL_H][FL_TX + 2 * FL_H];
-  __shared__ T mid[FL_TY + 2 * FL_H][FL_TX];
+  constexpr int NR = TY + 2 * FL_H, NC = FL_TX + 2 * FL_H, KR = NR / 4;
+  constexpr int NB = ORDER / 2;
+  __shared__ T in[NVAR][NR][NC];
+  __shared__ T mid[NVAR][NR][FL_TX];
   const int tile = xcd_tile(blockIdx.x, a.chunk);
   if (tile >= a.ntiles) return;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * FL_TY;
+  const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
-  const size_t voff = (size_t)a.vars[blockIdx.y] * n;
-  const T *src = a.src + voff;
+  const bool xtra = tx < 2 * FL_H;
+  const bool inner = !HOLES && i0 >= FL_H && i0 + FL_TX + FL_H <= ni && j0 >= FL_H && j0 + TY + FL_H <= nj;
 
-  for (int r = ty; r < FL_TY + 2 * FL_H; r += 4) {
+  T ra[NVAR][KR], rb[NVAR][KR];
+  if (inner) {
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int q0 = (j0 - FL_H + ty + 4 * k) * ni + (i0 - FL_H + tx);
+#pragma unroll
+      for (int q = 0; q < NVAR; q++) {
+        const T *src = a.src + a.vars[q] * n;
+        ra[q][k] = src[q0];
+        rb[q][k] = xtra ? src[q0 + FL_TX] : (T)0;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int gj = j0 - FL_H + ty + 4 * k;
+      const bool rowok = gj >= 0 && gj < nj;
+      const int gia = i0 - FL_H + tx, gib = i0 - FL_H + FL_TX + tx;
+      const bool oka = rowok && gia >= 0 && gia < ni, okb = rowok && xtra && gib < ni;
+#pragma unroll
+      for (int q = 0; q < NVAR; q++) {
+        const T *src = a.src + a.vars[q] * n;
+        ra[q][k] = oka ? src[gj * ni + gia] : (T)0;
+        rb[q][k] = okb ? src[gj * ni + gib] : (T)0;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KR; k++)
+#pragma unroll
+    for (int q = 0; q < NVAR; q++) {
+      in[q][ty + 4 * k][tx] = ra[q][k];
+      if (xtra) in[q][ty + 4 * k][FL_TX + tx] = rb[q][k];
+    }
+  __syncthreads();
+  const int gi = i0 + tx;
+#pragma unroll
+  for (int k = 0; k < KR; k++) {
+    const int r = ty + 4 * k;
     const int gj = j0 - FL_H + r;
-    for (int cc = tx; cc < FL_TX + 2 * FL_H; cc += 64) {
-      const int gi = i0 - FL_H + cc;
-      T v = 0;
-      if (gj >= 0 && gj < nj && gi >= 0 && gi < ni) v = src[(size_t)gj * ni + gi];
-      in[r][cc] = v;
+    int code = CODE_INTERIOR;
+    bool ok = true;
+    if (!inner) {
+      ok = gj >= 0 && gj < nj && gi < ni;
+      if (ok) code = HOLES ? (int)a.code[gj * ni + gi] : default_code(gi, ni, NB);
     }
+#pragma unroll
+    for (int q = 0; q < NVAR; q++) mid[q][r][tx] = ok ? filter_row<T, ORDER>(&in[q][r][tx + FL_H], 1, code) : (T)0;
   }
   __syncthreads();
-  for (int r = ty; r < FL_TY + 2 * FL_H; r += 4) {
-    const int gj = j0 - FL_H + r, gi = i0 + tx;
-    T v = 0;
-    if (gj >= 0 && gj < nj && gi < ni) {
-      const int code = HOLES ? (int)a.code[gj * ni + gi] : default_code(gi, ni, HALF);
-      v = lds_row<T, HALF>(&in[r][tx + FL_H], 1, code, a.c);
-    }
-    mid[r][tx] = v;
-  }
-  __syncthreads();
-  for (int r = ty; r < FL_TY; r += 4) {
-    const int gj = j0 + r, gi = i0 + tx;
-    if (gj >= nj || gi >= ni) continue;
+#pragma unroll
+  for (int k = 0; k < TY / 4; k++) {
+    const int r = ty + 4 * k;
+    const int gj = j0 + r;
+    if (!inner && (gj >= nj || gi >= ni)) continue;
     const int p = gj * ni + gi;
-    const int code = HOLES ? (int)a.code[n + p] : default_code(gj, nj, HALF);
-    const T f = lds_row<T, HALF>(&mid[r + FL_H][tx], FL_TX, code, a.c);
-    T out = a.rem * in[r + FL_H][tx + FL_H] + a.blend * f;
-    if (HOLES && a.iblank[p] == BLANKED) out = 0;
-    a.dst[voff + p] = out;
+    const int code = inner ? CODE_INTERIOR : (HOLES ? (int)a.code[n + p] : default_code(gj, nj, NB));
+    const bool hole = HOLES && a.iblank[p] == BLANKED;
+#pragma unroll
+    for (int q = 0; q < NVAR; q++) {
+      const T f = filter_row<T, ORDER>(&mid[q][r + FL_H][tx], FL_TX, code);
+      T out = a.rem * in[q][r + FL_H][tx + FL_H] + a.blend * f;
+      if (hole) out = 0;
+      a.dst[a.vars[q] * n + p] = out;
+    }
   }
 }
 
@@ -977,7 +1093,8 @@ static hipError_t fused_stage_t(hipStream_t s, lee_handle *h, Block &b, int stag
   a.ni = b.ni;
   a.nj = b.nj;
   a.ntx = (b.ni + ST_TX - 1) / ST_TX;
-  a.nty = (b.nj + ST_TY - 1) / ST_TY;
+  const int R = b.tune_r;
+  a.nty = (b.nj + 4 * R - 1) / (4 * R);
   a.ntiles = a.ntx * a.nty;
   a.chunk = (a.ntiles + 7) / 8;
   // buffers: stage 0 reads Y0; odd stages write Y2 ... see header comment
@@ -1022,10 +1139,14 @@ static hipError_t fused_stage_t(hipStream_t s, lee_handle *h, Block &b, int stag
   a.h = (T)sc.h;
   a.b = (T)sc.b;
   a.a = (T)sc.a;
-  a.c = make_coef<T>(h->fd);
   const dim3 grid(a.chunk * 8), block(64, 4);
   const bool mf = !b.uniform_mean, ho = b.has_holes;
-#define LEE_LAUNCH(MF, HO, K) hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K>), grid, block, 0, s, a)
+#define LEE_LAUNCH(MF, HO, K)                                                                     \
+  do {                                                                                            \
+    if (R == 1) hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K, 1>), grid, block, 0, s, a);      \
+    else if (R == 2) hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K, 2>), grid, block, 0, s, a); \
+    else hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K, 4>), grid, block, 0, s, a);             \
+  } while (0)
 #define LEE_KINDS(MF, HO)                                      \
   do {                                                         \
     if (kind == STAGE_FIRST) LEE_LAUNCH(MF, HO, STAGE_FIRST);  \
@@ -1051,33 +1172,52 @@ hipError_t fused_stage(hipStream_t s, lee_handle *h, Block &b, int stage, const 
 template <typename T>
 static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *const *Y, int src, int dst) {
   FilterArgs<T> a{};
+  const int TYF = b.tune_fty;
   a.ni = b.ni;
   a.nj = b.nj;
   a.ntx = (b.ni + FL_TX - 1) / FL_TX;
-  a.nty = (b.nj + FL_TY - 1) / FL_TY;
+  a.nty = (b.nj + TYF - 1) / TYF;
   a.ntiles = a.ntx * a.nty;
   a.chunk = (a.ntiles + 7) / 8;
   a.src = Y[src];
   a.dst = Y[dst];
   a.iblank = b.iblank;
   a.code = b.code_flt;
-  int nv = 0;
-  a.vars[nv++] = IS;
-  a.vars[nv++] = IUX;
-  a.vars[nv++] = IUY;
-  a.vars[nv++] = IP;
-  if (b.carry_w) a.vars[nv++] = IUZ;
   a.blend = (T)h->cfg.filter_blend;
   a.rem = (T)(1.0 - h->cfg.filter_blend);
-  a.c = make_coef<T>(h->flt);
-  const dim3 grid(a.chunk * 8, nv), block(64, 4);
-  const bool h8 = h->flt.half == 4;
-  if (b.has_holes) {
-    if (h8) hipLaunchKernelGGL((k_fused_filter<T, true, 4>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_fused_filter<T, true, 2>), grid, block, 0, s, a);
-  } else {
-    if (h8) hipLaunchKernelGGL((k_fused_filter<T, false, 4>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_fused_filter<T, false, 2>), grid, block, 0, s, a);
+  const dim3 grid(a.chunk * 8), block(64, 4);
+  const bool o8 = h->cfg.filter_order == 8;
+  // the four active variables together, then w' alone when it is carried
+  for (int pass = 0; pass < (b.carry_w ? 2 : 1); pass++) {
+    const bool four = pass == 0;
+    if (four) {
+      a.vars[0] = IS;
+      a.vars[1] = IUX;
+      a.vars[2] = IUY;
+      a.vars[3] = IP;
+    } else {
+      a.vars[0] = IUZ;
+    }
+#define LEE_FLT(HO, OR, NVV)                                                                        \
+  do {                                                                                              \
+    if (TYF == 4) hipLaunchKernelGGL((k_fused_filter<T, HO, OR, NVV, 4>), grid, block, 0, s, a);    \
+    else if (TYF == 8) hipLaunchKernelGGL((k_fused_filter<T, HO, OR, NVV, 8>), grid, block, 0, s, a); \
+    else hipLaunchKernelGGL((k_fused_filter<T, HO, OR, NVV, 16>), grid, block, 0, s, a);           \
+  } while (0)
+#define LEE_FLT_NV(HO, OR)            \
+  do {                                \
+    if (four) LEE_FLT(HO, OR, 4);     \
+    else LEE_FLT(HO, OR, 1);          \
+  } while (0)
+    if (b.has_holes) {
+      if (o8) LEE_FLT_NV(true, 8);
+      else LEE_FLT_NV(true, 4);
+    } else {
+      if (o8) LEE_FLT_NV(false, 8);
+      else LEE_FLT_NV(false, 4);
+    }
+#undef LEE_FLT_NV
+#undef LEE_FLT
   }
   return hipGetLastError();
 }

@@ -86,7 +86,7 @@ EXPORTS = [
     "lee_get_time", "lee_compute_cfl", "lee_time_advance", "lee_solve", "lee_synchronize", "lee_compute_rhs",
     "lee_apply_filter", "lee_take_derivative_xyz", "lee_update_boundary", "lee_get_metrics", "lee_get_iblank",
     "lee_get_mean_gradient", "lee_get_sponge", "lee_last_solve_ms", "lee_version",
-    "lee_set_kernel_timing", "lee_get_kernel_timing",
+    "lee_set_kernel_timing", "lee_get_kernel_timing", "lee_autotune", "lee_set_tile_shape", "lee_get_tile_shape",
 ]
 
 _lib = None
@@ -128,6 +128,9 @@ def library():
     L.lee_last_solve_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.lee_set_kernel_timing.argtypes = [vp, i]
     L.lee_get_kernel_timing.argtypes = [vp, dp, ip]
+    L.lee_autotune.argtypes = [vp]
+    L.lee_set_tile_shape.argtypes = [vp, i, i, i]
+    L.lee_get_tile_shape.argtypes = [vp, i, ip, ip]
     _lib = L
     return L
 
@@ -353,6 +356,17 @@ class Solver:
         return ms.value
 
     KERNEL_CLASSES = ("stage_first", "stage_mid", "stage_last", "filter", "boundary")
+
+    def autotune(self):
+        self._check(self.L.lee_autotune(self.h))
+
+    def set_tile_shape(self, stage_rows, filter_rows, ib=0):
+        self._check(self.L.lee_set_tile_shape(self.h, ib, int(stage_rows), int(filter_rows)))
+
+    def tile_shape(self, ib=0):
+        r, f = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.lee_get_tile_shape(self.h, ib, ctypes.byref(r), ctypes.byref(f)))
+        return r.value, f.value
 
     def set_kernel_timing(self, on):
         self._check(self.L.lee_set_kernel_timing(self.h, 1 if on else 0))

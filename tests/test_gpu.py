@@ -195,3 +195,25 @@ def test_handle_errors_are_reported():
     with pytest.raises(Exception):
         s.compute_rhs(np.zeros((5,) + c["x"].shape), 0.0, ib=3)
     s.close()
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_tile_shapes_bit_identical(prec):
+    """Every autotuner tile shape computes each point with the same
+    operations, so results agree bit for bit; the tuner picks one of them."""
+    c = cases.case_shear()
+    c["nsteps"] = 12
+    ref = None
+    for r in (1, 2, 4):
+        for f in ((4, 8, 16) if prec == FP32 else (4,)):
+            s = Solver.from_case(c, precision=prec, path=PATH_FUSED)
+            s.set_tile_shape(r, f)
+            s.solve(c["nsteps"])
+            sol = s.solution()
+            s.close()
+            if ref is None:
+                ref = sol
+            assert np.array_equal(sol, ref), (r, f)
+    s = Solver.from_case(c, precision=prec, path=PATH_FUSED)
+    s.solve(1)
+    assert s.tile_shape()[0] in (1, 2, 4) and s.tile_shape()[1] in (4, 8, 16)
