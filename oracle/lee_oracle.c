@@ -53,7 +53,8 @@ static char g_err[512];
  * show that the golden fixtures catch each quirk (0 = faithful):
  *  1 fabs instead of int abs in the CFL      2 sources at t_n instead of t_{n+1}
  *  3 aux density refreshed after the filter  4 metrics transposed in d/dx_i
- *  5 mean-gradient terms dropped             6 filter sweeps eta before xi */
+ *  5 mean-gradient terms dropped             6 filter sweeps eta before xi
+ *  7 right hole closures after any non-hole cell (`!=` for `/=`) */
 static int g_plant = 0;
 void orc_set_plant(int code) { g_plant = code; }
 static void set_err(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); }
@@ -269,7 +270,7 @@ static int iblank_correction(const blk_t *b, const op_t *op, int which, const do
         if (my == BLANKED) {
           if (last == BLANKED) {
             df[l0] = 0.0;
-          } else if (last != 0) { /* the reference's `last_iblank /= BLANKED` */
+          } else if (g_plant == 7 ? last != BLANKED : last != 0) { /* the reference's `last_iblank /= BLANKED` */
             if (i <= 0) {
             } else {
               if (check_fluid(b, op, ijk, dir, 0)) return -1;

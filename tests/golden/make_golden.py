@@ -67,7 +67,52 @@ def run_case(case, nranks=1):
         return res
 
 
+def run_overset_case(case, nranks=2):
+    """Two-block overset problem: one MPI rank per block (decomp.cpp:29-30)."""
+    import overset_cases
+    with tempfile.TemporaryDirectory(prefix=f"golden_{case['name']}_") as d:
+        blocks = case["blocks"]
+        refio.write_grid(os.path.join(d, "grid.xyz"),
+                         [(b["x"][None], b["y"][None], np.zeros_like(b["x"])[None]) for b in blocks])
+        refio.write_function(os.path.join(d, "mean.q"), [b["mean"][:, None] for b in blocks])
+        refio.write_solution(os.path.join(d, "ic.q"), [b["ic"][:, None] for b in blocks], 0, 0.0)
+        refio.write_bc(os.path.join(d, "bc.in"), case["bc"])
+        dims = [(b["x"].shape[1], b["x"].shape[0], 1) for b in blocks]
+        refio.write_overture(os.path.join(d, "overset.bin"), 2, dims, case["recv"], case["widths"], case["holes"])
+        items = cases.deck_items(case)
+        items = [(k, v) for k, v in items if k != "OVERSET"]
+        items.insert(12, ("OVERSET", f"OVERTURE, TYPE = EXPLICIT, ORDER_ACCURACY = {case['order']}"))
+        items.append(("OVERSET_FILE", "overset.bin"))
+        refio.write_deck(os.path.join(d, "camille.in"), items)
+        env = dict(os.environ)
+        env["LD_LIBRARY_PATH"] = "/usr/lib/x86_64-linux-gnu:/opt/conda/lib"
+        p = subprocess.run([MPIEXEC, "-n", str(nranks), REF_BIN, "ic.q"], cwd=d, env=env, capture_output=True,
+                           text=True, timeout=3600)
+        files = sorted(glob.glob(os.path.join(d, "camille.[0-9]*.q")))
+        if len(files) < 2:
+            raise RuntimeError(f"reference wrote no final solution:\n{p.stdout[-3000:]}\n{p.stderr[-2000:]}")
+        sols, taus = refio.read_solution(files[-1])
+        log = "\n".join(ln for ln in p.stdout.splitlines() if ln.startswith("Time step:"))
+        return sols, taus, log
+
+
 def main(names):
+    import overset_cases
+    if names and names[0] == "overset_small":
+        case = overset_cases.case_overset_small()
+        sols, taus, log = run_overset_case(case)
+        arrs = {f"sol{b}": sols[b][:, 0] for b in range(len(sols))}
+        arrs["tau"] = taus[0]
+        path = os.path.join(HERE, "overset_small.npz")
+        np.savez_compressed(path, **arrs)
+        mpath = os.path.join(HERE, "manifest.json")
+        old = json.load(open(mpath)) if os.path.exists(mpath) else {}
+        old["overset_small"] = dict(nsteps=case["nsteps"], blocks=[list(b["x"].shape) for b in case["blocks"]],
+                                    nrecv=len(case["recv"]), nholes=len(case["holes"]), time=float(taus[0][3]),
+                                    report=log, ranks=2)
+        json.dump(old, open(mpath, "w"), indent=1, sort_keys=True)
+        print("overset_small time", taus[0][3], "->", os.path.getsize(path), "bytes")
+        return
     names = names or list(cases.ALL)
     manifest = {}
     for name in names:

@@ -99,3 +99,34 @@ def sponge_lines(iblock, width, sides=("+xi", "-xi", "+eta", "-eta")):
         "-eta": ("0", "-0", f"-{w}", "-0", "0", "0"),
     }
     return [(iblock, "sponge_freund_ambient", s) + spec[s] for s in sides]
+
+
+def write_overture(path, ndim, dims, recv, widths, holes):
+    """Overture explicit-interpolation file as read by
+    src/io/overture.cpp:473-680.
+    dims: [(ni, nj, nk)] per block; recv: list of (rblock, (i, j[, k]),
+    dblock, (i, j[, k]) lower-left donor corner, coef[w3][w2][w1] flat),
+    grouped by receiver block; widths: [(w1, w2, w3)] per receiver block;
+    holes: list of (block, (i, j, k))."""
+    nb = len(dims)
+    recv = sorted(recv, key=lambda r: r[0])
+    with open(path, "wb") as f:
+        def ints(v):
+            np.asarray(v, dtype=np.int32).tofile(f)
+        ints([ndim, nb])
+        ints([d[0] * d[1] * d[2] for d in dims])
+        ints([sum(1 for r in recv if r[0] == b) for b in range(nb)])
+        ints([r[0] for r in recv])
+        ints([c for r in recv for c in r[1][:ndim]])
+        ints([r[2] for r in recv])
+        ints([c for r in recv for c in r[3][:ndim]])
+        for b in range(nb):
+            ints(widths[b])
+            for r in recv:
+                if r[0] == b:
+                    np.asarray(r[4], dtype=np.float64).tofile(f)
+        ints([sum(1 for h in holes if h[0] == b) for b in range(nb)])
+        for b in range(nb):
+            for h in holes:
+                if h[0] == b:
+                    ints(list(h[1]) + [0] * (3 - len(h[1])))

@@ -205,3 +205,35 @@ def test_hole_closures(ring):
     inner = ib[12] == 1
     first = np.argmax(inner)
     assert np.all(d[12][first + 1:first + 10] == 0.0)
+
+
+def _overset_run(plant=0):
+    import overset_cases
+    L = lib()
+    L.orc_set_plant.argtypes = [ctypes.c_int]
+    L.orc_set_plant(plant)
+    try:
+        c = overset_cases.case_overset_small()
+        o = overset_cases.oracle_for(c, Oracle)
+        o.run(c["nsteps"])
+        return o, [o.solution(b)[:, 0] for b in range(2)]
+    finally:
+        L.orc_set_plant(0)
+
+
+def test_overset_golden_bitwise():
+    """Two-block overset case (holes, interpolated fringe, fluid-to-hole rows,
+    ORDER-4 donor stencils), run by the reference on two MPI ranks."""
+    o, sols = _overset_run()
+    g = _golden("overset_small")
+    assert o.time == float(g["tau"][3])
+    for b in range(2):
+        assert np.array_equal(sols[b], g[f"sol{b}"]), b
+    ib = o.iblank(0)
+    assert (ib == 1).sum() > 300 and (ib == -1).sum() > 100
+
+
+def test_overset_planted_hole_closure_mistake_is_caught():
+    _, sols = _overset_run(plant=7)
+    g = _golden("overset_small")
+    assert not all(np.array_equal(sols[b], g[f"sol{b}"]) for b in range(2))
