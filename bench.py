@@ -51,6 +51,108 @@ def c2_inputs(n=N1D):
     return dict(x=x, y=y, mean=mean, bc=bc, source=src, ic=np.zeros((5, n, n)), dx=d)
 
 
+def c5_inputs(nsolve=64, seed=5, dx=0.5, sponge=20):
+    """Batched ensemble: `nsolve` independent Cartesian solves (spacing dx),
+    sizes drawn by numpy.random.default_rng(seed) with nx in [129, 1025] and
+    ny in [129, 513], each with a monopole (amplitude 1e-3, half-width 4 dx)
+    in the central 60 % at omega in [0.1 pi, 0.5 pi], uniform mean flow
+    M in [0, 0.6] along x and Freund sponges `sponge` points wide on every
+    face; RK4 at CFL 0.5 with the 8th-order filter."""
+    rng = np.random.default_rng(seed)
+    w = sponge - 1
+    bc = [(0, "sponge_freund_ambient", "+xi", "0", str(w), "0", "-0", "0", "0"),
+          (0, "sponge_freund_ambient", "-xi", f"-{w}", "-0", "0", "-0", "0", "0"),
+          (0, "sponge_freund_ambient", "+eta", "0", "-0", "0", str(w), "0", "0"),
+          (0, "sponge_freund_ambient", "-eta", "0", "-0", f"-{w}", "-0", "0", "0")]
+    out = []
+    for _ in range(nsolve):
+        nx = int(rng.integers(129, 1026))
+        ny = int(rng.integers(129, 514))
+        fx, fy = rng.uniform(0.2, 0.8, 2)
+        omega = float(rng.uniform(0.1, 0.5)) * math.pi
+        mach = float(rng.uniform(0.0, 0.6))
+        x, y = np.meshgrid(dx * np.arange(nx), dx * np.arange(ny))
+        mean = np.zeros((5, ny, nx))
+        mean[1] = mach
+        mean[4] = 1.0 / 1.4
+        src = (1e-3, omega, fx * dx * (nx - 1), fy * dx * (ny - 1), 0.0, 4.0 * dx)
+        out.append(dict(x=x, y=y, mean=mean, boundaries=bc, simulation="CASE_MONOPOLE", source_params=src,
+                        mode="FIX_CFL", value=0.5))
+    return out
+
+
+def c5_leg(steps, warmup, single_steps):
+    """Times the batched ensemble (FP32 and FP64 bulk), the FP32 ensemble run
+    one handle per solve for comparison, and each solve's FP32 pressure
+    error against its FP64 run."""
+    from camille_amd import FP32, FP64, PATH_FUSED, Batch, Solver
+    solves = c5_inputs()
+    npts = sum(sv["x"].size for sv in solves)
+    res = {"solves": len(solves), "points": npts, "steps": steps, "warmup": warmup}
+    finals = {}
+    for label, prec in (("fp32", FP32), ("fp64", FP64)):
+        print(f"[bench] C5 batch {label}: {len(solves)} solves, {npts} points, {steps} steps", file=sys.stderr,
+              flush=True)
+        b = Batch(solves, precision=prec)
+        for k, sv in enumerate(solves):
+            b.set_solution(np.zeros((5,) + sv["x"].shape), k)
+        t0 = time.perf_counter()
+        b.solve(warmup)  # includes the autotuner and the graph capture
+        b.synchronize()
+        t_warm = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b.solve(steps, sync=False)
+        b.synchronize()
+        wall = time.perf_counter() - t0
+        ups = npts * 4 * steps / wall
+        finals[label] = [b.solution(k)[4] for k in range(len(solves))]
+        kt = None
+        if label == "fp32":
+            g, gs, gf = b.launch()
+            b.set_launch(False, gs, gf)  # per-kernel events need direct launches
+            b.set_kernel_timing(True)
+            b.solve(20)
+            kt = b.kernel_timing()
+            b.set_kernel_timing(False)
+        res[label] = dict(updates_per_s=ups, ms_per_step=wall / steps * 1e3, warmup_s=t_warm,
+                          tile_shape=list(b.tile_shape()), launch=dict(zip(("graph", "grid_stage", "grid_filter"),
+                                                                           b.launch())))
+        if kt:
+            res[label]["kernel_ms"] = {k: [round(t, 4), c] for k, (t, c) in kt.items()}
+            per_point = {"stage_first": 92, "stage_mid": 124, "stage_last": 92, "filter": 32}
+            res[label]["kernel_hbm_frac"] = {
+                c: per_point[c] * npts / (kt[c][0] / kt[c][1] * 1e-3) / HBM_PEAK for c in per_point if kt[c][1]}
+        b.close()
+        print(f"[bench] C5 batch {label}: {ups:.3e} updates/s", file=sys.stderr, flush=True)
+    errs = []
+    for p32, p64 in zip(finals["fp32"], finals["fp64"]):
+        errs.append(float(np.max(np.abs(p32 - p64)) / np.max(np.abs(p64))))
+    res["p_rel_err_fp32_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs))}
+    # the same FP32 ensemble one handle per solve, launched one after another
+    print(f"[bench] C5 one handle per solve: {single_steps} steps", file=sys.stderr, flush=True)
+    hs = []
+    for sv in solves:
+        s = Solver([dict(x=sv["x"], y=sv["y"], mean=sv["mean"])], boundaries=sv["boundaries"], mode="FIX_CFL",
+                   value=0.5, simulation="CASE_MONOPOLE", source_params=sv["source_params"], precision=FP32,
+                   path=PATH_FUSED)
+        s.set_solution(np.zeros((5,) + sv["x"].shape))
+        s.solve(2)
+        hs.append(s)
+    for s in hs:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(single_steps):
+        for s in hs:
+            s.solve(1, sync=False)
+    for s in hs:
+        s.synchronize()
+    wall = time.perf_counter() - t0
+    res["fp32_one_handle_per_solve_updates_per_s"] = npts * 4 * single_steps / wall
+    for s in hs:
+        s.close()
+    return res
+
+
 def make_solver(inp, precision, path):
     from camille_amd import Solver
     s = Solver([dict(x=inp["x"], y=inp["y"], mean=inp["mean"])], simulation="CASE_MONOPOLE",
@@ -93,17 +195,38 @@ def cpu_baseline(inp, nsteps):
     return wall
 
 
+def copy_calibration():
+    """A plain device-to-device copy of known size: 256 MiB read and 256 MiB
+    written per launch.  FETCH_SIZE / WRITE_SIZE of this launch give the
+    factors that turn those counters into bytes."""
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--accuracy-steps", type=int, default=2000)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "plain"], default=None,
+    ap.add_argument("--c5-steps", type=int, default=1000)
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64", "plain", "c5"], default=None,
                     help="time one path only (for profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 256 MiB (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (N = 1); larger N is not supported",
               file=sys.stderr)
@@ -114,6 +237,9 @@ def main():
     npts = N1D * N1D
     upd_per_step = npts * 4
 
+    if args.only == "c5":
+        print(json.dumps({"c5": c5_leg(args.steps, args.warmup, 10)}))
+        return
     res = {}
     paths = [("fp32", FP32, PATH_FUSED, args.steps), ("fp64", FP64, PATH_FUSED, args.steps),
              ("plain", FP64, PATH_PLAIN, max(10, args.steps // 20))]
@@ -169,6 +295,8 @@ def main():
         except Exception:
             traffic = None
 
+    c5 = None if args.no_c5 else c5_leg(args.c5_steps, args.warmup, 50)
+
     cpu = None
     if not args.no_cpu_baseline:
         print(f"[bench] CPU oracle baseline: {args.cpu_steps} steps, 1 thread", file=sys.stderr, flush=True)
@@ -211,6 +339,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes_per_point": per_point, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
+        "batched_ensemble_c5": c5,
     }
     print(json.dumps(out))
 

@@ -74,6 +74,9 @@ void free_block(Block &b) {
   dfree(b.src.g);
 }
 
+int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks, int nblocks, const lee_overset *ov,
+                bool batch, lee_handle **out);
+
 bool is_fp32(const lee_handle *h) { return h->cfg.precision == LEE_FP32; }
 bool is_plain(const lee_handle *h) { return h->cfg.path == LEE_PATH_PLAIN; }
 
@@ -105,7 +108,7 @@ int enqueue_fused_step(lee_handle *h) {
     sc.h = dt;
     sc.b = h->b_i[st];
     sc.a = h->a_next[st];
-    source_factors(h, tn + h->c_i[st] * dt, sc.s1, sc.s2);
+    source_factors(h->cfg, tn + h->c_i[st] * dt, sc.s1, sc.s2);
     if (h->ktiming) HIPCHK(mark(h));
     for (int ib = 0; ib < h->nblk; ib++) HIPCHK(fused_stage(h->stream, h, h->b[ib], st, sc));
     if (h->ktiming) {
@@ -160,7 +163,7 @@ int enqueue_plain_step(lee_handle *h) {
   }
   for (int st = 0; st < S; st++) {
     double s1, s2;
-    source_factors(h, tn + h->c_i[st] * dt, s1, s2);
+    source_factors(h->cfg, tn + h->c_i[st] * dt, s1, s2);
     for (int ib = 0; ib < h->nblk; ib++) {
       Block &b = h->b[ib];
       HIPCHK(plain_rhs(s, h, b, b.Y64[2], b.Y64[3], s1, s2, b.rho64));
@@ -191,6 +194,229 @@ int enqueue_plain_step(lee_handle *h) {
 }
 
 int enqueue_step(lee_handle *h) { return is_plain(h) ? enqueue_plain_step(h) : enqueue_fused_step(h); }
+
+// One step of a batched ensemble: a persistent launch per RK stage over the
+// tiles of every solve, the boundary conditions of the solves that have
+// any, then the filter launch, which also advances the device step counter.
+// The launches only read operand tables and the schedule, so the sequence
+// can be captured once in a HIP graph and replayed.
+int enqueue_batch_step(lee_handle *h) {
+  const int S = h->cfg.rk_stages, last = S - 1;
+  const bool f32 = is_fp32(h);
+  for (int st = 0; st < S; st++) {
+    if (h->ktiming) HIPCHK(mark(h));
+    HIPCHK(batch_stage(h->stream, h, st));
+    if (h->ktiming) {
+      HIPCHK(mark(h));
+      h->evclass.push_back(st == 0 ? 0 : (st == last ? 2 : 1));
+    }
+    const int tgt = fused_target(h, st);
+    for (int ib = 0; ib < h->nblk; ib++) {
+      Block &b = h->b[ib];
+      if (b.faces.empty()) continue;
+      const bool rho = st == last && !b.uniform_mean;
+      HIPCHK(apply_bcs(h->stream, h, b, b.Y64[tgt], b.Y32[tgt], f32, rho ? b.rho64 : nullptr,
+                       rho ? b.rho32 : nullptr));
+    }
+  }
+  if (h->ktiming) HIPCHK(mark(h));
+  HIPCHK(batch_filter(h->stream, h));
+  if (h->ktiming) {
+    HIPCHK(mark(h));
+    h->evclass.push_back(3);
+  }
+  return LEE_OK;
+}
+
+// solver::solve's step prologue for each solve of a batch, `nsteps` steps
+// ahead: the clock never depends on the solution (the CFL maximum is a
+// function of metrics and mean flow), so dt, time and the source factors
+// of every stage are known in advance.
+void batch_schedule(lee_handle *h, int nsteps, BatchSched *out) {
+  const int S = h->cfg.rk_stages, ns = h->nblk;
+  for (int k = 0; k < ns; k++) {
+    Block &b = h->b[k];
+    for (int st = 0; st < nsteps; st++) {
+      b.clk_step++;
+      const double cmax = cfl_max_block(b, b.clk_dt, -1e99);
+      double dt = b.clk_dt;
+      if (b.cfg.fix_cfl) {
+        const double ratio = b.cfg.cfl_or_dt / cmax;
+        dt *= ratio;
+        b.clk_dt = dt;
+      }
+      b.clk_time += dt;
+      for (int i = 0; i < S; i++) {
+        BatchSched &q = out[((size_t)st * ns + k) * S + i];
+        q.dt = dt;
+        source_factors(b.cfg, b.clk_time + h->c_i[i] * dt, q.s1, q.s2);
+      }
+    }
+  }
+}
+
+// Makes room for `nsteps` steps of schedule on the device.
+int batch_reserve(lee_handle *h, size_t nsteps) {
+  BatchPlan &p = h->bp;
+  if (p.sched_steps >= nsteps) return LEE_OK;
+  // the graph holds the old address
+  if (p.exec) HIPCHK(hipGraphExecDestroy(p.exec));
+  p.exec = nullptr;
+  dfree(p.sched);
+  HIPCHK(dalloc(&p.sched, nsteps * h->nblk * h->cfg.rk_stages));
+  p.sched_steps = nsteps;
+  return LEE_OK;
+}
+
+// Captures one batched step in a HIP graph (once per table layout).
+int batch_capture(lee_handle *h) {
+  BatchPlan &p = h->bp;
+  if (p.exec) return LEE_OK;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_batch_step(h);
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIPCHK(e);
+  const hipError_t ei = hipGraphInstantiate(&p.exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(ei);
+  return LEE_OK;
+}
+
+// Tile shape and persistent grid of the batched kernels, timed with HIP
+// events on scratch tables (the state in Y0 is untouched).
+int batch_autotune(lee_handle *h) {
+  BatchPlan &p = h->bp;
+  if (p.tuned) return LEE_OK;
+  int rc = batch_reserve(h, 1);
+  if (rc) return rc;
+  std::vector<BatchSched> zero(h->nblk * h->cfg.rk_stages, BatchSched{0.0, 0.0, 0.0});
+  HIPCHK(hipMemcpy(p.sched, zero.data(), sizeof(BatchSched) * zero.size(), hipMemcpyHostToDevice));
+  const int z2[2] = {0, 0};
+  HIPCHK(hipMemcpy(p.counter, z2, sizeof(z2), hipMemcpyHostToDevice));
+  int dev_cu = 256;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && v > 0) dev_cu = v;
+  }
+  const int grids[4] = {0, 4 * dev_cu, 8 * dev_cu, 16 * dev_cu};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int reps = 5;
+  auto time_it = [&](bool stage, float &ms) -> int {
+    for (int k = 0; k < 2; k++) HIPCHK(stage ? batch_stage(h->stream, h, 1) : batch_filter(h->stream, h));
+    HIPCHK(hipEventRecord(e0, h->stream));
+    for (int k = 0; k < reps; k++) HIPCHK(stage ? batch_stage(h->stream, h, 1) : batch_filter(h->stream, h));
+    HIPCHK(hipEventRecord(e1, h->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    return LEE_OK;
+  };
+  p.tuning = true;
+  float best = 1e30f;
+  int bR = 1, bG = 0, bT = 4, bF = 0;
+  for (int r : {1, 2, 4}) {
+    p.R = r;
+    HIPCHK(batch_prepare(h->stream, h));
+    for (int g : grids) {
+      p.grid_stage = g;
+      float ms = 0.f;
+      if ((rc = time_it(true, ms))) break;
+      if (ms < best) {
+        best = ms;
+        bR = r;
+        bG = g;
+      }
+    }
+    if (rc) break;
+  }
+  best = 1e30f;
+  for (int ty : {4, 8, 16}) {
+    if (rc) break;
+    if (!is_fp32(h) && ty > 4) continue;
+    p.TY = ty;
+    HIPCHK(batch_prepare(h->stream, h));
+    for (int g : grids) {
+      p.grid_filter = g;
+      float ms = 0.f;
+      if ((rc = time_it(false, ms))) break;
+      if (ms < best) {
+        best = ms;
+        bT = ty;
+        bF = g;
+      }
+    }
+  }
+  p.tuning = false;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  batch_release(h);
+  if (rc) return rc;
+  p.R = bR;
+  p.grid_stage = bG;
+  p.TY = bT;
+  p.grid_filter = bF;
+  p.tuned = true;
+  return LEE_OK;
+}
+
+// lee_solve for a batch: per chunk of steps, the host schedule goes to the
+// device, the step counter to 0, then the captured step is replayed.
+int batch_solve(lee_handle *h, int nsteps) {
+  BatchPlan &p = h->bp;
+  int rc = batch_autotune(h);
+  if (rc) return rc;
+  if (!p.ready) {
+    HIPCHK(batch_prepare(h->stream, h));
+  }
+  const int chunk_max = 2048;
+  rc = batch_reserve(h, (size_t)std::min(nsteps, chunk_max));
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int done = 0; done < nsteps;) {
+    const int m = std::min(nsteps - done, chunk_max);
+    // next pinned staging buffer: wait until the device has read it
+    const int u = h->bp_host_cur;
+    h->bp_host_cur ^= 1;
+    if (h->bp_host_pending[u]) HIPCHK(hipEventSynchronize(h->bp_host_ev[u]));
+    h->bp_host_pending[u] = false;
+    const size_t cnt = (size_t)m * h->nblk * h->cfg.rk_stages;
+    if (h->bp_host_cap[u] < cnt) {
+      if (h->bp_host[u]) HIPCHK(hipHostFree(h->bp_host[u]));
+      h->bp_host[u] = nullptr;
+      h->bp_host_cap[u] = 0;
+      HIPCHK(hipHostMalloc((void **)&h->bp_host[u], sizeof(BatchSched) * cnt, hipHostMallocDefault));
+      h->bp_host_cap[u] = cnt;
+    }
+    if (!h->bp_host_ev[u]) HIPCHK(hipEventCreateWithFlags(&h->bp_host_ev[u], hipEventDisableTiming));
+    batch_schedule(h, m, h->bp_host[u]);
+    HIPCHK(hipMemcpyAsync(p.sched, h->bp_host[u], sizeof(BatchSched) * cnt, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(p.counter, h->bp_zero, sizeof(h->bp_zero), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipEventRecord(h->bp_host_ev[u], h->stream));
+    h->bp_host_pending[u] = true;
+    const bool graph = p.use_graph && !h->ktiming;
+    if (graph) {
+      rc = batch_capture(h);
+      if (rc) return rc;
+    }
+    for (int k = 0; k < m; k++) {
+      if (graph) {
+        HIPCHK(hipGraphLaunch(p.exec, h->stream));
+      } else {
+        rc = enqueue_batch_step(h);
+        if (rc) return rc;
+      }
+    }
+    done += m;
+  }
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  return LEE_OK;
+}
 
 // Tile-shape autotuner of the fused kernels.  For each block, every stage
 // tile height (rows per thread R = 1, 2, 4) and filter tile height (8, 16,
@@ -306,7 +532,7 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
     return fail(nullptr, LEE_EINVAL, "unknown source");
   if (c->source_kind == LEE_SOURCE_MONOPOLE && !(c->source_params[5] > 0.0))
     return fail(nullptr, LEE_EINVAL, "monopole half-width must be positive");
-  if (nblocks < 1 || nblocks > MAX_BLOCKS) return fail(nullptr, LEE_EINVAL, "block count must be 1..16");
+  if (nblocks < 1 || nblocks > MAX_BLOCKS) return fail(nullptr, LEE_EINVAL, "block count must be 1..4096");
   const int minlen = c->filter_order > 0 ? c->filter_order + 1 : 5;
   for (int ib = 0; ib < nblocks; ib++) {
     const lee_block &b = blocks[ib];
@@ -351,6 +577,13 @@ void lee_destroy(lee_handle *h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (int ib = 0; ib < h->nblk; ib++) free_block(h->b[ib]);
+  batch_release(h);
+  dfree(h->bp.sched);
+  dfree(h->bp.counter);
+  for (int u = 0; u < 2; u++) {
+    if (h->bp_host[u]) (void)hipHostFree(h->bp_host[u]);
+    if (h->bp_host_ev[u]) (void)hipEventDestroy(h->bp_host_ev[u]);
+  }
   dfree(h->ov.rb);
   dfree(h->ov.rl);
   dfree(h->ov.db);
@@ -371,11 +604,51 @@ int lee_create(const lee_config *cfg, const lee_block *blocks, int nblocks, cons
   *out = nullptr;
   int rc = validate(cfg, blocks, nblocks);
   if (rc) return rc;
+  return create_impl(cfg, 0, blocks, nblocks, ov, false, out);
+}
+
+int lee_batch_create(const lee_config *cfgs, const lee_block *blocks, int nsolve, lee_handle **out) {
+  if (!out) return fail(nullptr, LEE_EINVAL, "null output handle");
+  *out = nullptr;
+  if (!cfgs || !blocks || nsolve < 1) return fail(nullptr, LEE_EINVAL, "empty batch");
+  int rc = validate(&cfgs[0], blocks, nsolve);
+  if (rc) return rc;
+  for (int k = 0; k < nsolve; k++) {
+    const lee_config &a = cfgs[0], &c = cfgs[k];
+    rc = validate(&c, &blocks[k], 1);
+    if (rc) return rc;
+    if (c.num_dim != a.num_dim || c.gamma != a.gamma || c.fd_order != a.fd_order ||
+        c.filter_order != a.filter_order || c.filter_blend != a.filter_blend || c.rk_stages != a.rk_stages ||
+        c.sponge_order != a.sponge_order || c.sponge_constant != a.sponge_constant || c.precision != a.precision ||
+        c.path != a.path)
+      return fail(nullptr, LEE_EINVAL, "batched solves must share the scheme settings (solve " + std::to_string(k) + ")");
+  }
+  if (cfgs[0].path != LEE_PATH_FUSED) return fail(nullptr, LEE_EINVAL, "batched solves run on the fused path");
+  if (cfgs[0].filter_order == 0) return fail(nullptr, LEE_EINVAL, "batched solves need the filter");
+  return create_impl(cfgs, 1, blocks, nsolve, nullptr, true, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+// simulation::initialize from the base state onwards, for a multi-block run
+// (one config) or a batch of independent solves (one config per block,
+// cfg_stride = 1).
+int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks, int nblocks, const lee_overset *ov,
+                bool batch, lee_handle **out) {
+  const lee_config *cfg = cfgs;
   lee_handle *h = new (std::nothrow) lee_handle();
   if (!h) return fail(nullptr, LEE_ENOMEM, "out of host memory");
   h->cfg = *cfg;
   h->nblk = nblocks;
-  rc = [&]() -> int {
+  h->batch = batch;
+  h->b.resize(nblocks);
+  for (int ib = 0; ib < nblocks; ib++) {
+    h->b[ib].cfg = cfgs[(size_t)ib * cfg_stride];
+    h->b[ib].clk_dt = h->b[ib].cfg.fix_cfl ? 1e-6 : h->b[ib].cfg.cfl_or_dt;  // input.cpp:130
+  }
+  int rc = [&]() -> int {
     HIPCHK(hipGetDevice(&h->device));
     // a blocking stream: host-blocking copies order with the queued kernels
     HIPCHK(hipStreamCreate(&h->stream));
@@ -526,7 +799,7 @@ int lee_create(const lee_config *cfg, const lee_block *blocks, int nblocks, cons
         }
       }
       if ((int)b.zones.size() > MAX_ZONES) return fail(h, LEE_EINVAL, "more than 8 sponge zones in a block");
-      plan_source(h, b, in.xyz);
+      plan_source(b.cfg, b, in.xyz);
       if (!b.src.host_g.empty()) {
         HIPCHK(dalloc(&b.src.g, b.src.host_g.size()));
         HIPCHK(hipMemcpy(b.src.g, b.src.host_g.data(), sizeof(double) * b.src.host_g.size(),
@@ -578,6 +851,20 @@ int lee_create(const lee_config *cfg, const lee_block *blocks, int nblocks, cons
     }
     return LEE_OK;
   }();
+  if (rc == LEE_OK && batch) {
+    for (int ib = 1; ib < nblocks; ib++)
+      if (h->b[ib].uniform_mean != h->b[0].uniform_mean) {
+        h->err = "batched solves must all have a uniform mean or all a non-uniform one";
+        rc = LEE_EINVAL;
+      }
+    if (rc == LEE_OK) {
+      hipError_t e = hipMalloc((void **)&h->bp.counter, 2 * sizeof(int));
+      if (e != hipSuccess) {
+        h->err = std::string("HIP error ") + hipGetErrorString(e);
+        rc = LEE_EHIP;
+      }
+    }
+  }
   if (rc) {
     g_create_err = h->err;
     lee_destroy(h);
@@ -586,6 +873,10 @@ int lee_create(const lee_config *cfg, const lee_block *blocks, int nblocks, cons
   *out = h;
   return LEE_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int lee_set_solution(lee_handle *h, int ib, const double *sol) {
   if (!h || !sol || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
@@ -599,6 +890,8 @@ int lee_set_solution(lee_handle *h, int ib, const double *sol) {
       b.carry_w = true;
       break;
     }
+  if (h->batch && b.carry_w)
+    return fail(h, LEE_EINVAL, "batched 2-D solves carry s', u', v', p' only: w' must be zero");
   HIPCHK(hipMemcpy(b.Y64[0], sol, sizeof(double) * NV * n, hipMemcpyHostToDevice));
   // compute_dependent_variables after a restart (simulation.cpp:178-180)
   HIPCHK(aux_density(s, b, h->cfg.gamma, b.Y64[0], b.rho64));
@@ -631,26 +924,63 @@ int lee_set_time(lee_handle *h, int time_step, double time_sol, double dt) {
   h->time_step = time_step;
   h->time_sol = time_sol;
   h->dt = dt;
+  for (auto &b : h->b) {
+    b.clk_step = time_step;
+    b.clk_time = time_sol;
+    b.clk_dt = dt;
+  }
   return LEE_OK;
 }
 
 int lee_get_time(const lee_handle *h, int *time_step, double *time_sol, double *dt) {
   if (!h) return LEE_EINVAL;
-  if (time_step) *time_step = h->time_step;
-  if (time_sol) *time_sol = h->time_sol;
-  if (dt) *dt = h->dt;
+  return lee_get_solve_time(h, 0, time_step, time_sol, dt);
+}
+
+int lee_get_solve_time(const lee_handle *h, int isolve, int *time_step, double *time_sol, double *dt) {
+  if (!h || isolve < 0 || isolve >= h->nblk) return LEE_EINVAL;
+  const bool own = h->batch;
+  if (time_step) *time_step = own ? h->b[isolve].clk_step : h->time_step;
+  if (time_sol) *time_sol = own ? h->b[isolve].clk_time : h->time_sol;
+  if (dt) *dt = own ? h->b[isolve].clk_dt : h->dt;
   return LEE_OK;
 }
 
 int lee_compute_cfl(lee_handle *h, double *cmax) {
   if (!h || !cmax) return fail(h, LEE_EINVAL, "bad argument");
-  *cmax = cfl_max(h);
+  *cmax = h->batch ? cfl_max_block(h->b[0], h->b[0].clk_dt, -1e99) : cfl_max(h);
+  return LEE_OK;
+}
+
+int lee_set_batch_launch(lee_handle *h, int use_graph, int grid_stage, int grid_filter) {
+  if (!h || !h->batch) return fail(h, LEE_EINVAL, "not a batch");
+  if (grid_stage < 0 || grid_filter < 0) return fail(h, LEE_EINVAL, "grid sizes must be >= 0");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->bp.use_graph = use_graph != 0;
+  h->bp.grid_stage = grid_stage;
+  h->bp.grid_filter = grid_filter;
+  h->bp.tuned = true;
+  batch_release(h);
+  return LEE_OK;
+}
+
+int lee_get_batch_launch(lee_handle *h, int *use_graph, int *grid_stage, int *grid_filter) {
+  if (!h || !h->batch || !use_graph || !grid_stage || !grid_filter) return fail(h, LEE_EINVAL, "bad argument");
+  *use_graph = h->bp.use_graph;
+  *grid_stage = h->bp.grid_stage;
+  *grid_filter = h->bp.grid_filter;
   return LEE_OK;
 }
 
 int lee_time_advance(lee_handle *h) {
   if (!h) return LEE_EINVAL;
   if (!h->have_solution) return fail(h, LEE_ESTATE, "no solution set");
+  if (h->batch) {
+    int rc = batch_solve(h, 1);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return LEE_OK;
+  }
   int rc = enqueue_step(h);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -660,6 +990,7 @@ int lee_time_advance(lee_handle *h) {
 int lee_solve(lee_handle *h, int nsteps) {
   if (!h || nsteps < 0) return fail(h, LEE_EINVAL, "bad argument");
   if (!h->have_solution) return fail(h, LEE_ESTATE, "no solution set");
+  if (h->batch) return nsteps > 0 ? batch_solve(h, nsteps) : LEE_OK;
   if (nsteps > 0) {
     int rc = autotune(h);
     if (rc) return rc;
@@ -685,6 +1016,15 @@ int lee_autotune(lee_handle *h) {
   if (!h) return LEE_EINVAL;
   for (int ib = 0; ib < h->nblk; ib++) h->b[ib].tuned = false;
   g_tune_cache.clear();
+  if (h->batch) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->bp.tuned = false;
+    batch_release(h);
+    int rc = batch_autotune(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return LEE_OK;
+  }
   int rc = autotune(h);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -697,6 +1037,14 @@ int lee_set_tile_shape(lee_handle *h, int ib, int stage_rows, int filter_rows) {
   if (filter_rows != 4 && filter_rows != 8 && filter_rows != 16)
     return fail(h, LEE_EINVAL, "filter tile height must be 4, 8 or 16");
   if (!is_fp32(h) && filter_rows > 4) return fail(h, LEE_EINVAL, "FP64 filter tiles are 4 rows high");
+  if (h->batch) {  // one launch covers every solve: the shape is the batch's
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->bp.R = stage_rows;
+    h->bp.TY = filter_rows;
+    h->bp.tuned = true;
+    batch_release(h);
+    return LEE_OK;
+  }
   h->b[ib].tune_r = stage_rows;
   h->b[ib].tune_fty = filter_rows;
   h->b[ib].tuned = true;
@@ -705,8 +1053,8 @@ int lee_set_tile_shape(lee_handle *h, int ib, int stage_rows, int filter_rows) {
 
 int lee_get_tile_shape(lee_handle *h, int ib, int *stage_rows, int *filter_rows) {
   if (!h || ib < 0 || ib >= h->nblk || !stage_rows || !filter_rows) return fail(h, LEE_EINVAL, "bad argument");
-  *stage_rows = h->b[ib].tune_r;
-  *filter_rows = h->b[ib].tune_fty;
+  *stage_rows = h->batch ? h->bp.R : h->b[ib].tune_r;
+  *filter_rows = h->batch ? h->bp.TY : h->b[ib].tune_fty;
   return LEE_OK;
 }
 
@@ -749,7 +1097,7 @@ int lee_compute_rhs(lee_handle *h, int ib, const double *y, double time, double 
   const size_t cnt = (size_t)NV * b.n;
   hipStream_t s = h->stream;
   double s1, s2;
-  source_factors(h, time, s1, s2);
+  source_factors(b.cfg, time, s1, s2);
   HIPCHK(hipMemcpy(b.Y64[2], y, sizeof(double) * cnt, hipMemcpyHostToDevice));
   if (is_plain(h)) {
     HIPCHK(plain_rhs(s, h, b, b.Y64[2], b.Y64[3], s1, s2, b.dsum));

@@ -15,7 +15,7 @@ constexpr int NV = 5;          // s', u', v', w', p' (src/core/param.h:64-71)
 constexpr int IS = 0, IUX = 1, IUY = 2, IUZ = 3, IP = 4;
 constexpr int NOTBLANKED = 0, BLANKED = 1, INTERPOLATED = -1;  // param.h:126-128
 constexpr int MAX_ZONES = 8;   // sponge zones per block handled by the fused kernel
-constexpr int MAX_BLOCKS = 16;
+constexpr int MAX_BLOCKS = 4096;
 
 // Operator-code convention per point and direction (int8):
 //   0            interior row
@@ -92,6 +92,35 @@ struct Block {
   std::vector<std::pair<double, double>> cfl_cand;
   // host copies used by init checks
   std::vector<double> h_met, h_jac;
+  // batched ensembles: every block is an independent solve with its own
+  // settings (source, CFL or dt) and clock
+  lee_config cfg{};
+  int clk_step = 0;
+  double clk_time = 0.0, clk_dt = 1e-6;
+};
+
+// Per-step scalars of one solve in a batched ensemble.
+struct BatchSched {
+  double dt, s1, s2;
+};
+
+// Device tables of a batched ensemble: one persistent launch per RK stage
+// and one per filter over the tiles of all solves, replayed from a HIP graph.
+struct BatchPlan {
+  bool ready = false;          // tables match the tile shape and precision
+  bool tuned = false;
+  bool tuning = false;         // tables address scratch buffers only (autotuner)
+  int R = 1, TY = 4;           // stage rows per thread, filter tile height
+  int grid_stage = 0, grid_filter = 0;  // persistent grid sizes (0: one workgroup per tile)
+  void *stage_args = nullptr;  // device StageArgs<T>[stages][nsolve]
+  void *filter_args = nullptr; // device FilterArgs<T>[nsolve]
+  int *tile_start_stage = nullptr, *tile_start_filter = nullptr;
+  int total_stage = 0, total_filter = 0;
+  BatchSched *sched = nullptr;  // device [step][solve][stage]
+  size_t sched_steps = 0;
+  int *counter = nullptr;      // device: [0] step index within the schedule, [1] scratch
+  bool use_graph = true;
+  hipGraphExec_t exec = nullptr;
 };
 
 struct Overset {
@@ -111,7 +140,17 @@ struct Overset {
 struct lee_handle {
   lee_config cfg{};
   int nblk = 0;
-  lee::Block b[lee::MAX_BLOCKS];
+  std::vector<lee::Block> b;
+  bool batch = false;          // independent solves (lee_batch_create)
+  lee::BatchPlan bp;
+  // pinned host staging of the batch schedule, two buffers used in turn;
+  // an event marks when the device has copied each one
+  lee::BatchSched *bp_host[2] = {nullptr, nullptr};
+  size_t bp_host_cap[2] = {0, 0};
+  hipEvent_t bp_host_ev[2] = {nullptr, nullptr};
+  bool bp_host_pending[2] = {false, false};
+  int bp_host_cur = 0;
+  int bp_zero[2] = {0, 0};
   lee::Overset ov;
   lee::OpTable fd{}, flt{};
   int nb_fd = 2, nb_flt = 4;
@@ -145,9 +184,10 @@ void compute_sponge_zone(const Block &b, const double *xyz_host, SpongeZone &z, 
                          double constant);
 void plan_cfl(Block &b, const double *mean_host, double gamma);
 double cfl_max(const lee_handle *h);
-void plan_source(lee_handle *h, Block &b, const double *xyz_host);
+double cfl_max_block(const Block &b, double dt, double m);
+void plan_source(const lee_config &c, Block &b, const double *xyz_host);
 // time factors (s1, s2) of the source at time t: term = (s1 * g) * s2
-void source_factors(const lee_handle *h, double t, double &s1, double &s2);
+void source_factors(const lee_config &c, double t, double &s1, double &s2);
 
 // ---- kernels (lee_kernels.hip) ----
 struct StageScalars {
@@ -174,6 +214,10 @@ hipError_t metrics2d(hipStream_t s, Block &b, const OpTable &fd);
 
 hipError_t fused_stage(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc);
 hipError_t fused_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst);
+hipError_t batch_prepare(hipStream_t s, lee_handle *h);
+hipError_t batch_stage(hipStream_t s, lee_handle *h, int stage);
+hipError_t batch_filter(hipStream_t s, lee_handle *h);
+void batch_release(lee_handle *h);
 hipError_t cast_state(hipStream_t s, const Block &b, int nvars, const double *src, float *dst, size_t n);
 hipError_t cast_back(hipStream_t s, const Block &b, int nvars, const float *src, double *dst, size_t n);
 

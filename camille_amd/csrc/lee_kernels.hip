@@ -466,13 +466,19 @@ __device__ __forceinline__ int xcd_tile(int bid, int chunk) { return (bid & 7) *
 // y_n of its R points) is requested up front, so a launch that is a single
 // wave of tiles on the chip waits on HBM once instead of once per row.
 // R rows per thread (tile height 4 R) is chosen per block by the autotuner.
+template <typename T>
+struct StepT {       // per-stage scalars: dt, b_i, a_{i+1,i}, source time factors
+  T h, b, a;
+  double s1, s2;
+};
+
+template <typename T, int R>
+using StageLds = T[4][4 * R + 2 * ST_H][ST_TX + 2 * ST_H];
+
 template <typename T, bool MEANF, bool HOLES, int KIND, int R>
-__global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
+__device__ __forceinline__ void stage_tile(const StageArgs<T> &a, const StepT<T> &sc, int tile, StageLds<T, R> &s) {
   constexpr int TY = 4 * R;
-  constexpr int RS = ST_TX + 2 * ST_H, NR = TY + 2 * ST_H;  // NR = 4 (R + 1)
-  __shared__ T s[4][NR][RS];
-  const int tile = xcd_tile(blockIdx.x, a.chunk);
-  if (tile >= a.ntiles) return;
+  constexpr int RS = ST_TX + 2 * ST_H;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
@@ -629,7 +635,7 @@ __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
     // physical source (source.cpp:22-107), pressure equation
     if (gi >= a.si0 && gi <= a.si1 && gj >= a.sj0 && gj <= a.sj1) {
       const double gval = a.g[(gj - a.sj0) * (a.si1 - a.si0 + 1) + (gi - a.si0)];
-      rp += (T)(a.s1 * gval * a.s2);
+      rp += (T)(sc.s1 * gval * sc.s2);
     }
     // Freund sponges, zone by zone (source.cpp:136-201), sol_ref = 0
 #pragma unroll 1
@@ -651,12 +657,12 @@ __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
       const size_t q = (size_t)VOFF[v] + p;
       T accv = 0, outv = 0;
       if (!hole) {
-        const T rh = r[v] * a.h;
+        const T rh = r[v] * sc.h;
         const T acc_in = KIND == STAGE_FIRST ? yc[v] : pacc[k][v];
-        accv = acc_in + a.b * rh;
+        accv = acc_in + sc.b * rh;
         if (KIND != STAGE_LAST) {
           const T yn = KIND == STAGE_FIRST ? yc[v] : pyn[k][v];
-          outv = yn + a.a * rh;
+          outv = yn + sc.a * rh;
         }
       }
       a.acc[q] = accv;
@@ -669,6 +675,58 @@ __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
       q *= a.rhob[p];
       a.rho_out[p] = (T)q;
     }
+  }
+}
+
+template <typename T, bool MEANF, bool HOLES, int KIND, int R>
+__global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
+  __shared__ StageLds<T, R> s;
+  const int tile = xcd_tile(blockIdx.x, a.chunk);
+  if (tile >= a.ntiles) return;
+  const StepT<T> sc{a.h, a.b, a.a, a.s1, a.s2};
+  stage_tile<T, MEANF, HOLES, KIND, R>(a, sc, tile, s);
+}
+
+// Batched ensemble: the tiles of all solves form one list (solve k owns
+// tiles [tile_start[k], tile_start[k+1])).  A persistent grid walks the
+// list with a grid stride; virtual index v maps to list position
+// (v & 7) * chunk + (v >> 3), so with a grid that is a multiple of 8 each
+// XCD works through one contiguous eighth of the list.  Per-solve operands
+// come from `args`, the per-step scalars from `sched`, indexed by the step
+// counter that the step's last kernel advances.
+template <typename T>
+struct BatchStageArgs {
+  const StageArgs<T> *args;      // nsolve entries for this stage
+  const int *tile_start;         // nsolve + 1
+  const BatchSched *sched;       // [step][solve][stage]
+  const int *counter;            // step index within the schedule
+  int nsolve, nstage, stage, total, chunk;
+  T b, a;
+};
+
+__device__ __forceinline__ int find_solve(const int *start, int nsolve, int t) {
+  int lo = 0, hi = nsolve;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename T, bool MEANF, int KIND, int R>
+__global__ void __launch_bounds__(TPB) k_batch_stage(BatchStageArgs<T> ba) {
+  __shared__ StageLds<T, R> s;
+  // device-scope load: the counter was written by the previous step's filter launch
+  const int step = __hip_atomic_load(ba.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int v = blockIdx.x; v < 8 * ba.chunk; v += gridDim.x) {
+    const int t = (v & 7) * ba.chunk + (v >> 3);
+    if (t >= ba.total) continue;
+    const int k = find_solve(ba.tile_start, ba.nsolve, t);
+    const BatchSched &q = ba.sched[((size_t)step * ba.nsolve + k) * ba.nstage + ba.stage];
+    const StepT<T> sc{(T)q.dt, ba.b, ba.a, q.s1, q.s2};
+    stage_tile<T, MEANF, false, KIND, R>(ba.args[k], sc, t - ba.tile_start[k], s);
+    __syncthreads();  // the LDS tile is rewritten by the next iteration
   }
 }
 
@@ -750,14 +808,18 @@ __device__ __forceinline__ T filter_row(const T *s, int st, int code) {
 // sweep of all variables over the tile plus its eta halo goes to LDS, then
 // the eta sweep, the blend and temporal::iblank_correction; two barriers
 // per tile.  Tiles whose halo stays inside the block skip all edge logic.
+template <typename T, int NVAR, int TY>
+struct FilterLds {
+  T in[NVAR][TY + 2 * FL_H][FL_TX + 2 * FL_H];
+  T mid[NVAR][TY + 2 * FL_H][FL_TX];
+};
+
 template <typename T, bool HOLES, int ORDER, int NVAR, int TY>
-__global__ void __launch_bounds__(TPB) k_fused_filter(FilterArgs<T> a) {
-  constexpr int NR = TY + 2 * FL_H, NC = FL_TX + 2 * FL_H, KR = NR / 4;
+__device__ __forceinline__ void filter_tile(const FilterArgs<T> &a, int tile, FilterLds<T, NVAR, TY> &L) {
+  constexpr int NR = TY + 2 * FL_H, KR = NR / 4;
   constexpr int NB = ORDER / 2;
-  __shared__ T in[NVAR][NR][NC];
-  __shared__ T mid[NVAR][NR][FL_TX];
-  const int tile = xcd_tile(blockIdx.x, a.chunk);
-  if (tile >= a.ntiles) return;
+  auto &in = L.in;
+  auto &mid = L.mid;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
@@ -832,6 +894,42 @@ __global__ void __launch_bounds__(TPB) k_fused_filter(FilterArgs<T> a) {
   }
 }
 
+template <typename T, bool HOLES, int ORDER, int NVAR, int TY>
+__global__ void __launch_bounds__(TPB) k_fused_filter(FilterArgs<T> a) {
+  __shared__ FilterLds<T, NVAR, TY> L;
+  const int tile = xcd_tile(blockIdx.x, a.chunk);
+  if (tile >= a.ntiles) return;
+  filter_tile<T, HOLES, ORDER, NVAR, TY>(a, tile, L);
+}
+
+template <typename T>
+struct BatchFilterArgs {
+  const FilterArgs<T> *args;     // nsolve entries
+  const int *tile_start;         // nsolve + 1
+  int *counter;                  // advanced by one at the end of the step
+  int nsolve, total, chunk;
+};
+
+// The batched filter is the last kernel of a step: one thread advances the
+// step counter (no kernel of this launch reads it; the next step's stage
+// kernels see the new value).
+template <typename T, int ORDER, int TY>
+__global__ void __launch_bounds__(TPB) k_batch_filter(BatchFilterArgs<T> ba) {
+  __shared__ FilterLds<T, 4, TY> L;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    const int c = __hip_atomic_load(ba.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(ba.counter, c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // vector store
+  }
+  for (int v = blockIdx.x; v < 8 * ba.chunk; v += gridDim.x) {
+    const int t = (v & 7) * ba.chunk + (v >> 3);
+    if (t >= ba.total) continue;
+    const int k = find_solve(ba.tile_start, ba.nsolve, t);
+    filter_tile<T, false, ORDER, 4, TY>(ba.args[k], t - ba.tile_start[k], L);
+    __syncthreads();
+  }
+}
+
+
 // Passive w' in 2-D: only the sponge acts on it (the velocity loop of
 // spatial:2077-2103 stops at num_dim), then the RK update.
 template <typename T>
@@ -894,7 +992,7 @@ hipError_t metrics2d(hipStream_t s, Block &b, const OpTable &fd) {
   hipLaunchKernelGGL(k_metrics2d, dim3(nblocks(b.n)), dim3(TPB), 0, s, xx, b.met, b.jac, b.n);
   e = hipGetLastError();
   hipError_t e2 = hipStreamSynchronize(s);
-  hipFree(xx);
+  (void)hipFree(xx);
   return e != hipSuccess ? e : e2;
 }
 
@@ -927,27 +1025,32 @@ hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double 
     hipError_t e = aux_density(s, b, gam, y, rho);
     if (e != hipSuccess) return e;
   }
+  hipError_t err = hipSuccess;  // first failure of the derivative launches
+  auto deriv = [&](const double *f, int d) {
+    const hipError_t e = plain_take_derivative_xyz(s, b, h->fd, f, d);
+    if (err == hipSuccess) err = e;
+  };
   hipLaunchKernelGGL(k_fill, dim3(nblocks((size_t)NV * n)), t, 0, s, rhs, 0.0, (size_t)NV * n);
   for (int d = 0; d < 2; d++) {
-    plain_take_derivative_xyz(s, b, h->fd, Y(IS), d);
+    deriv(Y(IS), d);
     hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IS), M(IUX + d), b.dflux, n);
     hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IS), Y(IUX + d), G(IS, d), n);
   }
   for (int vel = 0; vel < 2; vel++) {
     for (int d = 0; d < 2; d++) {
-      plain_take_derivative_xyz(s, b, h->fd, Y(IUX + vel), d);
+      deriv(Y(IUX + vel), d);
       hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IUX + vel), M(IUX + d), b.dflux, n);
       hipLaunchKernelGGL(k_sub_velgrad, g, t, 0, s, R(IUX + vel), Y(IUX + d), rho, b.rhobar, M(IUX + d),
                          G(IUX + vel, d), n);
     }
-    plain_take_derivative_xyz(s, b, h->fd, Y(IP), vel);
+    deriv(Y(IP), vel);
     hipLaunchKernelGGL(k_sub_div, g, t, 0, s, R(IUX + vel), b.dflux, b.rhobar, n);
   }
   for (int d = 0; d < 2; d++) {
-    plain_take_derivative_xyz(s, b, h->fd, Y(IP), d);
+    deriv(Y(IP), d);
     hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IP), M(IUX + d), b.dflux, n);
     hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IP), Y(IUX + d), G(IP, d), n);
-    plain_take_derivative_xyz(s, b, h->fd, Y(IUX + d), d);
+    deriv(Y(IUX + d), d);
     hipLaunchKernelGGL(k_sub_gab, g, t, 0, s, R(IP), gam, M(IP), b.dflux, n);
     hipLaunchKernelGGL(k_sub_gab, g, t, 0, s, R(IP), gam, Y(IP), G(IUX + d, d), n);
   }
@@ -961,7 +1064,8 @@ hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double 
     hipLaunchKernelGGL(k_plain_sponge, dim3(nblocks(cnt)), t, 0, s, rhs, y, z.sigma64, b.ni, n, z.is[0], z.nz[0],
                        z.is[1], z.nz[1]);
   }
-  return hipGetLastError();
+  const hipError_t last = hipGetLastError();
+  return err != hipSuccess ? err : last;
 }
 
 hipError_t plain_rk_update(hipStream_t s, const Block &b, double *rhs, double *ynext, double *interim,
@@ -1082,22 +1186,27 @@ static void fill_zones(StageArgs<T> &a, const Block &b) {
   }
 }
 
-template <typename T>
-static hipError_t fused_stage_t(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc, T *const *Y,
-                                T *rho) {
+// Stage kind of RK stage `stage` (stage < 0: right-hand side only).
+static int stage_kind(const lee_handle *h, int stage) {
   const int last = h->cfg.rk_stages - 1;
-  // stage < 0: right-hand side only (lee_compute_rhs): y in Y[2], zeroed
-  // accumulator Y[3] with h = b = 1 receives the RHS, Y[1] is scratch
-  const int kind = stage <= 0 ? (stage == 0 ? STAGE_FIRST : STAGE_MID) : (stage == last ? STAGE_LAST : STAGE_MID);
+  return stage <= 0 ? (stage == 0 ? STAGE_FIRST : STAGE_MID) : (stage == last ? STAGE_LAST : STAGE_MID);
+}
+
+// Operands of one RK stage on block b with R rows per thread.
+template <typename T>
+static StageArgs<T> make_stage_args(const lee_handle *h, const Block &b, int stage, T *const *Y, T *rho, int R,
+                                    const StageScalars &sc) {
   StageArgs<T> a{};
   a.ni = b.ni;
   a.nj = b.nj;
   a.ntx = (b.ni + ST_TX - 1) / ST_TX;
-  const int R = b.tune_r;
   a.nty = (b.nj + 4 * R - 1) / (4 * R);
   a.ntiles = a.ntx * a.nty;
   a.chunk = (a.ntiles + 7) / 8;
-  // buffers: stage 0 reads Y0; odd stages write Y2 ... see header comment
+  // stage 0 reads Y0 and writes Y1 (accumulator) and Y2; odd stages read
+  // Y2 and write Y3, even stages read Y3 and write Y2; y_n stays in Y0.
+  // stage < 0 (lee_compute_rhs): y in Y[2], zeroed accumulator Y[3] with
+  // h = b = 1 receives the RHS, Y[1] is scratch
   const int in_idx = stage == 0 ? 0 : (stage % 2 == 1 ? 2 : 3);
   const int out_idx = stage % 2 == 0 ? 2 : 3;
   a.yin = Y[in_idx];
@@ -1139,6 +1248,15 @@ static hipError_t fused_stage_t(hipStream_t s, lee_handle *h, Block &b, int stag
   a.h = (T)sc.h;
   a.b = (T)sc.b;
   a.a = (T)sc.a;
+  return a;
+}
+
+template <typename T>
+static hipError_t fused_stage_t(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc, T *const *Y,
+                                T *rho) {
+  const int kind = stage_kind(h, stage);
+  const int R = b.tune_r;
+  const StageArgs<T> a = make_stage_args<T>(h, b, stage, Y, rho, R, sc);
   const dim3 grid(a.chunk * 8), block(64, 4);
   const bool mf = !b.uniform_mean, ho = b.has_holes;
 #define LEE_LAUNCH(MF, HO, K)                                                                     \
@@ -1169,10 +1287,10 @@ hipError_t fused_stage(hipStream_t s, lee_handle *h, Block &b, int stage, const 
   return fused_stage_t<double>(s, h, b, stage, sc, b.Y64, b.rho64);
 }
 
+// Operands of the filter on block b with tile height TYF (four variables).
 template <typename T>
-static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *const *Y, int src, int dst) {
+static FilterArgs<T> make_filter_args(const lee_handle *h, const Block &b, T *const *Y, int src, int dst, int TYF) {
   FilterArgs<T> a{};
-  const int TYF = b.tune_fty;
   a.ni = b.ni;
   a.nj = b.nj;
   a.ntx = (b.ni + FL_TX - 1) / FL_TX;
@@ -1185,19 +1303,23 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
   a.code = b.code_flt;
   a.blend = (T)h->cfg.filter_blend;
   a.rem = (T)(1.0 - h->cfg.filter_blend);
+  a.vars[0] = IS;
+  a.vars[1] = IUX;
+  a.vars[2] = IUY;
+  a.vars[3] = IP;
+  return a;
+}
+
+template <typename T>
+static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *const *Y, int src, int dst) {
+  const int TYF = b.tune_fty;
+  FilterArgs<T> a = make_filter_args<T>(h, b, Y, src, dst, TYF);
   const dim3 grid(a.chunk * 8), block(64, 4);
   const bool o8 = h->cfg.filter_order == 8;
   // the four active variables together, then w' alone when it is carried
   for (int pass = 0; pass < (b.carry_w ? 2 : 1); pass++) {
     const bool four = pass == 0;
-    if (four) {
-      a.vars[0] = IS;
-      a.vars[1] = IUX;
-      a.vars[2] = IUY;
-      a.vars[3] = IP;
-    } else {
-      a.vars[0] = IUZ;
-    }
+    if (!four) a.vars[0] = IUZ;
 #define LEE_FLT(HO, OR, NVV)                                                                        \
   do {                                                                                              \
     if (TYF == 4) hipLaunchKernelGGL((k_fused_filter<T, HO, OR, NVV, 4>), grid, block, 0, s, a);    \
@@ -1225,6 +1347,151 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
 hipError_t fused_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst) {
   if (h->cfg.precision == LEE_FP32) return fused_filter_t<float>(s, h, b, b.Y32, src, dst);
   return fused_filter_t<double>(s, h, b, b.Y64, src, dst);
+}
+
+// ------------------------------------------------------------ batched ensemble
+
+void batch_release(lee_handle *h) {
+  BatchPlan &p = h->bp;
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  p.exec = nullptr;
+  for (void *q : {p.stage_args, p.filter_args, (void *)p.tile_start_stage, (void *)p.tile_start_filter})
+    if (q) (void)hipFree(q);
+  p.stage_args = p.filter_args = nullptr;
+  p.tile_start_stage = p.tile_start_filter = nullptr;
+  p.ready = false;
+}
+
+// Uploads the per-solve operand tables of every stage and of the filter for
+// the current tile shape, and the tile prefix sums.  The schedule and the
+// step counter are allocated by the caller.
+// With p.tuning the tables describe runs on scratch buffers only (every
+// stage: the right-hand-side form, Y2 -> Y3 and Y1; filter: Y1 -> Y2), so
+// the autotuner can time launches without touching the state in Y0.
+template <typename T>
+static hipError_t batch_prepare_t(hipStream_t s, lee_handle *h) {
+  BatchPlan &p = h->bp;
+  const int ns = h->nblk, S = h->cfg.rk_stages;
+  std::vector<StageArgs<T>> sa((size_t)S * ns);
+  std::vector<FilterArgs<T>> fa(ns);
+  std::vector<int> ts(ns + 1, 0), tf(ns + 1, 0);
+  const StageScalars zero{0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < ns; k++) {
+    Block &b = h->b[k];
+    T *const *Y = sizeof(T) == 8 ? (T *const *)b.Y64 : (T *const *)b.Y32;
+    T *rho = sizeof(T) == 8 ? (T *)b.rho64 : (T *)b.rho32;
+    for (int st = 0; st < S; st++)
+      sa[(size_t)st * ns + k] = make_stage_args<T>(h, b, p.tuning ? -1 : st, Y, rho, p.R, zero);
+    fa[k] = make_filter_args<T>(h, b, Y, 1, p.tuning ? 2 : 0, p.TY);
+    ts[k + 1] = ts[k] + sa[k].ntiles;
+    tf[k + 1] = tf[k] + fa[k].ntiles;
+  }
+  p.total_stage = ts[ns];
+  p.total_filter = tf[ns];
+  hipError_t e;
+  if ((e = hipMalloc(&p.stage_args, sizeof(StageArgs<T>) * sa.size())) != hipSuccess) return e;
+  if ((e = hipMalloc(&p.filter_args, sizeof(FilterArgs<T>) * fa.size())) != hipSuccess) return e;
+  if ((e = hipMalloc((void **)&p.tile_start_stage, sizeof(int) * (ns + 1))) != hipSuccess) return e;
+  if ((e = hipMalloc((void **)&p.tile_start_filter, sizeof(int) * (ns + 1))) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(p.stage_args, sa.data(), sizeof(StageArgs<T>) * sa.size(), hipMemcpyHostToDevice, s)))
+    return e;
+  if ((e = hipMemcpyAsync(p.filter_args, fa.data(), sizeof(FilterArgs<T>) * fa.size(), hipMemcpyHostToDevice, s)))
+    return e;
+  if ((e = hipMemcpyAsync(p.tile_start_stage, ts.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s)))
+    return e;
+  if ((e = hipMemcpyAsync(p.tile_start_filter, tf.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s)))
+    return e;
+  // the host vectors die here: wait for the copies
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  p.ready = true;
+  return hipSuccess;
+}
+
+hipError_t batch_prepare(hipStream_t s, lee_handle *h) {
+  batch_release(h);
+  if (h->cfg.precision == LEE_FP32) return batch_prepare_t<float>(s, h);
+  return batch_prepare_t<double>(s, h);
+}
+
+static unsigned batch_grid(int total, int persistent) {
+  const int chunk = (total + 7) / 8;
+  const int full = 8 * chunk;
+  if (persistent <= 0 || persistent >= full) return (unsigned)full;
+  return (unsigned)((persistent + 7) / 8 * 8);
+}
+
+template <typename T>
+static hipError_t batch_stage_t(hipStream_t s, lee_handle *h, int stage) {
+  BatchPlan &p = h->bp;
+  const int ns = h->nblk;
+  BatchStageArgs<T> ba{};
+  ba.args = (const StageArgs<T> *)p.stage_args + (size_t)stage * ns;
+  ba.tile_start = p.tile_start_stage;
+  ba.sched = p.sched;
+  ba.counter = p.counter;
+  ba.nsolve = ns;
+  ba.nstage = h->cfg.rk_stages;
+  ba.stage = stage;
+  ba.total = p.total_stage;
+  ba.chunk = (p.total_stage + 7) / 8;
+  ba.b = (T)h->b_i[stage];
+  ba.a = (T)h->a_next[stage];
+  const int kind = p.tuning ? STAGE_MID : stage_kind(h, stage);
+  const dim3 grid(batch_grid(p.total_stage, p.grid_stage)), block(64, 4);
+  const bool mf = !h->b[0].uniform_mean;
+  const int R = p.R;
+#define LEE_B(MF, K)                                                                          \
+  do {                                                                                        \
+    if (R == 1) hipLaunchKernelGGL((k_batch_stage<T, MF, K, 1>), grid, block, 0, s, ba);      \
+    else if (R == 2) hipLaunchKernelGGL((k_batch_stage<T, MF, K, 2>), grid, block, 0, s, ba); \
+    else hipLaunchKernelGGL((k_batch_stage<T, MF, K, 4>), grid, block, 0, s, ba);             \
+  } while (0)
+#define LEE_BK(MF)                                  \
+  do {                                              \
+    if (kind == STAGE_FIRST) LEE_B(MF, STAGE_FIRST); \
+    else if (kind == STAGE_MID) LEE_B(MF, STAGE_MID); \
+    else LEE_B(MF, STAGE_LAST);                     \
+  } while (0)
+  if (mf) LEE_BK(true);
+  else LEE_BK(false);
+#undef LEE_BK
+#undef LEE_B
+  return hipGetLastError();
+}
+
+hipError_t batch_stage(hipStream_t s, lee_handle *h, int stage) {
+  if (h->cfg.precision == LEE_FP32) return batch_stage_t<float>(s, h, stage);
+  return batch_stage_t<double>(s, h, stage);
+}
+
+template <typename T>
+static hipError_t batch_filter_t(hipStream_t s, lee_handle *h) {
+  BatchPlan &p = h->bp;
+  BatchFilterArgs<T> ba{};
+  ba.args = (const FilterArgs<T> *)p.filter_args;
+  ba.tile_start = p.tile_start_filter;
+  ba.counter = p.tuning ? p.counter + 1 : p.counter;  // tuning runs leave the step counter alone
+  ba.nsolve = h->nblk;
+  ba.total = p.total_filter;
+  ba.chunk = (p.total_filter + 7) / 8;
+  const dim3 grid(batch_grid(p.total_filter, p.grid_filter)), block(64, 4);
+  const bool o8 = h->cfg.filter_order == 8;
+  const int TY = p.TY;
+#define LEE_BF(OR)                                                                          \
+  do {                                                                                      \
+    if (TY == 4) hipLaunchKernelGGL((k_batch_filter<T, OR, 4>), grid, block, 0, s, ba);     \
+    else if (TY == 8) hipLaunchKernelGGL((k_batch_filter<T, OR, 8>), grid, block, 0, s, ba); \
+    else hipLaunchKernelGGL((k_batch_filter<T, OR, 16>), grid, block, 0, s, ba);            \
+  } while (0)
+  if (o8) LEE_BF(8);
+  else LEE_BF(4);
+#undef LEE_BF
+  return hipGetLastError();
+}
+
+hipError_t batch_filter(hipStream_t s, lee_handle *h) {
+  if (h->cfg.precision == LEE_FP32) return batch_filter_t<float>(s, h);
+  return batch_filter_t<double>(s, h);
 }
 
 hipError_t cast_state(hipStream_t s, const Block &, int, const double *src, float *dst, size_t n) {

@@ -263,21 +263,24 @@ void plan_cfl(Block &b, const double *mean, double gamma) {
   b.cfl_cand = c;
 }
 
+double cfl_max_block(const Block &b, double dt, double m) {
+  for (const auto &c : b.cfl_cand) {
+    double cfl = c.first;
+    cfl *= dt * c.second;
+    m = std::max(m, cfl);
+  }
+  return m;
+}
+
 double cfl_max(const lee_handle *h) {
   double m = -1e99;  // -DUMMY_LARGE
-  for (int ib = 0; ib < h->nblk; ib++)
-    for (const auto &c : h->b[ib].cfl_cand) {
-      double cfl = c.first;
-      cfl *= h->dt * c.second;
-      m = std::max(m, cfl);
-    }
+  for (int ib = 0; ib < h->nblk; ib++) m = cfl_max_block(h->b[ib], h->dt, m);
   return m;
 }
 
 // Physical sources (source.cpp:22-107) as spatial factor g(x) times time
 // factors: term = (s1 * g) * s2, in the reference's multiplication order.
-static double source_shape(const lee_handle *h, double x, double y, double z) {
-  const lee_config &c = h->cfg;
+static double source_shape(const lee_config &c, double x, double y, double z) {
   switch (c.source_kind) {
     case LEE_SOURCE_TWOCYLINDER: {
       const double ef = -std::log(2.0) / (0.2 * 0.2);
@@ -302,20 +305,20 @@ static double source_shape(const lee_handle *h, double x, double y, double z) {
   }
 }
 
-void plan_source(lee_handle *h, Block &b, const double *xyz) {
+void plan_source(const lee_config &c, Block &b, const double *xyz) {
   b.src.host_g.clear();
   b.src.i0 = 0;
   b.src.i1 = -1;
   b.src.j0 = 0;
   b.src.j1 = -1;
-  if (h->cfg.source_kind == LEE_SOURCE_NONE) return;
+  if (c.source_kind == LEE_SOURCE_NONE) return;
   const int n = b.n;
   std::vector<double> g(n);
   int i0 = b.ni, i1 = -1, j0 = b.nj, j1 = -1;
   for (int j = 0; j < b.nj; j++)
     for (int i = 0; i < b.ni; i++) {
       const int p = j * b.ni + i;
-      g[p] = source_shape(h, xyz[p], xyz[(size_t)n + p], xyz[(size_t)2 * n + p]);
+      g[p] = source_shape(c, xyz[p], xyz[(size_t)n + p], xyz[(size_t)2 * n + p]);
       if (g[p] != 0.0) {
         i0 = std::min(i0, i);
         i1 = std::max(i1, i);
@@ -332,8 +335,7 @@ void plan_source(lee_handle *h, Block &b, const double *xyz) {
     for (int i = i0; i <= i1; i++) b.src.host_g.push_back(g[j * b.ni + i]);
 }
 
-void source_factors(const lee_handle *h, double t, double &s1, double &s2) {
-  const lee_config &c = h->cfg;
+void source_factors(const lee_config &c, double t, double &s1, double &s2) {
   const double pi = std::acos(-1.0), twopi = 2.0 * pi;
   s1 = 1.0;
   s2 = 0.0;

@@ -87,6 +87,7 @@ EXPORTS = [
     "lee_apply_filter", "lee_take_derivative_xyz", "lee_update_boundary", "lee_get_metrics", "lee_get_iblank",
     "lee_get_mean_gradient", "lee_get_sponge", "lee_last_solve_ms", "lee_version",
     "lee_set_kernel_timing", "lee_get_kernel_timing", "lee_autotune", "lee_set_tile_shape", "lee_get_tile_shape",
+    "lee_batch_create", "lee_get_solve_time", "lee_set_batch_launch", "lee_get_batch_launch",
 ]
 
 _lib = None
@@ -131,6 +132,10 @@ def library():
     L.lee_autotune.argtypes = [vp]
     L.lee_set_tile_shape.argtypes = [vp, i, i, i]
     L.lee_get_tile_shape.argtypes = [vp, i, ip, ip]
+    L.lee_batch_create.argtypes = [ctypes.POINTER(_Config), ctypes.POINTER(_Block), i, ctypes.POINTER(vp)]
+    L.lee_get_solve_time.argtypes = [vp, i, ip, dp, dp]
+    L.lee_set_batch_launch.argtypes = [vp, i, i, i]
+    L.lee_get_batch_launch.argtypes = [vp, ip, ip, ip]
     _lib = L
     return L
 
@@ -186,6 +191,29 @@ def read_bc_file(path, dims_per_block):
     return out
 
 
+def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4, mode="FIX_CFL", value=0.5,
+                 sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49", source_params=(),
+                 precision=FP64, path=PATH_FUSED):
+    """lee_config from the deck settings (src/core/input.cpp:99-284 keys)."""
+    cfg = _Config()
+    cfg.num_dim = 2
+    cfg.gamma = gamma
+    cfg.fd_order = fd_order
+    cfg.filter_order = filter_order
+    cfg.filter_blend = filter_blend
+    cfg.rk_stages = rk
+    cfg.fix_cfl = 1 if mode == "FIX_CFL" else 0
+    cfg.cfl_or_dt = value
+    cfg.sponge_order = sponge_order
+    cfg.sponge_constant = sponge_constant
+    cfg.source_kind = SOURCES[simulation]
+    for k, v in enumerate(source_params):
+        cfg.source_params[k] = v
+    cfg.precision = precision
+    cfg.path = path
+    return cfg
+
+
 class Solver:
     """One lee_handle: a set of blocks advanced on the current HIP device.
 
@@ -199,31 +227,29 @@ class Solver:
                  source_params=(), precision=FP64, path=PATH_FUSED, boundaries=(), overset=None):
         L = library()
         self.L = L
-        cfg = _Config()
-        cfg.num_dim = 2
-        cfg.gamma = gamma
-        cfg.fd_order = fd_order
-        cfg.filter_order = filter_order
-        cfg.filter_blend = filter_blend
-        cfg.rk_stages = rk
-        cfg.fix_cfl = 1 if mode == "FIX_CFL" else 0
-        cfg.cfl_or_dt = value
-        cfg.sponge_order = sponge_order
-        cfg.sponge_constant = sponge_constant
-        cfg.source_kind = SOURCES[simulation]
-        for k, v in enumerate(source_params):
-            cfg.source_params[k] = v
-        cfg.precision = precision
-        cfg.path = path
-        self.cfg = cfg
-        self.shapes = []
+        self.cfg = _make_config(gamma=gamma, fd_order=fd_order, filter_order=filter_order,
+                                filter_blend=filter_blend, rk=rk, mode=mode, value=value,
+                                sponge_order=sponge_order, sponge_constant=sponge_constant,
+                                simulation=simulation, source_params=source_params, precision=precision, path=path)
         self._keep = []
+        barr = self._blocks(blocks, boundaries)
+        ovp = None
+        if overset is not None:
+            ovp = self._overset_struct(overset)
+        h = ctypes.c_void_p()
+        rc = L.lee_create(ctypes.byref(self.cfg), barr, len(blocks), ovp, ctypes.byref(h))
+        if rc != 0:
+            raise LeeError(f"lee_create failed ({rc}): {L.lee_last_error(None).decode()}")
+        self.h = h
+
+    def _blocks(self, blocks, boundaries):
+        """lee_block array of `blocks`; boundaries are BC-file lines of any block."""
         nb = len(blocks)
         barr = (_Block * nb)()
+        self.shapes = []
         dims = []
-        for ib, b in enumerate(blocks):
-            x = np.asarray(b["x"], np.float64)
-            nj, ni = x.shape
+        for b in blocks:
+            nj, ni = np.asarray(b["x"]).shape
             dims.append((ni, nj, 1))
             self.shapes.append((nj, ni))
         parsed = [bc if isinstance(bc[1], int) else parse_bc_line(bc, dims[int(bc[0])]) for bc in boundaries]
@@ -246,14 +272,8 @@ class Solver:
             barr[ib].mean = _dp(mean)
             barr[ib].nboundary = len(mine)
             barr[ib].boundary = bar
-        ovp = None
-        if overset is not None:
-            ovp = self._overset_struct(overset)
-        h = ctypes.c_void_p()
-        rc = L.lee_create(ctypes.byref(cfg), barr, nb, ovp, ctypes.byref(h))
-        if rc != 0:
-            raise LeeError(f"lee_create failed ({rc}): {L.lee_last_error(None).decode()}")
-        self.h = h
+        self._keep.append(barr)
+        return barr
 
     def _overset_struct(self, ov):
         o = _Overset()
@@ -423,3 +443,56 @@ class Solver:
         out = np.empty(self.shapes[ib])
         self._check(self.L.lee_get_sponge(self.h, ib, _dp(out)))
         return out
+
+
+class Batch(Solver):
+    """A batched ensemble (lee_batch_create): independent single-block solves
+    advanced together, one persistent launch per RK stage over all of them,
+    replayed from a HIP graph.
+
+    solves: list of dict(x, y, mean, [boundaries], [simulation],
+    [source_params], [mode], [value]) -- grid, base state, BC-file lines,
+    source and CFL/dt of each solve.  The scheme settings are shared.
+    The per-block methods take the solve index as `ib`.
+    """
+
+    def __init__(self, solves, *, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4,
+                 sponge_order=2, sponge_constant=5.0, precision=FP64):
+        L = library()
+        self.L = L
+        self._keep = []
+        ns = len(solves)
+        cfgs = (_Config * ns)()
+        for k, sv in enumerate(solves):
+            cfgs[k] = _make_config(gamma=gamma, fd_order=fd_order, filter_order=filter_order,
+                                   filter_blend=filter_blend, rk=rk, mode=sv.get("mode", "FIX_CFL"),
+                                   value=sv.get("value", 0.5), sponge_order=sponge_order,
+                                   sponge_constant=sponge_constant,
+                                   simulation=sv.get("simulation", "CASE_TANNA_TPN49"),
+                                   source_params=sv.get("source_params", ()), precision=precision, path=PATH_FUSED)
+        self.cfg = cfgs[0]
+        self._keep.append(cfgs)
+        bcs = []
+        for k, sv in enumerate(solves):
+            for ln in sv.get("boundaries", ()):
+                bcs.append((k,) + tuple(ln[1:]))
+        barr = self._blocks(solves, bcs)
+        h = ctypes.c_void_p()
+        rc = L.lee_batch_create(cfgs, barr, ns, ctypes.byref(h))
+        if rc != 0:
+            raise LeeError(f"lee_batch_create failed ({rc}): {L.lee_last_error(None).decode()}")
+        self.h = h
+
+    def clock(self, ib=0):
+        st, t, dt = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.L.lee_get_solve_time(self.h, int(ib), ctypes.byref(st), ctypes.byref(t),
+                                              ctypes.byref(dt)))
+        return st.value, t.value, dt.value
+
+    def set_launch(self, use_graph=True, grid_stage=0, grid_filter=0):
+        self._check(self.L.lee_set_batch_launch(self.h, 1 if use_graph else 0, int(grid_stage), int(grid_filter)))
+
+    def launch(self):
+        g, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.lee_get_batch_launch(self.h, ctypes.byref(g), ctypes.byref(a), ctypes.byref(b)))
+        return bool(g.value), a.value, b.value

@@ -187,6 +187,26 @@ int lee_autotune(lee_handle *h);
 int lee_set_tile_shape(lee_handle *h, int iblock, int stage_rows, int filter_rows);
 int lee_get_tile_shape(lee_handle *h, int iblock, int *stage_rows, int *filter_rows);
 
+/* Batched ensemble (no reference counterpart: the reference runs one
+ * problem per process).  nsolve independent single-block problems, block k
+ * with its own lee_config cfgs[k], advanced together: per RK stage one
+ * persistent launch covers the tiles of every solve, and one step is
+ * captured once in a HIP graph and replayed.  The scheme settings (num_dim,
+ * gamma, orders, blend, stages, sponge law, precision, path) must agree;
+ * source, CFL or dt, grid, sponges and mean flow may differ (the means must
+ * be all uniform or all non-uniform).  Fused path only, filter required,
+ * no overset.  Each solve keeps its own clock (FIX_CFL scales its own dt);
+ * the per-block entry points take the solve index as `iblock`, lee_solve
+ * advances every solve by nsteps, lee_get_time reports solve 0. */
+int lee_batch_create(const lee_config *cfgs, const lee_block *blocks, int nsolve, lee_handle **out);
+int lee_get_solve_time(const lee_handle *h, int isolve, int *time_step, double *time_sol, double *dt);
+/* Launch settings of a batch: graph replay on/off, persistent grid sizes of
+ * the stage and filter launches (0 = one workgroup per tile).  Chosen by the
+ * autotuner at the first lee_solve unless set here; every setting gives
+ * bit-identical results. */
+int lee_set_batch_launch(lee_handle *h, int use_graph, int grid_stage, int grid_filter);
+int lee_get_batch_launch(lee_handle *h, int *use_graph, int *grid_stage, int *grid_filter);
+
 /* Library identification: returns a static string (build info). */
 const char *lee_version(void);
 
