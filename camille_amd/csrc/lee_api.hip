@@ -679,6 +679,7 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
       b.h_iblank.assign(b.n, (int8_t)NOTBLANKED);
     }
     if (ov && ov->nrecv > 0) {
+      if (nblocks > MAX_OV_BLOCKS) return fail(h, LEE_EINVAL, "an overset run has at most 16 blocks");
       for (int q = 0; q < ov->nhole; q++) {
         const int bb = ov->hole_block[q], l = ov->hole_l[q];
         if (bb < 0 || bb >= nblocks || l < 0 || l >= h->b[bb].n) return fail(h, LEE_EINVAL, "bad hole index");

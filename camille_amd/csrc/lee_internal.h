@@ -15,7 +15,8 @@ constexpr int NV = 5;          // s', u', v', w', p' (src/core/param.h:64-71)
 constexpr int IS = 0, IUX = 1, IUY = 2, IUZ = 3, IP = 4;
 constexpr int NOTBLANKED = 0, BLANKED = 1, INTERPOLATED = -1;  // param.h:126-128
 constexpr int MAX_ZONES = 8;   // sponge zones per block handled by the fused kernel
-constexpr int MAX_BLOCKS = 4096;
+constexpr int MAX_BLOCKS = 4096;     // blocks of a handle (solves of a batch)
+constexpr int MAX_OV_BLOCKS = 16;    // blocks of an overset run (per-block tables travel as kernel arguments)
 
 // Operator-code convention per point and direction (int8):
 //   0            interior row

@@ -327,11 +327,11 @@ __global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double
 
 template <typename T>
 struct OvTables {
-  T *y[MAX_BLOCKS];
-  int ni[MAX_BLOCKS], nn[MAX_BLOCKS];
-  T *rho[MAX_BLOCKS];
-  const double *mean[MAX_BLOCKS];
-  const double *rhob[MAX_BLOCKS];
+  T *y[MAX_OV_BLOCKS];
+  int ni[MAX_OV_BLOCKS], nn[MAX_OV_BLOCKS];
+  T *rho[MAX_OV_BLOCKS];
+  const double *mean[MAX_OV_BLOCKS];
+  const double *rhob[MAX_OV_BLOCKS];
 };
 
 // compute_weighted_sum (overset.cpp:886-974): every receiver's weighted
@@ -1150,7 +1150,7 @@ static hipError_t overset_t(hipStream_t s, lee_handle *h, T *const *ys, T *const
 hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_index, bool last_stage) {
   if (h->ov.nrecv == 0) return hipSuccess;
   if (fp32) {
-    float *ys[MAX_BLOCKS], *rh[MAX_BLOCKS];
+    float *ys[MAX_OV_BLOCKS], *rh[MAX_OV_BLOCKS];
     bool any_rho = false;
     for (int ib = 0; ib < h->nblk; ib++) {
       ys[ib] = h->b[ib].Y32[buf_index];
@@ -1159,7 +1159,7 @@ hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_
     }
     return overset_t<float>(s, h, ys, rh, last_stage && any_rho);
   }
-  double *ys[MAX_BLOCKS], *rh[MAX_BLOCKS];
+  double *ys[MAX_OV_BLOCKS], *rh[MAX_OV_BLOCKS];
   bool any_rho = false;
   for (int ib = 0; ib < h->nblk; ib++) {
     if (buf_index >= 0) ys[ib] = h->b[ib].Y64[buf_index];
