@@ -1,0 +1,201 @@
+// Device-side definitions shared by the forward and adjoint kernels of the
+// LEE time advance: tile geometry, operator rows as compile-time
+// constants, and the per-launch operand structs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lee_internal.h"
+
+namespace lee {
+
+namespace {
+
+constexpr int TPB = 256;
+
+__device__ __forceinline__ int default_code(int c, int len, int nb) {
+  if (c < nb) return CODE_LEFT + c;
+  if (c > len - 1 - nb) return CODE_LEFT + nb + (len - 1 - c);
+  return CODE_INTERIOR;
+}
+
+
+constexpr int ST_TX = 64, ST_H = 2;   // stage tile width and FD halo
+constexpr int FL_TX = 64, FL_H = 4;   // filter tile width and halo (height: autotuned)
+
+// OA4 finite-difference rows of StandardCentral::initialize (spatial:64-105)
+// as compile-time constants: interior {1/12, -8/12, 0, 8/12, -1/12}; left
+// rows 0 {-3/2, 2, -1/2} (anchored at the point) and 1 {-1/2, 0, 1/2}
+// (anchored one point left); right rows mirrored with the sign flipped.
+template <typename T>
+__device__ __forceinline__ T fd_row(const T *s, int st, int code) {
+  const T c1 = (T)(8.0 / 12.0), c2 = (T)(-1.0 / 12.0);
+  switch (code) {
+    case CODE_INTERIOR: {
+      T acc = -c2 * s[-2 * st];
+      acc += -c1 * s[-st];
+      acc += c1 * s[st];
+      acc += c2 * s[2 * st];
+      return acc;
+    }
+    case CODE_ZERO:
+      return (T)0;
+    case CODE_LEFT + 0: {
+      T acc = (T)(-3.0 / 2.0) * s[0];
+      acc += (T)(4.0 / 2.0) * s[st];
+      acc += (T)(-1.0 / 2.0) * s[2 * st];
+      return acc;
+    }
+    case CODE_LEFT + 1: {
+      T acc = (T)(-1.0 / 2.0) * s[-st];
+      acc += (T)(1.0 / 2.0) * s[st];
+      return acc;
+    }
+    case CODE_LEFT + 2: {
+      T acc = (T)(3.0 / 2.0) * s[0];
+      acc += (T)(-4.0 / 2.0) * s[-st];
+      acc += (T)(1.0 / 2.0) * s[-2 * st];
+      return acc;
+    }
+    default: {
+      T acc = (T)(1.0 / 2.0) * s[st];
+      acc += (T)(-1.0 / 2.0) * s[-st];
+      return acc;
+    }
+  }
+}
+
+template <typename T>
+struct ZoneT {
+  int i0, i1, j0, j1, nzi;
+  const T *sig;
+};
+
+template <typename T>
+struct StageArgs {
+  int ni, nj, ntx, nty, ntiles, chunk;
+  const T *yin;        // stage input (5n, var-major)
+  const T *yn;         // y_n
+  T *acc;              // y_next accumulator (read-modify-write)
+  T *yout;             // next stage input
+  const double *met, *jac;
+  const int8_t *iblank, *code;  // holes only
+  const double *mean, *mgrad, *rhob;   // non-uniform mean only
+  const T *rho_in;     // stale rho' (first stage, non-uniform mean)
+  T *rho_out;          // rho' of the new y_{n+1} (last stage, non-uniform mean)
+  double u0, v0, gp0, rhob0, gamma, gi;
+  int nz;
+  ZoneT<T> z[MAX_ZONES];
+  int si0, si1, sj0, sj1;
+  const double *g;
+  double s1, s2;
+  T h, b, a;
+};
+
+// Maps the launch index to a tile so that each XCD (dispatch is round-robin
+// over the 8 XCDs) walks a contiguous band of tile rows and keeps the halo
+// rows of vertically adjacent tiles in its own L2.
+__device__ __forceinline__ int xcd_tile(int bid, int chunk) { return (bid & 7) * chunk + (bid >> 3); }
+
+// One RK stage on a 64 x TY tile.  Every global operand of the thread (its
+// share of the LDS tile, the FP64 metrics and Jacobian, the accumulator and
+// y_n of its R points) is requested up front, so a launch that is a single
+// wave of tiles on the chip waits on HBM once instead of once per row.
+// R rows per thread (tile height 4 R) is chosen per block by the autotuner.
+template <typename T>
+struct StepT {       // per-stage scalars: dt, b_i, a_{i+1,i}, source time factors
+  T h, b, a;
+  double s1, s2;
+};
+
+template <typename T, int R>
+using StageLds = T[4][4 * R + 2 * ST_H][ST_TX + 2 * ST_H];
+
+
+template <typename T>
+struct FilterArgs {
+  int ni, nj, ntx, nty, ntiles, chunk;
+  const T *src;      // unfiltered y (5n)
+  T *dst;            // filtered y (5n)
+  const int8_t *iblank, *code;  // holes only (filter codes, xi then eta)
+  int vars[4];       // variables of this launch
+  T rem, blend;
+};
+
+// Filter rows with the reference's coefficients as compile-time constants
+// (StandardCentralFilter::initialize, spatial:113-224; the standard filter
+// always runs at strength 1, input.cpp:50-53, so the tables are fixed).
+// ORDER 8: interior 1 - 35/128 at the centre and 7/32, -7/64, 1/32, -1/256
+// off it; rows 0, 1 identity, row 2 Lele's 4th-order and row 3 his
+// 6th-order centred filter.  ORDER 4: Lele's 4th-order filter, rows 0, 1
+// identity.  Right rows are the mirror images (sign +1).  `s` points at the
+// point, `st` is the stride along the sweep.
+template <typename T, int ORDER>
+__device__ __forceinline__ T filter_row(const T *s, int st, int code) {
+  if (ORDER == 8) {
+    if (code == CODE_INTERIOR) {
+      const T c0 = (T)(93.0 / 128.0), c1 = (T)(7.0 / 32.0), c2 = (T)(-7.0 / 64.0), c3 = (T)(1.0 / 32.0),
+              c4 = (T)(-1.0 / 256.0);
+      T acc = c4 * s[-4 * st];
+      acc += c3 * s[-3 * st];
+      acc += c2 * s[-2 * st];
+      acc += c1 * s[-st];
+      acc += c0 * s[0];
+      acc += c1 * s[st];
+      acc += c2 * s[2 * st];
+      acc += c3 * s[3 * st];
+      acc += c4 * s[4 * st];
+      return acc;
+    }
+    if (code == CODE_ZERO) return (T)0;
+    const int k = code - CODE_LEFT;
+    const int row = k < 4 ? k : k - 4;
+    if (row <= 1) return s[0];
+    if (row == 2) {
+      const T a = (T)((-1.0 / 8.0) / 2.0), b = (T)((1.0 / 2.0) / 2.0), c = (T)(5.0 / 8.0);
+      T acc = a * s[-2 * st];
+      acc += b * s[-st];
+      acc += c * s[0];
+      acc += b * s[st];
+      acc += a * s[2 * st];
+      return acc;
+    }
+    const T a = (T)((1.0 / 32.0) / 2.0), b = (T)((-3.0 / 16.0) / 2.0), c = (T)((15.0 / 32.0) / 2.0),
+            d = (T)(11.0 / 16.0);
+    T acc = a * s[-3 * st];
+    acc += b * s[-2 * st];
+    acc += c * s[-st];
+    acc += d * s[0];
+    acc += c * s[st];
+    acc += b * s[2 * st];
+    acc += a * s[3 * st];
+    return acc;
+  } else {
+    if (code == CODE_INTERIOR) {
+      const T a = (T)((-1.0 / 8.0) / 2.0), b = (T)((1.0 / 2.0) / 2.0), c = (T)(5.0 / 8.0);
+      T acc = a * s[-2 * st];
+      acc += b * s[-st];
+      acc += c * s[0];
+      acc += b * s[st];
+      acc += a * s[2 * st];
+      return acc;
+    }
+    if (code == CODE_ZERO) return (T)0;
+    return s[0];
+  }
+}
+
+// spatial::apply_filter (spatial:2191-2246) on a 64 x TY tile for NVAR
+// variables at once: every global operand is requested up front; the xi
+// sweep of all variables over the tile plus its eta halo goes to LDS, then
+// the eta sweep, the blend and temporal::iblank_correction; two barriers
+// per tile.  Tiles whose halo stays inside the block skip all edge logic.
+template <typename T, int NVAR, int TY>
+struct FilterLds {
+  T in[NVAR][TY + 2 * FL_H][FL_TX + 2 * FL_H];
+  T mid[NVAR][TY + 2 * FL_H][FL_TX];
+};
+
+}  // namespace
+
+}  // namespace lee

@@ -24,6 +24,7 @@ constexpr int MAX_OV_BLOCKS = 16;    // blocks of an overset run (per-block tabl
 //   2 + k        left-closure row k, anchored k points to the left
 //   2 + nb + k   right-closure row k, anchored k points to the right (mirrored)
 constexpr int CODE_INTERIOR = 0, CODE_ZERO = 1, CODE_LEFT = 2;
+constexpr int STAGE_FIRST = 0, STAGE_MID = 1, STAGE_LAST = 2;  // RK stage kinds
 
 // One 1-D operator (interior stencil + closure rows), reference tables of
 // spatial_discretization.cpp:28-224.
@@ -93,11 +94,51 @@ struct Block {
   std::vector<std::pair<double, double>> cfl_cand;
   // host copies used by init checks
   std::vector<double> h_met, h_jac;
+  // discrete adjoint (allocated on first use): A[0] adjoint of the step
+  // state, A[1] of the unfiltered state / accumulator, A[2], A[3] of the
+  // stage inputs, A[4] running adjoint of y_n; adjoint of the stale aux
+  // density; source-field gradient (FP64)
+  double *A64[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  float *A32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double *amu64 = nullptr;
+  float *amu32 = nullptr;
+  double *agrad = nullptr;
+  int adj_r = 2;               // rows per thread of the adjoint stage kernel (1 or 2)
+  // observer points of the functional (device) and their weights
+  int nobs = 0;
+  int *obs_l = nullptr;
+  double *obs_w = nullptr;
+  // checkpoints of an objective run: slots of (state 5n + aux density n)
+  // in the bulk type; the last slot holds the state at the end of the run
+  char *ckpt = nullptr;
+  size_t ckpt_bytes = 0;
+  double *samp = nullptr;      // observer samples of one checkpoint segment
+  size_t samp_cap = 0;
   // batched ensembles: every block is an independent solve with its own
   // settings (source, CFL or dt) and clock
   lee_config cfg{};
   int clk_step = 0;
   double clk_time = 0.0, clk_dt = 1e-6;
+};
+
+// One reversed RK stage: buffer indices into Block::A (-1: none).
+struct AdjStage {
+  int kind;                    // 0 first, 1 middle, 2 last stage
+  int lacc, lin, lyin, lout, lyout;
+  double hb, ha, src;          // b_i h, a_i h, source time factor s1 s2
+  bool grad;                   // accumulate the source-field gradient
+};
+
+// Bookkeeping of the last lee_objective_forward.
+struct Objective {
+  bool ready = false;
+  int nsteps = 0, window = 0, every = 1, ncheck = 0;
+  std::vector<int> ck_step;            // clock at each checkpoint
+  std::vector<double> ck_time, ck_dt;
+  std::vector<double> hist_dt, hist_t; // per step: dt and time_sol after the increment
+  int end_step = 0;
+  double end_time = 0.0, end_dt = 0.0;
+  double *dJ = nullptr;                // device accumulator
 };
 
 // Per-step scalars of one solve in a batched ensemble.
@@ -142,6 +183,7 @@ struct lee_handle {
   lee_config cfg{};
   int nblk = 0;
   std::vector<lee::Block> b;
+  lee::Objective obj;
   bool batch = false;          // independent solves (lee_batch_create)
   lee::BatchPlan bp;
   // pinned host staging of the batch schedule, two buffers used in turn;
@@ -219,6 +261,14 @@ hipError_t batch_prepare(hipStream_t s, lee_handle *h);
 hipError_t batch_stage(hipStream_t s, lee_handle *h, int stage);
 hipError_t batch_filter(hipStream_t s, lee_handle *h);
 void batch_release(lee_handle *h);
+hipError_t adjoint_stage(hipStream_t s, lee_handle *h, Block &b, const AdjStage &st);
+hipError_t adjoint_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst, bool with_mu);
+hipError_t adjoint_zero(hipStream_t s, lee_handle *h, Block &b, int buf);  // buf < 0: the aux-density adjoint
+hipError_t adjoint_zero_grad(hipStream_t s, Block &b);
+hipError_t adjoint_add_rt_mu(hipStream_t s, lee_handle *h, Block &b, int buf);
+hipError_t observer_energy(hipStream_t s, lee_handle *h, Block &b, double *out);
+hipError_t observer_gather(hipStream_t s, lee_handle *h, Block &b, double *out);
+hipError_t observer_seed(hipStream_t s, lee_handle *h, Block &b, int buf, const double *samp, double scale);
 hipError_t cast_state(hipStream_t s, const Block &b, int nvars, const double *src, float *dst, size_t n);
 hipError_t cast_back(hipStream_t s, const Block &b, int nvars, const float *src, double *dst, size_t n);
 

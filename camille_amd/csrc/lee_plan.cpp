@@ -313,6 +313,14 @@ void plan_source(const lee_config &c, Block &b, const double *xyz) {
   b.src.j1 = -1;
   if (c.source_kind == LEE_SOURCE_NONE) return;
   const int n = b.n;
+  if (c.source_kind == LEE_SOURCE_FIELD) {  // the caller's field over the whole block, zero until set
+    b.src.i0 = 0;
+    b.src.i1 = b.ni - 1;
+    b.src.j0 = 0;
+    b.src.j1 = b.nj - 1;
+    b.src.host_g.assign(n, 0.0);
+    return;
+  }
   std::vector<double> g(n);
   int i0 = b.ni, i1 = -1, j0 = b.nj, j1 = -1;
   for (int j = 0; j < b.nj; j++)
@@ -361,6 +369,7 @@ void source_factors(const lee_config &c, double t, double &s1, double &s2) {
       break;
     }
     case LEE_SOURCE_MONOPOLE:
+    case LEE_SOURCE_FIELD:
       s2 = c.source_params[0] * std::sin(c.source_params[1] * t);
       break;
     default:

@@ -28,6 +28,7 @@ BC_TYPES = {
 }
 # SIMULATION key -> physical source
 SOURCES = {
+    "SOURCE_FIELD": 4,
     "CASE_TANNA_TPN49": 0,
     "CASE_KBK_COMBUSTOR": 0,
     "CASE_SCATTERING_TWOCYLINDER": 1,
@@ -88,6 +89,8 @@ EXPORTS = [
     "lee_get_mean_gradient", "lee_get_sponge", "lee_last_solve_ms", "lee_version",
     "lee_set_kernel_timing", "lee_get_kernel_timing", "lee_autotune", "lee_set_tile_shape", "lee_get_tile_shape",
     "lee_batch_create", "lee_get_solve_time", "lee_set_batch_launch", "lee_get_batch_launch",
+    "lee_set_source_field", "lee_set_observer", "lee_objective_forward", "lee_objective_backward",
+    "lee_get_source_gradient", "lee_get_state_gradient", "lee_compute_rhs_adjoint", "lee_apply_filter_adjoint",
 ]
 
 _lib = None
@@ -136,6 +139,14 @@ def library():
     L.lee_get_solve_time.argtypes = [vp, i, ip, dp, dp]
     L.lee_set_batch_launch.argtypes = [vp, i, i, i]
     L.lee_get_batch_launch.argtypes = [vp, ip, ip, ip]
+    L.lee_set_source_field.argtypes = [vp, i, dp]
+    L.lee_set_observer.argtypes = [vp, i, ip, ip, dp]
+    L.lee_objective_forward.argtypes = [vp, i, i, i, dp]
+    L.lee_objective_backward.argtypes = [vp, d]
+    L.lee_get_source_gradient.argtypes = [vp, i, dp]
+    L.lee_get_state_gradient.argtypes = [vp, i, dp]
+    L.lee_compute_rhs_adjoint.argtypes = [vp, i, dp, dp]
+    L.lee_apply_filter_adjoint.argtypes = [vp, i, dp]
     _lib = L
     return L
 
@@ -421,6 +432,53 @@ class Solver:
         ptrs = (ctypes.POINTER(ctypes.c_double) * len(arrs))(*[_dp(a) for a in arrs])
         self._check(self.L.lee_update_boundary(self.h, ptrs))
         return arrs
+
+    # discrete adjoint ----------------------------------------------------
+    def set_source_field(self, q, ib=0):
+        """Spatial field of a SOURCE_FIELD block: p-source q(x) A sin(omega t)."""
+        q = np.ascontiguousarray(np.asarray(q, np.float64).reshape(-1))
+        if q.size != self.n(ib):
+            raise LeeError("source field size mismatch")
+        self._check(self.L.lee_set_source_field(self.h, ib, _dp(q)))
+
+    def set_observer(self, points):
+        """points: iterable of (block, i, j, weight); J sums weight * p'^2."""
+        pts = list(points)
+        blk = np.array([p[0] for p in pts], np.int32)
+        idx = np.array([int(p[2]) * self.shapes[int(p[0])][1] + int(p[1]) for p in pts], np.int32)
+        w = np.array([p[3] for p in pts], np.float64)
+        self._keep += [blk, idx, w]
+        self._check(self.L.lee_set_observer(self.h, len(pts), _ip(blk), _ip(idx), _dp(w)))
+
+    def objective_forward(self, nsteps, window_start, ckpt_every):
+        J = ctypes.c_double()
+        self._check(self.L.lee_objective_forward(self.h, int(nsteps), int(window_start), int(ckpt_every),
+                                                 ctypes.byref(J)))
+        return J.value
+
+    def objective_backward(self, scale=1.0):
+        self._check(self.L.lee_objective_backward(self.h, float(scale)))
+
+    def source_gradient(self, ib=0):
+        out = np.empty(self.shapes[ib])
+        self._check(self.L.lee_get_source_gradient(self.h, ib, _dp(out)))
+        return out
+
+    def state_gradient(self, ib=0):
+        out = np.empty((NV,) + self.shapes[ib])
+        self._check(self.L.lee_get_state_gradient(self.h, ib, _dp(out)))
+        return out
+
+    def compute_rhs_adjoint(self, kappa, ib=0):
+        k = np.ascontiguousarray(np.asarray(kappa, np.float64))
+        out = np.empty_like(k)
+        self._check(self.L.lee_compute_rhs_adjoint(self.h, ib, _dp(k), _dp(out)))
+        return out
+
+    def apply_filter_adjoint(self, y, ib=0):
+        y = np.array(y, np.float64, copy=True, order="C")
+        self._check(self.L.lee_apply_filter_adjoint(self.h, ib, _dp(y)))
+        return y
 
     # init products -------------------------------------------------------
     def metrics(self, ib=0):

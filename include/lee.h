@@ -57,6 +57,8 @@ extern "C" {
 #define LEE_SOURCE_JET2D 2        /* CASE_2DJET_WITH_A_HARMONIC_SOURCE */
 #define LEE_SOURCE_MONOPOLE 3     /* extension: A exp(-ln2 |x-x0|^2/w^2) sin(omega t);
                                      params {A, omega, x0, y0, z0, w} */
+#define LEE_SOURCE_FIELD 4        /* extension: q(x) A sin(omega t) with a caller field q
+                                     (lee_set_source_field); params {A, omega} */
 
 typedef struct lee_handle lee_handle;
 
@@ -206,6 +208,35 @@ int lee_get_solve_time(const lee_handle *h, int isolve, int *time_step, double *
  * bit-identical results. */
 int lee_set_batch_launch(lee_handle *h, int use_graph, int grid_stage, int grid_filter);
 int lee_get_batch_launch(lee_handle *h, int *use_graph, int *grid_stage, int *grid_filter);
+
+/* Discrete adjoint (no reference counterpart: the reference has no
+ * adjoint).  Fused path, 2-D blocks with sponges and sources; no other
+ * boundary conditions and no overset.
+ *
+ * Source field of a LEE_SOURCE_FIELD block: p-equation source q(x) A sin(omega t). */
+int lee_set_source_field(lee_handle *h, int iblock, const double *q);
+/* Observer functional J = sum over steps k in (window_start, nsteps] of
+ * sum_l w_l p'_k(l)^2, p'_k the pressure after step k at the observer
+ * points (block[l], index[l]); nobs = 0 clears them. */
+int lee_set_observer(lee_handle *h, int nobs, const int *block, const int *index, const double *weight);
+/* Runs nsteps steps from the current state and clock (like lee_solve),
+ * keeping the state every ckpt_every steps on the device, and returns J. */
+int lee_objective_forward(lee_handle *h, int nsteps, int window_start, int ckpt_every, double *J);
+/* Reverse pass of the last lee_objective_forward for the gradient of
+ * scale * J: each checkpoint segment is recomputed forwards (observer
+ * samples kept), then the transposed filter, stages and RK updates run
+ * backwards through it.  Afterwards the state and clock are those at the
+ * end of the forward run again. */
+int lee_objective_backward(lee_handle *h, double scale);
+/* dJ/dq of a LEE_SOURCE_FIELD block (n doubles) and dJ/dy0 (5n doubles,
+ * the state the forward run started from). */
+int lee_get_source_gradient(lee_handle *h, int iblock, double *grad_q);
+int lee_get_state_gradient(lee_handle *h, int iblock, double *grad_y0);
+/* Transposed unit operators, for dot-product checks against
+ * lee_compute_rhs (without its source term) and lee_apply_filter:
+ * out = L^T kappa with the aux density of the argument, and y <- F^T y. */
+int lee_compute_rhs_adjoint(lee_handle *h, int iblock, const double *kappa, double *out);
+int lee_apply_filter_adjoint(lee_handle *h, int iblock, double *y);
 
 /* Library identification: returns a static string (build info). */
 const char *lee_version(void);
