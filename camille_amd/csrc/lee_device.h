@@ -13,6 +13,11 @@ namespace {
 
 constexpr int TPB = 256;
 
+// Read-only tables that kernels index at run time (the batched ensemble's
+// per-solve operands) are declared in the constant address space: loads
+// from it are scalar and the pointers they yield address global memory.
+#define LEE_CONST_AS __attribute__((address_space(4)))
+
 __device__ __forceinline__ int default_code(int c, int len, int nb) {
   if (c < nb) return CODE_LEFT + c;
   if (c > len - 1 - nb) return CODE_LEFT + nb + (len - 1 - c);
@@ -191,10 +196,15 @@ __device__ __forceinline__ T filter_row(const T *s, int st, int code) {
 // the eta sweep, the blend and temporal::iblank_correction; two barriers
 // per tile.  Tiles whose halo stays inside the block skip all edge logic.
 template <typename T, int NVAR, int TY>
-struct FilterLds {
+union alignas(32) FilterLds {  // the xi-sweep overwrites its input
   T in[NVAR][TY + 2 * FL_H][FL_TX + 2 * FL_H];
   T mid[NVAR][TY + 2 * FL_H][FL_TX];
 };
+
+// Four consecutive LDS values as one vector access (rows of the filter
+// tiles are padded to multiples of four values).
+template <typename T>
+using Vec4 = T __attribute__((ext_vector_type(4)));
 
 }  // namespace
 

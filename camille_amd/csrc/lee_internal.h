@@ -156,7 +156,7 @@ struct BatchPlan {
   int grid_stage = 0, grid_filter = 0;  // persistent grid sizes (0: one workgroup per tile)
   void *stage_args = nullptr;  // device StageArgs<T>[stages][nsolve]
   void *filter_args = nullptr; // device FilterArgs<T>[nsolve]
-  int *tile_start_stage = nullptr, *tile_start_filter = nullptr;
+  int *tile_map_stage = nullptr, *tile_map_filter = nullptr;  // (solve, tile) per list entry
   int total_stage = 0, total_filter = 0;
   BatchSched *sched = nullptr;  // device [step][solve][stage]
   size_t sched_steps = 0;

@@ -376,8 +376,8 @@ __global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__
 
 // ------------------------------------------------------------------ fused
 
-template <typename T, bool MEANF, bool HOLES, int KIND, int R>
-__device__ __forceinline__ void stage_tile(const StageArgs<T> &a, const StepT<T> &sc, int tile, StageLds<T, R> &s) {
+template <typename T, bool MEANF, bool HOLES, int KIND, int R, typename AT>
+__device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int tile, StageLds<T, R> &s) {
   constexpr int TY = 4 * R;
   constexpr int RS = ST_TX + 2 * ST_H;
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -541,7 +541,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &a, const StepT<T>
     // Freund sponges, zone by zone (source.cpp:136-201), sol_ref = 0
 #pragma unroll 1
     for (int z = 0; z < a.nz; z++) {
-      const ZoneT<T> &zz = a.z[z];
+      const auto &zz = a.z[z];
       if (gi >= zz.i0 && gi <= zz.i1 && gj >= zz.j0 && gj <= zz.j1) {
         const T sg = zz.sig[(gj - zz.j0) * zz.nzi + (gi - zz.i0)];
         rs += -sg * sv;
@@ -588,8 +588,9 @@ __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
   stage_tile<T, MEANF, HOLES, KIND, R>(a, sc, tile, s);
 }
 
-// Batched ensemble: the tiles of all solves form one list (solve k owns
-// tiles [tile_start[k], tile_start[k+1])).  A persistent grid walks the
+// Batched ensemble: the tiles of all solves form one list; map[2t] is the
+// solve of list entry t and map[2t + 1] its tile within the solve.  A
+// persistent grid walks the
 // list with a grid stride; virtual index v maps to list position
 // (v & 7) * chunk + (v >> 3), so with a grid that is a multiple of 8 each
 // XCD works through one contiguous eighth of the list.  Per-solve operands
@@ -598,43 +599,39 @@ __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
 template <typename T>
 struct BatchStageArgs {
   const StageArgs<T> *args;      // nsolve entries for this stage
-  const int *tile_start;         // nsolve + 1
+  const int *map;                // (solve, tile) per list entry
   const BatchSched *sched;       // [step][solve][stage]
   const int *counter;            // step index within the schedule
   int nsolve, nstage, stage, total, chunk;
   T b, a;
 };
 
-__device__ __forceinline__ int find_solve(const int *start, int nsolve, int t) {
-  int lo = 0, hi = nsolve;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (start[mid] <= t) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
+// The operand tables are read through the constant address space: they do
+// not change during a launch, so their loads are scalar and the pointers
+// loaded from them are known to address global memory (no flat accesses).
 template <typename T, bool MEANF, int KIND, int R>
-__global__ void __launch_bounds__(TPB) k_batch_stage(BatchStageArgs<T> ba) {
+__global__ void __launch_bounds__(TPB)
+    k_batch_stage(BatchStageArgs<T> ba, const LEE_CONST_AS StageArgs<T> *args, const LEE_CONST_AS int *map,
+                  const LEE_CONST_AS BatchSched *sched, const int *__restrict__ counter) {
   __shared__ StageLds<T, R> s;
   // device-scope load: the counter was written by the previous step's filter launch
-  const int step = __hip_atomic_load(ba.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int step = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (int v = blockIdx.x; v < 8 * ba.chunk; v += gridDim.x) {
     const int t = (v & 7) * ba.chunk + (v >> 3);
     if (t >= ba.total) continue;
-    const int k = find_solve(ba.tile_start, ba.nsolve, t);
-    const BatchSched &q = ba.sched[((size_t)step * ba.nsolve + k) * ba.nstage + ba.stage];
+    const int k = map[2 * t], local = map[2 * t + 1];
+    const auto &q = sched[((size_t)step * ba.nsolve + k) * ba.nstage + ba.stage];
     const StepT<T> sc{(T)q.dt, ba.b, ba.a, q.s1, q.s2};
-    stage_tile<T, MEANF, false, KIND, R>(ba.args[k], sc, t - ba.tile_start[k], s);
+    stage_tile<T, MEANF, false, KIND, R>(args[k], sc, local, s);
     __syncthreads();  // the LDS tile is rewritten by the next iteration
   }
 }
 
-template <typename T, bool HOLES, int ORDER, int NVAR, int TY>
-__device__ __forceinline__ void filter_tile(const FilterArgs<T> &a, int tile, FilterLds<T, NVAR, TY> &L) {
+template <typename T, bool HOLES, int ORDER, int NVAR, int TY, typename AT>
+__device__ __forceinline__ void filter_tile(const AT &a, int tile, FilterLds<T, NVAR, TY> &L) {
   constexpr int NR = TY + 2 * FL_H, KR = NR / 4;
   constexpr int NB = ORDER / 2;
+  static_assert(NR == 4 * KR && FL_H == 4, "tile rows and the halo come in fours");
   auto &in = L.in;
   auto &mid = L.mid;
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -678,23 +675,65 @@ __device__ __forceinline__ void filter_tile(const FilterArgs<T> &a, int tile, Fi
       if (xtra) in[q][ty + 4 * k][FL_TX + tx] = rb[q][k];
     }
   __syncthreads();
-  const int gi = i0 + tx;
+  // xi-sweep of every region row into registers: each item is 4 consecutive
+  // columns of one row and variable, from 12 values read as three 4-vectors
+  constexpr int NG = FL_TX / 4;
+  constexpr int NXI = NVAR * NR * NG;
+  constexpr int NS = (NXI + TPB - 1) / TPB;
+  const int tid = ty * FL_TX + tx;
+  Vec4<T> xo[NS];
 #pragma unroll
-  for (int k = 0; k < KR; k++) {
-    const int r = ty + 4 * k;
-    const int gj = j0 - FL_H + r;
-    int code = CODE_INTERIOR;
-    bool ok = true;
-    if (!inner) {
-      ok = gj >= 0 && gj < nj && gi < ni;
-      if (ok) code = HOLES ? (int)a.code[gj * ni + gi] : default_code(gi, ni, NB);
+  for (int s = 0; s < NS; s++) {
+    const int it = tid + s * TPB;
+    if (NXI % TPB != 0 && it >= NXI) break;
+    const int g = it % NG, r = (it / NG) % NR, q = it / (NG * NR);
+    T v[12];
+#pragma unroll
+    for (int h = 0; h < 3; h++) {
+      const Vec4<T> x = *reinterpret_cast<const Vec4<T> *>(&in[q][r][4 * g + 4 * h]);
+      v[4 * h] = x.x;
+      v[4 * h + 1] = x.y;
+      v[4 * h + 2] = x.z;
+      v[4 * h + 3] = x.w;
     }
+    const int gj = j0 - FL_H + r;
 #pragma unroll
-    for (int q = 0; q < NVAR; q++) mid[q][r][tx] = ok ? filter_row<T, ORDER>(&in[q][r][tx + FL_H], 1, code) : (T)0;
+    for (int m = 0; m < 4; m++) {
+      const int gi = i0 + 4 * g + m;
+      int code = CODE_INTERIOR;
+      bool ok = true;
+      if (!inner) {
+        ok = gj >= 0 && gj < nj && gi < ni;
+        if (ok) code = HOLES ? (int)a.code[gj * ni + gi] : default_code(gi, ni, NB);
+      }
+      xo[s][m] = ok ? filter_row<T, ORDER>(&v[m + FL_H], 1, code) : (T)0;
+    }
+  }
+  __syncthreads();  // every read of the input tile is done: mid overwrites it
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int it = tid + s * TPB;
+    if (NXI % TPB != 0 && it >= NXI) break;
+    const int g = it % NG, r = (it / NG) % NR, q = it / (NG * NR);
+    *reinterpret_cast<Vec4<T> *>(&mid[q][r][4 * g]) = xo[s];
   }
   __syncthreads();
+  // eta-sweep, blend with the unfiltered values kept in registers since the
+  // load, and hole reset.  Thread (tx, ty) loaded region row ty + 4k, i.e.
+  // tile row ty + 4(k - 1), at region column tx (block column i0 - 4 + tx)
+  // in ra[.][k] and column 64 + tx in rb[.][k]: the value at block column
+  // i0 + tx comes from lane tx + 4, or for tx >= 60 from rb of lane tx - 60.
+  const int gi = i0 + tx;
+  const int src_lane = tx < FL_TX - FL_H ? tx + FL_H : tx - (FL_TX - FL_H);
 #pragma unroll
   for (int k = 0; k < TY / 4; k++) {
+    T orig[NVAR];
+#pragma unroll
+    for (int q = 0; q < NVAR; q++) {
+      const T va = __shfl(ra[q][k + 1], src_lane, FL_TX);
+      const T vb = __shfl(rb[q][k + 1], src_lane, FL_TX);
+      orig[q] = tx < FL_TX - FL_H ? va : vb;
+    }
     const int r = ty + 4 * k;
     const int gj = j0 + r;
     if (!inner && (gj >= nj || gi >= ni)) continue;
@@ -704,7 +743,7 @@ __device__ __forceinline__ void filter_tile(const FilterArgs<T> &a, int tile, Fi
 #pragma unroll
     for (int q = 0; q < NVAR; q++) {
       const T f = filter_row<T, ORDER>(&mid[q][r + FL_H][tx], FL_TX, code);
-      T out = a.rem * in[q][r + FL_H][tx + FL_H] + a.blend * f;
+      T out = a.rem * orig[q] + a.blend * f;
       if (hole) out = 0;
       a.dst[a.vars[q] * n + p] = out;
     }
@@ -722,7 +761,7 @@ __global__ void __launch_bounds__(TPB) k_fused_filter(FilterArgs<T> a) {
 template <typename T>
 struct BatchFilterArgs {
   const FilterArgs<T> *args;     // nsolve entries
-  const int *tile_start;         // nsolve + 1
+  const int *map;                // (solve, tile) per list entry
   int *counter;                  // advanced by one at the end of the step
   int nsolve, total, chunk;
 };
@@ -731,7 +770,8 @@ struct BatchFilterArgs {
 // step counter (no kernel of this launch reads it; the next step's stage
 // kernels see the new value).
 template <typename T, int ORDER, int TY>
-__global__ void __launch_bounds__(TPB) k_batch_filter(BatchFilterArgs<T> ba) {
+__global__ void __launch_bounds__(TPB)
+    k_batch_filter(BatchFilterArgs<T> ba, const LEE_CONST_AS FilterArgs<T> *args, const LEE_CONST_AS int *map) {
   __shared__ FilterLds<T, 4, TY> L;
   if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
     const int c = __hip_atomic_load(ba.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -740,8 +780,7 @@ __global__ void __launch_bounds__(TPB) k_batch_filter(BatchFilterArgs<T> ba) {
   for (int v = blockIdx.x; v < 8 * ba.chunk; v += gridDim.x) {
     const int t = (v & 7) * ba.chunk + (v >> 3);
     if (t >= ba.total) continue;
-    const int k = find_solve(ba.tile_start, ba.nsolve, t);
-    filter_tile<T, false, ORDER, 4, TY>(ba.args[k], t - ba.tile_start[k], L);
+    filter_tile<T, false, ORDER, 4, TY>(args[map[2 * t]], map[2 * t + 1], L);
     __syncthreads();
   }
 }
@@ -1168,14 +1207,20 @@ hipError_t fused_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst
 
 // ------------------------------------------------------------ batched ensemble
 
+// A device table as a constant-address-space pointer (the kernels' view).
+template <typename P>
+static const LEE_CONST_AS P *CA(const P *p) {
+  return (const LEE_CONST_AS P *)p;
+}
+
 void batch_release(lee_handle *h) {
   BatchPlan &p = h->bp;
   if (p.exec) (void)hipGraphExecDestroy(p.exec);
   p.exec = nullptr;
-  for (void *q : {p.stage_args, p.filter_args, (void *)p.tile_start_stage, (void *)p.tile_start_filter})
+  for (void *q : {p.stage_args, p.filter_args, (void *)p.tile_map_stage, (void *)p.tile_map_filter})
     if (q) (void)hipFree(q);
   p.stage_args = p.filter_args = nullptr;
-  p.tile_start_stage = p.tile_start_filter = nullptr;
+  p.tile_map_stage = p.tile_map_filter = nullptr;
   p.ready = false;
 }
 
@@ -1191,7 +1236,7 @@ static hipError_t batch_prepare_t(hipStream_t s, lee_handle *h) {
   const int ns = h->nblk, S = h->cfg.rk_stages;
   std::vector<StageArgs<T>> sa((size_t)S * ns);
   std::vector<FilterArgs<T>> fa(ns);
-  std::vector<int> ts(ns + 1, 0), tf(ns + 1, 0);
+  std::vector<int> ts, tf;
   const StageScalars zero{0.0, 0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < ns; k++) {
     Block &b = h->b[k];
@@ -1200,23 +1245,29 @@ static hipError_t batch_prepare_t(hipStream_t s, lee_handle *h) {
     for (int st = 0; st < S; st++)
       sa[(size_t)st * ns + k] = make_stage_args<T>(h, b, p.tuning ? -1 : st, Y, rho, p.R, zero);
     fa[k] = make_filter_args<T>(h, b, Y, 1, p.tuning ? 2 : 0, p.TY);
-    ts[k + 1] = ts[k] + sa[k].ntiles;
-    tf[k + 1] = tf[k] + fa[k].ntiles;
+    for (int t = 0; t < sa[k].ntiles; t++) {
+      ts.push_back(k);
+      ts.push_back(t);
+    }
+    for (int t = 0; t < fa[k].ntiles; t++) {
+      tf.push_back(k);
+      tf.push_back(t);
+    }
   }
-  p.total_stage = ts[ns];
-  p.total_filter = tf[ns];
+  p.total_stage = (int)(ts.size() / 2);
+  p.total_filter = (int)(tf.size() / 2);
   hipError_t e;
   if ((e = hipMalloc(&p.stage_args, sizeof(StageArgs<T>) * sa.size())) != hipSuccess) return e;
   if ((e = hipMalloc(&p.filter_args, sizeof(FilterArgs<T>) * fa.size())) != hipSuccess) return e;
-  if ((e = hipMalloc((void **)&p.tile_start_stage, sizeof(int) * (ns + 1))) != hipSuccess) return e;
-  if ((e = hipMalloc((void **)&p.tile_start_filter, sizeof(int) * (ns + 1))) != hipSuccess) return e;
+  if ((e = hipMalloc((void **)&p.tile_map_stage, sizeof(int) * ts.size())) != hipSuccess) return e;
+  if ((e = hipMalloc((void **)&p.tile_map_filter, sizeof(int) * tf.size())) != hipSuccess) return e;
   if ((e = hipMemcpyAsync(p.stage_args, sa.data(), sizeof(StageArgs<T>) * sa.size(), hipMemcpyHostToDevice, s)))
     return e;
   if ((e = hipMemcpyAsync(p.filter_args, fa.data(), sizeof(FilterArgs<T>) * fa.size(), hipMemcpyHostToDevice, s)))
     return e;
-  if ((e = hipMemcpyAsync(p.tile_start_stage, ts.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s)))
+  if ((e = hipMemcpyAsync(p.tile_map_stage, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice, s)))
     return e;
-  if ((e = hipMemcpyAsync(p.tile_start_filter, tf.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s)))
+  if ((e = hipMemcpyAsync(p.tile_map_filter, tf.data(), sizeof(int) * tf.size(), hipMemcpyHostToDevice, s)))
     return e;
   // the host vectors die here: wait for the copies
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
@@ -1243,7 +1294,7 @@ static hipError_t batch_stage_t(hipStream_t s, lee_handle *h, int stage) {
   const int ns = h->nblk;
   BatchStageArgs<T> ba{};
   ba.args = (const StageArgs<T> *)p.stage_args + (size_t)stage * ns;
-  ba.tile_start = p.tile_start_stage;
+  ba.map = p.tile_map_stage;
   ba.sched = p.sched;
   ba.counter = p.counter;
   ba.nsolve = ns;
@@ -1257,11 +1308,14 @@ static hipError_t batch_stage_t(hipStream_t s, lee_handle *h, int stage) {
   const dim3 grid(batch_grid(p.total_stage, p.grid_stage)), block(64, 4);
   const bool mf = !h->b[0].uniform_mean;
   const int R = p.R;
-#define LEE_B(MF, K)                                                                          \
-  do {                                                                                        \
-    if (R == 1) hipLaunchKernelGGL((k_batch_stage<T, MF, K, 1>), grid, block, 0, s, ba);      \
-    else if (R == 2) hipLaunchKernelGGL((k_batch_stage<T, MF, K, 2>), grid, block, 0, s, ba); \
-    else hipLaunchKernelGGL((k_batch_stage<T, MF, K, 4>), grid, block, 0, s, ba);             \
+#define LEE_B(MF, K)                                                                                       \
+  do {                                                                                                     \
+    if (R == 1) hipLaunchKernelGGL((k_batch_stage<T, MF, K, 1>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map), \
+                                   CA(ba.sched), ba.counter);                                                  \
+    else if (R == 2) hipLaunchKernelGGL((k_batch_stage<T, MF, K, 2>), grid, block, 0, s, ba, CA(ba.args),  \
+                                        CA(ba.map), CA(ba.sched), ba.counter);                                     \
+    else hipLaunchKernelGGL((k_batch_stage<T, MF, K, 4>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map),  \
+                            CA(ba.sched), ba.counter);                                                         \
   } while (0)
 #define LEE_BK(MF)                                  \
   do {                                              \
@@ -1286,7 +1340,7 @@ static hipError_t batch_filter_t(hipStream_t s, lee_handle *h) {
   BatchPlan &p = h->bp;
   BatchFilterArgs<T> ba{};
   ba.args = (const FilterArgs<T> *)p.filter_args;
-  ba.tile_start = p.tile_start_filter;
+  ba.map = p.tile_map_filter;
   ba.counter = p.tuning ? p.counter + 1 : p.counter;  // tuning runs leave the step counter alone
   ba.nsolve = h->nblk;
   ba.total = p.total_filter;
@@ -1294,11 +1348,12 @@ static hipError_t batch_filter_t(hipStream_t s, lee_handle *h) {
   const dim3 grid(batch_grid(p.total_filter, p.grid_filter)), block(64, 4);
   const bool o8 = h->cfg.filter_order == 8;
   const int TY = p.TY;
-#define LEE_BF(OR)                                                                          \
-  do {                                                                                      \
-    if (TY == 4) hipLaunchKernelGGL((k_batch_filter<T, OR, 4>), grid, block, 0, s, ba);     \
-    else if (TY == 8) hipLaunchKernelGGL((k_batch_filter<T, OR, 8>), grid, block, 0, s, ba); \
-    else hipLaunchKernelGGL((k_batch_filter<T, OR, 16>), grid, block, 0, s, ba);            \
+#define LEE_BF(OR)                                                                                   \
+  do {                                                                                               \
+    if (TY == 4) hipLaunchKernelGGL((k_batch_filter<T, OR, 4>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map)); \
+    else if (TY == 8) hipLaunchKernelGGL((k_batch_filter<T, OR, 8>), grid, block, 0, s, ba, CA(ba.args), \
+                                         CA(ba.map));                                                       \
+    else hipLaunchKernelGGL((k_batch_filter<T, OR, 16>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map));        \
   } while (0)
   if (o8) LEE_BF(8);
   else LEE_BF(4);

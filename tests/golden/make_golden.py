@@ -30,6 +30,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, HERE)
+sys.path.insert(0, ROOT)
 
 import cases  # noqa: E402
 import refio  # noqa: E402
@@ -78,7 +79,12 @@ def run_overset_case(case, nranks=2):
         refio.write_solution(os.path.join(d, "ic.q"), [b["ic"][:, None] for b in blocks], 0, 0.0)
         refio.write_bc(os.path.join(d, "bc.in"), case["bc"])
         dims = [(b["x"].shape[1], b["x"].shape[0], 1) for b in blocks]
-        refio.write_overture(os.path.join(d, "overset.bin"), 2, dims, case["recv"], case["widths"], case["holes"])
+        if "geo" in case:
+            from camille_amd import overset_gen
+            overset_gen.write_overture(os.path.join(d, "overset.bin"), case["geo"])
+        else:
+            refio.write_overture(os.path.join(d, "overset.bin"), 2, dims, case["recv"], case["widths"],
+                                 case["holes"])
         items = cases.deck_items(case)
         items = [(k, v) for k, v in items if k != "OVERSET"]
         items.insert(12, ("OVERSET", f"OVERTURE, TYPE = EXPLICIT, ORDER_ACCURACY = {case['order']}"))
@@ -98,20 +104,26 @@ def run_overset_case(case, nranks=2):
 
 def main(names):
     import overset_cases
-    if names and names[0] == "overset_small":
-        case = overset_cases.case_overset_small()
+    overset = {"overset_small": overset_cases.case_overset_small,
+               "cylinder_small": overset_cases.case_cylinder_small}
+    if names and names[0] in overset:
+        name = names[0]
+        case = overset[name]()
         sols, taus, log = run_overset_case(case)
         arrs = {f"sol{b}": sols[b][:, 0] for b in range(len(sols))}
         arrs["tau"] = taus[0]
-        path = os.path.join(HERE, "overset_small.npz")
+        path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **arrs)
         mpath = os.path.join(HERE, "manifest.json")
         old = json.load(open(mpath)) if os.path.exists(mpath) else {}
-        old["overset_small"] = dict(nsteps=case["nsteps"], blocks=[list(b["x"].shape) for b in case["blocks"]],
-                                    nrecv=len(case["recv"]), nholes=len(case["holes"]), time=float(taus[0][3]),
-                                    report=log, ranks=2)
+        if "geo" in case:
+            nrecv, nholes = len(case["geo"]["recv"]["ri"]), len(case["geo"]["holes"]["i"])
+        else:
+            nrecv, nholes = len(case["recv"]), len(case["holes"])
+        old[name] = dict(nsteps=case["nsteps"], blocks=[list(b["x"].shape) for b in case["blocks"]],
+                         nrecv=nrecv, nholes=nholes, time=float(taus[0][3]), report=log, ranks=2)
         json.dump(old, open(mpath, "w"), indent=1, sort_keys=True)
-        print("overset_small time", taus[0][3], "->", os.path.getsize(path), "bytes")
+        print(name, "time", taus[0][3], "->", os.path.getsize(path), "bytes")
         return
     names = names or list(cases.ALL)
     manifest = {}

@@ -237,3 +237,17 @@ def test_overset_planted_hole_closure_mistake_is_caught():
     _, sols = _overset_run(plant=7)
     g = _golden("overset_small")
     assert not all(np.array_equal(sols[b], g[f"sol{b}"]) for b in range(2))
+
+
+def test_cylinder_golden_bitwise():
+    """Cylinder in a box (camille_amd.overset_gen): background holes and
+    fringe, O-grid with a seam interpolated from its own interior, wall-slip
+    on the body, potential-flow mean; reference on two MPI ranks."""
+    import overset_cases
+    c = overset_cases.case_cylinder_small()
+    o = overset_cases.oracle_for_cylinder(c, Oracle)
+    o.run(c["nsteps"])
+    g = _golden("cylinder_small")
+    assert o.time == float(g["tau"][3])
+    for b in range(2):
+        assert np.array_equal(o.solution(b)[:, 0], g[f"sol{b}"]), b
