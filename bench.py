@@ -153,6 +153,92 @@ def c5_leg(steps, warmup, single_steps):
     return res
 
 
+def c3_inputs():
+    """Overset cylinder: background Cartesian 2049x1025 on [-16, 16] x [-8, 8]
+    and an O-grid 513x129 from r = 0.5 to 2 around the cylinder
+    (camille_amd.overset_gen: holes for r < 1.5, 5-cell fringe, O-grid seam
+    from its own interior, ORDER-4 donor stencils); potential-flow mean at
+    M = 0.3 with p from Bernoulli; wall-slip on the cylinder; 64-point
+    Freund sponges on the background; Gaussian pressure pulse (1e-3,
+    half-width 0.2) upstream at (-3, 0); RK4 at CFL 0.5 with the 8th-order
+    filter.  Probes: 64 fluid background points in [-8, 8] x [-4, 4] drawn
+    by numpy.random.default_rng(3)."""
+    from camille_amd import overset_gen as og
+    geo = og.cylinder()
+    blocks = []
+    for x, y in geo["blocks"]:
+        ic = np.zeros((5,) + x.shape)
+        ic[4] = 1e-3 * np.exp(-math.log(2.0) * ((x + 3.0) ** 2 + y * y) / 0.2 ** 2)
+        blocks.append(dict(x=x, y=y, mean=og.potential_flow(x, y, 0.3, 0.5), ic=ic))
+    w = 64 - 1
+    bc = [(0, "sponge_freund_ambient", "+xi", "0", str(w), "0", "-0", "0", "0"),
+          (0, "sponge_freund_ambient", "-xi", f"-{w}", "-0", "0", "-0", "0", "0"),
+          (0, "sponge_freund_ambient", "+eta", "0", "-0", "0", str(w), "0", "0"),
+          (0, "sponge_freund_ambient", "-eta", "0", "-0", f"-{w}", "-0", "0", "0"),
+          (1, "wall_slip_kinematic", "+eta", "0", "-0", "0", "0", "0", "0")]
+    return dict(geo=geo, blocks=blocks, bc=bc, tables=og.abi_tables(geo))
+
+
+def c3_leg(steps, sample_every):
+    """Probe time series of the overset cylinder in FP32 bulk, FP64 fused
+    and FP64 plain (the reference's arithmetic), with throughput of each."""
+    from camille_amd import FP32, FP64, PATH_FUSED, PATH_PLAIN, Solver
+    inp = c3_inputs()
+    npts = sum(b["x"].size for b in inp["blocks"])
+    res = {"points": npts, "steps": steps, "sample_every": sample_every,
+           "receivers": int(len(inp["tables"]["recv_l"])), "holes": int(len(inp["tables"]["hole_l"]))}
+    series = {}
+    probes = None
+    for label, prec, path in (("fp32", FP32, PATH_FUSED), ("fp64", FP64, PATH_FUSED), ("plain", FP64, PATH_PLAIN)):
+        print(f"[bench] C3 {label}: {npts} points, {steps} steps", file=sys.stderr, flush=True)
+        s = Solver([dict(x=b["x"], y=b["y"], mean=b["mean"]) for b in inp["blocks"]], boundaries=inp["bc"],
+                   overset=inp["tables"], precision=prec, path=path, mode="FIX_CFL", value=0.5)
+        for ib, b in enumerate(inp["blocks"]):
+            s.set_solution(b["ic"], ib)
+        if probes is None:
+            ib0 = s.iblank(0)
+            x, y = inp["blocks"][0]["x"], inp["blocks"][0]["y"]
+            cand = np.flatnonzero((ib0 == 0) & (np.abs(x) <= 8.0) & (np.abs(y) <= 4.0))
+            pick = np.random.default_rng(3).choice(cand, 64, replace=False)
+            ni = x.shape[1]
+            probes = [(0, int(p % ni), int(p // ni), 1.0) for p in pick]
+        s.set_observer(probes)
+        s.solve(0)
+        samples = []
+        t0 = time.perf_counter()
+        for k in range(steps // sample_every):
+            s.solve(sample_every, sync=False)
+            samples.append(s.sample_observers())
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        series[label] = np.array(samples)
+        res[label] = dict(updates_per_s=npts * 4 * steps / wall, ms_per_step=wall / steps * 1e3,
+                          time=s.clock()[1], dt=s.clock()[2])
+        s.close()
+        print(f"[bench] C3 {label}: {res[label]['updates_per_s']:.3e} updates/s", file=sys.stderr, flush=True)
+    # The reference's discretization of this configuration is not stable for
+    # long times: a wall mode next to the rear stagnation point grows until
+    # it dominates.  Errors are taken over the samples before the probes'
+    # max |p| (plain path) first exceeds 10x the pulse amplitude; the growth
+    # itself is reported.
+    ref = series["plain"]
+    pmax = np.max(np.abs(ref), axis=1)
+    bad = np.flatnonzero(~(pmax <= 1e-2))
+    nwin = int(bad[0]) if bad.size else len(pmax)
+    res["probe_p_max_per_sample"] = [float(v) for v in pmax[::10]]
+    res["window_samples"] = nwin
+    res["window_steps"] = nwin * sample_every
+    if nwin > 0:
+        r = ref[:nwin]
+        scale = float(np.max(np.abs(r)))
+        res["p_rel_err_fp32_vs_plain"] = float(np.max(np.abs(series["fp32"][:nwin] - r)) / scale)
+        res["p_rel_err_fp64_fused_vs_plain"] = float(np.max(np.abs(series["fp64"][:nwin] - r)) / scale)
+    full = np.isfinite(series["fp64"]).all()
+    res["p_rel_err_fp64_fused_vs_plain_all_steps"] = (
+        float(np.max(np.abs(series["fp64"] - ref)) / np.max(np.abs(ref))) if full else None)
+    return res
+
+
 def make_solver(inp, precision, path):
     from camille_amd import Solver
     s = Solver([dict(x=inp["x"], y=inp["y"], mean=inp["mean"])], simulation="CASE_MONOPOLE",
@@ -219,7 +305,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c5-steps", type=int, default=1000)
     ap.add_argument("--no-c5", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "plain", "c5"], default=None,
+    ap.add_argument("--c3-steps", type=int, default=5000)
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64", "plain", "c5", "c3"], default=None,
                     help="time one path only (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (HBM counter calibration for rocprofv3 --pmc)")
@@ -239,6 +327,9 @@ def main():
 
     if args.only == "c5":
         print(json.dumps({"c5": c5_leg(args.steps, args.warmup, 10)}))
+        return
+    if args.only == "c3":
+        print(json.dumps({"c3": c3_leg(args.steps, 50)}))
         return
     res = {}
     paths = [("fp32", FP32, PATH_FUSED, args.steps), ("fp64", FP64, PATH_FUSED, args.steps),
@@ -296,6 +387,7 @@ def main():
             traffic = None
 
     c5 = None if args.no_c5 else c5_leg(args.c5_steps, args.warmup, 50)
+    c3 = None if args.no_c3 else c3_leg(args.c3_steps, 50)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -340,6 +432,7 @@ def main():
                      "algorithmic_bytes_per_point": per_point, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
         "batched_ensemble_c5": c5,
+        "overset_cylinder_c3": c3,
     }
     print(json.dumps(out))
 

@@ -57,6 +57,8 @@ void free_block(Block &b) {
   dfree(b.mean);
   dfree(b.mgrad);
   dfree(b.rhobar);
+  dfree(b.mc64);
+  dfree(b.mc32);
   dfree(b.rho64);
   dfree(b.rho32);
   for (int k = 0; k < 4; k++) {
@@ -81,6 +83,7 @@ void free_block(Block &b) {
   dfree(b.agrad);
   dfree(b.obs_l);
   dfree(b.obs_w);
+  dfree(b.obs_buf);
   dfree(b.ckpt);
   dfree(b.samp);
 }
@@ -867,6 +870,30 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
           HIPCHK(hipMemcpyAsync(b.mgrad + (size_t)(2 * v + d) * n, b.dflux, sizeof(double) * n,
                                 hipMemcpyDeviceToDevice, s));
         }
+      // coefficient fields of a non-uniform mean for the fused stage kernels
+      if (!b.uniform_mean) {
+        std::vector<double> g(2 * NV * (size_t)n), mc(MC_COUNT * (size_t)n);
+        HIPCHK(hipMemcpyAsync(g.data(), b.mgrad, sizeof(double) * g.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const double gam = cfg->gamma, gi = 1.0 / gam;
+        for (int p = 0; p < n; p++) {
+          const double pb = in.mean[(size_t)IP * n + p];
+          const double rb = std::pow(gam * pb, gi) * std::exp(-in.mean[(size_t)IS * n + p]);
+          auto G = [&](int v, int d) { return g[(size_t)(2 * v + d) * n + p]; };
+          const double f[MC_COUNT] = {in.mean[(size_t)IUX * n + p], in.mean[(size_t)IUY * n + p], gam * pb, 1.0 / rb,
+                                      G(IS, 0), G(IS, 1), G(IUX, 0), G(IUX, 1), G(IUY, 0), G(IUY, 1), G(IP, 0),
+                                      G(IP, 1), rb * gi / pb, rb};
+          for (int c = 0; c < MC_COUNT; c++) mc[(size_t)c * n + p] = f[c];
+        }
+        if (is_fp32(h)) {
+          std::vector<float> m32(mc.begin(), mc.end());
+          HIPCHK(dalloc(&b.mc32, mc.size()));
+          HIPCHK(hipMemcpy(b.mc32, m32.data(), sizeof(float) * m32.size(), hipMemcpyHostToDevice));
+        } else {
+          HIPCHK(dalloc(&b.mc64, mc.size()));
+          HIPCHK(hipMemcpy(b.mc64, mc.data(), sizeof(double) * mc.size(), hipMemcpyHostToDevice));
+        }
+      }
       // sponges and other boundaries (io.cpp:957-1369)
       for (int q = 0; q < in.nboundary; q++) {
         const lee_boundary &r = in.boundary[q];
@@ -1330,21 +1357,41 @@ int lee_set_observer(lee_handle *h, int nobs, const int *block, const int *index
     Block &b = h->b[ib];
     std::vector<int> l;
     std::vector<double> w;
+    b.obs_pos.clear();
     for (int q = 0; q < nobs; q++) {
       if (block[q] < 0 || block[q] >= h->nblk) return fail(h, LEE_EINVAL, "observer in a bad block");
       if (block[q] != ib) continue;
       if (index[q] < 0 || index[q] >= b.n) return fail(h, LEE_EINVAL, "observer index outside its block");
       l.push_back(index[q]);
       w.push_back(weight[q]);
+      b.obs_pos.push_back(q);
     }
     dfree(b.obs_l);
     dfree(b.obs_w);
+    dfree(b.obs_buf);
     b.nobs = (int)l.size();
+    h->nobs_total = nobs;
     if (b.nobs == 0) continue;
     HIPCHK(dalloc(&b.obs_l, l.size()));
     HIPCHK(dalloc(&b.obs_w, w.size()));
+    HIPCHK(dalloc(&b.obs_buf, l.size()));
     HIPCHK(hipMemcpy(b.obs_l, l.data(), sizeof(int) * l.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b.obs_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+  }
+  return LEE_OK;
+}
+
+int lee_sample_observers(lee_handle *h, double *out) {
+  if (!h || !out) return fail(h, LEE_EINVAL, "bad argument");
+  std::vector<double> tmp;
+  for (int ib = 0; ib < h->nblk; ib++) {
+    Block &b = h->b[ib];
+    if (b.nobs == 0) continue;
+    HIPCHK(observer_gather(h->stream, h, b, b.obs_buf));
+    tmp.resize(b.nobs);
+    HIPCHK(hipMemcpyAsync(tmp.data(), b.obs_buf, sizeof(double) * b.nobs, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int q = 0; q < b.nobs; q++) out[b.obs_pos[q]] = tmp[q];
   }
   return LEE_OK;
 }

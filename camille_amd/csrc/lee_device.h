@@ -86,6 +86,7 @@ struct StageArgs {
   const double *met, *jac;
   const int8_t *iblank, *code;  // holes only
   const double *mean, *mgrad, *rhob;   // non-uniform mean only
+  const T *mc;         // non-uniform mean: MC_COUNT coefficient fields
   const T *rho_in;     // stale rho' (first stage, non-uniform mean)
   T *rho_out;          // rho' of the new y_{n+1} (last stage, non-uniform mean)
   double u0, v0, gp0, rhob0, gamma, gi;

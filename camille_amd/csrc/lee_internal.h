@@ -25,6 +25,10 @@ constexpr int MAX_OV_BLOCKS = 16;    // blocks of an overset run (per-block tabl
 //   2 + nb + k   right-closure row k, anchored k points to the right (mirrored)
 constexpr int CODE_INTERIOR = 0, CODE_ZERO = 1, CODE_LEFT = 2;
 constexpr int STAGE_FIRST = 0, STAGE_MID = 1, STAGE_LAST = 2;  // RK stage kinds
+// Mean-flow coefficient fields of a non-uniform base state, in the bulk
+// type: u, v, gamma p, 1/rho, d(s, u, v, p)/d(x, y), rho gi / p, rho.
+enum { MC_U, MC_V, MC_GP, MC_IRB, MC_GSX, MC_GSY, MC_GUX, MC_GUY, MC_GVX, MC_GVY, MC_GPX, MC_GPY, MC_RP, MC_RB,
+       MC_COUNT };
 
 // One 1-D operator (interior stencil + closure rows), reference tables of
 // spatial_discretization.cpp:28-224.
@@ -72,6 +76,9 @@ struct Block {
   std::vector<int8_t> h_iblank, h_code_fd, h_code_flt;
   // base state (FP64): mean 5n, mean gradient 5*2n, rho_bar n
   double *mean = nullptr, *mgrad = nullptr, *rhobar = nullptr;
+  // non-uniform mean: MC_COUNT coefficient fields in the bulk type
+  double *mc64 = nullptr;
+  float *mc32 = nullptr;
   // aux density rho' carried between steps (stale after the filter)
   double *rho64 = nullptr;
   float *rho32 = nullptr;
@@ -108,6 +115,8 @@ struct Block {
   int nobs = 0;
   int *obs_l = nullptr;
   double *obs_w = nullptr;
+  std::vector<int> obs_pos;    // position of each observer in the caller's list
+  double *obs_buf = nullptr;   // device samples of the current state
   // checkpoints of an objective run: slots of (state 5n + aux density n)
   // in the bulk type; the last slot holds the state at the end of the run
   char *ckpt = nullptr;
@@ -184,6 +193,7 @@ struct lee_handle {
   int nblk = 0;
   std::vector<lee::Block> b;
   lee::Objective obj;
+  int nobs_total = 0;
   bool batch = false;          // independent solves (lee_batch_create)
   lee::BatchPlan bp;
   // pinned host staging of the batch schedule, two buffers used in turn;

@@ -491,25 +491,19 @@ __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int 
       rp -= v0 * fy[3];
       rp -= gp0 * fy[2];
     } else {
-      const T u0 = (T)a.mean[IUX * n + p], v0 = (T)a.mean[IUY * n + p];
-      const double pb = a.mean[IP * n + p];
-      const T rb = (T)a.rhob[p];
-      const T gsx = (T)a.mgrad[(2 * IS + 0) * n + p], gsy = (T)a.mgrad[(2 * IS + 1) * n + p];
-      const T gux = (T)a.mgrad[(2 * IUX + 0) * n + p], guy = (T)a.mgrad[(2 * IUX + 1) * n + p];
-      const T gvx = (T)a.mgrad[(2 * IUY + 0) * n + p], gvy = (T)a.mgrad[(2 * IUY + 1) * n + p];
-      const T gpx = (T)a.mgrad[(2 * IP + 0) * n + p], gpy = (T)a.mgrad[(2 * IP + 1) * n + p];
+      // per-point mean coefficients (MC_* order): u, v, gamma p, 1/rho,
+      // the eight mean gradients, rho gi / p and rho (all at the point)
+      const T *mc = a.mc;
+      const T u0 = mc[MC_U * n + p], v0 = mc[MC_V * n + p], gp = mc[MC_GP * n + p], irb = mc[MC_IRB * n + p];
+      const T gsx = mc[MC_GSX * n + p], gsy = mc[MC_GSY * n + p];
+      const T gux = mc[MC_GUX * n + p], guy = mc[MC_GUY * n + p];
+      const T gvx = mc[MC_GVX * n + p], gvy = mc[MC_GVY * n + p];
+      const T gpx = mc[MC_GPX * n + p], gpy = mc[MC_GPY * n + p];
       T rho;
-      if (KIND == STAGE_FIRST) {
-        rho = a.rho_in[p];
-      } else {
-        const double pbi = 1.0 / pb;
-        double q = (double)pv * a.gi * pbi - (double)sv;
-        q *= a.rhob[p];
-        rho = (T)q;
-      }
-      const T gp = (T)(a.gamma * pb);
+      if (KIND == STAGE_FIRST) rho = a.rho_in[p];
+      else rho = pv * mc[MC_RP * n + p] - sv * mc[MC_RB * n + p];
       const T g = (T)a.gamma;
-      const T ru_fac = rho / rb;
+      const T ru_fac = rho * irb;
       rs = -(u0 * fx[0]);
       rs -= uv * gsx;
       rs -= v0 * fy[0];
@@ -518,12 +512,12 @@ __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int 
       ru -= (uv + ru_fac * u0) * gux;
       ru -= v0 * fy[1];
       ru -= (vv + ru_fac * v0) * guy;
-      ru -= fx[3] / rb;
+      ru -= fx[3] * irb;
       rv = -(u0 * fx[2]);
       rv -= (uv + ru_fac * u0) * gvx;
       rv -= v0 * fy[2];
       rv -= (vv + ru_fac * v0) * gvy;
-      rv -= fy[3] / rb;
+      rv -= fy[3] * irb;
       rp = -(u0 * fx[3]);
       rp -= uv * gpx;
       rp -= gp * fx[1];
@@ -570,12 +564,7 @@ __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int 
       if (KIND != STAGE_LAST) a.yout[q] = outv;
       r[v] = accv;
     }
-    if (MEANF && KIND == STAGE_LAST) {
-      const double pbi = 1.0 / a.mean[IP * n + p];
-      double q = (double)r[3] * a.gi * pbi - (double)r[0];
-      q *= a.rhob[p];
-      a.rho_out[p] = (T)q;
-    }
+    if (MEANF && KIND == STAGE_LAST) a.rho_out[p] = r[3] * a.mc[MC_RP * n + p] - r[0] * a.mc[MC_RB * n + p];
   }
 }
 
@@ -1082,6 +1071,7 @@ static StageArgs<T> make_stage_args(const lee_handle *h, const Block &b, int sta
   a.mean = b.mean;
   a.mgrad = b.mgrad;
   a.rhob = b.rhobar;
+  a.mc = sizeof(T) == 8 ? (const T *)b.mc64 : (const T *)b.mc32;
   a.rho_in = rho;
   a.rho_out = rho;
   a.u0 = b.umean[IUX];

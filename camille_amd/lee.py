@@ -91,6 +91,7 @@ EXPORTS = [
     "lee_batch_create", "lee_get_solve_time", "lee_set_batch_launch", "lee_get_batch_launch",
     "lee_set_source_field", "lee_set_observer", "lee_objective_forward", "lee_objective_backward",
     "lee_get_source_gradient", "lee_get_state_gradient", "lee_compute_rhs_adjoint", "lee_apply_filter_adjoint",
+    "lee_sample_observers",
 ]
 
 _lib = None
@@ -147,6 +148,7 @@ def library():
     L.lee_get_state_gradient.argtypes = [vp, i, dp]
     L.lee_compute_rhs_adjoint.argtypes = [vp, i, dp, dp]
     L.lee_apply_filter_adjoint.argtypes = [vp, i, dp]
+    L.lee_sample_observers.argtypes = [vp, dp]
     _lib = L
     return L
 
@@ -449,6 +451,13 @@ class Solver:
         w = np.array([p[3] for p in pts], np.float64)
         self._keep += [blk, idx, w]
         self._check(self.L.lee_set_observer(self.h, len(pts), _ip(blk), _ip(idx), _dp(w)))
+        self._nobs = len(pts)
+
+    def sample_observers(self):
+        """p' at the observer points of set_observer, in its order."""
+        out = np.empty(getattr(self, "_nobs", 0))
+        self._check(self.L.lee_sample_observers(self.h, _dp(out)))
+        return out
 
     def objective_forward(self, nsteps, window_start, ckpt_every):
         J = ctypes.c_double()

@@ -7,9 +7,17 @@ coefficients w1 fastest, hole list).
 Two blocks:
   0  background Cartesian grid, uniform spacing;
   1  body-fitted O-grid around a cylinder of radius r_in at the origin,
-     theta_i = 2 pi (i - 2) / (ni - 5), r_j = r_in + j dr, so columns
+     x = r_j cos(theta_i), y = -r_j sin(theta_i) with
+     theta_i = theta_s + 2 pi (i - 2) / (ni - 5), r_j = r_in + j dr: clockwise in i
+     and outwards in j, so x_xi y_eta - x_eta y_xi > 0 like the background
+     (the reference's CFL evaluation multiplies by J and keeps the largest
+     value, so a left-handed block would drop out of it).  Columns
      i = 2 .. ni - 3 cover one period and the two columns on each side
-     repeat columns ni - 3, ni - 4 and 3, 4 exactly (seam).
+     repeat columns ni - 3, ni - 4 and 3, 4 exactly (seam).  The seam sits
+     at theta_s = 45 degrees by default, away from the stagnation points of
+     the mean flow: with the seam at the rear stagnation point the explicit
+     seam interpolation next to the one-sided closures of the block edges
+     feeds a fast-growing wall mode in the reference's discretization.
 
 Connectivity:
   * background points with r < r_hole are holes; points with
@@ -85,7 +93,7 @@ def potential_flow(x, y, U, a):
 
 
 def cylinder(nbg=(2049, 1025), extent=(-16.0, 16.0, -8.0, 8.0), nog=(513, 129), r_in=0.5, r_out=2.0,
-             r_hole=1.5, fringe=5, order=4):
+             r_hole=1.5, fringe=5, order=4, seam_angle=math.pi / 4):
     """Grids and connectivity of the cylinder problem.  nbg, nog = (ni, nj)
     of the background and the O-grid; `fringe` background cells of
     receivers around the holes.  Returns dict(blocks=[(x, y), (x, y)],
@@ -103,11 +111,11 @@ def cylinder(nbg=(2049, 1025), extent=(-16.0, 16.0, -8.0, 8.0), nog=(513, 129), 
     dr = (r_out - r_in) / (noj - 1)
     ii = np.arange(noi)
     ic = np.where(ii < 2, ii + period, np.where(ii > noi - 3, ii - period, ii))  # seam columns repeat
-    th = dth * (ic - 2)
+    th = seam_angle + dth * (ic - 2)
     rr = r_in + dr * np.arange(noj)
     TH, RR = np.meshgrid(th, rr)
     xo = RR * _ufunc("cos", TH)
-    yo = RR * _ufunc("sin", TH)
+    yo = -(RR * _ufunc("sin", TH))
 
     # background: holes and fringe receivers (polar donor search in the O-grid)
     rb = np.sqrt(xb * xb + yb * yb)
@@ -116,7 +124,7 @@ def cylinder(nbg=(2049, 1025), extent=(-16.0, 16.0, -8.0, 8.0), nog=(513, 129), 
     recv_b = (rb >= r_hole) & (rb < r_fr)
     jb, ib = np.nonzero(recv_b)
     xr, yr = xb[jb, ib], yb[jb, ib]
-    theta = np.mod(_ufunc("atan2", yr, xr), 2.0 * math.pi)
+    theta = np.mod(_ufunc("atan2", -yr, xr) - seam_angle, 2.0 * math.pi)
     fi = 2.0 + theta / dth
     fj = (np.sqrt(xr * xr + yr * yr) - r_in) / dr
     di, dj, cb = _stencils(fi, fj, noi, noj, m)

@@ -219,6 +219,9 @@ int lee_set_source_field(lee_handle *h, int iblock, const double *q);
  * sum_l w_l p'_k(l)^2, p'_k the pressure after step k at the observer
  * points (block[l], index[l]); nobs = 0 clears them. */
 int lee_set_observer(lee_handle *h, int nobs, const int *block, const int *index, const double *weight);
+/* p' of the current state at the observer points, in the order given to
+ * lee_set_observer (nobs doubles); for probe time series. */
+int lee_sample_observers(lee_handle *h, double *out);
 /* Runs nsteps steps from the current state and clock (like lee_solve),
  * keeping the state every ckpt_every steps on the device, and returns J. */
 int lee_objective_forward(lee_handle *h, int nsteps, int window_start, int ckpt_every, double *J);
