@@ -124,7 +124,9 @@ int enqueue_fused_step(lee_handle *h) {
     sc.a = h->a_next[st];
     source_factors(h->cfg, tn + h->c_i[st] * dt, sc.s1, sc.s2);
     if (h->ktiming) HIPCHK(mark(h));
-    for (int ib = 0; ib < h->nblk; ib++) HIPCHK(fused_stage(h->stream, h, h->b[ib], st, sc));
+    for (int ib = 0; ib < h->nblk; ib++)
+      HIPCHK(h->cfg.num_dim == 3 ? fused_stage3d(h->stream, h, h->b[ib], st, sc)
+                                 : fused_stage(h->stream, h, h->b[ib], st, sc));
     if (h->ktiming) {
       HIPCHK(mark(h));
       h->evclass.push_back(st == 0 ? 0 : (st == last ? 2 : 1));
@@ -150,7 +152,7 @@ int enqueue_fused_step(lee_handle *h) {
     Block &b = h->b[ib];
     if (h->cfg.filter_order > 0) {
       if (h->ktiming) HIPCHK(mark(h));
-      HIPCHK(fused_filter(h->stream, h, b, 1, 0));
+      HIPCHK(h->cfg.num_dim == 3 ? fused_filter3d(h->stream, h, b, 1, 0, 2) : fused_filter(h->stream, h, b, 1, 0));
       if (h->ktiming) {
         HIPCHK(mark(h));
         h->evclass.push_back(3);
@@ -505,7 +507,7 @@ int autotune_block(lee_handle *h, Block &b) {
 }
 
 int autotune(lee_handle *h) {
-  if (is_plain(h)) return LEE_OK;
+  if (is_plain(h) || h->cfg.num_dim == 3) return LEE_OK;  // the 3-D kernels have one tile shape
   for (int ib = 0; ib < h->nblk; ib++) {
     if (h->b[ib].tuned) continue;
     int rc = autotune_block(h, h->b[ib]);
@@ -532,6 +534,7 @@ void advance_clock(lee_handle *h) {
 
 int adjoint_check(lee_handle *h) {
   if (h->batch) return fail(h, LEE_EINVAL, "the adjoint runs on single handles, not batches");
+  if (h->cfg.num_dim == 3 && !h->adj3d) return fail(h, LEE_EINVAL, "no 3-D adjoint in this build");
   if (is_plain(h)) return fail(h, LEE_EINVAL, "the adjoint runs on the fused path");
   if (h->ov.nrecv > 0) return fail(h, LEE_EINVAL, "the adjoint does not cover overset interpolation");
   for (int ib = 0; ib < h->nblk; ib++) {
@@ -620,7 +623,9 @@ int adjoint_step(lee_handle *h, int k) {
 
 int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
   if (!c) return fail(nullptr, LEE_EINVAL, "null config");
-  if (c->num_dim != 2) return fail(nullptr, LEE_EINVAL, "only DIMENSION = 2 is supported");
+  if (c->num_dim != 2 && c->num_dim != 3) return fail(nullptr, LEE_EINVAL, "DIMENSION must be 2 or 3");
+  if (c->num_dim == 3 && c->path != LEE_PATH_FUSED)
+    return fail(nullptr, LEE_EINVAL, "3-D runs on the fused path (the reference has no 3-D Jacobian)");
   if (c->fd_order != 4)
     return fail(nullptr, LEE_EINVAL, "SPATIAL_SCHEME STANDARD_CENTRAL ORDER_ACCURACY must be 4");
   if (c->filter_order != 0 && c->filter_order != 4 && c->filter_order != 8)
@@ -641,13 +646,14 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
   for (int ib = 0; ib < nblocks; ib++) {
     const lee_block &b = blocks[ib];
     if (!b.xyz || !b.mean) return fail(nullptr, LEE_EINVAL, "block without coordinates or base state");
-    if (b.nk != 1) return fail(nullptr, LEE_EINVAL, "2-D blocks need nk = 1");
-    if (b.ni < minlen || b.nj < minlen) return fail(nullptr, LEE_EINVAL, "block too small for the stencils");
-    if ((long long)b.ni * b.nj > (1LL << 30)) return fail(nullptr, LEE_EINVAL, "block too large");
+    if (c->num_dim == 2 && b.nk != 1) return fail(nullptr, LEE_EINVAL, "2-D blocks need nk = 1");
+    if (b.ni < minlen || b.nj < minlen || (c->num_dim == 3 && b.nk < minlen))
+      return fail(nullptr, LEE_EINVAL, "block too small for the stencils");
+    if ((long long)b.ni * b.nj * b.nk > (1LL << 30)) return fail(nullptr, LEE_EINVAL, "block too large");
     for (int q = 0; q < b.nboundary; q++) {
       const lee_boundary &r = b.boundary[q];
       const int nd[3] = {b.ni, b.nj, b.nk};
-      if (r.dir < 0 || r.dir > 1 || (r.end != 0 && r.end != 1))
+      if (r.dir < 0 || r.dir >= c->num_dim || (r.end != 0 && r.end != 1))
         return fail(nullptr, LEE_EINVAL, "boundary with a bad direction or side");
       for (int d = 0; d < 3; d++)
         if (r.is[d] < 0 || r.ie[d] >= nd[d] || r.is[d] > r.ie[d])
@@ -664,6 +670,8 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
         default:
           return fail(nullptr, LEE_EINVAL, "unsupported boundary type " + std::to_string(r.type));
       }
+      if (c->num_dim == 3 && r.type != LEE_SPONGE_FREUND_AMBIENT)
+        return fail(nullptr, LEE_EINVAL, "3-D blocks take sponges only");
     }
   }
   return LEE_OK;
@@ -779,11 +787,12 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
       Block &b = h->b[ib];
       b.ni = blocks[ib].ni;
       b.nj = blocks[ib].nj;
-      b.nk = 1;
-      b.n = b.ni * b.nj;
+      b.nk = cfg->num_dim == 3 ? blocks[ib].nk : 1;
+      b.n = b.ni * b.nj * b.nk;
       b.h_iblank.assign(b.n, (int8_t)NOTBLANKED);
     }
     if (ov && ov->nrecv > 0) {
+      if (cfg->num_dim == 3) return fail(h, LEE_EINVAL, "overset interpolation is 2-D only");
       if (nblocks > MAX_OV_BLOCKS) return fail(h, LEE_EINVAL, "an overset run has at most 16 blocks");
       for (int q = 0; q < ov->nhole; q++) {
         const int bb = ov->hole_block[q], l = ov->hole_l[q];
@@ -808,10 +817,11 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
           if (in.mean[(size_t)v * n + p] != b.umean[v]) b.uniform_mean = false;
       }
       HIPCHK(dalloc(&b.xyz, 3 * (size_t)n));
-      HIPCHK(dalloc(&b.met, 4 * (size_t)n));
+      const int D = cfg->num_dim;
+      HIPCHK(dalloc(&b.met, (size_t)D * D * n));
       HIPCHK(dalloc(&b.jac, (size_t)n));
       HIPCHK(dalloc(&b.mean, NV * (size_t)n));
-      HIPCHK(dalloc(&b.mgrad, 2 * NV * (size_t)n));
+      HIPCHK(dalloc(&b.mgrad, (size_t)D * NV * n));
       HIPCHK(dalloc(&b.rhobar, (size_t)n));
       HIPCHK(dalloc(&b.rho64, (size_t)n));
       HIPCHK(dalloc(&b.dflux, (size_t)n));
@@ -839,10 +849,11 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
         HIPCHK(hipMemcpy(b.rhobar, rb.data(), sizeof(double) * n, hipMemcpyHostToDevice));
       }
       // metrics before any hole (simulation.cpp:207)
-      HIPCHK(metrics2d(s, b, h->fd));
-      b.h_met.resize(4 * (size_t)n);
+      if (D == 3) HIPCHK(metrics3d(s, b));
+      else HIPCHK(metrics2d(s, b, h->fd));
+      b.h_met.resize((size_t)D * D * n);
       b.h_jac.resize(n);
-      HIPCHK(hipMemcpyAsync(b.h_met.data(), b.met, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(b.h_met.data(), b.met, sizeof(double) * D * D * n, hipMemcpyDeviceToHost, s));
       HIPCHK(hipMemcpyAsync(b.h_jac.data(), b.jac, sizeof(double) * n, hipMemcpyDeviceToHost, s));
       // operator codes around holes
       if (b.has_holes) {
@@ -865,18 +876,32 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
       }
       // mean gradients after the holes exist (simulation.cpp:240, spatial:2250-2271)
       for (int v = 0; v < NV; v++)
-        for (int d = 0; d < 2; d++) {
-          HIPCHK(plain_take_derivative_xyz(s, b, h->fd, b.mean + (size_t)v * n, d));
-          HIPCHK(hipMemcpyAsync(b.mgrad + (size_t)(2 * v + d) * n, b.dflux, sizeof(double) * n,
+        for (int d = 0; d < D; d++) {
+          if (D == 3) HIPCHK(deriv_xyz3(s, b, b.mean + (size_t)v * n, b.dflux, d));
+          else HIPCHK(plain_take_derivative_xyz(s, b, h->fd, b.mean + (size_t)v * n, d));
+          HIPCHK(hipMemcpyAsync(b.mgrad + (size_t)(D * v + d) * n, b.dflux, sizeof(double) * n,
                                 hipMemcpyDeviceToDevice, s));
         }
       // coefficient fields of a non-uniform mean for the fused stage kernels
       if (!b.uniform_mean) {
-        std::vector<double> g(2 * NV * (size_t)n), mc(MC_COUNT * (size_t)n);
+        const int nmc = D == 3 ? MC3_COUNT : MC_COUNT;
+        std::vector<double> g((size_t)D * NV * n), mc((size_t)nmc * n);
         HIPCHK(hipMemcpyAsync(g.data(), b.mgrad, sizeof(double) * g.size(), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         const double gam = cfg->gamma, gi = 1.0 / gam;
-        for (int p = 0; p < n; p++) {
+        for (int p = 0; D == 3 && p < n; p++) {
+          const double pb = in.mean[(size_t)IP * n + p];
+          const double rb = std::pow(gam * pb, gi) * std::exp(-in.mean[(size_t)IS * n + p]);
+          mc[(size_t)MC3_U * n + p] = in.mean[(size_t)IUX * n + p];
+          mc[(size_t)MC3_V * n + p] = in.mean[(size_t)IUY * n + p];
+          mc[(size_t)MC3_W * n + p] = in.mean[(size_t)IUZ * n + p];
+          mc[(size_t)MC3_GP * n + p] = gam * pb;
+          mc[(size_t)MC3_IRB * n + p] = 1.0 / rb;
+          for (int q = 0; q < 15; q++) mc[(size_t)(MC3_G + q) * n + p] = g[(size_t)q * n + p];
+          mc[(size_t)MC3_RP * n + p] = rb * gi / pb;
+          mc[(size_t)MC3_RB * n + p] = rb;
+        }
+        for (int p = 0; D == 2 && p < n; p++) {
           const double pb = in.mean[(size_t)IP * n + p];
           const double rb = std::pow(gam * pb, gi) * std::exp(-in.mean[(size_t)IS * n + p]);
           auto G = [&](int v, int d) { return g[(size_t)(2 * v + d) * n + p]; };
@@ -1015,7 +1040,7 @@ int lee_set_solution(lee_handle *h, int ib, const double *sol) {
   hipStream_t s = h->stream;
   HIPCHK(hipStreamSynchronize(s));
   b.carry_w = false;
-  for (int p = 0; p < n; p++)
+  for (int p = 0; h->cfg.num_dim == 2 && p < n; p++)  // in 3-D w' is an active variable
     if (sol[(size_t)IUZ * n + p] != 0.0) {
       b.carry_w = true;
       break;
@@ -1237,11 +1262,11 @@ int lee_compute_rhs(lee_handle *h, int ib, const double *y, double time, double 
     if (is_fp32(h)) {
       HIPCHK(cast_state(s, b, NV, b.Y64[2], b.Y32[2], cnt));
       HIPCHK(hipMemsetAsync(b.Y32[3], 0, sizeof(float) * cnt, s));
-      HIPCHK(fused_stage(s, h, b, -1, sc));
+      HIPCHK(h->cfg.num_dim == 3 ? fused_stage3d(s, h, b, -1, sc) : fused_stage(s, h, b, -1, sc));
       HIPCHK(cast_back(s, b, NV, b.Y32[3], b.Y64[3], cnt));
     } else {
       HIPCHK(hipMemsetAsync(b.Y64[3], 0, sizeof(double) * cnt, s));
-      HIPCHK(fused_stage(s, h, b, -1, sc));
+      HIPCHK(h->cfg.num_dim == 3 ? fused_stage3d(s, h, b, -1, sc) : fused_stage(s, h, b, -1, sc));
     }
     HIPCHK(hipMemcpyAsync(rhs, b.Y64[3], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
   }
@@ -1262,12 +1287,12 @@ int lee_apply_filter(lee_handle *h, int ib, double *y) {
   } else if (is_fp32(h)) {
     HIPCHK(cast_state(s, b, NV, b.Y64[1], b.Y32[1], cnt));
     HIPCHK(cast_state(s, b, NV, b.Y64[1], b.Y32[2], cnt));
-    HIPCHK(fused_filter(s, h, b, 1, 2));
+    HIPCHK(h->cfg.num_dim == 3 ? fused_filter3d(s, h, b, 1, 2, 3) : fused_filter(s, h, b, 1, 2));
     HIPCHK(cast_back(s, b, NV, b.Y32[2], b.Y64[2], cnt));
     HIPCHK(hipMemcpyAsync(y, b.Y64[2], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
   } else {
     HIPCHK(hipMemcpyAsync(b.Y64[2], b.Y64[1], sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
-    HIPCHK(fused_filter(s, h, b, 1, 2));
+    HIPCHK(h->cfg.num_dim == 3 ? fused_filter3d(s, h, b, 1, 2, 3) : fused_filter(s, h, b, 1, 2));
     HIPCHK(hipMemcpyAsync(y, b.Y64[2], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
   }
   HIPCHK(hipStreamSynchronize(s));
@@ -1275,12 +1300,13 @@ int lee_apply_filter(lee_handle *h, int ib, double *y) {
 }
 
 int lee_take_derivative_xyz(lee_handle *h, int ib, const double *f, int dir_xyz, double *out) {
-  if (!h || !f || !out || ib < 0 || ib >= h->nblk || dir_xyz < 0 || dir_xyz > 1)
+  if (!h || !f || !out || ib < 0 || ib >= h->nblk || dir_xyz < 0 || dir_xyz >= h->cfg.num_dim)
     return fail(h, LEE_EINVAL, "bad argument");
   Block &b = h->b[ib];
   hipStream_t s = h->stream;
   HIPCHK(hipMemcpy(b.dsum, f, sizeof(double) * b.n, hipMemcpyHostToDevice));
-  HIPCHK(plain_take_derivative_xyz(s, b, h->fd, b.dsum, dir_xyz));
+  if (h->cfg.num_dim == 3) HIPCHK(deriv_xyz3(s, b, b.dsum, b.dflux, dir_xyz));
+  else HIPCHK(plain_take_derivative_xyz(s, b, h->fd, b.dsum, dir_xyz));
   HIPCHK(hipMemcpyAsync(out, b.dflux, sizeof(double) * b.n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return LEE_OK;
@@ -1305,10 +1331,15 @@ int lee_get_metrics(lee_handle *h, int ib, double *metrics, double *jac) {
   Block &b = h->b[ib];
   const size_t n = b.n;
   if (metrics) {
-    // metrics[l][i] for l, i < 3; the 2-D block fills the upper 2x2
+    // metrics[l][i] for l, i < 3; a 2-D block fills the upper 2x2
     std::fill(metrics, metrics + 9 * n, 0.0);
-    const int map[4] = {0, 1, 3, 4};
-    for (int q = 0; q < 4; q++) std::copy(b.h_met.begin() + q * n, b.h_met.begin() + (q + 1) * n, metrics + map[q] * n);
+    if (h->cfg.num_dim == 3) {
+      std::copy(b.h_met.begin(), b.h_met.end(), metrics);
+    } else {
+      const int map[4] = {0, 1, 3, 4};
+      for (int q = 0; q < 4; q++)
+        std::copy(b.h_met.begin() + q * n, b.h_met.begin() + (q + 1) * n, metrics + map[q] * n);
+    }
   }
   if (jac) std::copy(b.h_jac.begin(), b.h_jac.end(), jac);
   return LEE_OK;
@@ -1323,7 +1354,7 @@ int lee_get_iblank(lee_handle *h, int ib, int *iblank) {
 int lee_get_mean_gradient(lee_handle *h, int ib, double *grad) {
   if (!h || !grad || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
   Block &b = h->b[ib];
-  HIPCHK(hipMemcpy(grad, b.mgrad, sizeof(double) * 2 * NV * b.n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(grad, b.mgrad, sizeof(double) * h->cfg.num_dim * NV * b.n, hipMemcpyDeviceToHost));
   return LEE_OK;
 }
 

@@ -29,6 +29,8 @@ constexpr int STAGE_FIRST = 0, STAGE_MID = 1, STAGE_LAST = 2;  // RK stage kinds
 // type: u, v, gamma p, 1/rho, d(s, u, v, p)/d(x, y), rho gi / p, rho.
 enum { MC_U, MC_V, MC_GP, MC_IRB, MC_GSX, MC_GSY, MC_GUX, MC_GUY, MC_GVX, MC_GVY, MC_GPX, MC_GPY, MC_RP, MC_RB,
        MC_COUNT };
+// 3-D: u, v, w, gamma p, 1/rho, d(s, u, v, w, p)/d(x, y, z) (5 x 3), rho gi / p, rho.
+enum { MC3_U, MC3_V, MC3_W, MC3_GP, MC3_IRB, MC3_G, MC3_RP = MC3_G + 15, MC3_RB, MC3_COUNT };
 
 // One 1-D operator (interior stencil + closure rows), reference tables of
 // spatial_discretization.cpp:28-224.
@@ -56,7 +58,8 @@ struct BcFace {        // one boundary-condition face (BCs other than sponges)
 
 struct SourceBox {     // spatial factor of a physical source over its support
   int i0 = 0, i1 = -1, j0 = 0, j1 = -1;  // inclusive box; empty when i1 < i0
-  double *g = nullptr;                    // device (j-i order over the box)
+  int k0 = 0, k1 = 0;
+  double *g = nullptr;                    // device (k-j-i order over the box)
   std::vector<double> host_g;
 };
 
@@ -68,7 +71,7 @@ struct Block {
   double umean[NV] = {0};      // the constants when uniform
   // geometry (FP64)
   double *xyz = nullptr;       // 3n, kept for init
-  double *met = nullptr;       // 4n: xi_x, xi_y, eta_x, eta_y (2-D)
+  double *met = nullptr;       // 2-D 4n: xi_x, xi_y, eta_x, eta_y; 3-D 9n: m[l][i] at (3 l + i) n
   double *jac = nullptr;       // n
   int8_t *iblank = nullptr;    // n
   int8_t *code_fd = nullptr;   // 2n: xi codes then eta codes (holes only)
@@ -194,6 +197,7 @@ struct lee_handle {
   std::vector<lee::Block> b;
   lee::Objective obj;
   int nobs_total = 0;
+  bool adj3d = false;          // set when the 3-D adjoint kernels are present
   bool batch = false;          // independent solves (lee_batch_create)
   lee::BatchPlan bp;
   // pinned host staging of the batch schedule, two buffers used in turn;
@@ -266,6 +270,11 @@ hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_
 hipError_t metrics2d(hipStream_t s, Block &b, const OpTable &fd);
 
 hipError_t fused_stage(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc);
+// 3-D (lee_kernels3d.hip)
+hipError_t metrics3d(hipStream_t s, Block &b);
+hipError_t deriv_xyz3(hipStream_t s, const Block &b, const double *f, double *out, int d);
+hipError_t fused_stage3d(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc);
+hipError_t fused_filter3d(hipStream_t s, lee_handle *h, Block &b, int src, int dst, int tmp);
 hipError_t fused_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst);
 hipError_t batch_prepare(hipStream_t s, lee_handle *h);
 hipError_t batch_stage(hipStream_t s, lee_handle *h, int stage);

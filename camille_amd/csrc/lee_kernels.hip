@@ -616,129 +616,6 @@ __global__ void __launch_bounds__(TPB)
   }
 }
 
-template <typename T, bool HOLES, int ORDER, int NVAR, int TY, typename AT>
-__device__ __forceinline__ void filter_tile(const AT &a, int tile, FilterLds<T, NVAR, TY> &L) {
-  constexpr int NR = TY + 2 * FL_H, KR = NR / 4;
-  constexpr int NB = ORDER / 2;
-  static_assert(NR == 4 * KR && FL_H == 4, "tile rows and the halo come in fours");
-  auto &in = L.in;
-  auto &mid = L.mid;
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
-  const int ni = a.ni, nj = a.nj, n = ni * nj;
-  const bool xtra = tx < 2 * FL_H;
-  const bool inner = !HOLES && i0 >= FL_H && i0 + FL_TX + FL_H <= ni && j0 >= FL_H && j0 + TY + FL_H <= nj;
-
-  T ra[NVAR][KR], rb[NVAR][KR];
-  if (inner) {
-#pragma unroll
-    for (int k = 0; k < KR; k++) {
-      const int q0 = (j0 - FL_H + ty + 4 * k) * ni + (i0 - FL_H + tx);
-#pragma unroll
-      for (int q = 0; q < NVAR; q++) {
-        const T *src = a.src + a.vars[q] * n;
-        ra[q][k] = src[q0];
-        rb[q][k] = xtra ? src[q0 + FL_TX] : (T)0;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < KR; k++) {
-      const int gj = j0 - FL_H + ty + 4 * k;
-      const bool rowok = gj >= 0 && gj < nj;
-      const int gia = i0 - FL_H + tx, gib = i0 - FL_H + FL_TX + tx;
-      const bool oka = rowok && gia >= 0 && gia < ni, okb = rowok && xtra && gib < ni;
-#pragma unroll
-      for (int q = 0; q < NVAR; q++) {
-        const T *src = a.src + a.vars[q] * n;
-        ra[q][k] = oka ? src[gj * ni + gia] : (T)0;
-        rb[q][k] = okb ? src[gj * ni + gib] : (T)0;
-      }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < KR; k++)
-#pragma unroll
-    for (int q = 0; q < NVAR; q++) {
-      in[q][ty + 4 * k][tx] = ra[q][k];
-      if (xtra) in[q][ty + 4 * k][FL_TX + tx] = rb[q][k];
-    }
-  __syncthreads();
-  // xi-sweep of every region row into registers: each item is 4 consecutive
-  // columns of one row and variable, from 12 values read as three 4-vectors
-  constexpr int NG = FL_TX / 4;
-  constexpr int NXI = NVAR * NR * NG;
-  constexpr int NS = (NXI + TPB - 1) / TPB;
-  const int tid = ty * FL_TX + tx;
-  Vec4<T> xo[NS];
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    const int it = tid + s * TPB;
-    if (NXI % TPB != 0 && it >= NXI) break;
-    const int g = it % NG, r = (it / NG) % NR, q = it / (NG * NR);
-    T v[12];
-#pragma unroll
-    for (int h = 0; h < 3; h++) {
-      const Vec4<T> x = *reinterpret_cast<const Vec4<T> *>(&in[q][r][4 * g + 4 * h]);
-      v[4 * h] = x.x;
-      v[4 * h + 1] = x.y;
-      v[4 * h + 2] = x.z;
-      v[4 * h + 3] = x.w;
-    }
-    const int gj = j0 - FL_H + r;
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-      const int gi = i0 + 4 * g + m;
-      int code = CODE_INTERIOR;
-      bool ok = true;
-      if (!inner) {
-        ok = gj >= 0 && gj < nj && gi < ni;
-        if (ok) code = HOLES ? (int)a.code[gj * ni + gi] : default_code(gi, ni, NB);
-      }
-      xo[s][m] = ok ? filter_row<T, ORDER>(&v[m + FL_H], 1, code) : (T)0;
-    }
-  }
-  __syncthreads();  // every read of the input tile is done: mid overwrites it
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    const int it = tid + s * TPB;
-    if (NXI % TPB != 0 && it >= NXI) break;
-    const int g = it % NG, r = (it / NG) % NR, q = it / (NG * NR);
-    *reinterpret_cast<Vec4<T> *>(&mid[q][r][4 * g]) = xo[s];
-  }
-  __syncthreads();
-  // eta-sweep, blend with the unfiltered values kept in registers since the
-  // load, and hole reset.  Thread (tx, ty) loaded region row ty + 4k, i.e.
-  // tile row ty + 4(k - 1), at region column tx (block column i0 - 4 + tx)
-  // in ra[.][k] and column 64 + tx in rb[.][k]: the value at block column
-  // i0 + tx comes from lane tx + 4, or for tx >= 60 from rb of lane tx - 60.
-  const int gi = i0 + tx;
-  const int src_lane = tx < FL_TX - FL_H ? tx + FL_H : tx - (FL_TX - FL_H);
-#pragma unroll
-  for (int k = 0; k < TY / 4; k++) {
-    T orig[NVAR];
-#pragma unroll
-    for (int q = 0; q < NVAR; q++) {
-      const T va = __shfl(ra[q][k + 1], src_lane, FL_TX);
-      const T vb = __shfl(rb[q][k + 1], src_lane, FL_TX);
-      orig[q] = tx < FL_TX - FL_H ? va : vb;
-    }
-    const int r = ty + 4 * k;
-    const int gj = j0 + r;
-    if (!inner && (gj >= nj || gi >= ni)) continue;
-    const int p = gj * ni + gi;
-    const int code = inner ? CODE_INTERIOR : (HOLES ? (int)a.code[n + p] : default_code(gj, nj, NB));
-    const bool hole = HOLES && a.iblank[p] == BLANKED;
-#pragma unroll
-    for (int q = 0; q < NVAR; q++) {
-      const T f = filter_row<T, ORDER>(&mid[q][r + FL_H][tx], FL_TX, code);
-      T out = a.rem * orig[q] + a.blend * f;
-      if (hole) out = 0;
-      a.dst[a.vars[q] * n + p] = out;
-    }
-  }
-}
-
 template <typename T, bool HOLES, int ORDER, int NVAR, int TY>
 __global__ void __launch_bounds__(TPB) k_fused_filter(FilterArgs<T> a) {
   __shared__ FilterLds<T, NVAR, TY> L;

@@ -1,0 +1,481 @@
+// 3-D kernels of the fused LEE time advance (gfx950).
+//
+// The reference computes 3-D metrics but stops at the 3-D Jacobian
+// (metrics.cpp:292), so it never runs a 3-D case; these kernels follow
+// its 3-D operator definitions (take_derivative_xyz over xi, eta, zeta,
+// spatial:1802-1826; the LEE right-hand side summed over three directions,
+// spatial:2053-2130; filter sweeps xi, eta, zeta in turn, spatial:2191-2246)
+// and are checked against the 2-D path on z-invariant problems and by axis
+// permutations.
+//
+// Stage kernel: a 64 x 4 tile of (i, j) columns marches through a chunk of
+// z planes.  Each plane's tile plus its 2-point (i, j) halo is staged in
+// LDS for the xi and eta derivatives; the zeta derivative comes from a
+// 5-plane register window of the thread's own column.  Filter: one pass of
+// the 2-D xi-eta tile filter per plane, then a column pass in zeta with
+// the blend.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "lee_device.h"
+#include "lee_internal.h"
+
+namespace lee {
+
+namespace {
+
+constexpr int S3_TY = 4;      // tile rows in j
+constexpr int Z_CHUNK = 32;   // planes per z chunk
+
+inline unsigned nb3(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// ---------------------------------------------------------------- set-up
+
+// FD derivative along index direction dir (0 xi, 1 eta, 2 zeta), rows of
+// StandardCentral OA4 with the block-edge closures.
+__global__ void __launch_bounds__(TPB) k_op_dir3(const double *__restrict__ f, double *__restrict__ df, int ni,
+                                                 int nj, int nk, int dir) {
+  const size_t n = (size_t)ni * nj * nk;
+  const size_t p = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (p >= n) return;
+  const int i = (int)(p % ni), j = (int)((p / ni) % nj), k = (int)(p / ((size_t)ni * nj));
+  const int c = dir == 0 ? i : (dir == 1 ? j : k), len = dir == 0 ? ni : (dir == 1 ? nj : nk);
+  const int st = dir == 0 ? 1 : (dir == 1 ? ni : ni * nj);
+  df[p] = fd_row<double>(f + p, st, default_code(c, len, 2));
+}
+
+// m[l][i] = (1/J) d xi_l / d x_i as the adjugate of A = d x_i / d xi_l,
+// J = 1 / det A.  xx holds A row-major: xx[(3 i + l) n + p].
+__global__ void __launch_bounds__(TPB) k_metrics3d(const double *__restrict__ xx, double *__restrict__ met,
+                                                   double *__restrict__ jac, size_t n) {
+  const size_t p = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (p >= n) return;
+  double A[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int l = 0; l < 3; l++) A[i][l] = xx[(size_t)(3 * i + l) * n + p];
+  double m[3][3];
+  m[0][0] = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+  m[0][1] = -(A[0][1] * A[2][2] - A[0][2] * A[2][1]);
+  m[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+  m[1][0] = -(A[1][0] * A[2][2] - A[1][2] * A[2][0]);
+  m[1][1] = A[0][0] * A[2][2] - A[0][2] * A[2][0];
+  m[1][2] = -(A[0][0] * A[1][2] - A[0][2] * A[1][0]);
+  m[2][0] = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+  m[2][1] = -(A[0][0] * A[2][1] - A[0][1] * A[2][0]);
+  m[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+  for (int l = 0; l < 3; l++)
+    for (int i = 0; i < 3; i++) met[(size_t)(3 * l + i) * n + p] = m[l][i];
+  const double det = A[0][0] * m[0][0] + A[0][1] * m[1][0] + A[0][2] * m[2][0];
+  jac[p] = 1.0 / det;
+}
+
+// take_derivative_xyz in 3-D: out = (sum_l m[l][d] d f / d xi_l) * J.
+__global__ void __launch_bounds__(TPB) k_deriv_xyz3(const double *__restrict__ f, double *__restrict__ out,
+                                                    const double *__restrict__ met, const double *__restrict__ jac,
+                                                    int ni, int nj, int nk, int d) {
+  const size_t n = (size_t)ni * nj * nk;
+  const size_t p = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (p >= n) return;
+  const int i = (int)(p % ni), j = (int)((p / ni) % nj), k = (int)(p / ((size_t)ni * nj));
+  const double d0 = fd_row<double>(f + p, 1, default_code(i, ni, 2));
+  const double d1 = fd_row<double>(f + p, ni, default_code(j, nj, 2));
+  const double d2 = fd_row<double>(f + p, ni * nj, default_code(k, nk, 2));
+  double sum = 0.0;
+  sum += met[(size_t)(0 + d) * n + p] * d0;
+  sum += met[(size_t)(3 + d) * n + p] * d1;
+  sum += met[(size_t)(6 + d) * n + p] * d2;
+  out[p] = sum * jac[p];
+}
+
+// ------------------------------------------------------------ stage (3-D)
+
+template <typename T>
+struct Zone3 {
+  int i0, i1, j0, j1, k0, k1, nzi, nzj;
+  const T *sig;
+};
+
+template <typename T>
+struct Stage3Args {
+  int ni, nj, nk, ntx, nty, nzc;
+  const T *yin, *yn;
+  T *acc, *yout;
+  const double *met, *jac;
+  const T *mc;                      // non-uniform mean: MC3_COUNT fields
+  const T *rho_in;
+  T *rho_out;
+  double u0, v0, w0, gp0, rhob0, gamma;
+  int nz;
+  Zone3<T> z[MAX_ZONES];
+  int si0, si1, sj0, sj1, sk0, sk1;
+  const double *g;
+  double s1, s2;
+  T h, b, a;
+};
+
+// One RK stage on a 64 x 4 column tile through a chunk of Z_CHUNK planes.
+template <typename T, bool MEANF, int KIND>
+__global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
+  constexpr int RS = ST_TX + 2 * ST_H, NR = S3_TY + 2 * ST_H;
+  __shared__ T s[5][NR][RS];
+  const int tile = blockIdx.x;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * S3_TY;
+  const int k0 = blockIdx.y * Z_CHUNK, k1 = min(k0 + Z_CHUNK, a.nk);
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const size_t plane = (size_t)ni * nj, n = plane * nk;
+  const int gi = i0 + tx, gj = j0 + ty;
+  const bool own = gi < ni && gj < nj;
+  const size_t col = own ? (size_t)gj * ni + gi : 0;
+  // register window of the own column: planes k - 2 .. k + 2
+  T w[5][5];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int kk = k0 - 2 + q;
+    const bool ok = own && kk >= 0 && kk < nk;
+#pragma unroll
+    for (int v = 0; v < 5; v++) w[v][q] = ok ? a.yin[v * n + (size_t)kk * plane + col] : (T)0;
+  }
+  for (int k = k0; k < k1; k++) {
+    {
+      const int kk = k + 2;
+      const bool ok = own && kk < nk;
+#pragma unroll
+      for (int v = 0; v < 5; v++) w[v][4] = ok ? a.yin[v * n + (size_t)kk * plane + col] : (T)0;
+    }
+    // plane k with its (i, j) halo into LDS: rows ty + 4m, columns tx and 64 + tx
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+      const int r = ty + 4 * m;
+      const int rj = j0 - ST_H + r;
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+        const int c = half == 0 ? tx : ST_TX + tx;
+        if (half == 1 && tx >= 2 * ST_H) continue;
+        const int ri = i0 - ST_H + c;
+        const bool ok = rj >= 0 && rj < nj && ri >= 0 && ri < ni;
+        const size_t q = (size_t)k * plane + (size_t)(ok ? rj : 0) * ni + (ok ? ri : 0);
+#pragma unroll
+        for (int v = 0; v < 5; v++) s[v][r][c] = ok ? a.yin[v * n + q] : (T)0;
+      }
+    }
+    __syncthreads();
+    if (own) {
+      const size_t p = (size_t)k * plane + col;
+      const int cx = default_code(gi, ni, 2), cy = default_code(gj, nj, 2), cz = default_code(k, nk, 2);
+      T d[5][3], yc[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const T *sp = &s[v][ty + ST_H][tx + ST_H];
+        yc[v] = w[v][2];
+        d[v][0] = fd_row<T>(sp, 1, cx);
+        d[v][1] = fd_row<T>(sp, RS, cy);
+        d[v][2] = fd_row<T>(&w[v][2], 1, cz);
+      }
+      double m[3][3];
+#pragma unroll
+      for (int l = 0; l < 3; l++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) m[l][i] = a.met[(size_t)(3 * l + i) * n + p];
+      const double J = a.jac[p];
+      T f[5][3];  // f[v][x_i] = d var / d x_i
+#pragma unroll
+      for (int v = 0; v < 5; v++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          double acc = 0.0;
+          acc += m[0][i] * (double)d[v][0];
+          acc += m[1][i] * (double)d[v][1];
+          acc += m[2][i] * (double)d[v][2];
+          f[v][i] = (T)(acc * J);
+        }
+      T r[5];
+      const T up[3] = {yc[IUX], yc[IUY], yc[IUZ]};
+      if (!MEANF) {
+        const T ub[3] = {(T)a.u0, (T)a.v0, (T)a.w0};
+        const T gp = (T)a.gp0, irb = (T)(1.0 / a.rhob0);
+        r[IS] = 0;
+        for (int dd = 0; dd < 3; dd++) r[IS] -= ub[dd] * f[IS][dd];
+#pragma unroll
+        for (int vi = 0; vi < 3; vi++) {
+          T rv = 0;
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++) rv -= ub[dd] * f[IUX + vi][dd];
+          rv -= f[IP][vi] * irb;
+          r[IUX + vi] = rv;
+        }
+        T rp = 0;
+#pragma unroll
+        for (int dd = 0; dd < 3; dd++) {
+          rp -= ub[dd] * f[IP][dd];
+          rp -= gp * f[IUX + dd][dd];
+        }
+        r[IP] = rp;
+      } else {
+        const T *mc = a.mc;
+        const T ub[3] = {mc[MC3_U * n + p], mc[MC3_V * n + p], mc[MC3_W * n + p]};
+        const T gp = mc[MC3_GP * n + p], irb = mc[MC3_IRB * n + p];
+        auto G = [&](int v, int dd) { return mc[(MC3_G + 3 * v + dd) * n + p]; };
+        T rho;
+        if (KIND == STAGE_FIRST) rho = a.rho_in[p];
+        else rho = yc[IP] * mc[MC3_RP * n + p] - yc[IS] * mc[MC3_RB * n + p];
+        const T fr = rho * irb;
+        const T g = (T)a.gamma;
+        T rs = 0;
+#pragma unroll
+        for (int dd = 0; dd < 3; dd++) {
+          rs -= ub[dd] * f[IS][dd];
+          rs -= up[dd] * G(IS, dd);
+        }
+        r[IS] = rs;
+#pragma unroll
+        for (int vi = 0; vi < 3; vi++) {
+          T rv = 0;
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++) {
+            rv -= ub[dd] * f[IUX + vi][dd];
+            rv -= (up[dd] + fr * ub[dd]) * G(IUX + vi, dd);
+          }
+          rv -= f[IP][vi] * irb;
+          r[IUX + vi] = rv;
+        }
+        T rp = 0;
+#pragma unroll
+        for (int dd = 0; dd < 3; dd++) {
+          rp -= ub[dd] * f[IP][dd];
+          rp -= up[dd] * G(IP, dd);
+          rp -= gp * f[IUX + dd][dd];
+          rp -= g * yc[IP] * G(IUX + dd, dd);
+        }
+        r[IP] = rp;
+      }
+      if (gi >= a.si0 && gi <= a.si1 && gj >= a.sj0 && gj <= a.sj1 && k >= a.sk0 && k <= a.sk1) {
+        const double gval =
+            a.g[((size_t)(k - a.sk0) * (a.sj1 - a.sj0 + 1) + (gj - a.sj0)) * (a.si1 - a.si0 + 1) + (gi - a.si0)];
+        r[IP] += (T)(a.s1 * gval * a.s2);
+      }
+#pragma unroll 1
+      for (int z = 0; z < a.nz; z++) {
+        const Zone3<T> &zz = a.z[z];
+        if (gi >= zz.i0 && gi <= zz.i1 && gj >= zz.j0 && gj <= zz.j1 && k >= zz.k0 && k <= zz.k1) {
+          const T sg = zz.sig[((size_t)(k - zz.k0) * zz.nzj + (gj - zz.j0)) * zz.nzi + (gi - zz.i0)];
+#pragma unroll
+          for (int v = 0; v < 5; v++) r[v] += -sg * yc[v];
+        }
+      }
+      T accv[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const size_t q = v * n + p;
+        const T rh = r[v] * a.h;
+        const T acc_in = KIND == STAGE_FIRST ? yc[v] : a.acc[q];
+        accv[v] = acc_in + a.b * rh;
+        a.acc[q] = accv[v];
+        if (KIND != STAGE_LAST) {
+          const T yn = KIND == STAGE_FIRST ? yc[v] : a.yn[q];
+          a.yout[q] = yn + a.a * rh;
+        }
+      }
+      if (MEANF && KIND == STAGE_LAST)
+        a.rho_out[p] = accv[IP] * a.mc[MC3_RP * n + p] - accv[IS] * a.mc[MC3_RB * n + p];
+    }
+    __syncthreads();  // the next plane overwrites the LDS tile
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) w[v][q] = w[v][q + 1];
+  }
+}
+
+// ------------------------------------------------------------ filter (3-D)
+
+// Pass 1: the xi and eta sweeps of plane blockIdx.y (the 2-D tile filter
+// with blend 1) into tmp.
+template <typename T, int ORDER, int TY>
+__global__ void __launch_bounds__(TPB) k_filter3d_plane(FilterArgs<T> a, size_t plane) {
+  __shared__ FilterLds<T, 5, TY> L;
+  const int tile = xcd_tile(blockIdx.x, a.chunk);
+  if (tile >= a.ntiles) return;
+  FilterArgs<T> b = a;
+  b.src = a.src + (size_t)blockIdx.y * plane;
+  b.dst = a.dst + (size_t)blockIdx.y * plane;
+  filter_tile<T, false, ORDER, 5, TY>(b, tile, L);
+}
+
+// Pass 2: the zeta sweep of tmp along each (i, j) column through a chunk of
+// planes (9-plane register window), blend with y, into dst.
+template <typename T, int ORDER>
+__global__ void __launch_bounds__(TPB) k_filter3d_z(const T *__restrict__ tmp, const T *__restrict__ y,
+                                                    T *__restrict__ dst, int ni, int nj, int nk, T rem, T blend) {
+  constexpr int H = FL_H, NB = ORDER / 2;
+  const size_t plane = (size_t)ni * nj, n = plane * nk;
+  const size_t col = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (col >= plane) return;
+  const int k0 = blockIdx.y * Z_CHUNK, k1 = min(k0 + Z_CHUNK, nk);
+  T w[5][2 * H + 1];
+#pragma unroll
+  for (int q = 0; q < 2 * H; q++) {
+    const int kk = k0 - H + q;
+    const bool ok = kk >= 0 && kk < nk;
+#pragma unroll
+    for (int v = 0; v < 5; v++) w[v][q] = ok ? tmp[v * n + (size_t)kk * plane + col] : (T)0;
+  }
+  for (int k = k0; k < k1; k++) {
+    const int kk = k + H;
+    const bool ok = kk < nk;
+#pragma unroll
+    for (int v = 0; v < 5; v++) w[v][2 * H] = ok ? tmp[v * n + (size_t)kk * plane + col] : (T)0;
+    const int code = default_code(k, nk, NB);
+    const size_t p = (size_t)k * plane + col;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const T f = filter_row<T, ORDER>(&w[v][H], 1, code);
+      dst[v * n + p] = rem * y[v * n + p] + blend * f;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+#pragma unroll
+      for (int q = 0; q < 2 * H; q++) w[v][q] = w[v][q + 1];
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+hipError_t metrics3d(hipStream_t s, Block &b) {
+  double *xx = nullptr;
+  const size_t n = b.n;
+  hipError_t e = hipMalloc((void **)&xx, sizeof(double) * 9 * n);
+  if (e != hipSuccess) return e;
+  for (int i = 0; i < 3; i++)
+    for (int l = 0; l < 3; l++)
+      hipLaunchKernelGGL(k_op_dir3, dim3(nb3(n)), dim3(TPB), 0, s, b.xyz + (size_t)i * n, xx + (size_t)(3 * i + l) * n,
+                         b.ni, b.nj, b.nk, l);
+  hipLaunchKernelGGL(k_metrics3d, dim3(nb3(n)), dim3(TPB), 0, s, xx, b.met, b.jac, n);
+  e = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(s);
+  (void)hipFree(xx);
+  return e != hipSuccess ? e : e2;
+}
+
+hipError_t deriv_xyz3(hipStream_t s, const Block &b, const double *f, double *out, int d) {
+  hipLaunchKernelGGL(k_deriv_xyz3, dim3(nb3(b.n)), dim3(TPB), 0, s, f, out, b.met, b.jac, b.ni, b.nj, b.nk, d);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t stage3d_t(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc, T *const *Y,
+                            T *rho) {
+  const int last = h->cfg.rk_stages - 1;
+  const int kind = stage <= 0 ? (stage == 0 ? STAGE_FIRST : STAGE_MID) : (stage == last ? STAGE_LAST : STAGE_MID);
+  Stage3Args<T> a{};
+  a.ni = b.ni;
+  a.nj = b.nj;
+  a.nk = b.nk;
+  a.ntx = (b.ni + ST_TX - 1) / ST_TX;
+  a.nty = (b.nj + S3_TY - 1) / S3_TY;
+  a.nzc = (b.nk + Z_CHUNK - 1) / Z_CHUNK;
+  const int in_idx = stage == 0 ? 0 : (stage % 2 == 1 ? 2 : 3);
+  const int out_idx = stage % 2 == 0 ? 2 : 3;
+  a.yin = Y[in_idx];
+  a.yn = Y[0];
+  a.acc = Y[1];
+  a.yout = Y[out_idx];
+  if (stage < 0) {
+    a.yin = Y[2];
+    a.yn = Y[2];
+    a.acc = Y[3];
+    a.yout = Y[1];
+  }
+  a.met = b.met;
+  a.jac = b.jac;
+  a.mc = sizeof(T) == 8 ? (const T *)b.mc64 : (const T *)b.mc32;
+  a.rho_in = rho;
+  a.rho_out = rho;
+  a.u0 = b.umean[IUX];
+  a.v0 = b.umean[IUY];
+  a.w0 = b.umean[IUZ];
+  a.gamma = h->cfg.gamma;
+  a.gp0 = h->cfg.gamma * b.umean[IP];
+  a.rhob0 = std::pow(h->cfg.gamma * b.umean[IP], 1.0 / h->cfg.gamma) * std::exp(-b.umean[IS]);
+  a.nz = 0;
+  for (const auto &z : b.zones) {
+    if (a.nz >= MAX_ZONES) break;
+    Zone3<T> &q = a.z[a.nz++];
+    q.i0 = z.is[0];
+    q.i1 = z.ie[0];
+    q.j0 = z.is[1];
+    q.j1 = z.ie[1];
+    q.k0 = z.is[2];
+    q.k1 = z.ie[2];
+    q.nzi = z.nz[0];
+    q.nzj = z.nz[1];
+    q.sig = sizeof(T) == 8 ? (const T *)z.sigma64 : (const T *)z.sigma32;
+  }
+  a.si0 = b.src.i0;
+  a.si1 = b.src.i1;
+  a.sj0 = b.src.j0;
+  a.sj1 = b.src.j1;
+  a.sk0 = b.src.k0;
+  a.sk1 = b.src.k1;
+  a.g = b.src.g;
+  a.s1 = sc.s1;
+  a.s2 = sc.s2;
+  a.h = (T)sc.h;
+  a.b = (T)sc.b;
+  a.a = (T)sc.a;
+  const dim3 grid(a.ntx * a.nty, a.nzc), block(ST_TX, S3_TY);
+  const bool mf = !b.uniform_mean;
+#define LEE_S3(MF)                                                                                    \
+  do {                                                                                                \
+    if (kind == STAGE_FIRST) hipLaunchKernelGGL((k_stage3d<T, MF, STAGE_FIRST>), grid, block, 0, s, a); \
+    else if (kind == STAGE_MID) hipLaunchKernelGGL((k_stage3d<T, MF, STAGE_MID>), grid, block, 0, s, a); \
+    else hipLaunchKernelGGL((k_stage3d<T, MF, STAGE_LAST>), grid, block, 0, s, a);                     \
+  } while (0)
+  if (mf) LEE_S3(true);
+  else LEE_S3(false);
+#undef LEE_S3
+  return hipGetLastError();
+}
+
+hipError_t fused_stage3d(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc) {
+  if (h->cfg.precision == LEE_FP32) return stage3d_t<float>(s, h, b, stage, sc, b.Y32, b.rho32);
+  return stage3d_t<double>(s, h, b, stage, sc, b.Y64, b.rho64);
+}
+
+template <typename T>
+static hipError_t filter3d_t(hipStream_t s, lee_handle *h, Block &b, T *const *Y, int src, int dst, int tmp) {
+  const int TYF = sizeof(T) == 8 ? 4 : 8;
+  FilterArgs<T> a{};
+  a.ni = b.ni;
+  a.nj = b.nj;
+  a.vs = (size_t)b.n;
+  a.ntx = (b.ni + FL_TX - 1) / FL_TX;
+  a.nty = (b.nj + TYF - 1) / TYF;
+  a.ntiles = a.ntx * a.nty;
+  a.chunk = (a.ntiles + 7) / 8;
+  a.src = Y[src];
+  a.dst = Y[tmp];
+  a.blend = (T)1;
+  a.rem = (T)0;
+  for (int v = 0; v < 5; v++) a.vars[v] = v;
+  const size_t plane = (size_t)b.ni * b.nj;
+  const dim3 grid(a.chunk * 8, b.nk), block(64, 4);
+  const bool o8 = h->cfg.filter_order == 8;
+  if (o8) hipLaunchKernelGGL((k_filter3d_plane<T, 8, TYF>), grid, block, 0, s, a, plane);
+  else hipLaunchKernelGGL((k_filter3d_plane<T, 4, TYF>), grid, block, 0, s, a, plane);
+  const dim3 gz(nb3(plane), (b.nk + Z_CHUNK - 1) / Z_CHUNK);
+  const T blend = (T)h->cfg.filter_blend, rem = (T)(1.0 - h->cfg.filter_blend);
+  if (o8) hipLaunchKernelGGL((k_filter3d_z<T, 8>), gz, dim3(TPB), 0, s, Y[tmp], Y[src], Y[dst], b.ni, b.nj, b.nk, rem, blend);
+  else hipLaunchKernelGGL((k_filter3d_z<T, 4>), gz, dim3(TPB), 0, s, Y[tmp], Y[src], Y[dst], b.ni, b.nj, b.nk, rem, blend);
+  return hipGetLastError();
+}
+
+hipError_t fused_filter3d(hipStream_t s, lee_handle *h, Block &b, int src, int dst, int tmp) {
+  if (h->cfg.precision == LEE_FP32) return filter3d_t<float>(s, h, b, b.Y32, src, dst, tmp);
+  return filter3d_t<double>(s, h, b, b.Y64, src, dst, tmp);
+}
+
+}  // namespace lee

@@ -224,25 +224,23 @@ void plan_cfl(Block &b, const double *mean, double gamma) {
       prod[p] = -1e300;
       continue;
     }
-    const double m00 = b.h_met[p], m01 = b.h_met[(size_t)n + p], m10 = b.h_met[(size_t)2 * n + p],
-                 m11 = b.h_met[(size_t)3 * n + p];
-    const double u = mean[(size_t)IUX * n + p], v = mean[(size_t)IUY * n + p];
-    double vk0 = 0.0;
-    vk0 += m00 * u;
-    vk0 += m01 * v;
-    double vk1 = 0.0;
-    vk1 += m10 * u;
-    vk1 += m11 * v;
+    // contravariant mean velocity and the metric sums over num_dim directions
+    const int D = b.nk > 1 ? 3 : 2;
+    double m[3][3];
+    for (int l = 0; l < D; l++)
+      for (int i = 0; i < D; i++) m[l][i] = b.h_met[(size_t)(D * l + i) * n + p];
+    double uvec[3] = {mean[(size_t)IUX * n + p], mean[(size_t)IUY * n + p], mean[(size_t)IUZ * n + p]};
+    double vk[3];
+    for (int l = 0; l < D; l++) {
+      vk[l] = 0.0;
+      for (int i = 0; i < D; i++) vk[l] += m[l][i] * uvec[i];
+    }
     double cfl = 0.0;
-    cfl += std::abs((int)(vk0 * m00));
-    cfl += std::abs((int)(vk0 * m01));
-    cfl += std::abs((int)(vk1 * m10));
-    cfl += std::abs((int)(vk1 * m11));
+    for (int l = 0; l < D; l++)
+      for (int i = 0; i < D; i++) cfl += std::abs((int)(vk[l] * m[l][i]));
     double fac = 0.0;
-    fac += m00 * m00;
-    fac += m01 * m01;
-    fac += m10 * m10;
-    fac += m11 * m11;
+    for (int l = 0; l < D; l++)
+      for (int i = 0; i < D; i++) fac += m[l][i] * m[l][i];
     const double sbar = mean[(size_t)IS * n + p], pbar = mean[(size_t)IP * n + p];
     const double rhobar = std::pow(gamma * pbar, gi) * std::exp(-sbar);
     const double Tbar = pbar * gg * (1.0 / rhobar);
@@ -311,6 +309,8 @@ void plan_source(const lee_config &c, Block &b, const double *xyz) {
   b.src.i1 = -1;
   b.src.j0 = 0;
   b.src.j1 = -1;
+  b.src.k0 = 0;
+  b.src.k1 = 0;
   if (c.source_kind == LEE_SOURCE_NONE) return;
   const int n = b.n;
   if (c.source_kind == LEE_SOURCE_FIELD) {  // the caller's field over the whole block, zero until set
@@ -318,29 +318,36 @@ void plan_source(const lee_config &c, Block &b, const double *xyz) {
     b.src.i1 = b.ni - 1;
     b.src.j0 = 0;
     b.src.j1 = b.nj - 1;
+    b.src.k1 = b.nk - 1;
     b.src.host_g.assign(n, 0.0);
     return;
   }
   std::vector<double> g(n);
-  int i0 = b.ni, i1 = -1, j0 = b.nj, j1 = -1;
-  for (int j = 0; j < b.nj; j++)
-    for (int i = 0; i < b.ni; i++) {
-      const int p = j * b.ni + i;
-      g[p] = source_shape(c, xyz[p], xyz[(size_t)n + p], xyz[(size_t)2 * n + p]);
-      if (g[p] != 0.0) {
-        i0 = std::min(i0, i);
-        i1 = std::max(i1, i);
-        j0 = std::min(j0, j);
-        j1 = std::max(j1, j);
+  int i0 = b.ni, i1 = -1, j0 = b.nj, j1 = -1, k0 = b.nk, k1 = -1;
+  for (int k = 0; k < b.nk; k++)
+    for (int j = 0; j < b.nj; j++)
+      for (int i = 0; i < b.ni; i++) {
+        const int p = (k * b.nj + j) * b.ni + i;
+        g[p] = source_shape(c, xyz[p], xyz[(size_t)n + p], xyz[(size_t)2 * n + p]);
+        if (g[p] != 0.0) {
+          i0 = std::min(i0, i);
+          i1 = std::max(i1, i);
+          j0 = std::min(j0, j);
+          j1 = std::max(j1, j);
+          k0 = std::min(k0, k);
+          k1 = std::max(k1, k);
+        }
       }
-    }
   if (i1 < i0) return;
   b.src.i0 = i0;
   b.src.i1 = i1;
   b.src.j0 = j0;
   b.src.j1 = j1;
-  for (int j = j0; j <= j1; j++)
-    for (int i = i0; i <= i1; i++) b.src.host_g.push_back(g[j * b.ni + i]);
+  b.src.k0 = k0;
+  b.src.k1 = k1;
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) b.src.host_g.push_back(g[(k * b.nj + j) * b.ni + i]);
 }
 
 void source_factors(const lee_config &c, double t, double &s1, double &s2) {

@@ -206,10 +206,10 @@ def read_bc_file(path, dims_per_block):
 
 def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4, mode="FIX_CFL", value=0.5,
                  sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49", source_params=(),
-                 precision=FP64, path=PATH_FUSED):
+                 precision=FP64, path=PATH_FUSED, num_dim=2):
     """lee_config from the deck settings (src/core/input.cpp:99-284 keys)."""
     cfg = _Config()
-    cfg.num_dim = 2
+    cfg.num_dim = num_dim
     cfg.gamma = gamma
     cfg.fd_order = fd_order
     cfg.filter_order = filter_order
@@ -237,13 +237,15 @@ class Solver:
 
     def __init__(self, blocks, *, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4,
                  mode="FIX_CFL", value=0.5, sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49",
-                 source_params=(), precision=FP64, path=PATH_FUSED, boundaries=(), overset=None):
+                 source_params=(), precision=FP64, path=PATH_FUSED, boundaries=(), overset=None, num_dim=2):
         L = library()
         self.L = L
+        self.num_dim = num_dim
         self.cfg = _make_config(gamma=gamma, fd_order=fd_order, filter_order=filter_order,
                                 filter_blend=filter_blend, rk=rk, mode=mode, value=value,
                                 sponge_order=sponge_order, sponge_constant=sponge_constant,
-                                simulation=simulation, source_params=source_params, precision=precision, path=path)
+                                simulation=simulation, source_params=source_params, precision=precision, path=path,
+                                num_dim=num_dim)
         self._keep = []
         barr = self._blocks(blocks, boundaries)
         ovp = None
@@ -262,16 +264,17 @@ class Solver:
         self.shapes = []
         dims = []
         for b in blocks:
-            nj, ni = np.asarray(b["x"]).shape
-            dims.append((ni, nj, 1))
-            self.shapes.append((nj, ni))
+            sh = np.asarray(b["x"]).shape
+            nk, nj, ni = (1,) + sh if len(sh) == 2 else sh
+            dims.append((ni, nj, nk))
+            self.shapes.append(sh)
         parsed = [bc if isinstance(bc[1], int) else parse_bc_line(bc, dims[int(bc[0])]) for bc in boundaries]
         for ib, b in enumerate(blocks):
             x = np.asarray(b["x"], np.float64)
-            nj, ni = x.shape
+            ni, nj, nk = dims[ib]
             z = np.zeros_like(x) if b.get("z") is None else np.asarray(b["z"], np.float64)
             xyz = np.ascontiguousarray(np.stack([x, np.asarray(b["y"], np.float64), z]))
-            mean = np.ascontiguousarray(np.asarray(b["mean"], np.float64).reshape(5, nj, ni))
+            mean = np.ascontiguousarray(np.asarray(b["mean"], np.float64).reshape((5,) + x.shape))
             mine = [p for p in parsed if p[0] == ib]
             bar = (_Boundary * max(1, len(mine)))()
             for k, (_, typ, dr, end, is3, ie3) in enumerate(mine):
@@ -280,7 +283,7 @@ class Solver:
                     bar[k].is_[q] = is3[q]
                     bar[k].ie[q] = ie3[q]
             self._keep += [xyz, mean, bar]
-            barr[ib].ni, barr[ib].nj, barr[ib].nk = ni, nj, 1
+            barr[ib].ni, barr[ib].nj, barr[ib].nk = ni, nj, nk
             barr[ib].xyz = _dp(xyz)
             barr[ib].mean = _dp(mean)
             barr[ib].nboundary = len(mine)
@@ -331,8 +334,7 @@ class Solver:
         self.close()
 
     def n(self, ib=0):
-        nj, ni = self.shapes[ib]
-        return ni * nj
+        return int(np.prod(self.shapes[ib]))
 
     # state ---------------------------------------------------------------
     def set_solution(self, sol, ib=0):
@@ -444,11 +446,18 @@ class Solver:
         self._check(self.L.lee_set_source_field(self.h, ib, _dp(q)))
 
     def set_observer(self, points):
-        """points: iterable of (block, i, j, weight); J sums weight * p'^2."""
+        """points: iterable of (block, i, j, weight) in 2-D or (block, i, j, k,
+        weight) in 3-D; J sums weight * p'^2."""
         pts = list(points)
         blk = np.array([p[0] for p in pts], np.int32)
-        idx = np.array([int(p[2]) * self.shapes[int(p[0])][1] + int(p[1]) for p in pts], np.int32)
-        w = np.array([p[3] for p in pts], np.float64)
+
+        def lin(p):
+            sh = self.shapes[int(p[0])]
+            ni, nj = sh[-1], sh[-2]
+            k = int(p[3]) if len(p) == 5 else 0
+            return (k * nj + int(p[2])) * ni + int(p[1])
+        idx = np.array([lin(p) for p in pts], np.int32)
+        w = np.array([p[-1] for p in pts], np.float64)
         self._keep += [blk, idx, w]
         self._check(self.L.lee_set_observer(self.h, len(pts), _ip(blk), _ip(idx), _dp(w)))
         self._nobs = len(pts)
@@ -502,7 +511,7 @@ class Solver:
         return out
 
     def mean_gradient(self, ib=0):
-        out = np.empty((10,) + self.shapes[ib])
+        out = np.empty((5 * getattr(self, "num_dim", 2),) + self.shapes[ib])
         self._check(self.L.lee_get_mean_gradient(self.h, ib, _dp(out)))
         return out
 
