@@ -534,7 +534,6 @@ void advance_clock(lee_handle *h) {
 
 int adjoint_check(lee_handle *h) {
   if (h->batch) return fail(h, LEE_EINVAL, "the adjoint runs on single handles, not batches");
-  if (h->cfg.num_dim == 3 && !h->adj3d) return fail(h, LEE_EINVAL, "no 3-D adjoint in this build");
   if (is_plain(h)) return fail(h, LEE_EINVAL, "the adjoint runs on the fused path");
   if (h->ov.nrecv > 0) return fail(h, LEE_EINVAL, "the adjoint does not cover overset interpolation");
   for (int ib = 0; ib < h->nblk; ib++) {
@@ -600,7 +599,10 @@ int adjoint_step(lee_handle *h, int k) {
   for (int ib = 0; ib < h->nblk; ib++) {
     Block &b = h->b[ib];
     const bool field = b.cfg.source_kind == LEE_SOURCE_FIELD;
-    HIPCHK(adjoint_filter(h->stream, h, b, 0, 1, !b.uniform_mean));
+    const bool d3 = h->cfg.num_dim == 3;
+    HIPCHK(d3 ? adjoint_filter3d(h->stream, h, b, 0, 1, 2, !b.uniform_mean)
+              : adjoint_filter(h->stream, h, b, 0, 1, !b.uniform_mean));
+    auto stage = [&](const AdjStage &a) { return d3 ? adjoint_stage3d(h->stream, h, b, a) : adjoint_stage(h->stream, h, b, a); };
     auto src = [&](int i) {
       double s1 = 0.0, s2 = 0.0;
       source_factors(b.cfg, t + h->c_i[i] * dt, s1, s2);
@@ -608,15 +610,15 @@ int adjoint_step(lee_handle *h, int k) {
     };
     int prev = 2;
     AdjStage st{STAGE_LAST, 1, -1, -1, 2, -1, h->b_i[S - 1] * dt, 0.0, src(S - 1), field};
-    HIPCHK(adjoint_stage(h->stream, h, b, st));
+    HIPCHK(stage(st));
     for (int i = S - 2; i >= 1; i--) {
       const int out = prev == 2 ? 3 : 2;
       AdjStage m{STAGE_MID, 1, prev, i == S - 2 ? -1 : 4, out, 4, h->b_i[i] * dt, h->a_next[i] * dt, src(i), field};
-      HIPCHK(adjoint_stage(h->stream, h, b, m));
+      HIPCHK(stage(m));
       prev = out;
     }
     AdjStage f{STAGE_FIRST, 1, prev, S > 2 ? 4 : -1, 0, -1, h->b_i[0] * dt, h->a_next[0] * dt, src(0), field};
-    HIPCHK(adjoint_stage(h->stream, h, b, f));
+    HIPCHK(stage(f));
   }
   return LEE_OK;
 }
@@ -1544,7 +1546,8 @@ int lee_objective_backward(lee_handle *h, double scale) {
   }
   // the first step's stale aux density is R(y_0)
   for (int ib = 0; ib < h->nblk; ib++)
-    if (!h->b[ib].uniform_mean) HIPCHK(adjoint_add_rt_mu(s, h, h->b[ib], 0));
+    if (!h->b[ib].uniform_mean)
+      HIPCHK(h->cfg.num_dim == 3 ? adjoint_add_rt_mu3(s, h, h->b[ib], 0) : adjoint_add_rt_mu(s, h, h->b[ib], 0));
   rc = checkpoint_copy(h, o.ncheck, false);
   if (rc) return rc;
   h->time_step = o.end_step;
@@ -1594,7 +1597,7 @@ int lee_compute_rhs_adjoint(lee_handle *h, int ib, const double *kappa, double *
   else HIPCHK(hipMemcpyAsync(b.A64[1], stage, sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
   HIPCHK(adjoint_zero(s, h, b, 2));
   AdjStage st{STAGE_MID, 1, -1, -1, 2, -1, 1.0, 0.0, 0.0, false};
-  HIPCHK(adjoint_stage(s, h, b, st));
+  HIPCHK(h->cfg.num_dim == 3 ? adjoint_stage3d(s, h, b, st) : adjoint_stage(s, h, b, st));
   if (is_fp32(h)) HIPCHK(cast_back(s, b, NV, b.A32[2], stage, cnt));
   else HIPCHK(hipMemcpyAsync(stage, b.A64[2], sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(out, stage, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
@@ -1616,7 +1619,7 @@ int lee_apply_filter_adjoint(lee_handle *h, int ib, double *y) {
   if (is_fp32(h)) HIPCHK(cast_state(s, b, NV, stage, b.A32[0], cnt));
   else HIPCHK(hipMemcpyAsync(b.A64[0], stage, sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
   HIPCHK(adjoint_zero(s, h, b, 1));
-  HIPCHK(adjoint_filter(s, h, b, 0, 1, false));
+  HIPCHK(h->cfg.num_dim == 3 ? adjoint_filter3d(s, h, b, 0, 1, 2, false) : adjoint_filter(s, h, b, 0, 1, false));
   if (is_fp32(h)) HIPCHK(cast_back(s, b, NV, b.A32[1], stage, cnt));
   else HIPCHK(hipMemcpyAsync(stage, b.A64[1], sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
   // w' is not an adjoint variable in 2-D; it passes through the filter alone

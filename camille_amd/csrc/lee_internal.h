@@ -197,7 +197,6 @@ struct lee_handle {
   std::vector<lee::Block> b;
   lee::Objective obj;
   int nobs_total = 0;
-  bool adj3d = false;          // set when the 3-D adjoint kernels are present
   bool batch = false;          // independent solves (lee_batch_create)
   lee::BatchPlan bp;
   // pinned host staging of the batch schedule, two buffers used in turn;
@@ -285,6 +284,9 @@ hipError_t adjoint_filter(hipStream_t s, lee_handle *h, Block &b, int src, int d
 hipError_t adjoint_zero(hipStream_t s, lee_handle *h, Block &b, int buf);  // buf < 0: the aux-density adjoint
 hipError_t adjoint_zero_grad(hipStream_t s, Block &b);
 hipError_t adjoint_add_rt_mu(hipStream_t s, lee_handle *h, Block &b, int buf);
+hipError_t adjoint_stage3d(hipStream_t s, lee_handle *h, Block &b, const AdjStage &st);
+hipError_t adjoint_filter3d(hipStream_t s, lee_handle *h, Block &b, int src, int dst, int scratch, bool with_mu);
+hipError_t adjoint_add_rt_mu3(hipStream_t s, lee_handle *h, Block &b, int buf);
 hipError_t observer_energy(hipStream_t s, lee_handle *h, Block &b, double *out);
 hipError_t observer_gather(hipStream_t s, lee_handle *h, Block &b, double *out);
 hipError_t observer_seed(hipStream_t s, lee_handle *h, Block &b, int buf, const double *samp, double scale);
