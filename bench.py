@@ -239,6 +239,80 @@ def c3_leg(steps, sample_every):
     return res
 
 
+def c4_inputs(n=256):
+    """3-D Cartesian n^3 block on [-2.5, 2.5]^3 with the parallel shear mean
+    flow u(r) = 0.45 (1 - tanh((r - 0.5) / 0.05)), r = sqrt(y^2 + z^2),
+    p = 1/gamma, s = 0 (so the mean-gradient LEE terms act); observer points
+    within half a cell of the cylinder r = 1.5, weight h^2 each."""
+    h = 5.0 / (n - 1)
+    c = -2.5 + h * np.arange(n)
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    r = np.sqrt(y * y + z * z)
+    mean = np.zeros((5, n, n, n))
+    mean[1] = 0.45 * (1.0 - np.tanh((r - 0.5) / 0.05))
+    mean[4] = 1.0 / 1.4
+    obs = np.flatnonzero(np.abs(r - 1.5).reshape(-1) < 0.5 * h)
+    return dict(x=x, y=y, z=z, mean=mean, obs=obs, h=h, n=n)
+
+
+def c4_leg(nsteps=500, window=400, every=50):
+    """Gradient of J = sum over the last 100 of 500 steps of the observer
+    energy with respect to a 32^3 source-amplitude control (trilinear
+    embedding, p-source a(x) sin(omega t)), through the discrete adjoint
+    with checkpoints every 50 steps, in FP32 and FP64; central finite
+    difference (eps 1e-4) of the FP64 forward along one random direction."""
+    import torch
+
+    from camille_amd import FP32, FP64, Solver
+    from camille_amd.adjoint import embed_control3, observer_energy
+    inp = c4_inputs()
+    n = inp["n"]
+    shape = (n, n, n)
+    npts = n ** 3
+    rng = np.random.default_rng(4)
+    a0 = torch.tensor(rng.uniform(0.5, 1.5, (32, 32, 32)), dtype=torch.float64)
+    e = torch.tensor(rng.standard_normal((32, 32, 32)), dtype=torch.float64)
+    y0 = torch.zeros((5,) + shape, dtype=torch.float64)
+    res = {"grid": [n, n, n], "steps": nsteps, "window": [window + 1, nsteps], "checkpoint_every": every,
+           "observers": int(inp["obs"].size), "control": [32, 32, 32]}
+    grads = {}
+    solvers = {}
+    for label, prec in (("fp32", FP32), ("fp64", FP64)):
+        print(f"[bench] C4 {label}: {n}^3, {nsteps} steps forward + adjoint", file=sys.stderr, flush=True)
+        s = Solver([dict(x=inp["x"], y=inp["y"], z=inp["z"], mean=inp["mean"])], num_dim=3, precision=prec,
+                   simulation="SOURCE_FIELD", source_params=(1e-3, 2.0 * math.pi * 0.5), mode="FIX_CFL", value=0.5)
+        s.set_observer_points(np.zeros(inp["obs"].size), inp["obs"], np.full(inp["obs"].size, inp["h"] ** 2))
+        a = a0.clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        J = observer_energy(s, embed_control3(a, shape), y0, nsteps, window, every)
+        t1 = time.perf_counter()
+        J.backward()
+        t2 = time.perf_counter()
+        grads[label] = a.grad.detach().numpy().copy()
+        res[label] = dict(J=float(J.item()), forward_s=t1 - t0, backward_s=t2 - t1,
+                          forward_updates_per_s=npts * 4 * nsteps / (t1 - t0),
+                          dt=s.clock()[2])
+        solvers[label] = s
+        print(f"[bench] C4 {label}: J {J.item():.6e}, forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s",
+              file=sys.stderr, flush=True)
+    g64, g32 = grads["fp64"], grads["fp32"]
+    res["grad_rel_err_fp32_vs_fp64"] = float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64)))
+    res["J_rel_err_fp32_vs_fp64"] = abs(res["fp32"]["J"] - res["fp64"]["J"]) / res["fp64"]["J"]
+    s = solvers["fp64"]
+    eps = 1e-4
+    with torch.no_grad():
+        jp = observer_energy(s, embed_control3(a0 + eps * e, shape), y0, nsteps, window, every).item()
+        jm = observer_energy(s, embed_control3(a0 - eps * e, shape), y0, nsteps, window, every).item()
+    fd = (jp - jm) / (2 * eps)
+    ad = float(np.sum(g64 * e.numpy()))
+    res["fd_check"] = dict(eps=eps, fd=fd, adjoint=ad, rel_err=abs(fd - ad) / abs(fd),
+                           adjoint_fp32=float(np.sum(g32 * e.numpy())),
+                           rel_err_fp32=abs(fd - float(np.sum(g32 * e.numpy()))) / abs(fd))
+    for v in solvers.values():
+        v.close()
+    return res
+
+
 def make_solver(inp, precision, path):
     from camille_amd import Solver
     s = Solver([dict(x=inp["x"], y=inp["y"], mean=inp["mean"])], simulation="CASE_MONOPOLE",
@@ -307,7 +381,8 @@ def main():
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c3-steps", type=int, default=5000)
     ap.add_argument("--no-c3", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "plain", "c5", "c3"], default=None,
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64", "plain", "c5", "c3", "c4"], default=None,
                     help="time one path only (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (HBM counter calibration for rocprofv3 --pmc)")
@@ -330,6 +405,9 @@ def main():
         return
     if args.only == "c3":
         print(json.dumps({"c3": c3_leg(args.steps, 50)}))
+        return
+    if args.only == "c4":
+        print(json.dumps({"c4": c4_leg()}))
         return
     res = {}
     paths = [("fp32", FP32, PATH_FUSED, args.steps), ("fp64", FP64, PATH_FUSED, args.steps),
@@ -388,6 +466,7 @@ def main():
 
     c5 = None if args.no_c5 else c5_leg(args.c5_steps, args.warmup, 50)
     c3 = None if args.no_c3 else c3_leg(args.c3_steps, 50)
+    c4 = None if args.no_c4 else c4_leg()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -433,6 +512,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_ensemble_c5": c5,
         "overset_cylinder_c3": c3,
+        "adjoint_3d_c4": c4,
     }
     print(json.dumps(out))
 

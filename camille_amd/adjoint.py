@@ -40,7 +40,7 @@ class ObserverEnergy(torch.autograd.Function):
 
 def observer_energy(solver, q, y0, nsteps, window_start, ckpt_every, clock=(0, 0.0, 1e-6)):
     """J(q, y0) from the initial clock (step, time, dt); differentiable in
-    q (nj, ni) and y0 (5, nj, ni)."""
+    q (block shape) and y0 (5 + block shape)."""
     return ObserverEnergy.apply(q, y0, solver, nsteps, window_start, ckpt_every, clock)
 
 
@@ -48,4 +48,11 @@ def embed_control(a, shape):
     """Bilinear embedding of a coarse control field a (mj, mi) onto a block
     of `shape` (nj, ni), corners aligned; differentiable."""
     return torch.nn.functional.interpolate(a[None, None], size=tuple(shape), mode="bilinear",
+                                           align_corners=True)[0, 0]
+
+
+def embed_control3(a, shape):
+    """Trilinear embedding of a coarse control field a (mk, mj, mi) onto a
+    block of `shape` (nk, nj, ni), corners aligned; differentiable."""
+    return torch.nn.functional.interpolate(a[None, None], size=tuple(shape), mode="trilinear",
                                            align_corners=True)[0, 0]

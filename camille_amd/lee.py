@@ -468,6 +468,15 @@ class Solver:
         self._check(self.L.lee_sample_observers(self.h, _dp(out)))
         return out
 
+    def set_observer_points(self, block, index, weight):
+        """Observers as arrays: block, 1-D point index within the block, weight."""
+        blk = np.ascontiguousarray(np.asarray(block, np.int32))
+        idx = np.ascontiguousarray(np.asarray(index, np.int32))
+        w = np.ascontiguousarray(np.asarray(weight, np.float64))
+        self._keep += [blk, idx, w]
+        self._check(self.L.lee_set_observer(self.h, len(blk), _ip(blk), _ip(idx), _dp(w)))
+        self._nobs = len(blk)
+
     def objective_forward(self, nsteps, window_start, ckpt_every):
         J = ctypes.c_double()
         self._check(self.L.lee_objective_forward(self.h, int(nsteps), int(window_start), int(ckpt_every),
