@@ -446,14 +446,18 @@ struct TuneKey {
     return ni == o.ni && nj == o.nj && prec == o.prec && meanf == o.meanf && holes == o.holes;
   }
 };
-std::vector<std::pair<TuneKey, std::pair<int, int>>> g_tune_cache;
+struct TuneVal {
+  int r, fty, fgrid;
+};
+std::vector<std::pair<TuneKey, TuneVal>> g_tune_cache;
 
 int autotune_block(lee_handle *h, Block &b) {
   const TuneKey key{b.ni, b.nj, h->cfg.precision, b.uniform_mean ? 0 : 1, b.has_holes ? 1 : 0};
   for (const auto &e : g_tune_cache)
     if (e.first == key) {
-      b.tune_r = e.second.first;
-      b.tune_fty = e.second.second;
+      b.tune_r = e.second.r;
+      b.tune_fty = e.second.fty;
+      b.tune_fgrid = e.second.fgrid;
       b.tuned = true;
       return LEE_OK;
     }
@@ -480,29 +484,37 @@ int autotune_block(lee_handle *h, Block &b) {
   }
   b.tune_r = best_r;
   best = 1e30f;
-  int best_f = b.tune_fty;
+  int best_f = b.tune_fty, best_g = 0;
   if (h->cfg.filter_order > 0) {
+    // tile height, then one launch per tile or the persistent pipelined
+    // form with 2, 4 or 8 workgroups per CU
     for (int ty : {4, 8, 16}) {
       if (is_fp32(h) ? false : ty > 4) continue;  // FP64 tiles above 4 rows exceed 64 KB of LDS
-      b.tune_fty = ty;
-      for (int k = 0; k < 2; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
-      HIPCHK(hipEventRecord(e0, h->stream));
-      for (int k = 0; k < reps; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
-      HIPCHK(hipEventRecord(e1, h->stream));
-      HIPCHK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        best_f = ty;
+      for (int pw : {0, 2, 4, 8}) {
+        if (pw > 0 && b.cus <= 0) continue;
+        b.tune_fty = ty;
+        b.tune_fgrid = pw * b.cus;
+        for (int k = 0; k < 2; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
+        HIPCHK(hipEventRecord(e0, h->stream));
+        for (int k = 0; k < reps; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
+        HIPCHK(hipEventRecord(e1, h->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          best_f = ty;
+          best_g = b.tune_fgrid;
+        }
       }
     }
   }
   b.tune_fty = best_f;
+  b.tune_fgrid = best_g;
   b.tuned = true;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  g_tune_cache.push_back({key, {b.tune_r, b.tune_fty}});
+  g_tune_cache.push_back({key, {b.tune_r, b.tune_fty, b.tune_fgrid}});
   return LEE_OK;
 }
 
@@ -765,6 +777,9 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
   }
   int rc = [&]() -> int {
     HIPCHK(hipGetDevice(&h->device));
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    for (auto &blk : h->b) blk.cus = cus;
     // a blocking stream: host-blocking copies order with the queued kernels
     HIPCHK(hipStreamCreate(&h->stream));
     HIPCHK(hipEventCreate(&h->ev0));
@@ -1205,6 +1220,20 @@ int lee_set_tile_shape(lee_handle *h, int ib, int stage_rows, int filter_rows) {
   h->b[ib].tune_r = stage_rows;
   h->b[ib].tune_fty = filter_rows;
   h->b[ib].tuned = true;
+  return LEE_OK;
+}
+
+int lee_set_filter_grid(lee_handle *h, int ib, int workgroups) {
+  if (!h || ib < 0 || ib >= h->nblk || workgroups < 0) return fail(h, LEE_EINVAL, "bad argument");
+  if (h->batch) return fail(h, LEE_EINVAL, "a batch launches its filter through the batch launch settings");
+  h->b[ib].tune_fgrid = (workgroups + 7) / 8 * 8;
+  h->b[ib].tuned = true;
+  return LEE_OK;
+}
+
+int lee_get_filter_grid(lee_handle *h, int ib, int *workgroups) {
+  if (!h || ib < 0 || ib >= h->nblk || !workgroups) return fail(h, LEE_EINVAL, "bad argument");
+  *workgroups = h->batch ? 0 : h->b[ib].tune_fgrid;
   return LEE_OK;
 }
 

@@ -211,54 +211,80 @@ using Vec4 = T __attribute__((ext_vector_type(4)));
 // with the unfiltered values (kept in registers since the load) and
 // temporal::iblank_correction follow.  Tiles whose halo stays inside the
 // block skip all edge logic.
-template <typename T, bool HOLES, int ORDER, int NVAR, int TY, typename AT>
-__device__ __forceinline__ void filter_tile(const AT &a, int tile, FilterLds<T, NVAR, TY> &L) {
+// Global loads of one filter tile (tile plus its 4-point halo) into the
+// thread's registers: region rows ty + 4k, columns tx and, for tx < 8,
+// 64 + tx.
+template <typename T, int NVAR, int TY>
+struct FilterRegs {
+  T ra[NVAR][(TY + 2 * FL_H) / 4], rb[NVAR][(TY + 2 * FL_H) / 4];
+};
+
+template <typename T, bool HOLES, int NVAR, int TY, typename AT>
+__device__ __forceinline__ void filter_load(const AT &a, int tile, FilterRegs<T, NVAR, TY> &R) {
   constexpr int NR = TY + 2 * FL_H, KR = NR / 4;
-  constexpr int NB = ORDER / 2;
   static_assert(NR == 4 * KR && FL_H == 4, "tile rows and the halo come in fours");
-  auto &in = L.in;
-  auto &mid = L.mid;
+  auto &ra = R.ra;
+  auto &rb = R.rb;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
   const size_t vs = a.vs ? a.vs : (size_t)n;
   const bool xtra = tx < 2 * FL_H;
-  const bool inner = !HOLES && i0 >= FL_H && i0 + FL_TX + FL_H <= ni && j0 >= FL_H && j0 + TY + FL_H <= nj;
-
-  T ra[NVAR][KR], rb[NVAR][KR];
-  if (inner) {
+  // one straight-line sequence of loads for every tile, from clamped
+  // addresses (filter_compute puts zeros outside the block): the wait
+  // counters then let a caller keep the next tile's loads in flight while it
+  // filters the current one.  Lanes without a right-halo column repeat
+  // their own column.
+  const int gia = i0 - FL_H + tx, gib = xtra ? i0 - FL_H + FL_TX + tx : gia;
+  const int cia = min(max(gia, 0), ni - 1), cib = min(max(gib, 0), ni - 1);
 #pragma unroll
-    for (int k = 0; k < KR; k++) {
-      const int q0 = (j0 - FL_H + ty + 4 * k) * ni + (i0 - FL_H + tx);
-#pragma unroll
-      for (int q = 0; q < NVAR; q++) {
-        const T *src = a.src + a.vars[q] * vs;
-        ra[q][k] = src[q0];
-        rb[q][k] = xtra ? src[q0 + FL_TX] : (T)0;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < KR; k++) {
-      const int gj = j0 - FL_H + ty + 4 * k;
-      const bool rowok = gj >= 0 && gj < nj;
-      const int gia = i0 - FL_H + tx, gib = i0 - FL_H + FL_TX + tx;
-      const bool oka = rowok && gia >= 0 && gia < ni, okb = rowok && xtra && gib < ni;
-#pragma unroll
-      for (int q = 0; q < NVAR; q++) {
-        const T *src = a.src + a.vars[q] * vs;
-        ra[q][k] = oka ? src[gj * ni + gia] : (T)0;
-        rb[q][k] = okb ? src[gj * ni + gib] : (T)0;
-      }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < KR; k++)
+  for (int k = 0; k < KR; k++) {
+    const int gj = j0 - FL_H + ty + 4 * k;
+    const int cj = min(max(gj, 0), nj - 1);
+    const int pa = cj * ni + cia, pb = cj * ni + cib;
 #pragma unroll
     for (int q = 0; q < NVAR; q++) {
-      in[q][ty + 4 * k][tx] = ra[q][k];
-      if (xtra) in[q][ty + 4 * k][FL_TX + tx] = rb[q][k];
+      const T *src = a.src + a.vars[q] * vs;
+      ra[q][k] = src[pa];
+      rb[q][k] = src[pb];
     }
+  }
+}
+
+// The filter of one tile from the registers of filter_load (see filter_tile).
+// INNER: the tile and its halo lie inside a block without holes, so every
+// point takes the interior stencil and no bounds are tested.
+template <typename T, bool HOLES, int ORDER, int NVAR, int TY, bool INNER, typename AT>
+__device__ __forceinline__ void filter_compute_k(const AT &a, int tile, FilterLds<T, NVAR, TY> &L,
+                                                 const FilterRegs<T, NVAR, TY> &R) {
+  constexpr int NR = TY + 2 * FL_H, KR = NR / 4;
+  constexpr int NB = ORDER / 2;
+  auto &in = L.in;
+  auto &mid = L.mid;
+  const auto &ra = R.ra;
+  const auto &rb = R.rb;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
+  const int ni = a.ni, nj = a.nj, n = ni * nj;
+  const size_t vs = a.vs ? a.vs : (size_t)n;
+  const bool xtra = tx < 2 * FL_H;
+  constexpr bool inner = INNER;
+  // rb of a lane without a right-halo column holds its own column again
+  // (filter_load): writing it back there keeps the stores unconditional
+  const int colb = xtra ? FL_TX + tx : tx;
+  const int gia = i0 - FL_H + tx, gib = i0 - FL_H + colb;
+  const bool cola = inner || (gia >= 0 && gia < ni), colok_b = inner || (gib >= 0 && gib < ni);
+#pragma unroll
+  for (int k = 0; k < KR; k++) {
+    const int gj = j0 - FL_H + ty + 4 * k;
+    const bool rowok = inner || (gj >= 0 && gj < nj);
+    const bool oka = rowok && cola, okb = rowok && colok_b;
+#pragma unroll
+    for (int q = 0; q < NVAR; q++) {
+      in[q][ty + 4 * k][tx] = oka ? ra[q][k] : (T)0;
+      in[q][ty + 4 * k][colb] = okb ? rb[q][k] : (T)0;
+    }
+  }
   __syncthreads();
   // xi-sweep of every region row into registers: each item is 4 consecutive
   // columns of one row and variable, from 12 values read as three 4-vectors
@@ -333,6 +359,27 @@ __device__ __forceinline__ void filter_tile(const AT &a, int tile, FilterLds<T, 
       a.dst[a.vars[q] * vs + p] = out;
     }
   }
+}
+
+template <typename T, bool HOLES, int ORDER, int NVAR, int TY, typename AT>
+__device__ __forceinline__ void filter_compute(const AT &a, int tile, FilterLds<T, NVAR, TY> &L,
+                                               const FilterRegs<T, NVAR, TY> &R) {
+  if constexpr (!HOLES) {
+    const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
+    if (i0 >= FL_H && i0 + FL_TX + FL_H <= a.ni && j0 >= FL_H && j0 + TY + FL_H <= a.nj) {
+      filter_compute_k<T, HOLES, ORDER, NVAR, TY, true>(a, tile, L, R);
+      return;
+    }
+  }
+  filter_compute_k<T, HOLES, ORDER, NVAR, TY, false>(a, tile, L, R);
+}
+
+
+template <typename T, bool HOLES, int ORDER, int NVAR, int TY, typename AT>
+__device__ __forceinline__ void filter_tile(const AT &a, int tile, FilterLds<T, NVAR, TY> &L) {
+  FilterRegs<T, NVAR, TY> R;
+  filter_load<T, HOLES, NVAR, TY>(a, tile, R);
+  filter_compute<T, HOLES, ORDER, NVAR, TY>(a, tile, L, R);
 }
 
 }  // namespace

@@ -92,6 +92,8 @@ struct Block {
   // tile shape of the fused kernels: stage rows per thread (1, 2, 4; tile
   // height 4 R) and filter tile height (4, 8, 16; FP64 4); set by the autotuner
   int tune_r = 1, tune_fty = 4;
+  int tune_fgrid = 0;          // filter: 0 one workgroup per tile, else persistent pipelined grid
+  int cus = 0;                 // compute units of the device
   bool tuned = false;
   // plain-path scratch (FP64): dflux, sum, rhs (5n), func0, func, interim (5n), ynp1 (5n)
   double *dflux = nullptr, *dsum = nullptr, *rhs = nullptr, *func0 = nullptr, *func = nullptr;

@@ -624,6 +624,40 @@ __global__ void __launch_bounds__(TPB) k_fused_filter(FilterArgs<T> a) {
   filter_tile<T, HOLES, ORDER, NVAR, TY>(a, tile, L);
 }
 
+// Persistent, software-pipelined form: a grid of G workgroups (G a
+// multiple of 8, about one resident wave of workgroups) strides through
+// the XCD-permuted tile list; each workgroup issues the global loads of its
+// next tile before computing the current one, so the loads overlap the
+// LDS sweeps instead of waiting for a new launch round.  Two tiles' values
+// stay in registers: up to 256 VGPRs (two waves per SIMD) keep them out of
+// scratch.
+template <typename T, bool HOLES, int ORDER, int NVAR, int TY>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_fused_filter_p(FilterArgs<T> a) {
+  __shared__ FilterLds<T, NVAR, TY> L;
+  const int G = gridDim.x, total = 8 * a.chunk;
+  auto tile_of = [&](int v) { return (v & 7) * a.chunk + (v >> 3); };
+  auto live = [&](int v) { return v < total && tile_of(v) < a.ntiles; };
+  // loads are issued for every slot (past the end: the last tile again) so
+  // that no branch sits between a tile's loads and its use
+  auto ld = [&](int v) { return min(tile_of(v), a.ntiles - 1); };
+  // (the scheduling barrier keeps the compiler from sinking the prefetch
+  // into the sweeps, where each load would be waited for at once; the copy
+  // at the end of an iteration waits for loads that had the whole tile's
+  // sweeps to arrive)
+  FilterRegs<T, NVAR, TY> R;
+  int v = blockIdx.x;
+  filter_load<T, HOLES, NVAR, TY>(a, ld(v), R);
+  for (; v < total; v += G) {
+    FilterRegs<T, NVAR, TY> N;
+    filter_load<T, HOLES, NVAR, TY>(a, ld(v + G), N);
+    __builtin_amdgcn_sched_barrier(0);
+    if (live(v)) filter_compute<T, HOLES, ORDER, NVAR, TY>(a, tile_of(v), L, R);
+    __syncthreads();
+    R = N;
+  }
+}
+
 template <typename T>
 struct BatchFilterArgs {
   const FilterArgs<T> *args;     // nsolve entries
@@ -1038,16 +1072,24 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
   const int TYF = b.tune_fty;
   FilterArgs<T> a = make_filter_args<T>(h, b, Y, src, dst, TYF);
   const dim3 grid(a.chunk * 8), block(64, 4);
+  // persistent pipelined form: tune_fgrid workgroups (a multiple of 8) stride over the tiles
+  const bool pipe = b.tune_fgrid > 0;
+  const dim3 pgrid(std::min(a.chunk * 8, (b.tune_fgrid + 7) / 8 * 8));
   const bool o8 = h->cfg.filter_order == 8;
   // the four active variables together, then w' alone when it is carried
   for (int pass = 0; pass < (b.carry_w ? 2 : 1); pass++) {
     const bool four = pass == 0;
     if (!four) a.vars[0] = IUZ;
-#define LEE_FLT(HO, OR, NVV)                                                                        \
-  do {                                                                                              \
-    if (TYF == 4) hipLaunchKernelGGL((k_fused_filter<T, HO, OR, NVV, 4>), grid, block, 0, s, a);    \
-    else if (TYF == 8) hipLaunchKernelGGL((k_fused_filter<T, HO, OR, NVV, 8>), grid, block, 0, s, a); \
-    else hipLaunchKernelGGL((k_fused_filter<T, HO, OR, NVV, 16>), grid, block, 0, s, a);           \
+#define LEE_FLT1(HO, OR, NVV, TYV)                                                                       \
+  do {                                                                                                   \
+    if (pipe) hipLaunchKernelGGL((k_fused_filter_p<T, HO, OR, NVV, TYV>), pgrid, block, 0, s, a);        \
+    else hipLaunchKernelGGL((k_fused_filter<T, HO, OR, NVV, TYV>), grid, block, 0, s, a);                \
+  } while (0)
+#define LEE_FLT(HO, OR, NVV)            \
+  do {                                  \
+    if (TYF == 4) LEE_FLT1(HO, OR, NVV, 4);  \
+    else if (TYF == 8) LEE_FLT1(HO, OR, NVV, 8); \
+    else LEE_FLT1(HO, OR, NVV, 16);      \
   } while (0)
 #define LEE_FLT_NV(HO, OR)            \
   do {                                \
@@ -1063,6 +1105,7 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
     }
 #undef LEE_FLT_NV
 #undef LEE_FLT
+#undef LEE_FLT1
   }
   return hipGetLastError();
 }

@@ -88,6 +88,7 @@ EXPORTS = [
     "lee_apply_filter", "lee_take_derivative_xyz", "lee_update_boundary", "lee_get_metrics", "lee_get_iblank",
     "lee_get_mean_gradient", "lee_get_sponge", "lee_last_solve_ms", "lee_version",
     "lee_set_kernel_timing", "lee_get_kernel_timing", "lee_autotune", "lee_set_tile_shape", "lee_get_tile_shape",
+    "lee_set_filter_grid", "lee_get_filter_grid",
     "lee_batch_create", "lee_get_solve_time", "lee_set_batch_launch", "lee_get_batch_launch",
     "lee_set_source_field", "lee_set_observer", "lee_objective_forward", "lee_objective_backward",
     "lee_get_source_gradient", "lee_get_state_gradient", "lee_compute_rhs_adjoint", "lee_apply_filter_adjoint",
@@ -136,6 +137,8 @@ def library():
     L.lee_autotune.argtypes = [vp]
     L.lee_set_tile_shape.argtypes = [vp, i, i, i]
     L.lee_get_tile_shape.argtypes = [vp, i, ip, ip]
+    L.lee_set_filter_grid.argtypes = [vp, i, i]
+    L.lee_get_filter_grid.argtypes = [vp, i, ip]
     L.lee_batch_create.argtypes = [ctypes.POINTER(_Config), ctypes.POINTER(_Block), i, ctypes.POINTER(vp)]
     L.lee_get_solve_time.argtypes = [vp, i, ip, dp, dp]
     L.lee_set_batch_launch.argtypes = [vp, i, i, i]
@@ -397,6 +400,16 @@ class Solver:
 
     def set_tile_shape(self, stage_rows, filter_rows, ib=0):
         self._check(self.L.lee_set_tile_shape(self.h, ib, int(stage_rows), int(filter_rows)))
+
+    def set_filter_grid(self, workgroups, ib=0):
+        """0: one filter workgroup per tile; >0: that many persistent
+        pipelined workgroups (rounded up to a multiple of 8)."""
+        self._check(self.L.lee_set_filter_grid(self.h, ib, int(workgroups)))
+
+    def filter_grid(self, ib=0):
+        g = ctypes.c_int()
+        self._check(self.L.lee_get_filter_grid(self.h, ib, ctypes.byref(g)))
+        return g.value
 
     def tile_shape(self, ib=0):
         r, f = ctypes.c_int(), ctypes.c_int()

@@ -64,7 +64,7 @@ typedef struct lee_handle lee_handle;
 
 /* Scheme and run settings: the deck keys of src/core/input.cpp:99-284. */
 typedef struct {
-  int num_dim;             /* DIMENSION: 2 */
+  int num_dim;             /* DIMENSION: 2, or 3 (fused path, sponges only, no overset) */
   double gamma;            /* GAMMA_SPECIFICHEAT */
   int fd_order;            /* SPATIAL_SCHEME STANDARD_CENTRAL ORDER_ACCURACY: 4 */
   int filter_order;        /* SPATIAL_FILTER STANDARD_CENTRAL ORDER_ACCURACY: 0 (NONE), 4, 8 */
@@ -89,7 +89,7 @@ typedef struct {
 } lee_boundary;
 
 /* One block: grid coordinates (3n: x, y, z) and base state (5n: s, u, v,
- * w, p) as read from GRID_FILE and BASESTATE_FILE. */
+ * w, p) as read from GRID_FILE and BASESTATE_FILE; nk = 1 in 2-D. */
 typedef struct {
   int ni, nj, nk;
   const double *xyz;
@@ -188,6 +188,13 @@ int lee_get_kernel_timing(lee_handle *h, double *ms, int *count);
 int lee_autotune(lee_handle *h);
 int lee_set_tile_shape(lee_handle *h, int iblock, int stage_rows, int filter_rows);
 int lee_get_tile_shape(lee_handle *h, int iblock, int *stage_rows, int *filter_rows);
+/* Launch form of the fused filter (no reference counterpart): workgroups 0
+ * launches one workgroup per tile; a positive count (rounded up to a
+ * multiple of 8) launches that many persistent workgroups, each loading its
+ * next tile while it filters the current one.  The autotuner also tries 2, 4
+ * and 8 workgroups per CU.  Results are bit-identical in every form. */
+int lee_set_filter_grid(lee_handle *h, int iblock, int workgroups);
+int lee_get_filter_grid(lee_handle *h, int iblock, int *workgroups);
 
 /* Batched ensemble (no reference counterpart: the reference runs one
  * problem per process).  nsolve independent single-block problems, block k
