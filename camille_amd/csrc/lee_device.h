@@ -32,39 +32,45 @@ constexpr int FL_TX = 64, FL_H = 4;   // filter tile width and halo (height: aut
 // as compile-time constants: interior {1/12, -8/12, 0, 8/12, -1/12}; left
 // rows 0 {-3/2, 2, -1/2} (anchored at the point) and 1 {-1/2, 0, 1/2}
 // (anchored one point left); right rows mirrored with the sign flipped.
+// Explicit fused multiply-add: the fused kernels spell out every fusion so
+// that their instances (interior tiles, edge tiles, batched) round alike.
+__device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T>
 __device__ __forceinline__ T fd_row(const T *s, int st, int code) {
+#pragma clang fp contract(off)
   const T c1 = (T)(8.0 / 12.0), c2 = (T)(-1.0 / 12.0);
   switch (code) {
     case CODE_INTERIOR: {
       T acc = -c2 * s[-2 * st];
-      acc += -c1 * s[-st];
-      acc += c1 * s[st];
-      acc += c2 * s[2 * st];
+      acc = fmx(-c1, s[-st], acc);
+      acc = fmx(c1, s[st], acc);
+      acc = fmx(c2, s[2 * st], acc);
       return acc;
     }
     case CODE_ZERO:
       return (T)0;
     case CODE_LEFT + 0: {
       T acc = (T)(-3.0 / 2.0) * s[0];
-      acc += (T)(4.0 / 2.0) * s[st];
-      acc += (T)(-1.0 / 2.0) * s[2 * st];
+      acc = fmx((T)(4.0 / 2.0), s[st], acc);
+      acc = fmx((T)(-1.0 / 2.0), s[2 * st], acc);
       return acc;
     }
     case CODE_LEFT + 1: {
       T acc = (T)(-1.0 / 2.0) * s[-st];
-      acc += (T)(1.0 / 2.0) * s[st];
+      acc = fmx((T)(1.0 / 2.0), s[st], acc);
       return acc;
     }
     case CODE_LEFT + 2: {
       T acc = (T)(3.0 / 2.0) * s[0];
-      acc += (T)(-4.0 / 2.0) * s[-st];
-      acc += (T)(1.0 / 2.0) * s[-2 * st];
+      acc = fmx((T)(-4.0 / 2.0), s[-st], acc);
+      acc = fmx((T)(1.0 / 2.0), s[-2 * st], acc);
       return acc;
     }
     default: {
       T acc = (T)(1.0 / 2.0) * s[st];
-      acc += (T)(-1.0 / 2.0) * s[-st];
+      acc = fmx((T)(-1.0 / 2.0), s[-st], acc);
       return acc;
     }
   }

@@ -376,8 +376,13 @@ __global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__
 
 // ------------------------------------------------------------------ fused
 
-template <typename T, bool MEANF, bool HOLES, int KIND, int R, typename AT>
-__device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int tile, StageLds<T, R> &s) {
+// INNER: the tile and its halo lie inside a block without holes (interior
+// stencils only, no bounds tests); stage_tile below picks the instance.
+template <typename T, bool MEANF, bool HOLES, int KIND, int R, bool INNER, typename AT>
+__device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, int tile, StageLds<T, R> &s) {
+  // every fusion is written out (fmx): the interior and edge instances of
+  // this tile, and the batched kernels, round alike
+#pragma clang fp contract(off)
   constexpr int TY = 4 * R;
   constexpr int RS = ST_TX + 2 * ST_H;
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -387,7 +392,7 @@ __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int 
 
   // a tile whose halo lies inside the block needs no bounds checks and has
   // interior stencils only
-  const bool inner = !HOLES && i0 >= ST_H && i0 + ST_TX + ST_H <= ni && j0 >= ST_H && j0 + TY + ST_H <= nj;
+  constexpr bool inner = INNER;
   // (1) tile values: rows ty + 4k (k = 0..R), columns tx and, for tx < 4, 64 + tx
   T ta[4][R + 1], tb[4][R + 1];
   const bool xtra = tx < 2 * ST_H;
@@ -464,8 +469,8 @@ __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int 
     T fx[4], fy[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      fx[v] = (T)((mxx * (double)dx[v] + myx * (double)dy[v]) * J);
-      fy[v] = (T)((mxy * (double)dx[v] + myy * (double)dy[v]) * J);
+      fx[v] = (T)(fmx(mxx, (double)dx[v], myx * (double)dy[v]) * J);
+      fy[v] = (T)(fmx(mxy, (double)dx[v], myy * (double)dy[v]) * J);
     }
     // LEE right-hand side (spatial:2053-2130), reference term order
     const T sv = yc[0], uv = yc[1], vv = yc[2], pv = yc[3];
@@ -473,23 +478,23 @@ __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int 
     if (!MEANF) {
       const T u0 = (T)a.u0, v0 = (T)a.v0, gp0 = (T)a.gp0;
       rs = -(u0 * fx[0]);
-      rs -= v0 * fy[0];
+      rs = fmx(-v0, fy[0], rs);
       ru = -(u0 * fx[1]);
-      ru -= v0 * fy[1];
+      ru = fmx(-v0, fy[1], ru);
       rv = -(u0 * fx[2]);
-      rv -= v0 * fy[2];
+      rv = fmx(-v0, fy[2], rv);
       if (sizeof(T) == 8) {
         ru -= fx[3] / (T)a.rhob0;
         rv -= fy[3] / (T)a.rhob0;
       } else {
         const T irb = (T)(1.0 / a.rhob0);
-        ru -= fx[3] * irb;
-        rv -= fy[3] * irb;
+        ru = fmx(-fx[3], irb, ru);
+        rv = fmx(-fy[3], irb, rv);
       }
       rp = -(u0 * fx[3]);
-      rp -= gp0 * fx[1];
-      rp -= v0 * fy[3];
-      rp -= gp0 * fy[2];
+      rp = fmx(-gp0, fx[1], rp);
+      rp = fmx(-v0, fy[3], rp);
+      rp = fmx(-gp0, fy[2], rp);
     } else {
       // per-point mean coefficients (MC_* order): u, v, gamma p, 1/rho,
       // the eight mean gradients, rho gi / p and rho (all at the point)
@@ -501,31 +506,32 @@ __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int 
       const T gpx = mc[MC_GPX * n + p], gpy = mc[MC_GPY * n + p];
       T rho;
       if (KIND == STAGE_FIRST) rho = a.rho_in[p];
-      else rho = pv * mc[MC_RP * n + p] - sv * mc[MC_RB * n + p];
+      else rho = fmx(pv, mc[MC_RP * n + p], -(sv * mc[MC_RB * n + p]));
       const T g = (T)a.gamma;
       const T ru_fac = rho * irb;
+      const T uf = fmx(ru_fac, u0, uv), vf = fmx(ru_fac, v0, vv), gpv = g * pv;
       rs = -(u0 * fx[0]);
-      rs -= uv * gsx;
-      rs -= v0 * fy[0];
-      rs -= vv * gsy;
+      rs = fmx(-uv, gsx, rs);
+      rs = fmx(-v0, fy[0], rs);
+      rs = fmx(-vv, gsy, rs);
       ru = -(u0 * fx[1]);
-      ru -= (uv + ru_fac * u0) * gux;
-      ru -= v0 * fy[1];
-      ru -= (vv + ru_fac * v0) * guy;
-      ru -= fx[3] * irb;
+      ru = fmx(-uf, gux, ru);
+      ru = fmx(-v0, fy[1], ru);
+      ru = fmx(-vf, guy, ru);
+      ru = fmx(-fx[3], irb, ru);
       rv = -(u0 * fx[2]);
-      rv -= (uv + ru_fac * u0) * gvx;
-      rv -= v0 * fy[2];
-      rv -= (vv + ru_fac * v0) * gvy;
-      rv -= fy[3] * irb;
+      rv = fmx(-uf, gvx, rv);
+      rv = fmx(-v0, fy[2], rv);
+      rv = fmx(-vf, gvy, rv);
+      rv = fmx(-fy[3], irb, rv);
       rp = -(u0 * fx[3]);
-      rp -= uv * gpx;
-      rp -= gp * fx[1];
-      rp -= g * pv * gux;
-      rp -= v0 * fy[3];
-      rp -= vv * gpy;
-      rp -= gp * fy[2];
-      rp -= g * pv * gvy;
+      rp = fmx(-uv, gpx, rp);
+      rp = fmx(-gp, fx[1], rp);
+      rp = fmx(-gpv, gux, rp);
+      rp = fmx(-v0, fy[3], rp);
+      rp = fmx(-vv, gpy, rp);
+      rp = fmx(-gp, fy[2], rp);
+      rp = fmx(-gpv, gvy, rp);
     }
     // physical source (source.cpp:22-107), pressure equation
     if (gi >= a.si0 && gi <= a.si1 && gj >= a.sj0 && gj <= a.sj1) {
@@ -538,10 +544,10 @@ __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int 
       const auto &zz = a.z[z];
       if (gi >= zz.i0 && gi <= zz.i1 && gj >= zz.j0 && gj <= zz.j1) {
         const T sg = zz.sig[(gj - zz.j0) * zz.nzi + (gi - zz.i0)];
-        rs += -sg * sv;
-        ru += -sg * uv;
-        rv += -sg * vv;
-        rp += -sg * pv;
+        rs = fmx(-sg, sv, rs);
+        ru = fmx(-sg, uv, ru);
+        rv = fmx(-sg, vv, rv);
+        rp = fmx(-sg, pv, rp);
       }
     }
     // RK stage update (temporal:208-230)
@@ -554,18 +560,31 @@ __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int 
       if (!hole) {
         const T rh = r[v] * sc.h;
         const T acc_in = KIND == STAGE_FIRST ? yc[v] : pacc[k][v];
-        accv = acc_in + sc.b * rh;
+        accv = fmx(sc.b, rh, acc_in);
         if (KIND != STAGE_LAST) {
           const T yn = KIND == STAGE_FIRST ? yc[v] : pyn[k][v];
-          outv = yn + sc.a * rh;
+          outv = fmx(sc.a, rh, yn);
         }
       }
       a.acc[q] = accv;
       if (KIND != STAGE_LAST) a.yout[q] = outv;
       r[v] = accv;
     }
-    if (MEANF && KIND == STAGE_LAST) a.rho_out[p] = r[3] * a.mc[MC_RP * n + p] - r[0] * a.mc[MC_RB * n + p];
+    if (MEANF && KIND == STAGE_LAST) a.rho_out[p] = fmx(r[3], a.mc[MC_RP * n + p], -(r[0] * a.mc[MC_RB * n + p]));
   }
+}
+
+template <typename T, bool MEANF, bool HOLES, int KIND, int R, typename AT>
+__device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int tile, StageLds<T, R> &s) {
+  if constexpr (!HOLES) {
+    constexpr int TY = 4 * R;
+    const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
+    if (i0 >= ST_H && i0 + ST_TX + ST_H <= a.ni && j0 >= ST_H && j0 + TY + ST_H <= a.nj) {
+      stage_tile_k<T, MEANF, HOLES, KIND, R, true>(a, sc, tile, s);
+      return;
+    }
+  }
+  stage_tile_k<T, MEANF, HOLES, KIND, R, false>(a, sc, tile, s);
 }
 
 template <typename T, bool MEANF, bool HOLES, int KIND, int R>
