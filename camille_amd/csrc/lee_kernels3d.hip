@@ -26,7 +26,10 @@ namespace lee {
 namespace {
 
 constexpr int S3_TY = 4;      // tile rows in j
-constexpr int Z_CHUNK = 32;   // planes per z chunk
+#ifndef LEE_Z_CHUNK
+#define LEE_Z_CHUNK 32
+#endif
+constexpr int Z_CHUNK = LEE_Z_CHUNK;  // planes per z chunk
 
 inline unsigned nb3(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
@@ -138,31 +141,61 @@ __global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
     for (int v = 0; v < 5; v++) w[v][q] = ok ? a.yin[v * n + (size_t)kk * plane + col] : (T)0;
   }
   for (int k = k0; k < k1; k++) {
+    // every global operand of plane k is requested before the barrier, so
+    // one memory latency per plane is exposed instead of three
     {
       const int kk = k + 2;
       const bool ok = own && kk < nk;
 #pragma unroll
       for (int v = 0; v < 5; v++) w[v][4] = ok ? a.yin[v * n + (size_t)kk * plane + col] : (T)0;
     }
-    // plane k with its (i, j) halo into LDS: rows ty + 4m, columns tx and 64 + tx
+    // plane k with its (i, j) halo: rows ty + 4m, columns tx and 64 + tx
+    T tv[2][2][5];
 #pragma unroll
     for (int m = 0; m < 2; m++) {
-      const int r = ty + 4 * m;
-      const int rj = j0 - ST_H + r;
+      const int rj = j0 - ST_H + ty + 4 * m;
 #pragma unroll
       for (int half = 0; half < 2; half++) {
         const int c = half == 0 ? tx : ST_TX + tx;
-        if (half == 1 && tx >= 2 * ST_H) continue;
         const int ri = i0 - ST_H + c;
-        const bool ok = rj >= 0 && rj < nj && ri >= 0 && ri < ni;
+        const bool ok = (half == 0 || tx < 2 * ST_H) && rj >= 0 && rj < nj && ri >= 0 && ri < ni;
         const size_t q = (size_t)k * plane + (size_t)(ok ? rj : 0) * ni + (ok ? ri : 0);
 #pragma unroll
-        for (int v = 0; v < 5; v++) s[v][r][c] = ok ? a.yin[v * n + q] : (T)0;
+        for (int v = 0; v < 5; v++) tv[m][half][v] = ok ? a.yin[v * n + q] : (T)0;
       }
     }
+    // per-point operands (a clamped column for threads outside the block)
+    const size_t p = (size_t)k * plane + col;
+    double m[3][3];
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) m[l][i] = a.met[(size_t)(3 * l + i) * n + p];
+    const double J = a.jac[p];
+    T mcv[MC3_COUNT];
+    if (MEANF) {
+#pragma unroll
+      for (int f = 0; f < MC3_COUNT; f++)
+        mcv[f] = (f < MC3_RP || KIND != STAGE_FIRST) ? a.mc[(size_t)f * n + p] : (T)0;
+    }
+    const T rho_first = (MEANF && KIND == STAGE_FIRST) ? a.rho_in[p] : (T)0;
+    T accp[5], ynp[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      accp[v] = KIND != STAGE_FIRST ? a.acc[v * n + p] : (T)0;
+      ynp[v] = KIND == STAGE_MID ? a.yn[v * n + p] : (T)0;
+    }
+#pragma unroll
+    for (int m2 = 0; m2 < 2; m2++)
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+        if (half == 1 && tx >= 2 * ST_H) continue;
+        const int c = half == 0 ? tx : ST_TX + tx;
+#pragma unroll
+        for (int v = 0; v < 5; v++) s[v][ty + 4 * m2][c] = tv[m2][half][v];
+      }
     __syncthreads();
     if (own) {
-      const size_t p = (size_t)k * plane + col;
       const int cx = default_code(gi, ni, 2), cy = default_code(gj, nj, 2), cz = default_code(k, nk, 2);
       T d[5][3], yc[5];
 #pragma unroll
@@ -173,12 +206,6 @@ __global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
         d[v][1] = fd_row<T>(sp, RS, cy);
         d[v][2] = fd_row<T>(&w[v][2], 1, cz);
       }
-      double m[3][3];
-#pragma unroll
-      for (int l = 0; l < 3; l++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) m[l][i] = a.met[(size_t)(3 * l + i) * n + p];
-      const double J = a.jac[p];
       T f[5][3];  // f[v][x_i] = d var / d x_i
 #pragma unroll
       for (int v = 0; v < 5; v++)
@@ -213,13 +240,12 @@ __global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
         }
         r[IP] = rp;
       } else {
-        const T *mc = a.mc;
-        const T ub[3] = {mc[MC3_U * n + p], mc[MC3_V * n + p], mc[MC3_W * n + p]};
-        const T gp = mc[MC3_GP * n + p], irb = mc[MC3_IRB * n + p];
-        auto G = [&](int v, int dd) { return mc[(MC3_G + 3 * v + dd) * n + p]; };
+        const T ub[3] = {mcv[MC3_U], mcv[MC3_V], mcv[MC3_W]};
+        const T gp = mcv[MC3_GP], irb = mcv[MC3_IRB];
+        auto G = [&](int v, int dd) { return mcv[MC3_G + 3 * v + dd]; };
         T rho;
-        if (KIND == STAGE_FIRST) rho = a.rho_in[p];
-        else rho = yc[IP] * mc[MC3_RP * n + p] - yc[IS] * mc[MC3_RB * n + p];
+        if (KIND == STAGE_FIRST) rho = rho_first;
+        else rho = yc[IP] * mcv[MC3_RP] - yc[IS] * mcv[MC3_RB];
         const T fr = rho * irb;
         const T g = (T)a.gamma;
         T rs = 0;
@@ -269,16 +295,15 @@ __global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
       for (int v = 0; v < 5; v++) {
         const size_t q = v * n + p;
         const T rh = r[v] * a.h;
-        const T acc_in = KIND == STAGE_FIRST ? yc[v] : a.acc[q];
+        const T acc_in = KIND == STAGE_FIRST ? yc[v] : accp[v];
         accv[v] = acc_in + a.b * rh;
         a.acc[q] = accv[v];
         if (KIND != STAGE_LAST) {
-          const T yn = KIND == STAGE_FIRST ? yc[v] : a.yn[q];
+          const T yn = KIND == STAGE_FIRST ? yc[v] : ynp[v];
           a.yout[q] = yn + a.a * rh;
         }
       }
-      if (MEANF && KIND == STAGE_LAST)
-        a.rho_out[p] = accv[IP] * a.mc[MC3_RP * n + p] - accv[IS] * a.mc[MC3_RB * n + p];
+      if (MEANF && KIND == STAGE_LAST) a.rho_out[p] = accv[IP] * mcv[MC3_RP] - accv[IS] * mcv[MC3_RB];
     }
     __syncthreads();  // the next plane overwrites the LDS tile
 #pragma unroll
