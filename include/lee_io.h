@@ -1,0 +1,124 @@
+/*
+ * lee_io.h -- host file layer of camille_amd: the reference's input deck,
+ * PLOT3D grid / solution / function files, BC file and Overture overset
+ * file, and its step loop with report and solution-file cadence.  With
+ * these a caller runs an unchanged case directory through the GPU advance
+ * and gets the same output files.
+ *
+ * Pure host code (no device memory except inside lee_run_deck).  Every call
+ * returns an int status (LEE_OK = 0, LEE_EINVAL for a bad file or an
+ * unsupported option) and never exits; the message of the last failed call
+ * on the calling thread is lee_io_last_error().
+ */
+#ifndef CAMILLE_AMD_LEE_IO_H
+#define CAMILLE_AMD_LEE_IO_H
+
+#include "lee.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *lee_io_last_error(void);
+
+/* ---------------------------------------------------------------- deck
+ * The input deck (camille.in) as io::read_inputDeck (src/io/io.cpp:565-625)
+ * and inputDeck::parse_linesInputDeck (src/core/input.cpp:870-999) read it:
+ * blank and '#' lines skipped, a last line without a newline dropped, '#'
+ * starts a comment, a backslash merges the line (minus the character before
+ * the backslash) into the next, tokens split at " =,;:\t", and a name given
+ * twice keeps its later line. */
+typedef struct lee_deck lee_deck;
+int lee_deck_open(const char *path, lee_deck **out);
+void lee_deck_close(lee_deck *d);
+
+/* inputDeck::get_userInput (input.cpp:1040-1290): the `index`-th item after
+ * `name` (keyword NULL) or after `keyword` within name's line, copied to buf
+ * (NUL-terminated, at most cap bytes).  LEE_EINVAL when the name, keyword or
+ * item is missing. */
+int lee_deck_get(const lee_deck *d, const char *name, const char *keyword, int index, char *buf, int cap);
+
+/* The settings UserInput::set_inputDeck (input.cpp:99-284) reads for a LEE
+ * run, mapped to a lee_config with precision FP64 and the fused path (the
+ * caller may change both), plus the run and file settings.  Options outside the
+ * advance's coverage are rejected with the reason. */
+typedef struct {
+  lee_config cfg;
+  int num_time_steps;              /* NUM_TOTAL_TIMESTEP */
+  int report_freq;                 /* REPORT_TIMESTEP */
+  double time_writing_solutions;   /* SOLUTION_WRITING_TIME */
+  int overset;                     /* OVERSET = OVERTURE */
+  int overset_order;               /* OVERSET ORDER_ACCURACY */
+  char grid_file[512];             /* GRID_FILE */
+  char mean_file[512];             /* BASESTATE_FILE */
+  char bc_file[512];               /* BC_FILE */
+  char overset_file[512];          /* OVERSET_FILE */
+} lee_run_settings;
+int lee_deck_settings(const lee_deck *d, lee_run_settings *s);
+
+/* --------------------------------------------------------------- PLOT3D
+ * Whole multi-block binary files, native-endian int32 / float64, no record
+ * markers, i fastest (src/io/plot3d.cpp:183-441 headers, 443-672 grid,
+ * 715-833 solution, 881-989 function, 993-1458 writers):
+ *   header  int nblocks; per block ni, nj, nk (function: ni, nj, nk, nvar)
+ *   grid    per block x[n], y[n], z[n]
+ *   solution per block tau[4] = {0, 1, time_step, time_sol}, then 5 x [n]
+ *   function per block nvar x [n] */
+#define LEE_P3D_GRID 0
+#define LEE_P3D_SOLUTION 1
+#define LEE_P3D_FUNCTION 2
+
+/* Block count and, for up to `cap` blocks, dims[4 b + 0..3] = ni, nj, nk,
+ * nvar (3 for a grid, 5 for a solution). */
+int lee_plot3d_dims(const char *path, int kind, int *nblocks, int *dims, int cap);
+/* Block `iblock` into data (nvar * n doubles); tau (4 doubles) for a
+ * solution file, may be NULL otherwise. */
+int lee_plot3d_read(const char *path, int kind, int iblock, double *data, double *tau);
+/* Writes a whole file: dims as lee_plot3d_dims gives them, data[b] the
+ * nvar * n doubles of block b, tau (solution files) written before each
+ * block's data. */
+int lee_plot3d_write(const char *path, int kind, int nblocks, const int *dims, const double *const *data,
+                     const double *tau);
+
+/* -------------------------------------------------------------- BC file
+ * io::read_bc / extract_bcData_out_of_ (src/io/io.cpp:957-1290): lines of
+ * `iblock type +dir is ie js je ks ke` separated by blanks; '+' the low
+ * face, '-' the high one; an index written '-k' means n-1-k of that block.
+ * dims: 3 per block (ni, nj, nk).  Regions go to out[0..count) and their
+ * block numbers to iblock[]; LEE_EINVAL if more than cap lines. */
+int lee_bc_file_read(const char *path, int nblocks, const int *dims, lee_boundary *out, int *iblock, int cap,
+                     int *count);
+
+/* -------------------------------------------------------------- Overture
+ * The explicit-interpolation file of overset::read_overset_at_headrank
+ * (src/io/overture.cpp:473-680), checked against the blocks' dimension,
+ * count and sizes and the stencil width ORDER + 1, as lee_overset tables
+ * (receivers in file order, which groups them by block). */
+typedef struct lee_overture lee_overture;
+int lee_overture_open(const char *path, int num_dim, int nblocks, const int *dims, int order, lee_overture **out);
+const lee_overset *lee_overture_tables(const lee_overture *o);
+void lee_overture_close(lee_overture *o);
+
+/* ------------------------------------------------------------------ run
+ * The reference's executable for a LEE case (simulation::initialize,
+ * solver::solve src/solver/solver.cpp:41-89, io::report_timeadvancing and
+ * io::write_solution_on_the_fly src/io/io.cpp:442-518, 850-877): reads the
+ * deck and the files it names (relative names resolve against the deck's
+ * folder), the restart file (NULL: zero state at step 0, time 0), writes
+ * the set-up files of simulation.cpp:221-226 into out_dir (camille.xyz,
+ * camille.000000.q, mean_camille.q and their .nam files; not the metrics
+ * or auxiliary files), advances to NUM_TOTAL_TIMESTEP, and at every
+ * REPORT_TIMESTEP step prints
+ * "Time step: ..., time: ..., dt: ..., CFL: ..." (unless quiet), appends the
+ * min/max line to out_dir/report/solution/solution and writes
+ * out_dir/camille.NNNNNN.q plus camille.nam when the writing cadence says so.
+ * precision / path select the advance (LEE_FP64 with LEE_PATH_PLAIN writes
+ * the reference's bits). */
+int lee_run_deck(const char *deck_path, const char *restart_path, int precision, int path, const char *out_dir,
+                 int quiet);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

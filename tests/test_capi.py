@@ -13,8 +13,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    txt = open(os.path.join(ROOT, "include", "lee.h")).read()
-    return sorted(set(re.findall(r"\b(lee_[a-z_]+)\s*\(", txt)))
+    txt = "".join(open(os.path.join(ROOT, "include", f)).read() for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+                  if f.endswith(".h"))
+    return sorted(set(re.findall(r"\b(lee_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_loads_and_exports_every_declared_symbol():

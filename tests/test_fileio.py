@@ -1,0 +1,232 @@
+"""Host file layer (include/lee_io.h): the reference's input deck, PLOT3D,
+BC and Overture files, and lee_run_deck over a case folder.
+
+The readers are checked against the reference's reading rules
+(src/core/input.cpp:870-999, src/io/io.cpp:520-625, 957-1290,
+src/io/plot3d.cpp, src/io/overture.cpp:473-680) on hand-made files whose
+expected values follow from those rules, and against the independent
+Python writers/readers the golden fixtures were made with (refio,
+overset_gen).  The GPU test runs whole golden case folders through
+lee_run_deck on the plain path and compares the written solution file
+with the one the reference binary wrote, bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import refio
+from camille_amd import FP64, PATH_PLAIN, LeeError, parse_bc_line
+from camille_amd import fileio
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+    return str(path)
+
+
+def test_deck_reading_rules(tmp_path):
+    text = ("# a comment line\n"
+            "   # an indented comment line\n"
+            "\n"
+            "DIMENSION = 2\n"
+            "SPATIAL_SCHEME = STANDARD_CENTRAL, ORDER_ACCURACY = 4   # trailing comment\n"
+            "BUFFER_ZONE = POLYNOMIAL_ORDER = 2, \\\n"
+            "  CONSTANT = 5.0\n"
+            "SPLIT = 12\\\n"
+            "3\n"
+            "CFL = 0.3\n"
+            "CFL=0.5;\n"
+            "TABS\t:\t7\n"
+            "LAST = 9")
+    d = fileio.Deck(_write(tmp_path / "camille.in", text))
+    assert d.get("DIMENSION") == "2"
+    assert d.get("SPATIAL_SCHEME") == "STANDARD_CENTRAL"
+    assert d.get("SPATIAL_SCHEME", "ORDER_ACCURACY") == "4"
+    # the continued line: the blank before the backslash is dropped with it
+    assert d.get("BUFFER_ZONE", "POLYNOMIAL_ORDER") == "2"
+    assert d.get("BUFFER_ZONE", "CONSTANT") == "5.0"
+    # ... and so is a digit: "12\" + "3" reads 13 (input.cpp:930, substr(0, ibackslash-1))
+    assert d.get("SPLIT") == "13"
+    assert d.get("CFL") == "0.5"          # a repeated name keeps its later line
+    assert d.get("TABS") == "7"
+    with pytest.raises(LeeError, match="does not match any of the names"):
+        d.get("LAST")                     # last line without a newline is never read
+    with pytest.raises(LeeError, match="keyword BLEND"):
+        d.get("SPATIAL_SCHEME", "BLEND")
+    with pytest.raises(LeeError, match="too few values"):
+        d.get("DIMENSION", index=1)
+    d.close()
+
+
+def _deck_for(case, tmp_path, **over):
+    items = cases.deck_items(case)
+    items = [(k, over.pop(k, v)) for k, v in items] + list(over.items())
+    refio.write_deck(str(tmp_path / "camille.in"), items)
+    return str(tmp_path / "camille.in")
+
+
+def test_deck_settings_map_the_case(tmp_path):
+    c = cases.case_shear()
+    st = fileio.Deck(_deck_for(c, tmp_path)).settings()
+    cfg = st["cfg"]
+    assert cfg["num_dim"] == 2 and cfg["gamma"] == cases.GAMMA and cfg["fd_order"] == c["fd_order"]
+    assert cfg["filter_order"] == c["filter_order"] and cfg["filter_blend"] == c["filter_blend"]
+    assert cfg["rk_stages"] == c["rk"] and cfg["fix_cfl"] == (c["mode"] == "FIX_CFL")
+    assert cfg["cfl_or_dt"] == (c["cfl"] if c["mode"] == "FIX_CFL" else c["dt"])
+    assert cfg["sponge_order"] == c["sponge_order"] and cfg["sponge_constant"] == c["sponge_constant"]
+    assert st["num_time_steps"] == c["nsteps"] and st["report_freq"] == c["nsteps"]
+    assert (st["grid_file"], st["mean_file"], st["bc_file"]) == ("grid.xyz", "mean.q", "bc.in")
+    assert st["overset"] == 0 and st["time_writing_solutions"] == 1.0e6
+    jet = fileio.Deck(_deck_for(cases.case_jet(), tmp_path)).settings()
+    assert jet["cfg"]["source_kind"] == 2
+
+
+@pytest.mark.parametrize("key,value,msg", [
+    ("SPATIAL_FILTER", "SFO11P, STRENGTH = 0.2, BLEND = 1.0", "The filter SFO11P is not supported"),
+    ("SPATIAL_SCHEME", "FDO11P", "SPATIAL_SCHEME = FDO11P"),
+    ("AXISYMMETRY", "TRUE", "AXISYMMETRY = TRUE"),
+    ("FIX_DT_OR_CFL", "FIX_NOTHING", "You may fix either DT or CFL"),
+    ("HARMONIC_WAVE", "PLANE", "HARMONIC_WAVE"),
+    ("OVERSET", "CHIMERA", "unsupported format for overset"),
+])
+def test_deck_settings_reject_what_the_advance_lacks(tmp_path, key, value, msg):
+    with pytest.raises(LeeError, match=msg):
+        fileio.Deck(_deck_for(cases.case_c1(), tmp_path, **{key: value})).settings()
+
+
+def test_plot3d_round_trips_with_the_fixture_writers(tmp_path):
+    rng = np.random.default_rng(1)
+    dims = [(7, 5, 1), (4, 6, 3)]
+    grids = [tuple(rng.standard_normal((nk, nj, ni)) for _ in range(3)) for ni, nj, nk in dims]
+    sols = [rng.standard_normal((5, nk, nj, ni)) for ni, nj, nk in dims]
+    funcs = [rng.standard_normal((11, nk, nj, ni)) for ni, nj, nk in dims]
+    # fixture writers -> C++ readers
+    refio.write_grid(str(tmp_path / "a.xyz"), grids)
+    refio.write_solution(str(tmp_path / "a.q"), sols, 17, 2.5)
+    refio.write_function(str(tmp_path / "a.f"), funcs)
+    assert fileio.plot3d_dims(str(tmp_path / "a.xyz"), fileio.GRID) == [d + (3,) for d in dims]
+    assert fileio.plot3d_dims(str(tmp_path / "a.f"), fileio.FUNCTION) == [d + (11,) for d in dims]
+    for b in range(2):
+        assert np.array_equal(fileio.plot3d_read(str(tmp_path / "a.xyz"), fileio.GRID, b), np.stack(grids[b]))
+        q, tau = fileio.plot3d_read(str(tmp_path / "a.q"), fileio.SOLUTION, b)
+        assert np.array_equal(q, sols[b]) and list(tau) == [0.0, 1.0, 17.0, 2.5]
+        assert np.array_equal(fileio.plot3d_read(str(tmp_path / "a.f"), fileio.FUNCTION, b), funcs[b])
+    # C++ writers -> fixture readers, byte-identical files
+    fileio.plot3d_write(str(tmp_path / "b.q"), fileio.SOLUTION, sols, [0.0, 1.0, 17.0, 2.5])
+    fileio.plot3d_write(str(tmp_path / "b.f"), fileio.FUNCTION, funcs)
+    fileio.plot3d_write(str(tmp_path / "b.xyz"), fileio.GRID, [np.stack(g) for g in grids])
+    for a, b in (("a.q", "b.q"), ("a.f", "b.f"), ("a.xyz", "b.xyz")):
+        assert open(tmp_path / a, "rb").read() == open(tmp_path / b, "rb").read()
+    with pytest.raises(LeeError, match="block 2 is not in"):
+        fileio.plot3d_read(str(tmp_path / "a.q"), fileio.SOLUTION, 2)
+
+
+def test_bc_file_matches_the_python_reader(tmp_path):
+    lines = ["# regions", "0 sponge_freund_ambient +xi 0 9 0 -0 0 0",
+             "  0 SPONGE_FREUND_AMBIENT -eta 0 -0 -9 -0 0 0", "", "1 wall_slip_kinematic +eta 0 -0 0 0 0 0",
+             "1 neumann -xi -0 -0 +2 -3 0 0 extra entries are ignored"]
+    path = _write(tmp_path / "bc.in", "\n".join(lines) + "\n")
+    dims = [(30, 20, 1), (12, 8, 1)]
+    got = fileio.bc_file_read(path, dims)
+    want = [parse_bc_line(ln.split()[:9], dims[int(ln.split()[0])]) for ln in lines[1:] if ln.strip()
+            and not ln.strip().startswith("#")]
+    assert [(g[0], g[1], g[2], g[3], g[4], g[5]) for g in got] == [tuple(w) for w in want]
+    for bad, msg in (("0 sponge_freund_ambient +xi 0 9 0 -0 0", "Number of entries"),
+                     ("0 sponge_freund_ambient *xi 0 9 0 -0 0 0", "Unknown side"),
+                     ("0 sponge_freund_ambient +chi 0 9 0 -0 0 0", "Unknown direction"),
+                     ("0 sponge_freund_ambient +xi 0 9 0 -0 0 #0", "comment sign"),
+                     ("5 sponge_freund_ambient +xi 0 9 0 -0 0 0", "names block 5"),
+                     ("0 dirichlet_harmonicwave +xi 0 9 0 -0 0 0", "unsupported boundary type")):
+        with pytest.raises(LeeError, match=msg):
+            fileio.bc_file_read(_write(tmp_path / "bad.in", bad + "\n"), dims)
+
+
+def test_overture_reader_matches_the_tables(tmp_path):
+    import overset_cases
+    c = overset_cases.case_overset_small()
+    dims = [(b["x"].shape[1], b["x"].shape[0], 1) for b in c["blocks"]]
+    path = str(tmp_path / "overset.bin")
+    refio.write_overture(path, 2, dims, c["recv"], c["widths"], c["holes"])
+    got = fileio.overture_read(path, 2, dims, c["order"])
+    want = overset_cases.abi_tables(c)
+    for k in ("recv_block", "recv_l", "donor_block", "donor_l", "hole_block", "hole_l"):
+        assert np.array_equal(got[k], np.asarray(want[k])), k
+    assert np.array_equal(got["widths"], np.asarray(want["widths"]))
+    assert np.array_equal(got["coef"], np.asarray(want["coef"]))
+    with pytest.raises(LeeError, match="ORDER_ACCURACY"):
+        fileio.overture_read(path, 2, dims, 2)
+    with pytest.raises(LeeError, match="block size"):
+        fileio.overture_read(path, 2, [(d[0] + 1, d[1], 1) for d in dims], c["order"])
+
+
+def _golden(name):
+    return np.load(os.path.join(HERE, "golden", f"{name}.npz"), allow_pickle=False)
+
+
+def _case_folder(name, d):
+    """The reference's input files for a golden case, as make_golden.py
+    writes them."""
+    import overset_cases
+    if name in ("overset_small", "cylinder_small"):
+        c = overset_cases.case_overset_small() if name == "overset_small" else overset_cases.case_cylinder_small()
+        blocks = c["blocks"]
+        refio.write_grid(os.path.join(d, "grid.xyz"),
+                         [(b["x"][None], b["y"][None], np.zeros_like(b["x"])[None]) for b in blocks])
+        refio.write_function(os.path.join(d, "mean.q"), [b["mean"][:, None] for b in blocks])
+        refio.write_solution(os.path.join(d, "ic.q"), [b["ic"][:, None] for b in blocks], 0, 0.0)
+        refio.write_bc(os.path.join(d, "bc.in"), c["bc"])
+        if "geo" in c:
+            from camille_amd import overset_gen
+            overset_gen.write_overture(os.path.join(d, "overset.bin"), c["geo"])
+        else:
+            dims = [(b["x"].shape[1], b["x"].shape[0], 1) for b in blocks]
+            refio.write_overture(os.path.join(d, "overset.bin"), 2, dims, c["recv"], c["widths"], c["holes"])
+        items = [(k, v) for k, v in cases.deck_items(c) if k != "OVERSET"]
+        items.insert(12, ("OVERSET", f"OVERTURE, TYPE = EXPLICIT, ORDER_ACCURACY = {c['order']}"))
+        items.append(("OVERSET_FILE", "overset.bin"))
+        nb = len(blocks)
+    else:
+        c = cases.ALL[name]()
+        x, y = c["x"], c["y"]
+        refio.write_grid(os.path.join(d, "grid.xyz"), [(x[None], y[None], np.zeros_like(x)[None])])
+        refio.write_function(os.path.join(d, "mean.q"), [c["mean"][:, None]])
+        refio.write_solution(os.path.join(d, "ic.q"), [c["ic"][:, None]], 0, 0.0)
+        refio.write_bc(os.path.join(d, "bc.in"), c["bc"])
+        items = cases.deck_items(c)
+        nb = 1
+    refio.write_deck(os.path.join(d, "camille.in"), items)
+    return c, nb
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["c1", "shear", "wall", "jet", "overset_small", "cylinder_small"])
+def test_run_deck_writes_the_reference_solution_file(tmp_path, name):
+    """A golden case folder through lee_run_deck (plain FP64): the final
+    camille.000001.q holds the reference's bits and tau; the set-up files
+    hold the inputs."""
+    d = str(tmp_path)
+    c, nb = _case_folder(name, d)
+    g = _golden(name)
+    fileio.run_deck(os.path.join(d, "camille.in"), os.path.join(d, "ic.q"), FP64, PATH_PLAIN, d)
+    out = os.path.join(d, "camille.000001.q")
+    for b in range(nb):
+        q, tau = fileio.plot3d_read(out, fileio.SOLUTION, b)
+        if nb == 1:  # single-block fixtures keep the variables that are not all zero
+            ref = np.zeros(q.shape[:1] + q.shape[2:])
+            ref[g["vars"]] = g["sol"]
+        else:
+            ref = g[f"sol{b}"]
+        assert np.array_equal(q[:, 0], ref), f"block {b}"
+        assert np.array_equal(tau, g["tau"])
+        q0, tau0 = fileio.plot3d_read(os.path.join(d, "camille.000000.q"), fileio.SOLUTION, b)
+        ic = c["ic"] if nb == 1 else c["blocks"][b]["ic"]
+        assert np.array_equal(q0[:, 0], ic) and list(tau0) == [0.0, 1.0, 0.0, 0.0]
+    rep = open(os.path.join(d, "report", "solution", "solution")).read().splitlines()
+    assert len(rep) == 1 and rep[0].startswith(f"Time step: {c['nsteps']:10d}, time: ")
+    assert open(os.path.join(d, "camille.nam")).read().split() == ["S'", "UX'", "UY'", "UZ'", "P'"]
+    print(f"run_deck {name}: {nb} block(s), {c['nsteps']} steps, bit-identical", flush=True)
