@@ -640,10 +640,14 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
   if (c->num_dim != 2 && c->num_dim != 3) return fail(nullptr, LEE_EINVAL, "DIMENSION must be 2 or 3");
   if (c->num_dim == 3 && c->path != LEE_PATH_FUSED)
     return fail(nullptr, LEE_EINVAL, "3-D runs on the fused path (the reference has no 3-D Jacobian)");
-  if (c->fd_order != 4)
-    return fail(nullptr, LEE_EINVAL, "SPATIAL_SCHEME STANDARD_CENTRAL ORDER_ACCURACY must be 4");
-  if (c->filter_order != 0 && c->filter_order != 4 && c->filter_order != 8)
-    return fail(nullptr, LEE_EINVAL, "filter order must be 0 (NONE), 4 or 8");
+  if (c->fd_order != 4 && c->fd_order != 11)
+    return fail(nullptr, LEE_EINVAL, "SPATIAL_SCHEME must be STANDARD_CENTRAL ORDER_ACCURACY 4 or FDO11P (11)");
+  if (c->filter_order != 0 && c->filter_order != 4 && c->filter_order != 8 && c->filter_order != 11)
+    return fail(nullptr, LEE_EINVAL, "filter order must be 0 (NONE), 4, 8 or 11 (SFO11P)");
+  if ((c->fd_order == 11 || c->filter_order == 11) && c->path != LEE_PATH_PLAIN)
+    return fail(nullptr, LEE_EINVAL, "the FDO11P and SFO11P schemes run on the plain path");
+  if (c->filter_order == 11 && !(c->filter_strength >= 0.0 && c->filter_strength <= 1.0))
+    return fail(nullptr, LEE_EINVAL, "SFO11P STRENGTH must lie in [0, 1]");
   if (c->filter_blend < 0.0 || c->filter_blend > 1.0)
     return fail(nullptr, LEE_EINVAL, "The filter-blending parameter is out-of-bound.");
   if (c->rk_stages != 2 && c->rk_stages != 4) return fail(nullptr, LEE_EINVAL, "Unknown temporal scheme.");
@@ -656,7 +660,10 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
   if (c->source_kind == LEE_SOURCE_MONOPOLE && !(c->source_params[5] > 0.0))
     return fail(nullptr, LEE_EINVAL, "monopole half-width must be positive");
   if (nblocks < 1 || nblocks > MAX_BLOCKS) return fail(nullptr, LEE_EINVAL, "block count must be 1..4096");
-  const int minlen = c->filter_order > 0 ? c->filter_order + 1 : 5;
+  // an interior row must fit: 2 half + 1 points for each operator
+  const int fd_len = c->fd_order == 11 ? 11 : 5;
+  const int flt_len = c->filter_order == 11 ? 11 : c->filter_order + 1;
+  const int minlen = std::max(fd_len, flt_len);
   for (int ib = 0; ib < nblocks; ib++) {
     const lee_block &b = blocks[ib];
     if (!b.xyz || !b.mean) return fail(nullptr, LEE_EINVAL, "block without coordinates or base state");
@@ -784,8 +791,8 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
     HIPCHK(hipStreamCreate(&h->stream));
     HIPCHK(hipEventCreate(&h->ev0));
     HIPCHK(hipEventCreate(&h->ev1));
-    make_fd_table(h->fd);
-    if (cfg->filter_order > 0) make_filter_table(h->flt, cfg->filter_order);
+    make_fd_table(h->fd, cfg->fd_order);
+    if (cfg->filter_order > 0) make_filter_table(h->flt, cfg->filter_order, cfg->filter_strength);
     // Butcher tableau (RungeKutta::initialize, temporal:33-85)
     if (cfg->rk_stages == 4) {
       h->b_i[0] = 1.0 / 6.0; h->b_i[1] = 1.0 / 3.0; h->b_i[2] = 1.0 / 3.0; h->b_i[3] = 1.0 / 6.0;

@@ -232,8 +232,8 @@ struct lee_handle {
 namespace lee {
 
 // ---- host-side planning (lee_plan.cpp) ----
-void make_fd_table(OpTable &t);
-void make_filter_table(OpTable &t, int order);
+void make_fd_table(OpTable &t, int order);
+void make_filter_table(OpTable &t, int order, double strength);
 // Replays the reference's operator coverage + iblank_correction scan for one
 // direction; writes per-point codes.  Returns false (and a message) when a
 // closure would cover a hole (the reference asserts, spatial:1794).

@@ -226,14 +226,19 @@ extern "C" int lee_deck_settings(const lee_deck *d, lee_run_settings *s) {
   if (ts == "RUNGE_KUTTA4") c.rk_stages = 4;
   else if (ts == "RUNGE_KUTTA2") c.rk_stages = 2;
   else return io_fail("TEMPORAL_SCHEME = %s is not supported.", ts.c_str());
-  if (fd != "STANDARD_CENTRAL") return io_fail("SPATIAL_SCHEME = %s is not supported by the GPU advance.", fd.c_str());
-  c.fd_order = r.i("SPATIAL_SCHEME", "ORDER_ACCURACY");
+  if (fd == "STANDARD_CENTRAL") c.fd_order = r.i("SPATIAL_SCHEME", "ORDER_ACCURACY");
+  else if (fd == "FDO11P") c.fd_order = 11;
+  else return io_fail("SPATIAL_SCHEME = %s is not supported.", fd.c_str());
   const std::string flt = r.str("SPATIAL_FILTER");
   if (r.rc) return r.rc;
   if (flt == "NONE") {
     c.filter_order = 0;
   } else if (flt == "STANDARD_CENTRAL") {
     c.filter_order = r.i("SPATIAL_FILTER", "ORDER_ACCURACY");
+    c.filter_blend = r.f("SPATIAL_FILTER", "BLEND");
+  } else if (flt == "SFO11P") {
+    c.filter_order = 11;
+    c.filter_strength = r.f("SPATIAL_FILTER", "STRENGTH");
     c.filter_blend = r.f("SPATIAL_FILTER", "BLEND");
   } else {
     return io_fail("The filter %s is not supported.", flt.c_str());
@@ -271,7 +276,8 @@ extern "C" int lee_deck_settings(const lee_deck *d, lee_run_settings *s) {
                   : sim == "CASE_2DJET_WITH_A_HARMONIC_SOURCE" ? LEE_SOURCE_JET2D
                                                                : LEE_SOURCE_NONE;
   c.precision = LEE_FP64;
-  c.path = LEE_PATH_FUSED;
+  // the optimized schemes run on the plain path
+  c.path = (c.fd_order == 11 || c.filter_order == 11) ? LEE_PATH_PLAIN : LEE_PATH_FUSED;
   if (s->report_freq <= 0) return io_fail("REPORT_TIMESTEP must be positive.");
   return LEE_OK;
 }

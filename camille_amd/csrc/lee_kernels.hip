@@ -237,7 +237,7 @@ struct BcArgs {
   int type, dir, end;
   int i0, i1, j0, j1;   // face box (collapsed along dir)
   int ni, nj;
-  double neu[3];        // closure row 0 of the FD operator
+  double neu[9];        // closure row 0 of the FD operator (sb entries)
   int sb;
 };
 
@@ -888,7 +888,7 @@ hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, 
     a.ni = b.ni;
     a.nj = b.nj;
     a.sb = h->fd.sb;
-    for (int q = 0; q < 3; q++) a.neu[q] = h->fd.bnd[0][q];
+    for (int q = 0; q < 9; q++) a.neu[q] = h->fd.bnd[0][q];
     const size_t cnt = (size_t)(a.i1 - a.i0 + 1) * (a.j1 - a.j0 + 1);
     const int8_t *ib = b.has_holes ? b.iblank : nullptr;
     if (fp32)

@@ -14,7 +14,44 @@ namespace lee {
 // StandardCentral::initialize, ORDER_ACCURACY 4 (spatial_discretization.cpp:
 // 28-109): interior {+-1: +-8/12, +-2: -+1/12}; closure rows i=0 {-3/2, 2,
 // -1/2}, i=1 {-1/2, 0, 1/2}; mirrored with a sign flip on the right.
-void make_fd_table(OpTable &t) {
+// OptimizedCentralExplicit::initialize, FDO11P (spatial_discretization.cpp:
+// 228-344): Bogey-Bailly FDo11p interior; closure rows i=0, 1 Berland's FD06
+// and FD15 (7 points), i=2 the standard 5-point row, i=3 FDo7p, i=4 FDo9p,
+// all anchored at the boundary, 9 entries per row; right side mirrored with
+// a sign flip.
+static void make_fdo11p_table(OpTable &t) {
+  t = OpTable{};
+  t.half = 5;
+  t.nb = 5;
+  t.sb = 9;
+  t.sign_right = -1.0;
+  double *c = t.st + t.half;
+  const double in[6] = {0.0, 0.872756993962, -0.286511173973, 0.090320001280, -0.020779405824, 0.002484594688};
+  for (int s = 0; s <= 5; s++) c[s] = in[s];
+  for (int s = 1; s <= 5; s++) c[-s] = -c[s];
+  const double r0[7] = {-2.225833963270, 4.827779580575, -5.001388453836, 3.911103941646, -2.115267458633,
+                        0.718882784412, -0.115276430894};
+  const double r1[7] = {-0.212932721951, -1.060320390770, 2.078926116439, -1.287179452384, 0.685176395471,
+                        -0.245320613994, 0.041650667189};
+  const double r2[5] = {1.0 / 12.0, -8.0 / 12.0, 0.0, 8.0 / 12.0, -1.0 / 12.0};
+  const double r3[7] = {-0.02625099337517627, 0.18833730683403838, -0.797921633542548, 0.0,
+                        0.797921633542548,    -0.18833730683403838, 0.02625099337517627};
+  const double r4[9] = {0.007650904064, -0.059463584768, 0.244678631765, -0.841570125482, 0.0,
+                        0.841570125482, -0.244678631765, 0.059463584768, -0.007650904064};
+  for (int s = 0; s < 7; s++) {
+    t.bnd[0][s] = r0[s];
+    t.bnd[1][s] = r1[s];
+    t.bnd[3][s] = r3[s];
+  }
+  for (int s = 0; s < 5; s++) t.bnd[2][s] = r2[s];
+  for (int s = 0; s < 9; s++) t.bnd[4][s] = r4[s];
+}
+
+void make_fd_table(OpTable &t, int order) {
+  if (order == 11) {
+    make_fdo11p_table(t);
+    return;
+  }
   t = OpTable{};
   t.half = 2;
   t.nb = 2;
@@ -40,7 +77,39 @@ void make_fd_table(OpTable &t) {
 // 1 - d0 at the centre and -d_k off it; closure rows 0 and 1 identity, row 2
 // Lele's 4th-order and row 3 his 6th-order centred filters.  Order 4: Lele's
 // 4th-order centred filter with identity rows 0 and 1.  Right side mirrored.
-void make_filter_table(OpTable &t, int order) {
+// OptimizedCentralFilterExplicit::initialize, SFO11P (spatial_discretization.
+// cpp:348-492): Bogey-Bailly SFo11p interior at strength s (1 - s d0 at the
+// centre, -s d_k off it); closure rows 0 and 1 identity, 2 and 3 Lele's 4th-
+// and 6th-order filters, row 4 the 9-point optimized filter at strength s.
+static void make_sfo11p_table(OpTable &t, double strength) {
+  t = OpTable{};
+  t.half = 5;
+  t.nb = 5;
+  t.sb = 9;
+  t.sign_right = 1.0;
+  double *c = t.st + t.half;
+  const double d[6] = {0.215044884112, -0.187772883589, 0.123755948787, -0.059227575576, 0.018721609157,
+                       -0.002999540835};
+  c[0] = 1.0 - strength * d[0];
+  for (int s = 1; s <= 5; s++) c[s] = d[s] * -strength;
+  for (int s = 1; s <= 5; s++) c[-s] = c[s];
+  t.bnd[0][0] = 1.0;
+  t.bnd[1][1] = 1.0;
+  const double lele4[5] = {(-1.0 / 8.0) / 2.0, (1.0 / 2.0) / 2.0, 5.0 / 8.0, (1.0 / 2.0) / 2.0, (-1.0 / 8.0) / 2.0};
+  const double lele6[7] = {(1.0 / 32.0) / 2.0, (-3.0 / 16.0) / 2.0, (15.0 / 32.0) / 2.0, 11.0 / 16.0,
+                           (15.0 / 32.0) / 2.0, (-3.0 / 16.0) / 2.0, (1.0 / 32.0) / 2.0};
+  for (int s = 0; s < 5; s++) t.bnd[2][s] = lele4[s];
+  for (int s = 0; s < 7; s++) t.bnd[3][s] = lele6[s];
+  const double r4[9] = {0.008228661760, -0.045211119360, 0.120007591680, -0.204788880640, 0.243527493120,
+                        -0.204788880640, 0.120007591680, -0.045211119360, 0.008228661760};
+  for (int s = 0; s < 9; s++) t.bnd[4][s] = s == 4 ? 1.0 - strength * r4[s] : r4[s] * -strength;
+}
+
+void make_filter_table(OpTable &t, int order, double strength) {
+  if (order == 11) {
+    make_sfo11p_table(t, strength);
+    return;
+  }
   t = OpTable{};
   t.half = order / 2;
   t.nb = order / 2;

@@ -59,6 +59,7 @@ class _Config(ctypes.Structure):
         ("source_params", ctypes.c_double * 8),
         ("precision", ctypes.c_int),
         ("path", ctypes.c_int),
+        ("filter_strength", ctypes.c_double),
     ]
 
 
@@ -207,7 +208,8 @@ def read_bc_file(path, dims_per_block):
     return out
 
 
-def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4, mode="FIX_CFL", value=0.5,
+def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, filter_strength=1.0, rk=4,
+                 mode="FIX_CFL", value=0.5,
                  sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49", source_params=(),
                  precision=FP64, path=PATH_FUSED, num_dim=2):
     """lee_config from the deck settings (src/core/input.cpp:99-284 keys)."""
@@ -217,6 +219,7 @@ def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=
     cfg.fd_order = fd_order
     cfg.filter_order = filter_order
     cfg.filter_blend = filter_blend
+    cfg.filter_strength = filter_strength
     cfg.rk_stages = rk
     cfg.fix_cfl = 1 if mode == "FIX_CFL" else 0
     cfg.cfl_or_dt = value
@@ -240,12 +243,14 @@ class Solver:
 
     def __init__(self, blocks, *, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4,
                  mode="FIX_CFL", value=0.5, sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49",
-                 source_params=(), precision=FP64, path=PATH_FUSED, boundaries=(), overset=None, num_dim=2):
+                 source_params=(), precision=FP64, path=PATH_FUSED, boundaries=(), overset=None, num_dim=2,
+                 filter_strength=1.0):
         L = library()
         self.L = L
         self.num_dim = num_dim
         self.cfg = _make_config(gamma=gamma, fd_order=fd_order, filter_order=filter_order,
-                                filter_blend=filter_blend, rk=rk, mode=mode, value=value,
+                                filter_blend=filter_blend, filter_strength=filter_strength, rk=rk, mode=mode,
+                                value=value,
                                 sponge_order=sponge_order, sponge_constant=sponge_constant,
                                 simulation=simulation, source_params=source_params, precision=precision, path=path,
                                 num_dim=num_dim)
@@ -316,7 +321,8 @@ class Solver:
     def from_case(cls, case, precision=FP64, path=PATH_FUSED):
         value = case["cfl"] if case["mode"] == "FIX_CFL" else case["dt"]
         s = cls([dict(x=case["x"], y=case["y"], mean=case["mean"])], fd_order=case["fd_order"],
-                filter_order=case["filter_order"], filter_blend=case["filter_blend"], rk=case["rk"],
+                filter_order=case["filter_order"], filter_blend=case["filter_blend"],
+                filter_strength=case.get("filter_strength", 1.0), rk=case["rk"],
                 mode=case["mode"], value=value, sponge_order=case["sponge_order"],
                 sponge_constant=case["sponge_constant"], simulation=case["simulation"],
                 source_params=case.get("source_params", ()), precision=precision, path=path,

@@ -66,8 +66,10 @@ typedef struct lee_handle lee_handle;
 typedef struct {
   int num_dim;             /* DIMENSION: 2, or 3 (fused path, sponges only, no overset) */
   double gamma;            /* GAMMA_SPECIFICHEAT */
-  int fd_order;            /* SPATIAL_SCHEME STANDARD_CENTRAL ORDER_ACCURACY: 4 */
-  int filter_order;        /* SPATIAL_FILTER STANDARD_CENTRAL ORDER_ACCURACY: 0 (NONE), 4, 8 */
+  int fd_order;            /* SPATIAL_SCHEME STANDARD_CENTRAL ORDER_ACCURACY: 4;
+                              11 = FDO11P (plain path) */
+  int filter_order;        /* SPATIAL_FILTER STANDARD_CENTRAL ORDER_ACCURACY: 0 (NONE), 4, 8;
+                              11 = SFO11P (plain path) */
   double filter_blend;     /* SPATIAL_FILTER BLEND */
   int rk_stages;           /* TEMPORAL_SCHEME: 2 (RUNGE_KUTTA2) or 4 (RUNGE_KUTTA4) */
   int fix_cfl;             /* FIX_DT_OR_CFL: 1 = FIX_CFL, 0 = FIX_DT */
@@ -78,6 +80,8 @@ typedef struct {
   double source_params[8];
   int precision;           /* LEE_FP64 / LEE_FP32 */
   int path;                /* LEE_PATH_FUSED / LEE_PATH_PLAIN */
+  double filter_strength;  /* SPATIAL_FILTER SFO11P STRENGTH (unused by the
+                              standard filter, which runs at strength 1) */
 } lee_config;
 
 /* One BC-file region (src/io/io.cpp:1158-1268) with negative indices

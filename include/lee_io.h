@@ -40,7 +40,8 @@ int lee_deck_get(const lee_deck *d, const char *name, const char *keyword, int i
 
 /* The settings UserInput::set_inputDeck (input.cpp:99-284) reads for a LEE
  * run, mapped to a lee_config with precision FP64 and the fused path (the
- * caller may change both), plus the run and file settings.  Options outside the
+ * plain path for FDO11P / SFO11P; the caller may change both), plus the run
+ * and file settings.  Options outside the
  * advance's coverage are rejected with the reason. */
 typedef struct {
   lee_config cfg;

@@ -126,9 +126,69 @@ typedef struct {
 
 /* ---------------------------------------------------------------- tables */
 
-/* StandardCentral::initialize, solver/spatial_discretization.cpp:28-109 */
+/* OptimizedCentralExplicit::initialize, FDO11P (spatial_discretization.cpp:
+ * 228-344): Bogey-Bailly FDo11p interior; closure rows i=0 and i=1 Berland's
+ * FD06 and FD15 (7 points from the boundary), i=2 the standard 5-point
+ * centred row, i=3 FDo7p, i=4 FDo9p, every row anchored at the boundary;
+ * 9 entries per row, mirrored with a sign flip on the right. */
+static void init_fdo11p(op_t *op) {
+  op->half = 5;
+  op->nb = 5;
+  op->sb = 9;
+  op->sign_right = -1.0;
+  op->ghost = 5; /* input.cpp:698-699 */
+  double *st = op->st + op->half;
+  st[0] = 0.0; st[1] = 0.872756993962; st[2] = -0.286511173973; st[3] = 0.090320001280;
+  st[4] = -0.020779405824; st[5] = 0.002484594688;
+  const double r0[7] = {-2.225833963270, 4.827779580575, -5.001388453836, 3.911103941646, -2.115267458633,
+                        0.718882784412, -0.115276430894};
+  const double r1[7] = {-0.212932721951, -1.060320390770, 2.078926116439, -1.287179452384, 0.685176395471,
+                        -0.245320613994, 0.041650667189};
+  const double r2[5] = {1.0 / 12.0, -8.0 / 12.0, 0.0, 8.0 / 12.0, -1.0 / 12.0};
+  const double r3[7] = {-0.02625099337517627, 0.18833730683403838, -0.797921633542548, 0.0, 0.797921633542548,
+                        -0.18833730683403838, 0.02625099337517627};
+  const double r4[9] = {0.007650904064, -0.059463584768, 0.244678631765, -0.841570125482, 0.0,
+                        0.841570125482, -0.244678631765, 0.059463584768, -0.007650904064};
+  for (int k = 0; k < 7; k++) { op->bnd[0][k] = r0[k]; op->bnd[1][k] = r1[k]; op->bnd[3][k] = r3[k]; }
+  for (int k = 0; k < 5; k++) op->bnd[2][k] = r2[k];
+  for (int k = 0; k < 9; k++) op->bnd[4][k] = r4[k];
+  for (int s = 1; s <= op->half; s++) st[-s] = -st[s];
+}
+
+/* OptimizedCentralFilterExplicit::initialize, SFO11P (spatial_discretization.
+ * cpp:348-492): Bogey-Bailly SFo11p interior at strength sigma (1 - sigma d0
+ * at the centre, -sigma d_k off it); closure rows 0 and 1 identity, 2 and 3
+ * Lele's 4th- and 6th-order filters (at full strength), row 4 the 9-point
+ * optimized filter at strength sigma; mirrored on the right. */
+static void init_sfo11p(op_t *op, double strength) {
+  op->half = 5;
+  op->nb = 5;
+  op->sb = 9;
+  op->sign_right = 1.0;
+  op->ghost = 5; /* input.cpp:718-719 */
+  double *st = op->st + op->half;
+  st[0] = 0.215044884112; st[1] = -0.187772883589; st[2] = 0.123755948787; st[3] = -0.059227575576;
+  st[4] = 0.018721609157; st[5] = -0.002999540835;
+  st[0] = 1.0 - strength * st[0];
+  for (int s = 1; s <= op->half; s++) st[s] *= -strength;
+  op->bnd[0][0] = 1.0;
+  op->bnd[1][1] = 1.0;
+  op->bnd[2][0] = (-1.0 / 8.0) / 2.0; op->bnd[2][1] = (1.0 / 2.0) / 2.0; op->bnd[2][2] = 5.0 / 8.0;
+  op->bnd[2][3] = (1.0 / 2.0) / 2.0; op->bnd[2][4] = (-1.0 / 8.0) / 2.0;
+  op->bnd[3][0] = (1.0 / 32.0) / 2.0; op->bnd[3][1] = (-3.0 / 16.0) / 2.0; op->bnd[3][2] = (15.0 / 32.0) / 2.0;
+  op->bnd[3][3] = 11.0 / 16.0; op->bnd[3][4] = (15.0 / 32.0) / 2.0; op->bnd[3][5] = (-3.0 / 16.0) / 2.0;
+  op->bnd[3][6] = (1.0 / 32.0) / 2.0;
+  const double r4[9] = {0.008228661760, -0.045211119360, 0.120007591680, -0.204788880640, 0.243527493120,
+                        -0.204788880640, 0.120007591680, -0.045211119360, 0.008228661760};
+  for (int k = 0; k < 9; k++) op->bnd[4][k] = k == 4 ? 1.0 - strength * r4[k] : r4[k] * -strength;
+  for (int s = 1; s <= op->half; s++) st[-s] = st[s];
+}
+
+/* StandardCentral::initialize, solver/spatial_discretization.cpp:28-109;
+ * order 11 selects FDO11P */
 static int init_fd(op_t *op, int order) {
   memset(op, 0, sizeof *op);
+  if (order == 11) { init_fdo11p(op); return 0; }
   /* order 2 exists in the reference in name only: its one-entry closure
    * table is written at index 1 (spatial:52,70-71) and runs diverge */
   if (order != 4) { set_err("FD order must be 4 (the reference's order-2 closure is broken)"); return -1; }
@@ -150,9 +210,11 @@ static int init_fd(op_t *op, int order) {
   return 0;
 }
 
-/* StandardCentralFilter::initialize, spatial_discretization.cpp:113-224 */
+/* StandardCentralFilter::initialize, spatial_discretization.cpp:113-224;
+ * order 11 selects SFO11P at the given strength */
 static int init_filter(op_t *op, int order, double strength) {
   memset(op, 0, sizeof *op);
+  if (order == 11) { init_sfo11p(op, strength); return 0; }
   if (order != 4 && order != 8) { set_err("filter order must be 4 or 8"); return -1; }
   op->half = order / 2;
   op->nb = op->half;
@@ -767,9 +829,9 @@ void *orc_create(int nblk, int ndim, double gamma, int fd_order, int filter_orde
   if (init_fd(&o->fd, fd_order)) { free(o); return NULL; }
   o->do_filter = filter_order > 0;
   if (o->do_filter) {
-    /* the standard central filter always runs at full strength (input.cpp:50-53) */
-    (void)filter_strength;
-    if (init_filter(&o->flt, filter_order, 1.0)) { free(o); return NULL; }
+    /* the standard central filter always runs at full strength (input.cpp:50-53);
+     * SFO11P takes the deck's STRENGTH */
+    if (init_filter(&o->flt, filter_order, filter_order == 11 ? filter_strength : 1.0)) { free(o); return NULL; }
   } else {
     o->flt.ghost = 0;
   }

@@ -99,11 +99,13 @@ class Oracle:
     def __init__(self, blocks, scheme, bc_lines=(), source=None, mode="FIX_CFL", value=0.5,
                  overset=None, ndim=2, gamma=1.4):
         """blocks: list of dict(x, y[, z], mean (5,...), ic (5,...)) arrays shaped (nk?, nj, ni)
-        scheme: dict(fd_order, filter_order, filter_blend, rk, sponge_order, sponge_constant)
+        scheme: dict(fd_order, filter_order, filter_blend, rk, sponge_order, sponge_constant
+        [, filter_strength]); order 11 selects FDO11P / SFO11P
         source: None or (kind_name, params[8])."""
         L = lib()
         self.L = L
-        self.h = L.orc_create(len(blocks), ndim, gamma, scheme["fd_order"], scheme["filter_order"], 1.0,
+        self.h = L.orc_create(len(blocks), ndim, gamma, scheme["fd_order"], scheme["filter_order"],
+                              scheme.get("filter_strength", 1.0),
                               scheme.get("filter_blend", 1.0), scheme["rk"], scheme["sponge_order"],
                               scheme["sponge_constant"])
         if not self.h:
@@ -160,6 +162,7 @@ class Oracle:
         blocks = [dict(x=case["x"], y=case["y"], mean=case["mean"], ic=case["ic"])]
         scheme = {k: case[k] for k in ("fd_order", "filter_order", "filter_blend", "rk", "sponge_order",
                                        "sponge_constant")}
+        scheme["filter_strength"] = case.get("filter_strength", 1.0)
         return cls(blocks, scheme, case["bc"], src, case["mode"], value, **kw)
 
     def __del__(self):

@@ -205,8 +205,34 @@ def case_jet():
                 simulation="CASE_2DJET_WITH_A_HARMONIC_SOURCE", **BASE_SCHEME)
 
 
+def case_opt11():
+    """The optimized 11-point schemes (FDO11P derivative, SFO11P filter at
+    strength 0.6) on a 97x81 wavy curvilinear grid with a non-uniform mean
+    (shear layer, entropy and pressure bumps), a Neumann face and a
+    slip wall, 12-point sponges on the other faces; RK4 CFL 0.5, 40 steps."""
+    ni, nj = 97, 81
+    x, y = _wavy(ni, nj, 48.0, 40.0, 0.04)
+    mean = np.zeros((5, nj, ni))
+    mean[0] = _field(x, y, lambda a, b: 0.1 * math.exp(-(b * b) / 16.0))
+    mean[1] = _field(x, y, lambda a, b: 0.25 * (1.0 + math.tanh(b / 3.0)))
+    mean[2] = _field(x, y, lambda a, b: 0.02 * math.sin(0.15 * a))
+    mean[4] = _field(x, y, lambda a, b: (1.0 / GAMMA) * (1.0 + 0.05 * math.exp(-(a * a + b * b) / 64.0)))
+    ic = np.zeros((5, nj, ni))
+    ic[0] = _field(x, y, _pulse(1e-4, 3.0, -2.0, 2.0))
+    ic[4] = _field(x, y, _pulse(1e-3, -4.0, 1.0, 2.0))
+    bc = _sponges(12, sides=("+xi", "-xi")) + [
+        (0, "neumann", "+eta", "0", "-0", "0", "0", "0", "0"),
+        (0, "wall_slip_kinematic", "-eta", "0", "-0", "-0", "-0", "0", "0")]
+    scheme = dict(BASE_SCHEME, fd_order=11, filter_order=11, filter_strength=0.6)
+    return dict(name="opt11", ni=ni, nj=nj, x=x, y=y, mean=mean, ic=ic, bc=bc, mode="FIX_CFL", cfl=0.5,
+                dt=None, nsteps=40, simulation="CASE_TANNA_TPN49", **scheme)
+
+
 ALL = {f.__name__[5:]: f for f in (case_c1, case_curv, case_filt, case_shear, case_cylsrc, case_wall,
-                                    case_variants, case_jet)}
+                                    case_variants, case_jet, case_opt11)}
+
+# cases whose schemes run on the plain path only (FDO11P / SFO11P)
+PLAIN_ONLY = {"opt11"}
 
 
 def deck_items(case, grid_file="grid.xyz", mean_file="mean.q", bc_file="bc.in"):
@@ -228,9 +254,13 @@ def deck_items(case, grid_file="grid.xyz", mean_file="mean.q", bc_file="bc.in"):
         ("TEMPORAL_SCHEME", "RUNGE_KUTTA4" if case["rk"] == 4 else "RUNGE_KUTTA2"),
         ("NUM_TOTAL_TIMESTEP", str(case["nsteps"])),
         ("REPORT_TIMESTEP", str(case["nsteps"])),
-        ("SPATIAL_SCHEME", f"STANDARD_CENTRAL, ORDER_ACCURACY = {case['fd_order']}"),
+        ("SPATIAL_SCHEME", "FDO11P" if case["fd_order"] == 11
+         else f"STANDARD_CENTRAL, ORDER_ACCURACY = {case['fd_order']}"),
     ]
-    if case["filter_order"]:
+    if case["filter_order"] == 11:
+        items.append(("SPATIAL_FILTER", f"SFO11P, STRENGTH = {case['filter_strength']!r}, "
+                                        f"BLEND = {case['filter_blend']!r}"))
+    elif case["filter_order"]:
         items.append(("SPATIAL_FILTER",
                       f"STANDARD_CENTRAL, ORDER_ACCURACY = {case['filter_order']}, BLEND = {case['filter_blend']!r}"))
     else:

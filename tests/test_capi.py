@@ -65,7 +65,7 @@ def _create(cfg, ni=16, nj=16, boundary=None):
 
 @pytest.mark.parametrize("field,value,msg", [
     ("num_dim", 4, "DIMENSION must be 2 or 3"),
-    ("fd_order", 2, "ORDER_ACCURACY must be 4"),
+    ("fd_order", 2, "ORDER_ACCURACY 4 or FDO11P"),
     ("filter_order", 6, "filter order"),
     ("filter_blend", 1.5, "out-of-bound"),
     ("rk_stages", 3, "temporal scheme"),
@@ -80,6 +80,12 @@ def test_create_rejects_bad_config(field, value, msg):
 def test_create_rejects_plain_fp32_and_small_blocks():
     rc, err, _ = _create(_cfg(precision=1, path=1))
     assert rc == 1 and "FP64 only" in err
+    rc, err, _ = _create(_cfg(fd_order=11))
+    assert rc == 1 and "plain path" in err
+    rc, err, _ = _create(_cfg(filter_order=11, filter_strength=1.5, path=1))
+    assert rc == 1 and "STRENGTH" in err
+    rc, err, _ = _create(_cfg(fd_order=11, path=1), ni=10, nj=20)
+    assert rc == 1 and "too small" in err
     rc, err, _ = _create(_cfg(num_dim=3, path=1))
     assert rc == 1 and "fused path" in err
     rc, err, _ = _create(_cfg(), ni=6, nj=20)

@@ -86,9 +86,16 @@ def test_deck_settings_map_the_case(tmp_path):
     assert jet["cfg"]["source_kind"] == 2
 
 
+def test_deck_settings_optimized_schemes(tmp_path):
+    st = fileio.Deck(_deck_for(cases.case_opt11(), tmp_path)).settings()
+    cfg = st["cfg"]
+    assert (cfg["fd_order"], cfg["filter_order"], cfg["filter_strength"]) == (11, 11, 0.6)
+    assert cfg["path"] == 1  # the optimized schemes select the plain path
+
+
 @pytest.mark.parametrize("key,value,msg", [
-    ("SPATIAL_FILTER", "SFO11P, STRENGTH = 0.2, BLEND = 1.0", "The filter SFO11P is not supported"),
-    ("SPATIAL_SCHEME", "FDO11P", "SPATIAL_SCHEME = FDO11P"),
+    ("SPATIAL_FILTER", "COMPACT, BLEND = 1.0", "The filter COMPACT is not supported"),
+    ("SPATIAL_SCHEME", "COMPACT6", "SPATIAL_SCHEME = COMPACT6"),
     ("AXISYMMETRY", "TRUE", "AXISYMMETRY = TRUE"),
     ("FIX_DT_OR_CFL", "FIX_NOTHING", "You may fix either DT or CFL"),
     ("HARMONIC_WAVE", "PLANE", "HARMONIC_WAVE"),
@@ -204,7 +211,7 @@ def _case_folder(name, d):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["c1", "shear", "wall", "jet", "overset_small", "cylinder_small"])
+@pytest.mark.parametrize("name", ["c1", "shear", "wall", "jet", "opt11", "overset_small", "cylinder_small"])
 def test_run_deck_writes_the_reference_solution_file(tmp_path, name):
     """A golden case folder through lee_run_deck (plain FP64): the final
     camille.000001.q holds the reference's bits and tau; the set-up files

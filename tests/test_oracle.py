@@ -125,6 +125,36 @@ def test_filter_properties():
         assert np.array_equal(h[:, col], r[0][:, col])
 
 
+def test_optimized_schemes_properties():
+    """FDO11P (Bogey-Bailly FDo11p interior, Berland FD06/FD15, standard,
+    FDo7p and FDo9p closure rows) is 4th order in every row: exact for
+    degree <= 4 up to the 12-digit coefficients.  SFO11P at strength s
+    passes constants, leaves the odd-even mode at 1 - s in the interior and
+    in closure row 4 (both optimized filters remove it at full strength),
+    and keeps rows 0 and 1 the identity."""
+    s_ = 0.6
+    sch = dict(cases.BASE_SCHEME, fd_order=11, filter_order=11, filter_strength=s_)
+    o, x, y = _box(30, 12, scheme=sch)
+    i = x.copy()
+    for k in range(6):
+        d = o.apply_dir((i ** k)[None], 0, 0)[0]
+        exact = k * i ** (k - 1) if k > 0 else np.zeros_like(i)
+        err = np.max(np.abs(d - exact)) / max(1.0, np.max(np.abs(exact)))
+        if k <= 4:
+            assert err < 1e-10, (k, err)
+        else:
+            assert err > 1e-6  # not 5th order
+    f = o.apply_dir(np.ones((1,) + x.shape), 1, 0)
+    assert np.max(np.abs(f - 1.0)) < 1e-14
+    nyq = ((-1.0) ** x.astype(int))[None]
+    g = o.apply_dir(nyq, 1, 0)[0]
+    assert np.max(np.abs(np.abs(g[:, 4:-4]) - (1.0 - s_))) < 1e-11
+    r = np.random.default_rng(0).standard_normal(x.shape)[None]
+    h = o.apply_dir(r, 1, 0)[0]
+    for col in (0, 1, -1, -2):
+        assert np.array_equal(h[:, col], r[0][:, col])
+
+
 def test_sponge_strength_closed_form():
     """On a Cartesian grid, sigma = C ((W-1-i)/(W-1))^2 across a +xi zone;
     overlapping zones add."""
