@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -548,6 +549,7 @@ int adjoint_check(lee_handle *h) {
   if (h->batch) return fail(h, LEE_EINVAL, "the adjoint runs on single handles, not batches");
   if (is_plain(h)) return fail(h, LEE_EINVAL, "the adjoint runs on the fused path");
   if (h->ov.nrecv > 0) return fail(h, LEE_EINVAL, "the adjoint does not cover overset interpolation");
+  if (h->cfg.axisym) return fail(h, LEE_EINVAL, "the adjoint does not cover the axisymmetric term");
   for (int ib = 0; ib < h->nblk; ib++) {
     if (!h->b[ib].faces.empty())
       return fail(h, LEE_EINVAL, "the adjoint covers sponges only, not other boundary conditions");
@@ -646,6 +648,8 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
     return fail(nullptr, LEE_EINVAL, "filter order must be 0 (NONE), 4, 8 or 11 (SFO11P)");
   if ((c->fd_order == 11 || c->filter_order == 11) && c->path != LEE_PATH_PLAIN)
     return fail(nullptr, LEE_EINVAL, "the FDO11P and SFO11P schemes run on the plain path");
+  if (c->axisym != 0 && c->axisym != 1) return fail(nullptr, LEE_EINVAL, "AXISYMMETRY must be 0 or 1");
+  if (c->axisym && c->num_dim != 2) return fail(nullptr, LEE_EINVAL, "Axisymmetry requires 2-D.");
   if (c->filter_order == 11 && !(c->filter_strength >= 0.0 && c->filter_strength <= 1.0))
     return fail(nullptr, LEE_EINVAL, "SFO11P STRENGTH must lie in [0, 1]");
   if (c->filter_blend < 0.0 || c->filter_blend > 1.0)
@@ -686,6 +690,7 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
         case LEE_BC_CENTERLINE_CART_NORM2X:
         case LEE_BC_CENTERLINE_CART_NORM2Y:
         case LEE_BC_CENTERLINE_CART_NORM2Z:
+        case LEE_BC_CENTERLINE_AXISYM:
         case LEE_SPONGE_FREUND_AMBIENT:
           break;
         default:
@@ -840,6 +845,8 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
         for (int p = 1; p < n && b.uniform_mean; p++)
           if (in.mean[(size_t)v * n + p] != b.umean[v]) b.uniform_mean = false;
       }
+      // the axisymmetric term reads the radius from the coefficient fields
+      if (cfg->axisym) b.uniform_mean = false;
       HIPCHK(dalloc(&b.xyz, 3 * (size_t)n));
       const int D = cfg->num_dim;
       HIPCHK(dalloc(&b.met, (size_t)D * D * n));
@@ -931,11 +938,15 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
           auto G = [&](int v, int d) { return g[(size_t)(2 * v + d) * n + p]; };
           const double f[MC_COUNT] = {in.mean[(size_t)IUX * n + p], in.mean[(size_t)IUY * n + p], gam * pb, 1.0 / rb,
                                       G(IS, 0), G(IS, 1), G(IUX, 0), G(IUX, 1), G(IUY, 0), G(IUY, 1), G(IP, 0),
-                                      G(IP, 1), rb * gi / pb, rb};
+                                      G(IP, 1), rb * gi / pb, rb, in.xyz[(size_t)n + p] + 1e-99};
           for (int c = 0; c < MC_COUNT; c++) mc[(size_t)c * n + p] = f[c];
         }
         if (is_fp32(h)) {
           std::vector<float> m32(mc.begin(), mc.end());
+          // r + 1e-99 is 0 in FP32 on the axis, where the numerator vanishes
+          // (u_r' = ubar_r = 0): the least normal float keeps 0 / r at 0
+          for (int p = 0; D == 2 && p < n; p++)
+            m32[(size_t)MC_R * n + p] = std::max(m32[(size_t)MC_R * n + p], FLT_MIN);
           HIPCHK(dalloc(&b.mc32, mc.size()));
           HIPCHK(hipMemcpy(b.mc32, m32.data(), sizeof(float) * m32.size(), hipMemcpyHostToDevice));
         } else {

@@ -96,6 +96,7 @@ struct StageArgs {
   const T *rho_in;     // stale rho' (first stage, non-uniform mean)
   T *rho_out;          // rho' of the new y_{n+1} (last stage, non-uniform mean)
   double u0, v0, gp0, rhob0, gamma, gi;
+  int axi;             // axisymmetric pressure term (non-uniform mean path)
   int nz;
   ZoneT<T> z[MAX_ZONES];
   int si0, si1, sj0, sj1;

@@ -26,9 +26,10 @@ constexpr int MAX_OV_BLOCKS = 16;    // blocks of an overset run (per-block tabl
 constexpr int CODE_INTERIOR = 0, CODE_ZERO = 1, CODE_LEFT = 2;
 constexpr int STAGE_FIRST = 0, STAGE_MID = 1, STAGE_LAST = 2;  // RK stage kinds
 // Mean-flow coefficient fields of a non-uniform base state, in the bulk
-// type: u, v, gamma p, 1/rho, d(s, u, v, p)/d(x, y), rho gi / p, rho.
+// type: u, v, gamma p, 1/rho, d(s, u, v, p)/d(x, y), rho gi / p, rho, and
+// the radius r + 1e-99 of an axisymmetric run.
 enum { MC_U, MC_V, MC_GP, MC_IRB, MC_GSX, MC_GSY, MC_GUX, MC_GUY, MC_GVX, MC_GVY, MC_GPX, MC_GPY, MC_RP, MC_RB,
-       MC_COUNT };
+       MC_R, MC_COUNT };
 // 3-D: u, v, w, gamma p, 1/rho, d(s, u, v, w, p)/d(x, y, z) (5 x 3), rho gi / p, rho.
 enum { MC3_U, MC3_V, MC3_W, MC3_GP, MC3_IRB, MC3_G, MC3_RP = MC3_G + 15, MC3_RB, MC3_COUNT };
 

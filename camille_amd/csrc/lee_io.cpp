@@ -205,8 +205,16 @@ extern "C" int lee_deck_settings(const lee_deck *d, lee_run_settings *s) {
   if (r.rc) return r.rc;
   std::string ax = axisym;  // truefalse_2int (input.cpp:786-800) compares in upper case
   for (char &ch : ax) ch = (char)toupper((unsigned char)ch);
-  if (ax == "TRUE") return io_fail("AXISYMMETRY = TRUE is not supported by the GPU advance.");
-  if (ax != "FALSE") return io_fail("AXISYMMETRY must be TRUE or FALSE.");
+  if (ax == "TRUE") {
+    std::string axis = r.str("AXISYMMETRY", "AXIS_OF_SYMMETRY");  // xyz_2int (input.cpp:820-838)
+    if (r.rc) return r.rc;
+    for (char &ch : axis) ch = (char)toupper((unsigned char)ch);
+    if (axis != "X") return io_fail("The axis of symmetry should be x by definition of the coordinate system.");
+    if (c.num_dim != 2) return io_fail("Axisymmetry requires 2-D.");
+    c.axisym = 1;
+  } else if (ax != "FALSE") {
+    return io_fail("AXISYMMETRY must be TRUE or FALSE.");
+  }
   if (model != "LEE") return io_fail("PHYSICAL_MODEL = %s is not supported by the GPU advance.", model.c_str());
   if (!(c.gamma > 0.0)) return io_fail("GAMMA_SPECIFICHEAT must be positive.");
   if (fix == "FIX_DT") {
@@ -390,6 +398,7 @@ int bc_type(std::string s) {
   if (s == "centerline_cart_norm2x") return LEE_BC_CENTERLINE_CART_NORM2X;
   if (s == "centerline_cart_norm2y") return LEE_BC_CENTERLINE_CART_NORM2Y;
   if (s == "centerline_cart_norm2z") return LEE_BC_CENTERLINE_CART_NORM2Z;
+  if (s == "centerline_axisym") return LEE_BC_CENTERLINE_AXISYM;
   if (s == "sponge_freund_ambient") return LEE_SPONGE_FREUND_AMBIENT;
   return -1;
 }
@@ -692,8 +701,11 @@ extern "C" int lee_run_deck(const char *deck_path, const char *restart_path, int
   // auxiliary variables are not written)
   const std::string od(out_dir);
   mkdirs(od + "/report/solution");
-  const char *names[5] = {"S'", "UX'", "UY'", "UZ'", "P'"};
-  const char *mnames[5] = {"Sbar", "UXbar", "UYbar", "UZbar", "Pbar"};
+  // State::initialize_state_linearizedEuler (state.cpp:284-312): radial and
+  // azimuthal names in an axisymmetric run
+  const bool ax = st.cfg.axisym != 0;
+  const char *names[5] = {"S'", "UX'", ax ? "UR'" : "UY'", ax ? "UTHETA'" : "UZ'", "P'"};
+  const char *mnames[5] = {"Sbar", "UXbar", ax ? "URbar" : "UYbar", ax ? "UTHETAbar" : "UZbar", "Pbar"};
   auto write_names = [&](const std::string &fn, const char *const *nm) {
     if (FILE *f = fopen(fn.c_str(), "w")) {
       for (int v = 0; v < 5; v++) fprintf(f, "%s\n", nm[v]);

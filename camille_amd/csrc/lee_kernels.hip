@@ -150,6 +150,20 @@ __global__ void __launch_bounds__(TPB) k_sub_velgrad(double *__restrict__ r, con
 }
 
 // apply_physicalSource: rhs[P] += (s1 * g) * s2 over the source support
+// compute_RHS_linearized_Euler's axisymmetric term (spatial:2119-2126):
+// rhs_p -= gamma (pbar u_r' + p' ubar_r) / (r + 1e-99), with u_r = u_y and
+// r = y (the axis of symmetry is x, input.cpp:523-531).
+__global__ void __launch_bounds__(TPB) k_plain_axisym(double *__restrict__ rp, double gam,
+                                                      const double *__restrict__ pb, const double *__restrict__ ur,
+                                                      const double *__restrict__ pp, const double *__restrict__ urb,
+                                                      const double *__restrict__ r, int n) {
+#pragma clang fp contract(off)
+  const int q = blockIdx.x * TPB + threadIdx.x;
+  if (q >= n) return;
+  const double DUMMY_SMALL = 1e-99;
+  rp[q] -= gam * (pb[q] * ur[q] + pp[q] * urb[q]) / (r[q] + DUMMY_SMALL);
+}
+
 __global__ void __launch_bounds__(TPB) k_plain_source(double *__restrict__ rp, const double *__restrict__ g,
                                                       int ni, int i0, int i1, int j0, int j1, double s1,
                                                       double s2) {
@@ -265,7 +279,7 @@ __global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double
   if (a.type == LEE_BC_DIRICHLET_ALLZERO)
     for (int v = 0; v < NV; v++) data[v] = 0.0;
   const bool neumann = a.type == LEE_BC_NEUMANN || a.type == LEE_BC_WALL_SLIP_KINEMATIC ||
-                       (a.type >= LEE_BC_CENTERLINE_CART_NORM2X && a.type <= LEE_BC_CENTERLINE_CART_NORM2Z);
+                       (a.type >= LEE_BC_CENTERLINE_CART_NORM2X && a.type <= LEE_BC_CENTERLINE_AXISYM);
   if (neumann) {
     for (int v = 0; v < NV; v++) {
       if (a.type == LEE_BC_WALL_SLIP_KINEMATIC && v >= IUX && v <= IUZ) continue;
@@ -307,6 +321,7 @@ __global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double
   }
   if (a.type >= LEE_BC_CENTERLINE_CART_NORM2X && a.type <= LEE_BC_CENTERLINE_CART_NORM2Z)
     data[IUX + (a.type - LEE_BC_CENTERLINE_CART_NORM2X)] = 0.0;
+  if (a.type == LEE_BC_CENTERLINE_AXISYM) data[IUY] = 0.0;  // u_r on the axis (bc:498-545, IVAR_UR)
   for (int v = 0; v < NV; v++) y[(size_t)v * n + p] = (T)data[v];
   if (rho) {
     const double pbi = 1.0 / mean[(size_t)IP * n + p];
@@ -532,6 +547,8 @@ __device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, in
       rp = fmx(-vv, gpy, rp);
       rp = fmx(-gp, fy[2], rp);
       rp = fmx(-gpv, gvy, rp);
+      // axisymmetry (spatial:2119-2126): -gamma (pbar v' + p' vbar) / (r + 1e-99)
+      if (a.axi) rp -= fmx(gp, vv, g * v0 * pv) / mc[MC_R * n + p];
     }
     // physical source (source.cpp:22-107), pressure equation
     if (gi >= a.si0 && gi <= a.si1 && gj >= a.sj0 && gj <= a.sj1) {
@@ -829,6 +846,8 @@ hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double 
     hipLaunchKernelGGL(k_sub_gab, g, t, 0, s, R(IP), gam, M(IP), b.dflux, n);
     hipLaunchKernelGGL(k_sub_gab, g, t, 0, s, R(IP), gam, Y(IP), G(IUX + d, d), n);
   }
+  if (h->cfg.axisym)
+    hipLaunchKernelGGL(k_plain_axisym, g, t, 0, s, R(IP), gam, M(IP), Y(IUY), Y(IP), M(IUY), b.xyz + n, n);
   if (b.src.i1 >= b.src.i0) {
     const size_t cnt = (size_t)(b.src.i1 - b.src.i0 + 1) * (b.src.j1 - b.src.j0 + 1);
     hipLaunchKernelGGL(k_plain_source, dim3(nblocks(cnt)), t, 0, s, R(IP), b.src.g, b.ni, b.src.i0, b.src.i1,
@@ -1007,6 +1026,7 @@ static StageArgs<T> make_stage_args(const lee_handle *h, const Block &b, int sta
   a.u0 = b.umean[IUX];
   a.v0 = b.umean[IUY];
   a.gamma = h->cfg.gamma;
+  a.axi = h->cfg.axisym;
   a.gi = 1.0 / h->cfg.gamma;
   a.gp0 = h->cfg.gamma * b.umean[IP];
   {

@@ -24,6 +24,7 @@ BC_TYPES = {
     "centerline_cart_norm2x": 900,
     "centerline_cart_norm2y": 901,
     "centerline_cart_norm2z": 902,
+    "centerline_axisym": 903,
     "sponge_freund_ambient": 1000,
 }
 # SIMULATION key -> physical source
@@ -60,6 +61,7 @@ class _Config(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("path", ctypes.c_int),
         ("filter_strength", ctypes.c_double),
+        ("axisym", ctypes.c_int),
     ]
 
 
@@ -211,7 +213,7 @@ def read_bc_file(path, dims_per_block):
 def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, filter_strength=1.0, rk=4,
                  mode="FIX_CFL", value=0.5,
                  sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49", source_params=(),
-                 precision=FP64, path=PATH_FUSED, num_dim=2):
+                 precision=FP64, path=PATH_FUSED, num_dim=2, axisym=0):
     """lee_config from the deck settings (src/core/input.cpp:99-284 keys)."""
     cfg = _Config()
     cfg.num_dim = num_dim
@@ -220,6 +222,7 @@ def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, fil
     cfg.filter_order = filter_order
     cfg.filter_blend = filter_blend
     cfg.filter_strength = filter_strength
+    cfg.axisym = 1 if axisym else 0
     cfg.rk_stages = rk
     cfg.fix_cfl = 1 if mode == "FIX_CFL" else 0
     cfg.cfl_or_dt = value
@@ -244,7 +247,7 @@ class Solver:
     def __init__(self, blocks, *, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4,
                  mode="FIX_CFL", value=0.5, sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49",
                  source_params=(), precision=FP64, path=PATH_FUSED, boundaries=(), overset=None, num_dim=2,
-                 filter_strength=1.0):
+                 filter_strength=1.0, axisym=0):
         L = library()
         self.L = L
         self.num_dim = num_dim
@@ -253,7 +256,7 @@ class Solver:
                                 value=value,
                                 sponge_order=sponge_order, sponge_constant=sponge_constant,
                                 simulation=simulation, source_params=source_params, precision=precision, path=path,
-                                num_dim=num_dim)
+                                num_dim=num_dim, axisym=axisym)
         self._keep = []
         barr = self._blocks(blocks, boundaries)
         ovp = None
@@ -322,7 +325,7 @@ class Solver:
         value = case["cfl"] if case["mode"] == "FIX_CFL" else case["dt"]
         s = cls([dict(x=case["x"], y=case["y"], mean=case["mean"])], fd_order=case["fd_order"],
                 filter_order=case["filter_order"], filter_blend=case["filter_blend"],
-                filter_strength=case.get("filter_strength", 1.0), rk=case["rk"],
+                filter_strength=case.get("filter_strength", 1.0), axisym=case.get("axisym", 0), rk=case["rk"],
                 mode=case["mode"], value=value, sponge_order=case["sponge_order"],
                 sponge_constant=case["sponge_constant"], simulation=case["simulation"],
                 source_params=case.get("source_params", ()), precision=precision, path=path,

@@ -49,6 +49,7 @@ extern "C" {
 #define LEE_BC_CENTERLINE_CART_NORM2X 900
 #define LEE_BC_CENTERLINE_CART_NORM2Y 901
 #define LEE_BC_CENTERLINE_CART_NORM2Z 902
+#define LEE_BC_CENTERLINE_AXISYM 903         /* the axis r = 0 of an axisymmetric run */
 #define LEE_SPONGE_FREUND_AMBIENT 1000
 
 /* physical sources (SIMULATION key; src/solver/source.cpp:22-107) */
@@ -82,6 +83,8 @@ typedef struct {
   int path;                /* LEE_PATH_FUSED / LEE_PATH_PLAIN */
   double filter_strength;  /* SPATIAL_FILTER SFO11P STRENGTH (unused by the
                               standard filter, which runs at strength 1) */
+  int axisym;              /* AXISYMMETRY TRUE (axis x, r = y; 2-D): the pressure
+                              equation gains -gamma (pbar u_r' + p' ubar_r) / r */
 } lee_config;
 
 /* One BC-file region (src/io/io.cpp:1158-1268) with negative indices

@@ -30,6 +30,7 @@
 #define NV 5
 #define IS_ 0 /* s'  (param.h IVAR_S)  */
 #define IUX 1
+#define IUY 2 /* u_y, also IVAR_UR of an axisymmetric run (param.h:74) */
 #define IP 4
 #define NOTBLANKED 0
 #define BLANKED 1
@@ -117,6 +118,7 @@ typedef struct {
   int src_kind;
   double src[8];
   int do_overset;
+  int axisym;
   ovs_t ov;
   double *dflux, *dflux_sum, *flux, *func0, *func; /* scratch of max n */
   int nmax;
@@ -508,7 +510,7 @@ static void compute_sigma(orc_t *o, blk_t *b, bnd_t *z) {
 
 /* ---------------------------------------------------------------- physics */
 
-/* compute_RHS_linearized_Euler, spatial:2053-2130 (non-axisymmetric) */
+/* compute_RHS_linearized_Euler, spatial:2053-2130 */
 static int rhs_lee(orc_t *o, blk_t *b, double **y, double **rhs) {
   int n = b->n, nd = o->ndim;
   double *df = o->dflux;
@@ -538,6 +540,11 @@ static int rhs_lee(orc_t *o, blk_t *b, double **y, double **rhs) {
     for (int p = 0; p < n; p++) rhs[IP][p] -= g * b->mean[IP][p] * df[p];
     for (int p = 0; p < n; p++) rhs[IP][p] -= g * y[IP][p] * b->mgrad[(IUX + d) * nd + d][p];
   }
+  /* additional terms due to axisymmetry (spatial:2119-2126): IVAR_UR = 2,
+   * RDIR = y, DUMMY_SMALL = 1e-99 (param.h:40-74) */
+  if (o->axisym)
+    for (int p = 0; p < n; p++)
+      rhs[IP][p] -= g * (b->mean[IP][p] * y[IUY][p] + y[IP][p] * b->mean[IUY][p]) / (b->xyz[1][p] + 1e-99);
   return 0;
 }
 
@@ -668,6 +675,8 @@ static void apply_bc(orc_t *o, blk_t *b, double **y) {
             for (int d = 0; d < 3; d++) data[IUX + d] = vc[d];
           }
           if (bc->type >= BC_CL_X && bc->type <= BC_CL_Z) data[IUX + (bc->type - BC_CL_X)] = 0.0;
+          /* BC_CENTERLINE_AXISYM: Neumann, then u_r = 0 (bc:150-156, 534-541) */
+          if (bc->type == 903) data[IUY] = 0.0;
           for (int v = 0; v < NV; v++) y[v][l0] = data[v];
         }
   }
@@ -879,6 +888,14 @@ int orc_set_block(void *h, int ib, int ni, int nj, int nk, const double *xyz, co
     o->dflux = dalloc(n); o->dflux_sum = dalloc(n); o->flux = dalloc(n); o->func0 = dalloc(n); o->func = dalloc(n);
   }
   set_extents(o, b);
+  return 0;
+}
+
+/* AXISYMMETRY = TRUE with AXIS_OF_SYMMETRY = X (input.cpp:105-108, 523-531) */
+int orc_set_axisym(void *h, int on) {
+  orc_t *o = h;
+  if (on && o->ndim != 2) { set_err("Axisymmetry requires 2-D."); return -1; }
+  o->axisym = on;
   return 0;
 }
 

@@ -18,6 +18,7 @@ BC_TYPES = {
     "centerline_cart_norm2x": 900,
     "centerline_cart_norm2y": 901,
     "centerline_cart_norm2z": 902,
+    "centerline_axisym": 903,
     "sponge_freund_ambient": 1000,
 }
 SRC = {"CASE_TANNA_TPN49": 0, "CASE_SCATTERING_TWOCYLINDER": 1,
@@ -38,6 +39,7 @@ def lib():
         L.orc_set_block.argtypes = [vp, i, i, i, i, dp, dp]
         L.orc_add_boundary.argtypes = [vp, i, i, i, i, ip, ip]
         L.orc_set_source.argtypes = [vp, i, dp]
+        L.orc_set_axisym.argtypes = [vp, i]
         L.orc_set_timestep.argtypes = [vp, i, d]
         L.orc_set_overset.argtypes = [vp, i, ip, ip, ip, ip, ip, dp, i, i, ip, ip]
         L.orc_init.argtypes = [vp, dp, i, d]
@@ -100,7 +102,7 @@ class Oracle:
                  overset=None, ndim=2, gamma=1.4):
         """blocks: list of dict(x, y[, z], mean (5,...), ic (5,...)) arrays shaped (nk?, nj, ni)
         scheme: dict(fd_order, filter_order, filter_blend, rk, sponge_order, sponge_constant
-        [, filter_strength]); order 11 selects FDO11P / SFO11P
+        [, filter_strength, axisym]); order 11 selects FDO11P / SFO11P
         source: None or (kind_name, params[8])."""
         L = lib()
         self.L = L
@@ -109,6 +111,8 @@ class Oracle:
                               scheme.get("filter_blend", 1.0), scheme["rk"], scheme["sponge_order"],
                               scheme["sponge_constant"])
         if not self.h:
+            raise RuntimeError(L.orc_error().decode())
+        if scheme.get("axisym") and L.orc_set_axisym(self.h, 1):
             raise RuntimeError(L.orc_error().decode())
         self.shapes = []
         for ib, b in enumerate(blocks):
@@ -163,6 +167,7 @@ class Oracle:
         scheme = {k: case[k] for k in ("fd_order", "filter_order", "filter_blend", "rk", "sponge_order",
                                        "sponge_constant")}
         scheme["filter_strength"] = case.get("filter_strength", 1.0)
+        scheme["axisym"] = case.get("axisym", 0)
         return cls(blocks, scheme, case["bc"], src, case["mode"], value, **kw)
 
     def __del__(self):

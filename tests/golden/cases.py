@@ -228,8 +228,26 @@ def case_opt11():
                 dt=None, nsteps=40, simulation="CASE_TANNA_TPN49", **scheme)
 
 
+def case_axisym():
+    """Axisymmetric LEE (axis x, r = y): 81x49 box x in [-10, 10], r in
+    [0, 12] with the axis at j = 0 (centerline_axisym), a jet-like mean
+    u(r) = 0.15 (1 - tanh(r - 3)), an off-axis pressure pulse at (0, 3),
+    sponges on the x ends and the outer radius; RK4 CFL 0.5, 40 steps."""
+    ni, nj = 81, 49
+    x, y = _grid(ni, nj, lambda i, j: -10.0 + 0.25 * i, lambda i, j: 0.25 * j)
+    mean = np.zeros((5, nj, ni))
+    mean[1] = _field(x, y, lambda a, b: 0.15 * (1.0 - math.tanh(b - 3.0)))
+    mean[4] = 1.0 / GAMMA
+    ic = np.zeros((5, nj, ni))
+    ic[4] = _field(x, y, _pulse(1e-3, 0.0, 3.0, 1.0))
+    bc = _sponges(10, sides=("+xi", "-xi", "-eta")) + [
+        (0, "centerline_axisym", "+eta", "0", "-0", "0", "0", "0", "0")]
+    return dict(name="axisym", ni=ni, nj=nj, x=x, y=y, mean=mean, ic=ic, bc=bc, mode="FIX_CFL", cfl=0.5,
+                dt=None, nsteps=40, simulation="CASE_TANNA_TPN49", axisym=1, **BASE_SCHEME)
+
+
 ALL = {f.__name__[5:]: f for f in (case_c1, case_curv, case_filt, case_shear, case_cylsrc, case_wall,
-                                    case_variants, case_jet, case_opt11)}
+                                    case_variants, case_jet, case_opt11, case_axisym)}
 
 # cases whose schemes run on the plain path only (FDO11P / SFO11P)
 PLAIN_ONLY = {"opt11"}
@@ -239,7 +257,7 @@ def deck_items(case, grid_file="grid.xyz", mean_file="mean.q", bc_file="bc.in"):
     """Deck lines for the reference solver (keys of src/core/input.cpp:99-284)."""
     items = [
         ("DIMENSION", "2"),
-        ("AXISYMMETRY", "FALSE"),
+        ("AXISYMMETRY", "TRUE, AXIS_OF_SYMMETRY = X" if case.get("axisym") else "FALSE"),
         ("PHYSICAL_MODEL", "LEE"),
         ("FLUID_MODEL", "IDEAL_GAS_CALORIC"),
         ("SIMULATION", case["simulation"]),

@@ -86,6 +86,8 @@ def test_create_rejects_plain_fp32_and_small_blocks():
     assert rc == 1 and "STRENGTH" in err
     rc, err, _ = _create(_cfg(fd_order=11, path=1), ni=10, nj=20)
     assert rc == 1 and "too small" in err
+    rc, err, _ = _create(_cfg(num_dim=3, axisym=1))
+    assert rc == 1 and "requires 2-D" in err
     rc, err, _ = _create(_cfg(num_dim=3, path=1))
     assert rc == 1 and "fused path" in err
     rc, err, _ = _create(_cfg(), ni=6, nj=20)

@@ -83,7 +83,8 @@ def test_deck_settings_map_the_case(tmp_path):
     assert (st["grid_file"], st["mean_file"], st["bc_file"]) == ("grid.xyz", "mean.q", "bc.in")
     assert st["overset"] == 0 and st["time_writing_solutions"] == 1.0e6
     jet = fileio.Deck(_deck_for(cases.case_jet(), tmp_path)).settings()
-    assert jet["cfg"]["source_kind"] == 2
+    assert jet["cfg"]["source_kind"] == 2 and jet["cfg"]["axisym"] == 0
+    assert fileio.Deck(_deck_for(cases.case_axisym(), tmp_path)).settings()["cfg"]["axisym"] == 1
 
 
 def test_deck_settings_optimized_schemes(tmp_path):
@@ -96,7 +97,7 @@ def test_deck_settings_optimized_schemes(tmp_path):
 @pytest.mark.parametrize("key,value,msg", [
     ("SPATIAL_FILTER", "COMPACT, BLEND = 1.0", "The filter COMPACT is not supported"),
     ("SPATIAL_SCHEME", "COMPACT6", "SPATIAL_SCHEME = COMPACT6"),
-    ("AXISYMMETRY", "TRUE", "AXISYMMETRY = TRUE"),
+    ("AXISYMMETRY", "TRUE, AXIS_OF_SYMMETRY = Y", "axis of symmetry should be x"),
     ("FIX_DT_OR_CFL", "FIX_NOTHING", "You may fix either DT or CFL"),
     ("HARMONIC_WAVE", "PLANE", "HARMONIC_WAVE"),
     ("OVERSET", "CHIMERA", "unsupported format for overset"),
@@ -211,7 +212,8 @@ def _case_folder(name, d):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["c1", "shear", "wall", "jet", "opt11", "overset_small", "cylinder_small"])
+@pytest.mark.parametrize("name", ["c1", "shear", "wall", "jet", "opt11", "axisym", "overset_small",
+                                  "cylinder_small"])
 def test_run_deck_writes_the_reference_solution_file(tmp_path, name):
     """A golden case folder through lee_run_deck (plain FP64): the final
     camille.000001.q holds the reference's bits and tau; the set-up files
@@ -235,5 +237,7 @@ def test_run_deck_writes_the_reference_solution_file(tmp_path, name):
         assert np.array_equal(q0[:, 0], ic) and list(tau0) == [0.0, 1.0, 0.0, 0.0]
     rep = open(os.path.join(d, "report", "solution", "solution")).read().splitlines()
     assert len(rep) == 1 and rep[0].startswith(f"Time step: {c['nsteps']:10d}, time: ")
-    assert open(os.path.join(d, "camille.nam")).read().split() == ["S'", "UX'", "UY'", "UZ'", "P'"]
+    ax = c.get("axisym", 0)
+    assert open(os.path.join(d, "camille.nam")).read().split() == \
+        ["S'", "UX'", "UR'" if ax else "UY'", "UTHETA'" if ax else "UZ'", "P'"]
     print(f"run_deck {name}: {nb} block(s), {c['nsteps']} steps, bit-identical", flush=True)
