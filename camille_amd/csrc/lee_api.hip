@@ -353,9 +353,8 @@ int batch_autotune(lee_handle *h) {
     if (rc) break;
   }
   best = 1e30f;
-  for (int ty : {4, 8, 16}) {
+  for (int ty : {4, 8, 12, 16}) {
     if (rc) break;
-    if (!is_fp32(h) && ty > 4) continue;
     p.TY = ty;
     HIPCHK(batch_prepare(h->stream, h));
     for (int g : grids) {
@@ -436,11 +435,12 @@ int batch_solve(lee_handle *h, int nsteps) {
 }
 
 // Tile-shape autotuner of the fused kernels.  For each block, every stage
-// tile height (rows per thread R = 1, 2, 4) and filter tile height (8, 16,
-// 32) is timed with HIP events on the block itself, in buffers that carry no
-// state between steps (stage: input Y2, accumulator Y3, output Y1; filter:
-// Y1 -> Y2); the fastest is kept and cached per (ni, nj, precision, mean
-// kind, holes) for the life of the process.
+// tile height (rows per thread R = 1, 2, 4) and filter tile height (4, 8,
+// 12, 16; one workgroup per tile or the persistent pipelined form with 2, 4
+// or 8 workgroups per CU) is timed with HIP events on the block itself, in
+// buffers that carry no state between steps (stage: input Y2, accumulator
+// Y3, output Y1; filter: Y1 -> Y2); the fastest is kept and cached per (ni,
+// nj, precision, mean kind, holes) for the life of the process.
 struct TuneKey {
   int ni, nj, prec, meanf, holes;
   bool operator==(const TuneKey &o) const {
@@ -489,8 +489,9 @@ int autotune_block(lee_handle *h, Block &b) {
   if (h->cfg.filter_order > 0) {
     // tile height, then one launch per tile or the persistent pipelined
     // form with 2, 4 or 8 workgroups per CU
-    for (int ty : {4, 8, 16}) {
-      if (is_fp32(h) ? false : ty > 4) continue;  // FP64 tiles above 4 rows exceed 64 KB of LDS
+    // a height whose tile count just fits the resident workgroups saves a
+    // second round: 12 rows are offered besides 4, 8 and 16
+    for (int ty : {4, 8, 12, 16}) {
       for (int pw : {0, 2, 4, 8}) {
         if (pw > 0 && b.cus <= 0) continue;
         b.tune_fty = ty;
@@ -1224,9 +1225,8 @@ int lee_autotune(lee_handle *h) {
 int lee_set_tile_shape(lee_handle *h, int ib, int stage_rows, int filter_rows) {
   if (!h || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
   if (stage_rows != 1 && stage_rows != 2 && stage_rows != 4) return fail(h, LEE_EINVAL, "stage rows must be 1, 2 or 4");
-  if (filter_rows != 4 && filter_rows != 8 && filter_rows != 16)
-    return fail(h, LEE_EINVAL, "filter tile height must be 4, 8 or 16");
-  if (!is_fp32(h) && filter_rows > 4) return fail(h, LEE_EINVAL, "FP64 filter tiles are 4 rows high");
+  if (filter_rows != 4 && filter_rows != 8 && filter_rows != 12 && filter_rows != 16)
+    return fail(h, LEE_EINVAL, "filter tile height must be 4, 8, 12 or 16");
   if (h->batch) {  // one launch covers every solve: the shape is the batch's
     HIPCHK(hipStreamSynchronize(h->stream));
     h->bp.R = stage_rows;

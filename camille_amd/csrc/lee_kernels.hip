@@ -1128,6 +1128,7 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
   do {                                  \
     if (TYF == 4) LEE_FLT1(HO, OR, NVV, 4);  \
     else if (TYF == 8) LEE_FLT1(HO, OR, NVV, 8); \
+    else if (TYF == 12) LEE_FLT1(HO, OR, NVV, 12); \
     else LEE_FLT1(HO, OR, NVV, 16);      \
   } while (0)
 #define LEE_FLT_NV(HO, OR)            \
@@ -1302,6 +1303,8 @@ static hipError_t batch_filter_t(hipStream_t s, lee_handle *h) {
     if (TY == 4) hipLaunchKernelGGL((k_batch_filter<T, OR, 4>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map)); \
     else if (TY == 8) hipLaunchKernelGGL((k_batch_filter<T, OR, 8>), grid, block, 0, s, ba, CA(ba.args), \
                                          CA(ba.map));                                                       \
+    else if (TY == 12) hipLaunchKernelGGL((k_batch_filter<T, OR, 12>), grid, block, 0, s, ba, CA(ba.args), \
+                                          CA(ba.map));                                                      \
     else hipLaunchKernelGGL((k_batch_filter<T, OR, 16>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map));        \
   } while (0)
   if (o8) LEE_BF(8);

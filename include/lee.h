@@ -188,7 +188,7 @@ int lee_get_kernel_timing(lee_handle *h, double *ms, int *count);
 
 /* Tile shapes of the fused kernels (no reference counterpart).  The first
  * lee_solve of a handle times every stage tile height (4 * stage_rows, with
- * stage_rows 1, 2 or 4) and filter tile height (4, 8 or 16; FP64: 4) on each block
+ * stage_rows 1, 2 or 4) and filter tile height (4, 8, 12 or 16) on each block
  * and keeps the fastest, cached per (ni, nj, precision, mean kind, holes).
  * lee_autotune re-runs the search; lee_set_tile_shape pins a shape (every
  * shape gives bit-identical results). */

@@ -138,7 +138,7 @@ def test_batch_launch_modes_bit_identical():
     solves = _ensemble(seed=11, n=4)
     ref = None
     for use_graph, gs, gf, shape in [(True, 0, 0, (1, 4)), (False, 0, 0, (1, 4)), (True, 256, 512, (2, 8)),
-                                     (True, 1024, 0, (4, 16)), (False, 8, 8, (1, 8))]:
+                                     (True, 1024, 0, (4, 16)), (False, 8, 8, (1, 8)), (True, 0, 64, (2, 12))]:
         b = _batch(solves, FP32)
         b.set_tile_shape(*shape)
         b.set_launch(use_graph, gs, gf)
