@@ -54,6 +54,8 @@ void free_block(Block &b) {
   dfree(b.jac);
   dfree(b.iblank);
   dfree(b.code_fd);
+  dfree(b.clean_fd);
+  dfree(b.clean_flt);
   dfree(b.code_flt);
   dfree(b.mean);
   dfree(b.mgrad);
@@ -730,6 +732,7 @@ void lee_destroy(lee_handle *h) {
   dfree(h->ov.dl);
   dfree(h->ov.w);
   dfree(h->ov.coef);
+  dfree(h->ov.coef5);
   dfree(h->ov.buf);
   for (auto e : h->evpool) (void)hipEventDestroy(e);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -905,6 +908,28 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
           HIPCHK(dalloc(&b.code_flt, 2 * (size_t)n));
           HIPCHK(hipMemcpy(b.code_flt, b.h_code_flt.data(), 2 * (size_t)n, hipMemcpyHostToDevice));
         }
+        // integral images of the points off the interior rows (or holes): the
+        // fused kernels run hole-free interior tiles without per-point checks
+        auto integral = [&](const std::vector<int8_t> &codes, int **dst) -> int {
+          const int w = b.ni + 1;
+          std::vector<int> S((size_t)w * (b.nj + 1), 0);
+          for (int j = 0; j < b.nj; j++)
+            for (int i = 0; i < b.ni; i++) {
+              const int p = j * b.ni + i;
+              const int bad = codes[p] != CODE_INTERIOR || codes[(size_t)n + p] != CODE_INTERIOR ||
+                              b.h_iblank[p] == BLANKED;
+              S[(size_t)(j + 1) * w + i + 1] = bad + S[(size_t)j * w + i + 1] + S[(size_t)(j + 1) * w + i] -
+                                               S[(size_t)j * w + i];
+            }
+          HIPCHK(dalloc(dst, S.size()));
+          HIPCHK(hipMemcpy(*dst, S.data(), sizeof(int) * S.size(), hipMemcpyHostToDevice));
+          return LEE_OK;
+        };
+        if (D == 2) {
+          if (int e = integral(b.h_code_fd, &b.clean_fd)) return e;
+          if (cfg->filter_order > 0)
+            if (int e = integral(b.h_code_flt, &b.clean_flt)) return e;
+        }
       }
       // mean gradients after the holes exist (simulation.cpp:240, spatial:2250-2271)
       for (int v = 0; v < NV; v++)
@@ -1039,6 +1064,18 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
       HIPCHK(hipMemcpy(o.dl, o.h_dl.data(), sizeof(int) * o.nrecv, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(o.w, o.h_w.data(), sizeof(int) * 3 * nblocks, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(o.coef, o.h_coef.data(), sizeof(double) * o.h_coef.size(), hipMemcpyHostToDevice));
+      // ORDER 4 in 2-D everywhere: a receiver-minor copy of the coefficients
+      // for the (receiver, variable)-parallel gather
+      bool all5 = true;
+      for (int b2 = 0; b2 < nblocks; b2++)
+        all5 &= o.h_w[3 * b2] == 5 && o.h_w[3 * b2 + 1] == 5 && o.h_w[3 * b2 + 2] == 1;
+      if (all5) {
+        std::vector<double> t((size_t)25 * o.nrecv);
+        for (int r = 0; r < o.nrecv; r++)
+          for (int q = 0; q < 25; q++) t[(size_t)q * o.nrecv + r] = o.h_coef[(size_t)r * o.stride + q];
+        HIPCHK(dalloc(&o.coef5, t.size()));
+        HIPCHK(hipMemcpy(o.coef5, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+      }
     }
     return LEE_OK;
   }();

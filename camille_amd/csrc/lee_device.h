@@ -32,6 +32,15 @@ constexpr int FL_TX = 64, FL_H = 4;   // filter tile width and halo (height: aut
 // as compile-time constants: interior {1/12, -8/12, 0, 8/12, -1/12}; left
 // rows 0 {-3/2, 2, -1/2} (anchored at the point) and 1 {-1/2, 0, 1/2}
 // (anchored one point left); right rows mirrored with the sign flipped.
+// Whether the box [i0, i1) x [j0, j1) lies inside an ni x nj block and holds
+// no point flagged in the integral image S ((ni + 1) x (nj + 1), S[j][i] =
+// flagged points with j' < j and i' < i).  Uniform per workgroup.
+__device__ __forceinline__ bool box_clean(const int *S, int ni, int nj, int i0, int i1, int j0, int j1) {
+  if (i0 < 0 || j0 < 0 || i1 > ni || j1 > nj) return false;
+  const int w = ni + 1;
+  return S[j1 * w + i1] - S[j0 * w + i1] - S[j1 * w + i0] + S[j0 * w + i0] == 0;
+}
+
 // Explicit fused multiply-add: the fused kernels spell out every fusion so
 // that their instances (interior tiles, edge tiles, batched) round alike.
 __device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -91,6 +100,7 @@ struct StageArgs {
   T *yout;             // next stage input
   const double *met, *jac;
   const int8_t *iblank, *code;  // holes only
+  const int *clean;    // holes only: integral image of points off the interior stencil
   const double *mean, *mgrad, *rhob;   // non-uniform mean only
   const T *mc;         // non-uniform mean: MC_COUNT coefficient fields
   const T *rho_in;     // stale rho' (first stage, non-uniform mean)
@@ -131,6 +141,7 @@ struct FilterArgs {
   const T *src;      // unfiltered y (5n)
   T *dst;            // filtered y (5n)
   const int8_t *iblank, *code;  // holes only (filter codes, xi then eta)
+  const int *clean;  // holes only: integral image of points off the interior filter
   int vars[5];       // variables of this launch
   size_t vs;         // variable stride (0: ni nj)
   T rem, blend;
@@ -371,10 +382,17 @@ __device__ __forceinline__ void filter_compute_k(const AT &a, int tile, FilterLd
 template <typename T, bool HOLES, int ORDER, int NVAR, int TY, typename AT>
 __device__ __forceinline__ void filter_compute(const AT &a, int tile, FilterLds<T, NVAR, TY> &L,
                                                const FilterRegs<T, NVAR, TY> &R) {
+  const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
   if constexpr (!HOLES) {
-    const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
     if (i0 >= FL_H && i0 + FL_TX + FL_H <= a.ni && j0 >= FL_H && j0 + TY + FL_H <= a.nj) {
       filter_compute_k<T, HOLES, ORDER, NVAR, TY, true>(a, tile, L, R);
+      return;
+    }
+  } else {
+    // a tile of a block with holes whose points and halo rows all take the
+    // interior filter (no hole, no closure row) runs the interior instance
+    if (a.clean && box_clean(a.clean, a.ni, a.nj, i0, i0 + FL_TX, j0 - FL_H, j0 + TY + FL_H)) {
+      filter_compute_k<T, false, ORDER, NVAR, TY, true>(a, tile, L, R);
       return;
     }
   }

@@ -77,6 +77,8 @@ struct Block {
   int8_t *iblank = nullptr;    // n
   int8_t *code_fd = nullptr;   // 2n: xi codes then eta codes (holes only)
   int8_t *code_flt = nullptr;  // 2n
+  int *clean_fd = nullptr;     // (ni+1)(nj+1) integral images of the points whose FD /
+  int *clean_flt = nullptr;    // filter row is not the interior one or that are holes
   std::vector<int8_t> h_iblank, h_code_fd, h_code_flt;
   // base state (FP64): mean 5n, mean gradient 5*2n, rho_bar n
   double *mean = nullptr, *mgrad = nullptr, *rhobar = nullptr;
@@ -185,6 +187,7 @@ struct Overset {
   int *rb = nullptr, *rl = nullptr, *db = nullptr, *dl = nullptr;  // device
   int *w = nullptr;            // device widths 3 per block
   double *coef = nullptr;      // device normalised coefficients nrecv * stride
+  double *coef5 = nullptr;     // when every stencil is 5 x 5: coefficient q of receiver r at q * nrecv + r
   int stride = 0;
   double *buf = nullptr;       // device nrecv * NV
   std::vector<int> h_rb, h_rl, h_db, h_dl, h_w;

@@ -281,18 +281,29 @@ __global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double
   const bool neumann = a.type == LEE_BC_NEUMANN || a.type == LEE_BC_WALL_SLIP_KINEMATIC ||
                        (a.type >= LEE_BC_CENTERLINE_CART_NORM2X && a.type <= LEE_BC_CENTERLINE_AXISYM);
   if (neumann) {
+    // every neighbour value is requested first (clamped addresses), then the
+    // closure row is solved in the reference's order with the skipped
+    // entries masked: one memory latency instead of one per entry
+    const int sg = a.end == 0 ? 1 : -1;
+    T nb[NV][8];
+#pragma unroll
+    for (int s = 1; s <= 8; s++) {
+      const int cs = c + sg * s;
+      const int q = p + sg * mf * (cs >= 0 && cs < len ? s : 0);
+#pragma unroll
+      for (int v = 0; v < NV; v++) nb[v][s - 1] = y[(size_t)v * n + q];
+    }
     for (int v = 0; v < NV; v++) {
       if (a.type == LEE_BC_WALL_SLIP_KINEMATIC && v >= IUX && v <= IUZ) continue;
       double d = 0.0;
-      if (a.end == 0) {
-        for (int s = 1; s < a.sb; s++)
-          if (c + s < len) d -= a.neu[s] * (double)y[(size_t)v * n + p + mf * s];
-        d /= a.neu[0];
-      } else {
-        for (int s = 1; s < a.sb; s++)
-          if (c - s >= 0) d += a.neu[s] * (double)y[(size_t)v * n + p - mf * s];
-        d /= -a.neu[0];
+#pragma unroll
+      for (int s = 1; s <= 8; s++) {
+        const int cs = c + sg * s;
+        const bool use = s < a.sb && cs >= 0 && cs < len;
+        const double t1 = a.end == 0 ? d - a.neu[s] * (double)nb[v][s - 1] : d + a.neu[s] * (double)nb[v][s - 1];
+        d = use ? t1 : d;
       }
+      d /= a.end == 0 ? a.neu[0] : -a.neu[0];
       data[v] = d;
     }
   }
@@ -357,6 +368,37 @@ __global__ void __launch_bounds__(TPB) k_ov_gather(OvTables<T> t, const int *__r
   const int ni = t.ni[b], n = t.nn[b];
   const int w1 = w[3 * b], w2 = w[3 * b + 1];
   const double *c = coef + (size_t)r * stride;
+  if (w1 == 5 && w2 == 5 && w[3 * b + 2] == 1) {
+    // ORDER 4 in 2-D: the 25 donors of one variable are requested together
+    // (clamped addresses, skipped donors masked), then summed in the
+    // reference's order; one memory latency per variable instead of one
+    // per donor
+    const int d0 = dl[r];
+    double cc[25];
+    int li[25];
+    bool ok[25];
+#pragma unroll
+    for (int q = 0; q < 25; q++) {
+      const int l1 = d0 + (q % 5) + (q / 5) * ni;
+      ok[q] = l1 >= 0 && l1 < n;
+      li[q] = ok[q] ? l1 : 0;
+      cc[q] = c[q];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      T yv[25];
+#pragma unroll
+      for (int q = 0; q < 25; q++) yv[q] = y[(size_t)v * n + li[q]];
+      double a = 0.0;
+#pragma unroll
+      for (int q = 0; q < 25; q++) {
+        const double t1 = a + cc[q] * (double)yv[q];
+        a = ok[q] ? t1 : a;
+      }
+      buf[(size_t)r * NV + v] = a;
+    }
+    return;
+  }
   double acc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int s = 0;
   for (int b2 = 0; b2 < w2; b2++)
@@ -367,6 +409,40 @@ __global__ void __launch_bounds__(TPB) k_ov_gather(OvTables<T> t, const int *__r
       s++;
     }
   for (int v = 0; v < NV; v++) buf[(size_t)r * NV + v] = acc[v];
+}
+
+// compute_weighted_sum for 5 x 5 stencils, one thread per (receiver,
+// variable): the 25 donor values and coefficients (receiver-minor, so
+// coalesced) are requested together, then summed in the reference's order
+// with the donors outside the block skipped.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_ov_gather5(OvTables<T> t, const int *__restrict__ db,
+                                                    const int *__restrict__ dl, const double *__restrict__ coef5,
+                                                    double *__restrict__ buf, int nrecv) {
+#pragma clang fp contract(off)
+  const int gid = blockIdx.x * TPB + threadIdx.x;
+  if (gid >= NV * nrecv) return;
+  const int v = gid / nrecv, r = gid - v * nrecv;
+  const int b = db[r];
+  const T *y = t.y[b] + (size_t)v * t.nn[b];
+  const int ni = t.ni[b], n = t.nn[b], d0 = dl[r];
+  double cc[25];
+  T yv[25];
+  bool ok[25];
+#pragma unroll
+  for (int q = 0; q < 25; q++) {
+    const int l1 = d0 + (q % 5) + (q / 5) * ni;
+    ok[q] = l1 >= 0 && l1 < n;
+    cc[q] = coef5[(size_t)q * nrecv + r];
+    yv[q] = y[ok[q] ? l1 : 0];
+  }
+  double a = 0.0;
+#pragma unroll
+  for (int q = 0; q < 25; q++) {
+    const double t1 = a + cc[q] * (double)yv[q];
+    a = ok[q] ? t1 : a;
+  }
+  buf[(size_t)r * NV + v] = a;
 }
 
 // update_each_grid (overset.cpp:1241-1278); optionally refreshes rho'.
@@ -593,11 +669,18 @@ __device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, in
 
 template <typename T, bool MEANF, bool HOLES, int KIND, int R, typename AT>
 __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int tile, StageLds<T, R> &s) {
+  constexpr int TY = 4 * R;
+  const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
   if constexpr (!HOLES) {
-    constexpr int TY = 4 * R;
-    const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
     if (i0 >= ST_H && i0 + ST_TX + ST_H <= a.ni && j0 >= ST_H && j0 + TY + ST_H <= a.nj) {
       stage_tile_k<T, MEANF, HOLES, KIND, R, true>(a, sc, tile, s);
+      return;
+    }
+  } else {
+    // a tile of a block with holes whose points all take the interior
+    // stencil (no hole, no closure row) runs the interior instance
+    if (a.clean && box_clean(a.clean, a.ni, a.nj, i0, i0 + ST_TX, j0, j0 + TY)) {
+      stage_tile_k<T, MEANF, false, KIND, R, true>(a, sc, tile, s);
       return;
     }
   }
@@ -932,8 +1015,12 @@ static hipError_t overset_t(hipStream_t s, lee_handle *h, T *const *ys, T *const
     t.rhob[ib] = h->b[ib].rhobar;
   }
   const Overset &o = h->ov;
-  hipLaunchKernelGGL(k_ov_gather<T>, dim3(nblocks(o.nrecv)), dim3(TPB), 0, s, t, o.db, o.dl, o.w, o.coef, o.stride,
-                     o.buf, o.nrecv);
+  if (o.coef5)
+    hipLaunchKernelGGL(k_ov_gather5<T>, dim3(nblocks((size_t)NV * o.nrecv)), dim3(TPB), 0, s, t, o.db, o.dl, o.coef5,
+                       o.buf, o.nrecv);
+  else
+    hipLaunchKernelGGL(k_ov_gather<T>, dim3(nblocks(o.nrecv)), dim3(TPB), 0, s, t, o.db, o.dl, o.w, o.coef, o.stride,
+                       o.buf, o.nrecv);
   hipLaunchKernelGGL(k_ov_scatter<T>, dim3(nblocks(o.nrecv)), dim3(TPB), 0, s, t, o.rb, o.rl, o.buf, o.nrecv,
                      with_rho ? 1 : 0, 1.0 / h->cfg.gamma);
   return hipGetLastError();
@@ -1017,6 +1104,7 @@ static StageArgs<T> make_stage_args(const lee_handle *h, const Block &b, int sta
   a.jac = b.jac;
   a.iblank = b.iblank;
   a.code = b.code_fd;
+  a.clean = b.clean_fd;
   a.mean = b.mean;
   a.mgrad = b.mgrad;
   a.rhob = b.rhobar;
@@ -1097,6 +1185,7 @@ static FilterArgs<T> make_filter_args(const lee_handle *h, const Block &b, T *co
   a.dst = Y[dst];
   a.iblank = b.iblank;
   a.code = b.code_flt;
+  a.clean = b.clean_flt;
   a.blend = (T)h->cfg.filter_blend;
   a.rem = (T)(1.0 - h->cfg.filter_blend);
   a.vars[0] = IS;
