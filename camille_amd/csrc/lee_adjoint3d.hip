@@ -339,17 +339,40 @@ __global__ void __launch_bounds__(TPB) k_adj_filter3d_z(const T *__restrict__ la
 #pragma unroll
     for (int v = 0; v < 5; v++) w[v][q] = ok ? lam[v * n + (size_t)kk * plane + col] : (T)0;
   }
-  for (int k = k0; k < k1; k++) {
-    const int kk = k + H;
-    const bool ok = kk < nk;
+  // four planes per pass, their window planes requested together (clamped
+  // plane index, zeros selected afterwards)
+  auto plane_out = [&](int k, const T (&wn)[5]) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) w[v][2 * H] = ok ? lam[v * n + (size_t)kk * plane + col] : (T)0;
+    for (int v = 0; v < 5; v++) w[v][2 * H] = wn[v];
 #pragma unroll
     for (int v = 0; v < 5; v++) tmp[v * n + (size_t)k * plane + col] = flt_t3<T, ORDER>(&w[v][H], 1, k, nk);
 #pragma unroll
     for (int v = 0; v < 5; v++)
 #pragma unroll
       for (int q = 0; q < 2 * H; q++) w[v][q] = w[v][q + 1];
+  };
+  int k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    T wn[4][5];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int kk = k + u + H;
+      const size_t pc = (size_t)min(kk, nk - 1) * plane + col;
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const T t = lam[v * n + pc];
+        wn[u][v] = kk < nk ? t : (T)0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) plane_out(k + u, wn[u]);
+  }
+  for (; k < k1; k++) {
+    const int kk = k + H;
+    T wn[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) wn[v] = kk < nk ? lam[v * n + (size_t)kk * plane + col] : (T)0;
+    plane_out(k, wn);
   }
 }
 

@@ -346,22 +346,50 @@ __global__ void __launch_bounds__(TPB) k_filter3d_z(const T *__restrict__ tmp, c
 #pragma unroll
     for (int v = 0; v < 5; v++) w[v][q] = ok ? tmp[v * n + (size_t)kk * plane + col] : (T)0;
   }
-  for (int k = k0; k < k1; k++) {
-    const int kk = k + H;
-    const bool ok = kk < nk;
+  // four planes per pass: their window planes and unfiltered values are
+  // requested together (clamped plane index, zeros selected afterwards),
+  // one memory latency per four planes
+  auto plane_out = [&](int k, const T (&wn)[5], const T (&yk)[5]) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) w[v][2 * H] = ok ? tmp[v * n + (size_t)kk * plane + col] : (T)0;
+    for (int v = 0; v < 5; v++) w[v][2 * H] = wn[v];
     const int code = default_code(k, nk, NB);
     const size_t p = (size_t)k * plane + col;
 #pragma unroll
     for (int v = 0; v < 5; v++) {
       const T f = filter_row<T, ORDER>(&w[v][H], 1, code);
-      dst[v * n + p] = rem * y[v * n + p] + blend * f;
+      dst[v * n + p] = rem * yk[v] + blend * f;
     }
 #pragma unroll
     for (int v = 0; v < 5; v++)
 #pragma unroll
       for (int q = 0; q < 2 * H; q++) w[v][q] = w[v][q + 1];
+  };
+  int k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    T wn[4][5], yk[4][5];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int kk = k + u + H;
+      const size_t pc = (size_t)min(kk, nk - 1) * plane + col, pk = (size_t)(k + u) * plane + col;
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const T t = tmp[v * n + pc];
+        wn[u][v] = kk < nk ? t : (T)0;
+        yk[u][v] = y[v * n + pk];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) plane_out(k + u, wn[u], yk[u]);
+  }
+  for (; k < k1; k++) {
+    const int kk = k + H;
+    T wn[5], yk[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      wn[v] = kk < nk ? tmp[v * n + (size_t)kk * plane + col] : (T)0;
+      yk[v] = y[v * n + (size_t)k * plane + col];
+    }
+    plane_out(k, wn, yk);
   }
 }
 
