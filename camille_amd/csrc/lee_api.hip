@@ -450,7 +450,7 @@ struct TuneKey {
   }
 };
 struct TuneVal {
-  int r, fty, fgrid;
+  int r, fty, fgrid, mc, ml;
 };
 std::vector<std::pair<TuneKey, TuneVal>> g_tune_cache;
 
@@ -461,6 +461,8 @@ int autotune_block(lee_handle *h, Block &b) {
       b.tune_r = e.second.r;
       b.tune_fty = e.second.fty;
       b.tune_fgrid = e.second.fgrid;
+      b.tune_mc = e.second.mc;
+      b.tune_ml = e.second.ml;
       b.tuned = true;
       return LEE_OK;
     }
@@ -488,6 +490,7 @@ int autotune_block(lee_handle *h, Block &b) {
   b.tune_r = best_r;
   best = 1e30f;
   int best_f = b.tune_fty, best_g = 0;
+  b.tune_mc = b.tune_ml = 0;  // tile forms first
   if (h->cfg.filter_order > 0) {
     // tile height, then one launch per tile or the persistent pipelined
     // form with 2, 4 or 8 workgroups per CU
@@ -515,10 +518,35 @@ int autotune_block(lee_handle *h, Block &b) {
   }
   b.tune_fty = best_f;
   b.tune_fgrid = best_g;
+  // column-march form (blocks without holes): 1 or 2 columns per lane, 8,
+  // 16 or 32 rows per wave
+  int best_mc = 0, best_ml = 0;
+  if (h->cfg.filter_order > 0 && !b.has_holes) {
+    for (int mc : {1, 2}) {
+      for (int ml : {8, 16, 32}) {
+        b.tune_mc = mc;
+        b.tune_ml = ml;
+        for (int k = 0; k < 2; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
+        HIPCHK(hipEventRecord(e0, h->stream));
+        for (int k = 0; k < reps; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
+        HIPCHK(hipEventRecord(e1, h->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          best_mc = mc;
+          best_ml = ml;
+        }
+      }
+    }
+  }
+  b.tune_mc = best_mc;
+  b.tune_ml = best_ml;
   b.tuned = true;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  g_tune_cache.push_back({key, {b.tune_r, b.tune_fty, b.tune_fgrid}});
+  g_tune_cache.push_back({key, {b.tune_r, b.tune_fty, b.tune_fgrid, b.tune_mc, b.tune_ml}});
   return LEE_OK;
 }
 
@@ -1274,6 +1302,7 @@ int lee_set_tile_shape(lee_handle *h, int ib, int stage_rows, int filter_rows) {
   }
   h->b[ib].tune_r = stage_rows;
   h->b[ib].tune_fty = filter_rows;
+  h->b[ib].tune_mc = h->b[ib].tune_ml = 0;  // a tile height selects the tile forms
   h->b[ib].tuned = true;
   return LEE_OK;
 }
@@ -1282,7 +1311,28 @@ int lee_set_filter_grid(lee_handle *h, int ib, int workgroups) {
   if (!h || ib < 0 || ib >= h->nblk || workgroups < 0) return fail(h, LEE_EINVAL, "bad argument");
   if (h->batch) return fail(h, LEE_EINVAL, "a batch launches its filter through the batch launch settings");
   h->b[ib].tune_fgrid = (workgroups + 7) / 8 * 8;
+  h->b[ib].tune_mc = h->b[ib].tune_ml = 0;
   h->b[ib].tuned = true;
+  return LEE_OK;
+}
+
+int lee_set_filter_march(lee_handle *h, int ib, int columns, int rows) {
+  if (!h || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
+  if (h->batch) return fail(h, LEE_EINVAL, "a batch launches its filter through the batch launch settings");
+  const bool off = columns == 0 && rows == 0;
+  if (!off && ((columns != 1 && columns != 2) || (rows != 8 && rows != 16 && rows != 32)))
+    return fail(h, LEE_EINVAL, "column-march filter: columns 1 or 2 and rows 8, 16 or 32, or both 0");
+  if (!off && h->b[ib].has_holes) return fail(h, LEE_EINVAL, "the column-march filter serves blocks without holes");
+  h->b[ib].tune_mc = off ? 0 : columns;
+  h->b[ib].tune_ml = off ? 0 : rows;
+  h->b[ib].tuned = true;
+  return LEE_OK;
+}
+
+int lee_get_filter_march(lee_handle *h, int ib, int *columns, int *rows) {
+  if (!h || ib < 0 || ib >= h->nblk || !columns || !rows) return fail(h, LEE_EINVAL, "bad argument");
+  *columns = h->batch ? 0 : h->b[ib].tune_mc;
+  *rows = h->batch ? 0 : h->b[ib].tune_ml;
   return LEE_OK;
 }
 

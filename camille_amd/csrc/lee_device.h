@@ -157,19 +157,22 @@ struct FilterArgs {
 // point, `st` is the stride along the sweep.
 template <typename T, int ORDER>
 __device__ __forceinline__ T filter_row(const T *s, int st, int code) {
+#pragma clang fp contract(off)
+  // every fusion is spelled out, so that the tile forms and the column-march
+  // form round each point alike
   if (ORDER == 8) {
     if (code == CODE_INTERIOR) {
       const T c0 = (T)(93.0 / 128.0), c1 = (T)(7.0 / 32.0), c2 = (T)(-7.0 / 64.0), c3 = (T)(1.0 / 32.0),
               c4 = (T)(-1.0 / 256.0);
       T acc = c4 * s[-4 * st];
-      acc += c3 * s[-3 * st];
-      acc += c2 * s[-2 * st];
-      acc += c1 * s[-st];
-      acc += c0 * s[0];
-      acc += c1 * s[st];
-      acc += c2 * s[2 * st];
-      acc += c3 * s[3 * st];
-      acc += c4 * s[4 * st];
+      acc = fmx(c3, s[-3 * st], acc);
+      acc = fmx(c2, s[-2 * st], acc);
+      acc = fmx(c1, s[-st], acc);
+      acc = fmx(c0, s[0], acc);
+      acc = fmx(c1, s[st], acc);
+      acc = fmx(c2, s[2 * st], acc);
+      acc = fmx(c3, s[3 * st], acc);
+      acc = fmx(c4, s[4 * st], acc);
       return acc;
     }
     if (code == CODE_ZERO) return (T)0;
@@ -179,35 +182,43 @@ __device__ __forceinline__ T filter_row(const T *s, int st, int code) {
     if (row == 2) {
       const T a = (T)((-1.0 / 8.0) / 2.0), b = (T)((1.0 / 2.0) / 2.0), c = (T)(5.0 / 8.0);
       T acc = a * s[-2 * st];
-      acc += b * s[-st];
-      acc += c * s[0];
-      acc += b * s[st];
-      acc += a * s[2 * st];
+      acc = fmx(b, s[-st], acc);
+      acc = fmx(c, s[0], acc);
+      acc = fmx(b, s[st], acc);
+      acc = fmx(a, s[2 * st], acc);
       return acc;
     }
     const T a = (T)((1.0 / 32.0) / 2.0), b = (T)((-3.0 / 16.0) / 2.0), c = (T)((15.0 / 32.0) / 2.0),
             d = (T)(11.0 / 16.0);
     T acc = a * s[-3 * st];
-    acc += b * s[-2 * st];
-    acc += c * s[-st];
-    acc += d * s[0];
-    acc += c * s[st];
-    acc += b * s[2 * st];
-    acc += a * s[3 * st];
+    acc = fmx(b, s[-2 * st], acc);
+    acc = fmx(c, s[-st], acc);
+    acc = fmx(d, s[0], acc);
+    acc = fmx(c, s[st], acc);
+    acc = fmx(b, s[2 * st], acc);
+    acc = fmx(a, s[3 * st], acc);
     return acc;
   } else {
     if (code == CODE_INTERIOR) {
       const T a = (T)((-1.0 / 8.0) / 2.0), b = (T)((1.0 / 2.0) / 2.0), c = (T)(5.0 / 8.0);
       T acc = a * s[-2 * st];
-      acc += b * s[-st];
-      acc += c * s[0];
-      acc += b * s[st];
-      acc += a * s[2 * st];
+      acc = fmx(b, s[-st], acc);
+      acc = fmx(c, s[0], acc);
+      acc = fmx(b, s[st], acc);
+      acc = fmx(a, s[2 * st], acc);
       return acc;
     }
     if (code == CODE_ZERO) return (T)0;
     return s[0];
   }
+}
+
+// The filter's blend of the unfiltered value u with the filtered f
+// (rem = 1 - blend), fused as one FMA.
+template <typename T>
+__device__ __forceinline__ T filter_blend(T rem, T u, T blend, T f) {
+#pragma clang fp contract(off)
+  return fmx(rem, u, blend * f);
 }
 
 template <typename T, int NVAR, int TY>
@@ -372,7 +383,7 @@ __device__ __forceinline__ void filter_compute_k(const AT &a, int tile, FilterLd
 #pragma unroll
     for (int q = 0; q < NVAR; q++) {
       const T f = filter_row<T, ORDER>(&mid[q][r + FL_H][tx], FL_TX, code);
-      T out = a.rem * orig[q] + a.blend * f;
+      T out = filter_blend(a.rem, orig[q], a.blend, f);
       if (hole) out = 0;
       a.dst[a.vars[q] * vs + p] = out;
     }
@@ -405,6 +416,185 @@ __device__ __forceinline__ void filter_tile(const AT &a, int tile, FilterLds<T, 
   FilterRegs<T, NVAR, TY> R;
   filter_load<T, HOLES, NVAR, TY>(a, tile, R);
   filter_compute<T, HOLES, ORDER, NVAR, TY>(a, tile, L, R);
+}
+
+// ---------------------------------------------------------------------------
+// Column-march form of the filter (blocks without holes).  One wave filters
+// one variable on a strip of 64 C columns over L rows.  Lane l holds the C
+// consecutive columns i_s + C l; the outer 4 columns on each side are halo,
+// so strips advance by 64 C - 8 columns and overlap by 8.  The wave requests
+// its L + 8 raw rows at once, then per row takes the xi neighbours of its
+// columns from the adjacent lanes (DPP wave shifts: no LDS, no barrier),
+// keeps the xi-filtered rows in registers and runs the eta sweep and the
+// blend as soon as the nine rows of an output row are there.  Each point
+// gets the operations of filter_tile in the same order, so both forms give
+// the same bits.
+
+// Whole-wave lane shifts by one lane (DPP wave_shr:1, control 0x138, and
+// wave_shl:1, 0x130): UP gives lane l the value of lane l - 1, !UP that of
+// lane l + 1; the lane without a source gets 0.
+template <bool UP>
+__device__ __forceinline__ int wave_shift1(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, UP ? 0x138 : 0x130, 0xF, 0xF, true);
+}
+template <bool UP>
+__device__ __forceinline__ float wave_shift1(float x) {
+  return __builtin_bit_cast(float, wave_shift1<UP>(__builtin_bit_cast(int, x)));
+}
+template <bool UP>
+__device__ __forceinline__ double wave_shift1(double x) {
+  const long long v = __builtin_bit_cast(long long, x);
+  const int lo = wave_shift1<UP>((int)(v & 0xffffffffll)), hi = wave_shift1<UP>((int)(v >> 32));
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// C consecutive values at p (aligned to one element only) as one vector
+// access.
+template <typename T, int C>
+__device__ __forceinline__ void load_cols(const T *p, T (&v)[C]) {
+  if constexpr (C == 1) {
+    v[0] = *p;
+  } else {
+    typedef T VT __attribute__((ext_vector_type(C)));
+    VT x;
+    __builtin_memcpy(&x, p, sizeof(VT));
+#pragma unroll
+    for (int e = 0; e < C; e++) v[e] = x[e];
+  }
+}
+template <typename T, int C>
+__device__ __forceinline__ void store_cols(T *p, const T (&v)[C]) {
+  if constexpr (C == 1) {
+    *p = v[0];
+  } else {
+    typedef T VT __attribute__((ext_vector_type(C)));
+    VT x;
+#pragma unroll
+    for (int e = 0; e < C; e++) x[e] = v[e];
+    __builtin_memcpy(p, &x, sizeof(VT));
+  }
+}
+
+// Strip geometry of the column-march form.
+template <int C>
+struct March {
+  static constexpr int HL = FL_H / C;              // halo lanes on each side
+  static constexpr int STRIDE = (64 - 2 * HL) * C;  // output columns per strip
+};
+
+// One wave's share: variable slot q, strip s, row chunk c.  INNER: every
+// load of the wave lies inside the block, so every output point takes the
+// interior rows and no bounds are tested.
+template <typename T, int ORDER, int C, int L, bool INNER>
+__device__ __forceinline__ void march_wave(const FilterArgs<T> &a, int q, int s, int c) {
+  static_assert(FL_H % C == 0, "the halo is a whole number of lanes");
+  constexpr int NB = ORDER / 2, HL = March<C>::HL, NR = L + 2 * FL_H, W = C + 2 * FL_H;
+  const int lane = threadIdx.x;
+  const int ni = a.ni, nj = a.nj;
+  const size_t vs = a.vs ? a.vs : (size_t)ni * nj;
+  const T *src = a.src + a.vars[q] * vs;
+  T *dst = a.dst + a.vars[q] * vs;
+  const int gi0 = s * March<C>::STRIDE - FL_H + C * lane;
+  const int j0 = c * L;
+  const bool out_lane = lane >= HL && lane < 64 - HL;
+  // raw rows j0 - 4 .. j0 + L + 3 (outside the block: 0), all requested here
+  // from clamped addresses so the sequence has no branch
+  T xr[NR][C];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int gj = j0 - FL_H + r;
+    if constexpr (INNER) {
+      load_cols<T, C>(src + (size_t)gj * ni + gi0, xr[r]);
+    } else {
+      const int cj = min(max(gj, 0), nj - 1);
+#pragma unroll
+      for (int e = 0; e < C; e++) {
+        const int gi = gi0 + e;
+        const T v = src[(size_t)cj * ni + min(max(gi, 0), ni - 1)];
+        xr[r][e] = (gj >= 0 && gj < nj && gi >= 0 && gi < ni) ? v : (T)0;
+      }
+    }
+  }
+  int cx[C];
+#pragma unroll
+  for (int e = 0; e < C; e++) {
+    const int gi = gi0 + e;
+    cx[e] = (gi >= 0 && gi < ni) ? default_code(gi, ni, NB) : CODE_ZERO;
+  }
+  T xi[NR][C];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    // w: columns gi0 - 4 .. gi0 + C + 3 of row r; lane l - k holds columns
+    // gi0 - k C .. gi0 - k C + C - 1
+    T w[W];
+    T tl[C], tr[C];
+#pragma unroll
+    for (int e = 0; e < C; e++) {
+      w[FL_H + e] = xr[r][e];
+      tl[e] = xr[r][e];
+      tr[e] = xr[r][e];
+    }
+#pragma unroll
+    for (int k = 1; k <= HL; k++) {
+#pragma unroll
+      for (int e = 0; e < C; e++) {
+        tl[e] = wave_shift1<true>(tl[e]);
+        tr[e] = wave_shift1<false>(tr[e]);
+        w[FL_H - k * C + e] = tl[e];
+        w[FL_H + k * C + e] = tr[e];
+      }
+    }
+    const int gj = j0 - FL_H + r;
+#pragma unroll
+    for (int e = 0; e < C; e++) {
+      if constexpr (INNER) {
+        xi[r][e] = filter_row<T, ORDER>(&w[FL_H + e], 1, CODE_INTERIOR);
+      } else {
+        const bool ok = gj >= 0 && gj < nj && cx[e] != CODE_ZERO;
+        xi[r][e] = ok ? filter_row<T, ORDER>(&w[FL_H + e], 1, cx[e]) : (T)0;
+      }
+    }
+    if (r < 2 * FL_H) continue;
+    // eta sweep, blend and store of output row o
+    const int o = r - 2 * FL_H, go = j0 + o;
+    if (!INNER && go >= nj) continue;
+    const int cy = INNER ? CODE_INTERIOR : default_code(go, nj, NB);
+    T out[C];
+#pragma unroll
+    for (int e = 0; e < C; e++) {
+      T col[2 * FL_H + 1];
+#pragma unroll
+      for (int k = 0; k <= 2 * FL_H; k++) col[k] = xi[o + k][e];
+      const T f = filter_row<T, ORDER>(&col[FL_H], 1, cy);
+      out[e] = filter_blend(a.rem, xr[o + FL_H][e], a.blend, f);
+    }
+    if constexpr (INNER) {
+      if (out_lane) store_cols<T, C>(dst + (size_t)go * ni + gi0, out);
+    } else {
+#pragma unroll
+      for (int e = 0; e < C; e++) {
+        const int gi = gi0 + e;
+        if (out_lane && gi >= 0 && gi < ni) dst[(size_t)go * ni + gi] = out[e];
+      }
+    }
+  }
+}
+
+// The wave's share from its index within the launch: four consecutive row
+// chunks of one strip per workgroup, workgroups spread over the XCDs by
+// xcd_tile so that each XCD walks whole strips (adjacent chunks share their
+// halo rows in its L2).  a.ntx strips, a.nty chunks, a.ntiles = nvar * ntx *
+// nty waves, a.chunk = workgroups per XCD.
+template <typename T, int ORDER, int C, int L>
+__device__ __forceinline__ void filter_march(const FilterArgs<T> &a) {
+  const int wv = 4 * xcd_tile(blockIdx.x, a.chunk) + (int)threadIdx.y;
+  if (wv >= a.ntiles) return;
+  const int c = wv % a.nty, s = (wv / a.nty) % a.ntx, q = wv / (a.nty * a.ntx);
+  const int i_s = s * March<C>::STRIDE - FL_H, j_s = c * L - FL_H;
+  if (i_s >= 0 && i_s + 64 * C <= a.ni && j_s >= 0 && j_s + L + 2 * FL_H <= a.nj)
+    march_wave<T, ORDER, C, L, true>(a, q, s, c);
+  else
+    march_wave<T, ORDER, C, L, false>(a, q, s, c);
 }
 
 }  // namespace

@@ -777,6 +777,13 @@ k_fused_filter_p(FilterArgs<T> a) {
   }
 }
 
+// Column-march form (blocks without holes): one wave per (variable, strip,
+// row chunk), four waves per workgroup, no LDS (filter_march).
+template <typename T, int ORDER, int C, int L>
+__global__ void __launch_bounds__(TPB) k_fused_filter_m(FilterArgs<T> a) {
+  filter_march<T, ORDER, C, L>(a);
+}
+
 template <typename T>
 struct BatchFilterArgs {
   const FilterArgs<T> *args;     // nsolve entries
@@ -1204,6 +1211,33 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
   const bool pipe = b.tune_fgrid > 0;
   const dim3 pgrid(std::min(a.chunk * 8, (b.tune_fgrid + 7) / 8 * 8));
   const bool o8 = h->cfg.filter_order == 8;
+  if (b.tune_mc > 0 && !b.has_holes) {
+    // column-march form: every carried variable in one launch
+    const int nv = b.carry_w ? 5 : 4;
+    a.vars[4] = IUZ;
+    const int C = b.tune_mc, L = b.tune_ml;
+    const int stride = C == 1 ? March<1>::STRIDE : March<2>::STRIDE;
+    a.ntx = (b.ni + stride - 1) / stride;
+    a.nty = (b.nj + L - 1) / L;
+    a.ntiles = nv * a.ntx * a.nty;
+    a.chunk = ((a.ntiles + 3) / 4 + 7) / 8;
+    const dim3 mgrid(a.chunk * 8);
+#define LEE_FLM(OR, CC, LL) hipLaunchKernelGGL((k_fused_filter_m<T, OR, CC, LL>), mgrid, block, 0, s, a)
+#define LEE_FLM_O(OR)                                     \
+  do {                                                    \
+    if (C == 1 && L == 8) LEE_FLM(OR, 1, 8);              \
+    else if (C == 1 && L == 16) LEE_FLM(OR, 1, 16);       \
+    else if (C == 1) LEE_FLM(OR, 1, 32);                  \
+    else if (L == 8) LEE_FLM(OR, 2, 8);                   \
+    else if (L == 16) LEE_FLM(OR, 2, 16);                 \
+    else LEE_FLM(OR, 2, 32);                              \
+  } while (0)
+    if (o8) LEE_FLM_O(8);
+    else LEE_FLM_O(4);
+#undef LEE_FLM_O
+#undef LEE_FLM
+    return hipGetLastError();
+  }
   // the four active variables together, then w' alone when it is carried
   for (int pass = 0; pass < (b.carry_w ? 2 : 1); pass++) {
     const bool four = pass == 0;

@@ -91,7 +91,7 @@ EXPORTS = [
     "lee_apply_filter", "lee_take_derivative_xyz", "lee_update_boundary", "lee_get_metrics", "lee_get_iblank",
     "lee_get_mean_gradient", "lee_get_sponge", "lee_last_solve_ms", "lee_version",
     "lee_set_kernel_timing", "lee_get_kernel_timing", "lee_autotune", "lee_set_tile_shape", "lee_get_tile_shape",
-    "lee_set_filter_grid", "lee_get_filter_grid",
+    "lee_set_filter_grid", "lee_get_filter_grid", "lee_set_filter_march", "lee_get_filter_march",
     "lee_batch_create", "lee_get_solve_time", "lee_set_batch_launch", "lee_get_batch_launch",
     "lee_set_source_field", "lee_set_observer", "lee_objective_forward", "lee_objective_backward",
     "lee_get_source_gradient", "lee_get_state_gradient", "lee_compute_rhs_adjoint", "lee_apply_filter_adjoint",
@@ -142,6 +142,8 @@ def library():
     L.lee_get_tile_shape.argtypes = [vp, i, ip, ip]
     L.lee_set_filter_grid.argtypes = [vp, i, i]
     L.lee_get_filter_grid.argtypes = [vp, i, ip]
+    L.lee_set_filter_march.argtypes = [vp, i, i, i]
+    L.lee_get_filter_march.argtypes = [vp, i, ip, ip]
     L.lee_batch_create.argtypes = [ctypes.POINTER(_Config), ctypes.POINTER(_Block), i, ctypes.POINTER(vp)]
     L.lee_get_solve_time.argtypes = [vp, i, ip, dp, dp]
     L.lee_set_batch_launch.argtypes = [vp, i, i, i]
@@ -419,6 +421,16 @@ class Solver:
         g = ctypes.c_int()
         self._check(self.L.lee_get_filter_grid(self.h, ib, ctypes.byref(g)))
         return g.value
+
+    def set_filter_march(self, columns, rows, ib=0):
+        """Column-march filter form (blocks without holes): columns per lane
+        1 or 2, rows per wave 8, 16 or 32; (0, 0) returns to the tile forms."""
+        self._check(self.L.lee_set_filter_march(self.h, ib, int(columns), int(rows)))
+
+    def filter_march(self, ib=0):
+        c, r = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.lee_get_filter_march(self.h, ib, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
 
     def tile_shape(self, ib=0):
         r, f = ctypes.c_int(), ctypes.c_int()

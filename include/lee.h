@@ -199,9 +199,19 @@ int lee_get_tile_shape(lee_handle *h, int iblock, int *stage_rows, int *filter_r
  * launches one workgroup per tile; a positive count (rounded up to a
  * multiple of 8) launches that many persistent workgroups, each loading its
  * next tile while it filters the current one.  The autotuner also tries 2, 4
- * and 8 workgroups per CU.  Results are bit-identical in every form. */
+ * and 8 workgroups per CU.  Results are bit-identical in every form.  Setting
+ * the tile shape or the grid selects the tile forms. */
 int lee_set_filter_grid(lee_handle *h, int iblock, int workgroups);
 int lee_get_filter_grid(lee_handle *h, int iblock, int *workgroups);
+/* Column-march form of the fused filter, for blocks without holes (no
+ * reference counterpart): one wave filters one variable on a strip of
+ * 64 * columns grid columns over `rows` rows, taking the xi neighbours from
+ * adjacent lanes and keeping the eta window in registers (no LDS).  columns
+ * 1 or 2, rows 8, 16 or 32; (0, 0) returns to the tile forms above.  LEE_EINVAL
+ * for a block with holes or a batch.  The autotuner tries it on blocks
+ * without holes; results are bit-identical to the tile forms. */
+int lee_set_filter_march(lee_handle *h, int iblock, int columns, int rows);
+int lee_get_filter_march(lee_handle *h, int iblock, int *columns, int *rows);
 
 /* Batched ensemble (no reference counterpart: the reference runs one
  * problem per process).  nsolve independent single-block problems, block k

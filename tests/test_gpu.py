@@ -223,7 +223,49 @@ def test_tile_shapes_bit_identical(prec):
     s = Solver.from_case(c, precision=prec, path=PATH_FUSED)
     s.solve(1)
     assert s.tile_shape()[0] in (1, 2, 4) and s.tile_shape()[1] in (4, 8, 12, 16)
-    print(f"tuned prec {prec}: tile {s.tile_shape()} filter grid {s.filter_grid()}", flush=True)
+    print(f"tuned prec {prec}: tile {s.tile_shape()} filter grid {s.filter_grid()} march {s.filter_march()}",
+          flush=True)
+
+
+def _march_case():
+    """301 x 150 wavy grid, uniform mean flow, off-centre pulse: wide and tall
+    enough that the column-march filter has interior waves (all loads inside
+    the block) besides edge waves for both lane widths and both row counts."""
+    ni, nj = 301, 150
+    x, y = cases._wavy(ni, nj, 150.0, 75.0, 0.05)
+    ic = np.zeros((5, nj, ni))
+    ic[4] = cases._field(x, y, cases._pulse(1e-3, -20.0, 8.0, 4.0))
+    ic[1] = cases._field(x, y, cases._pulse(3e-4, 10.0, -5.0, 3.0))
+    return dict(name="march", ni=ni, nj=nj, x=x, y=y, mean=cases._uniform_mean(x, u=0.3, v=0.1), ic=ic,
+                bc=_sponges(16), mode="FIX_CFL", cfl=0.5, dt=None, nsteps=20,
+                simulation="CASE_TANNA_TPN49", **cases.BASE_SCHEME)
+
+
+MARCH_FORMS = [(1, 8), (1, 16), (1, 32), (2, 8), (2, 16), (2, 32)]
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("name", [n for n in sorted(cases.ALL) if n not in cases.PLAIN_ONLY] + ["march"])
+def test_filter_march_bit_identical(prec, name):
+    """The column-march filter form computes each point with the operations
+    of the tile forms: every lane width and row count gives the tile form's
+    bits, over whole runs (edges, closures, carried w', non-uniform mean)."""
+    c = _march_case() if name == "march" else cases.ALL[name]()
+    s = Solver.from_case(c, precision=prec, path=PATH_FUSED)
+    s.set_tile_shape(1, 8)
+    s.solve(c["nsteps"])
+    ref = s.solution()
+    s.close()
+    for mc, ml in MARCH_FORMS:
+        s = Solver.from_case(c, precision=prec, path=PATH_FUSED)
+        s.set_tile_shape(1, 8)
+        s.set_filter_march(mc, ml)
+        assert s.filter_march() == (mc, ml)
+        s.solve(c["nsteps"])
+        sol = s.solution()
+        s.close()
+        assert np.array_equal(sol, ref), (name, prec, mc, ml, float(np.max(np.abs(sol - ref))))
+    print(f"march {name} prec {prec}: {len(MARCH_FORMS)} forms bit-identical", flush=True)
 
 
 def _overset_solver(prec, path, name="overset_small"):
