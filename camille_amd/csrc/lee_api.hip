@@ -518,12 +518,14 @@ int autotune_block(lee_handle *h, Block &b) {
   }
   b.tune_fty = best_f;
   b.tune_fgrid = best_g;
-  // column-march form (blocks without holes): 1 or 2 columns per lane, 8,
-  // 16 or 32 rows per wave
+  // column-march forms (blocks without holes): single waves of 8 or 16
+  // rows, stacks of four waves of 4, 8 or 16 rows or of eight of 8 rows
   int best_mc = 0, best_ml = 0;
   if (h->cfg.filter_order > 0 && !b.has_holes) {
-    for (int mc : {1, 2}) {
-      for (int ml : {8, 16, 32}) {
+    static const int forms[][2] = {{1, 8}, {1, 16}, {4, 4}, {4, 8}, {4, 16}, {8, 8}};
+    for (const auto &fm : forms) {
+      {
+        const int mc = fm[0], ml = fm[1];
         b.tune_mc = mc;
         b.tune_ml = ml;
         for (int k = 0; k < 2; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
@@ -1316,22 +1318,26 @@ int lee_set_filter_grid(lee_handle *h, int ib, int workgroups) {
   return LEE_OK;
 }
 
-int lee_set_filter_march(lee_handle *h, int ib, int columns, int rows) {
+int lee_set_filter_march(lee_handle *h, int ib, int waves, int rows) {
   if (!h || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
   if (h->batch) return fail(h, LEE_EINVAL, "a batch launches its filter through the batch launch settings");
-  const bool off = columns == 0 && rows == 0;
-  if (!off && ((columns != 1 && columns != 2) || (rows != 8 && rows != 16 && rows != 32)))
-    return fail(h, LEE_EINVAL, "column-march filter: columns 1 or 2 and rows 8, 16 or 32, or both 0");
+  const bool off = waves == 0 && rows == 0;
+  const bool ok = (waves == 1 && (rows == 8 || rows == 16)) ||
+                  (waves == 4 && (rows == 4 || rows == 8 || rows == 16)) || (waves == 8 && rows == 8);
+  if (!off && !ok)
+    return fail(h, LEE_EINVAL,
+                "column-march filter: 1 wave of 8 or 16 rows, stacks of 4 waves of 4, 8 or 16 rows or of 8 waves of "
+                "8 rows, or (0, 0)");
   if (!off && h->b[ib].has_holes) return fail(h, LEE_EINVAL, "the column-march filter serves blocks without holes");
-  h->b[ib].tune_mc = off ? 0 : columns;
+  h->b[ib].tune_mc = off ? 0 : waves;
   h->b[ib].tune_ml = off ? 0 : rows;
   h->b[ib].tuned = true;
   return LEE_OK;
 }
 
-int lee_get_filter_march(lee_handle *h, int ib, int *columns, int *rows) {
-  if (!h || ib < 0 || ib >= h->nblk || !columns || !rows) return fail(h, LEE_EINVAL, "bad argument");
-  *columns = h->batch ? 0 : h->b[ib].tune_mc;
+int lee_get_filter_march(lee_handle *h, int ib, int *waves, int *rows) {
+  if (!h || ib < 0 || ib >= h->nblk || !waves || !rows) return fail(h, LEE_EINVAL, "bad argument");
+  *waves = h->batch ? 0 : h->b[ib].tune_mc;
   *rows = h->batch ? 0 : h->b[ib].tune_ml;
   return LEE_OK;
 }

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lee_internal.h"
 
 namespace lee {
@@ -420,7 +422,8 @@ __device__ __forceinline__ void filter_tile(const AT &a, int tile, FilterLds<T, 
 
 // ---------------------------------------------------------------------------
 // Column-march form of the filter (blocks without holes).  One wave filters
-// one variable on a strip of 64 C columns over L rows.  Lane l holds the C
+// one variable on a strip of 64 C columns over L rows (the launcher uses
+// C = 1: two columns per lane halve the waves and were measured slower).  Lane l holds the C
 // consecutive columns i_s + C l; the outer 4 columns on each side are halo,
 // so strips advance by 64 C - 8 columns and overlap by 8.  The wave requests
 // its L + 8 raw rows at once, then per row takes the xi neighbours of its
@@ -580,6 +583,118 @@ __device__ __forceinline__ void march_wave(const FilterArgs<T> &a, int q, int s,
   }
 }
 
+// Stacked form of the column march: the NW waves of a workgroup filter one
+// variable on one strip (64 lanes, 56 output columns) over NW R consecutive
+// rows, R rows each.  Each wave computes the xi rows of its own R rows and
+// 8 / NW of the 8 halo rows of the stack (4 above, 4 below), passes them to
+// its neighbours through LDS, and takes the 4 xi rows on each side of its
+// range from there; so a stack filters NW R + 8 xi rows instead of
+// NW (R + 8).
+template <typename T, int NW, int R>
+struct StackLds {
+  T xi[NW * R + 2 * FL_H][64];
+};
+
+template <typename T, int ORDER, int NW, int R, bool INNER>
+__device__ __forceinline__ void stack_wave(const FilterArgs<T> &a, int q, int s, int j0, int w,
+                                           StackLds<T, NW, R> &L) {
+  constexpr int NB = ORDER / 2, HL = March<1>::HL, NH = 2 * FL_H / NW;  // halo xi rows per wave
+  static_assert(NW * NH == 2 * FL_H && R >= FL_H, "stack shape");
+  const int lane = threadIdx.x;
+  const int ni = a.ni, nj = a.nj;
+  const size_t vs = a.vs ? a.vs : (size_t)ni * nj;
+  const T *src = a.src + a.vars[q] * vs;
+  T *dst = a.dst + a.vars[q] * vs;
+  const int gi = s * March<1>::STRIDE - FL_H + lane;
+  const bool out_lane = lane >= HL && lane < 64 - HL;
+  const bool cok = gi >= 0 && gi < ni;
+  const int ci = min(max(gi, 0), ni - 1);
+  const int cx = cok ? default_code(gi, ni, NB) : CODE_ZERO;
+  // stack rows (LDS index): t = block row - (j0 - 4); own rows 4 + w R + k,
+  // halo rows h = NH w .. NH w + NH - 1 of {0, .., 3, NW R + 4, .., NW R + 7}
+  int tr[R + NH];
+#pragma unroll
+  for (int k = 0; k < R; k++) tr[k] = FL_H + w * R + k;
+#pragma unroll
+  for (int k = 0; k < NH; k++) {
+    const int h = NH * w + k;
+    tr[R + k] = h < FL_H ? h : NW * R + h;
+  }
+  T raw[R + NH];
+#pragma unroll
+  for (int k = 0; k < R + NH; k++) {
+    const int gj = j0 - FL_H + tr[k];
+    if constexpr (INNER) {
+      raw[k] = src[(size_t)gj * ni + gi];
+    } else {
+      raw[k] = src[(size_t)min(max(gj, 0), nj - 1) * ni + ci];
+    }
+  }
+  // xi rows (values outside the block are 0, selected here at the use)
+  T xo[R + NH];
+#pragma unroll
+  for (int k = 0; k < R + NH; k++) {
+    const int gj = j0 - FL_H + tr[k];
+    const bool rok = INNER || (gj >= 0 && gj < nj);
+    const T u = (rok && (INNER || cok)) ? raw[k] : (T)0;
+    T wv[2 * FL_H + 1];
+    T tl = u, trr = u;
+    wv[FL_H] = u;
+#pragma unroll
+    for (int m = 1; m <= FL_H; m++) {
+      tl = wave_shift1<true>(tl);
+      trr = wave_shift1<false>(trr);
+      wv[FL_H - m] = tl;
+      wv[FL_H + m] = trr;
+    }
+    if constexpr (INNER) {
+      xo[k] = filter_row<T, ORDER>(&wv[FL_H], 1, CODE_INTERIOR);
+    } else {
+      xo[k] = (rok && cx != CODE_ZERO) ? filter_row<T, ORDER>(&wv[FL_H], 1, cx) : (T)0;
+    }
+  }
+  // neighbours need the first and last 4 own rows and the halo rows
+#pragma unroll
+  for (int k = 0; k < R + NH; k++)
+    if (k >= R || k < FL_H || k >= R - FL_H) L.xi[tr[k]][lane] = xo[k];
+  __syncthreads();
+  T above[FL_H], below[FL_H];
+#pragma unroll
+  for (int m = 0; m < FL_H; m++) {
+    above[m] = L.xi[w * R + m][lane];
+    below[m] = L.xi[FL_H + (w + 1) * R + m][lane];
+  }
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int go = j0 + w * R + k;
+    if (!INNER && go >= nj) break;
+    T col[2 * FL_H + 1];
+#pragma unroll
+    for (int m = 0; m <= 2 * FL_H; m++) {
+      const int r = k - FL_H + m;  // own-row index of the window row
+      col[m] = r < 0 ? above[r + FL_H] : (r >= R ? below[r - R] : xo[r]);
+    }
+    const int cy = INNER ? CODE_INTERIOR : default_code(go, nj, NB);
+    const T out = filter_blend(a.rem, raw[k], a.blend, filter_row<T, ORDER>(&col[FL_H], 1, cy));
+    if (out_lane && (INNER || cok)) dst[(size_t)go * ni + (INNER ? gi : ci)] = out;
+  }
+}
+
+// Stacked launch: a.ntx strips, a.nty stacks of NW R rows per strip,
+// a.ntiles = nvar * ntx * nty workgroups, a.chunk workgroups per XCD.
+template <typename T, int ORDER, int NW, int R>
+__device__ __forceinline__ void filter_stack(const FilterArgs<T> &a, StackLds<T, NW, R> &L) {
+  const int t = xcd_tile(blockIdx.x, a.chunk);
+  if (t >= a.ntiles) return;
+  const int c = t % a.nty, s = (t / a.nty) % a.ntx, q = t / (a.nty * a.ntx);
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const int j0 = c * NW * R, i_s = s * March<1>::STRIDE - FL_H;
+  if (i_s >= 0 && i_s + 64 <= a.ni && j0 - FL_H >= 0 && j0 + NW * R + FL_H <= a.nj)
+    stack_wave<T, ORDER, NW, R, true>(a, q, s, j0, w, L);
+  else
+    stack_wave<T, ORDER, NW, R, false>(a, q, s, j0, w, L);
+}
+
 // The wave's share from its index within the launch: four consecutive row
 // chunks of one strip per workgroup, workgroups spread over the XCDs by
 // xcd_tile so that each XCD walks whole strips (adjacent chunks share their
@@ -587,7 +702,8 @@ __device__ __forceinline__ void march_wave(const FilterArgs<T> &a, int q, int s,
 // nty waves, a.chunk = workgroups per XCD.
 template <typename T, int ORDER, int C, int L>
 __device__ __forceinline__ void filter_march(const FilterArgs<T> &a) {
-  const int wv = 4 * xcd_tile(blockIdx.x, a.chunk) + (int)threadIdx.y;
+  // the wave index is uniform: keep it (and every row index) in SGPRs
+  const int wv = 4 * xcd_tile(blockIdx.x, a.chunk) + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   if (wv >= a.ntiles) return;
   const int c = wv % a.nty, s = (wv / a.nty) % a.ntx, q = wv / (a.nty * a.ntx);
   const int i_s = s * March<C>::STRIDE - FL_H, j_s = c * L - FL_H;

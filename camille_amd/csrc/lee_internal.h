@@ -96,7 +96,7 @@ struct Block {
   // height 4 R) and filter tile height (4, 8, 16; FP64 4); set by the autotuner
   int tune_r = 1, tune_fty = 4;
   int tune_fgrid = 0;          // filter: 0 one workgroup per tile, else persistent pipelined grid
-  int tune_mc = 0, tune_ml = 0;  // filter column-march form (no holes): columns per lane, rows per wave; 0 off
+  int tune_mc = 0, tune_ml = 0;  // filter column-march form (no holes): waves per stack, rows per wave; 0 off
   int cus = 0;                 // compute units of the device
   bool tuned = false;
   // plain-path scratch (FP64): dflux, sum, rhs (5n), func0, func, interim (5n), ynp1 (5n)

@@ -784,6 +784,14 @@ __global__ void __launch_bounds__(TPB) k_fused_filter_m(FilterArgs<T> a) {
   filter_march<T, ORDER, C, L>(a);
 }
 
+// Stacked column march (blocks without holes): NW waves per (variable,
+// strip, stack of NW R rows), xi rows shared through LDS (filter_stack).
+template <typename T, int ORDER, int NW, int R>
+__global__ void __launch_bounds__(64 * NW) k_fused_filter_s(FilterArgs<T> a) {
+  __shared__ StackLds<T, NW, R> L;
+  filter_stack<T, ORDER, NW, R>(a, L);
+}
+
 template <typename T>
 struct BatchFilterArgs {
   const FilterArgs<T> *args;     // nsolve entries
@@ -1212,29 +1220,30 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
   const dim3 pgrid(std::min(a.chunk * 8, (b.tune_fgrid + 7) / 8 * 8));
   const bool o8 = h->cfg.filter_order == 8;
   if (b.tune_mc > 0 && !b.has_holes) {
-    // column-march form: every carried variable in one launch
+    // column-march forms: every carried variable in one launch
     const int nv = b.carry_w ? 5 : 4;
     a.vars[4] = IUZ;
-    const int C = b.tune_mc, L = b.tune_ml;
-    const int stride = C == 1 ? March<1>::STRIDE : March<2>::STRIDE;
-    a.ntx = (b.ni + stride - 1) / stride;
-    a.nty = (b.nj + L - 1) / L;
-    a.ntiles = nv * a.ntx * a.nty;
-    a.chunk = ((a.ntiles + 3) / 4 + 7) / 8;
-    const dim3 mgrid(a.chunk * 8);
-#define LEE_FLM(OR, CC, LL) hipLaunchKernelGGL((k_fused_filter_m<T, OR, CC, LL>), mgrid, block, 0, s, a)
-#define LEE_FLM_O(OR)                                     \
-  do {                                                    \
-    if (C == 1 && L == 8) LEE_FLM(OR, 1, 8);              \
-    else if (C == 1 && L == 16) LEE_FLM(OR, 1, 16);       \
-    else if (C == 1) LEE_FLM(OR, 1, 32);                  \
-    else if (L == 8) LEE_FLM(OR, 2, 8);                   \
-    else if (L == 16) LEE_FLM(OR, 2, 16);                 \
-    else LEE_FLM(OR, 2, 32);                              \
+    const int NW = b.tune_mc, L = b.tune_ml;
+    a.ntx = (b.ni + March<1>::STRIDE - 1) / March<1>::STRIDE;
+    a.nty = (b.nj + NW * L - 1) / (NW * L);
+    a.ntiles = nv * a.ntx * a.nty;  // waves (single-wave form) or stacks
+    a.chunk = NW == 1 ? ((a.ntiles + 3) / 4 + 7) / 8 : (a.ntiles + 7) / 8;
+    const dim3 mgrid(a.chunk * 8), sblock(64, NW == 1 ? 4 : NW);
+#define LEE_FLM(OR, LL) hipLaunchKernelGGL((k_fused_filter_m<T, OR, 1, LL>), mgrid, sblock, 0, s, a)
+#define LEE_FLS(OR, NWV, RR) hipLaunchKernelGGL((k_fused_filter_s<T, OR, NWV, RR>), mgrid, sblock, 0, s, a)
+#define LEE_FLM_O(OR)                              \
+  do {                                             \
+    if (NW == 1 && L == 8) LEE_FLM(OR, 8);         \
+    else if (NW == 1) LEE_FLM(OR, 16);             \
+    else if (NW == 8) LEE_FLS(OR, 8, 8);           \
+    else if (L == 4) LEE_FLS(OR, 4, 4);            \
+    else if (L == 8) LEE_FLS(OR, 4, 8);            \
+    else LEE_FLS(OR, 4, 16);                       \
   } while (0)
     if (o8) LEE_FLM_O(8);
     else LEE_FLM_O(4);
 #undef LEE_FLM_O
+#undef LEE_FLS
 #undef LEE_FLM
     return hipGetLastError();
   }

@@ -422,10 +422,11 @@ class Solver:
         self._check(self.L.lee_get_filter_grid(self.h, ib, ctypes.byref(g)))
         return g.value
 
-    def set_filter_march(self, columns, rows, ib=0):
-        """Column-march filter form (blocks without holes): columns per lane
-        1 or 2, rows per wave 8, 16 or 32; (0, 0) returns to the tile forms."""
-        self._check(self.L.lee_set_filter_march(self.h, ib, int(columns), int(rows)))
+    def set_filter_march(self, waves, rows, ib=0):
+        """Column-march filter forms (blocks without holes): waves 1 with
+        rows 8 or 16, stacks of 4 waves with rows 4, 8 or 16 or of 8 waves
+        with rows 8; (0, 0) returns to the tile forms."""
+        self._check(self.L.lee_set_filter_march(self.h, ib, int(waves), int(rows)))
 
     def filter_march(self, ib=0):
         c, r = ctypes.c_int(), ctypes.c_int()

@@ -203,15 +203,18 @@ int lee_get_tile_shape(lee_handle *h, int iblock, int *stage_rows, int *filter_r
  * the tile shape or the grid selects the tile forms. */
 int lee_set_filter_grid(lee_handle *h, int iblock, int workgroups);
 int lee_get_filter_grid(lee_handle *h, int iblock, int *workgroups);
-/* Column-march form of the fused filter, for blocks without holes (no
- * reference counterpart): one wave filters one variable on a strip of
- * 64 * columns grid columns over `rows` rows, taking the xi neighbours from
- * adjacent lanes and keeping the eta window in registers (no LDS).  columns
- * 1 or 2, rows 8, 16 or 32; (0, 0) returns to the tile forms above.  LEE_EINVAL
- * for a block with holes or a batch.  The autotuner tries it on blocks
- * without holes; results are bit-identical to the tile forms. */
-int lee_set_filter_march(lee_handle *h, int iblock, int columns, int rows);
-int lee_get_filter_march(lee_handle *h, int iblock, int *columns, int *rows);
+/* Column-march forms of the fused filter, for blocks without holes (no
+ * reference counterpart).  A wave filters one variable on a strip of 64
+ * grid columns (56 of them output), taking the xi neighbours from adjacent
+ * lanes and keeping the eta window in registers.  waves 1: each wave filters
+ * `rows` rows alone (8 or 16); waves 4 or 8: a workgroup stacks that many
+ * waves of `rows` rows each (4, 8 or 16 for 4 waves; 8 for 8) and shares
+ * their xi rows through LDS.  (0, 0) returns to the tile forms above.
+ * LEE_EINVAL for a block with holes or a batch.  The autotuner tries these
+ * forms on blocks without holes; results are bit-identical to the tile
+ * forms. */
+int lee_set_filter_march(lee_handle *h, int iblock, int waves, int rows);
+int lee_get_filter_march(lee_handle *h, int iblock, int *waves, int *rows);
 
 /* Batched ensemble (no reference counterpart: the reference runs one
  * problem per process).  nsolve independent single-block problems, block k

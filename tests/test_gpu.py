@@ -229,8 +229,8 @@ def test_tile_shapes_bit_identical(prec):
 
 def _march_case():
     """301 x 150 wavy grid, uniform mean flow, off-centre pulse: wide and tall
-    enough that the column-march filter has interior waves (all loads inside
-    the block) besides edge waves for both lane widths and both row counts."""
+    enough that every column-march filter form has interior waves or stacks
+    (all loads inside the block) besides edge ones."""
     ni, nj = 301, 150
     x, y = cases._wavy(ni, nj, 150.0, 75.0, 0.05)
     ic = np.zeros((5, nj, ni))
@@ -241,14 +241,16 @@ def _march_case():
                 simulation="CASE_TANNA_TPN49", **cases.BASE_SCHEME)
 
 
-MARCH_FORMS = [(1, 8), (1, 16), (1, 32), (2, 8), (2, 16), (2, 32)]
+# (waves, rows per wave): single-wave chunks, and stacks of four or eight
+# waves sharing their xi rows
+MARCH_FORMS = [(1, 8), (1, 16), (4, 4), (4, 8), (4, 16), (8, 8)]
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
 @pytest.mark.parametrize("name", [n for n in sorted(cases.ALL) if n not in cases.PLAIN_ONLY] + ["march"])
 def test_filter_march_bit_identical(prec, name):
-    """The column-march filter form computes each point with the operations
-    of the tile forms: every lane width and row count gives the tile form's
+    """The column-march filter forms compute each point with the operations
+    of the tile forms: every stack shape and row count gives the tile form's
     bits, over whole runs (edges, closures, carried w', non-uniform mean)."""
     c = _march_case() if name == "march" else cases.ALL[name]()
     s = Solver.from_case(c, precision=prec, path=PATH_FUSED)
