@@ -355,6 +355,8 @@ int batch_autotune(lee_handle *h) {
     if (rc) break;
   }
   best = 1e30f;
+  int bS = 0;
+  p.stack = 0;
   for (int ty : {4, 8, 12, 16}) {
     if (rc) break;
     p.TY = ty;
@@ -370,6 +372,25 @@ int batch_autotune(lee_handle *h) {
       }
     }
   }
+  // column-march stacks of four waves (solves without holes)
+  bool holes = false;
+  for (int k = 0; k < h->nblk; k++) holes = holes || h->b[k].has_holes;
+  for (int st : {4, 8, 16}) {
+    if (rc || holes) break;
+    p.stack = st;
+    HIPCHK(batch_prepare(h->stream, h));
+    for (int g : grids) {
+      p.grid_filter = g;
+      float ms = 0.f;
+      if ((rc = time_it(false, ms))) break;
+      if (ms < best) {
+        best = ms;
+        bS = st;
+        bF = g;
+      }
+    }
+  }
+  p.stack = 0;
   p.tuning = false;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -378,6 +399,7 @@ int batch_autotune(lee_handle *h) {
   p.R = bR;
   p.grid_stage = bG;
   p.TY = bT;
+  p.stack = bS;
   p.grid_filter = bF;
   p.tuned = true;
   return LEE_OK;
@@ -1298,6 +1320,7 @@ int lee_set_tile_shape(lee_handle *h, int ib, int stage_rows, int filter_rows) {
     HIPCHK(hipStreamSynchronize(h->stream));
     h->bp.R = stage_rows;
     h->bp.TY = filter_rows;
+    h->bp.stack = 0;  // a tile height selects the tile forms
     h->bp.tuned = true;
     batch_release(h);
     return LEE_OK;
@@ -1320,8 +1343,18 @@ int lee_set_filter_grid(lee_handle *h, int ib, int workgroups) {
 
 int lee_set_filter_march(lee_handle *h, int ib, int waves, int rows) {
   if (!h || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
-  if (h->batch) return fail(h, LEE_EINVAL, "a batch launches its filter through the batch launch settings");
   const bool off = waves == 0 && rows == 0;
+  if (h->batch) {  // one launch covers every solve: stacks of four waves only
+    if (!off && !(waves == 4 && (rows == 4 || rows == 8 || rows == 16)))
+      return fail(h, LEE_EINVAL, "a batch takes stacks of 4 waves of 4, 8 or 16 rows, or (0, 0)");
+    for (int k = 0; k < h->nblk && !off; k++)
+      if (h->b[k].has_holes) return fail(h, LEE_EINVAL, "the column-march filter serves blocks without holes");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->bp.stack = off ? 0 : rows;
+    h->bp.tuned = true;
+    batch_release(h);
+    return LEE_OK;
+  }
   const bool ok = (waves == 1 && (rows == 8 || rows == 16)) ||
                   (waves == 4 && (rows == 4 || rows == 8 || rows == 16)) || (waves == 8 && rows == 8);
   if (!off && !ok)
@@ -1337,8 +1370,8 @@ int lee_set_filter_march(lee_handle *h, int ib, int waves, int rows) {
 
 int lee_get_filter_march(lee_handle *h, int ib, int *waves, int *rows) {
   if (!h || ib < 0 || ib >= h->nblk || !waves || !rows) return fail(h, LEE_EINVAL, "bad argument");
-  *waves = h->batch ? 0 : h->b[ib].tune_mc;
-  *rows = h->batch ? 0 : h->b[ib].tune_ml;
+  *waves = h->batch ? (h->bp.stack ? 4 : 0) : h->b[ib].tune_mc;
+  *rows = h->batch ? h->bp.stack : h->b[ib].tune_ml;
   return LEE_OK;
 }
 

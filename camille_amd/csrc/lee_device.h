@@ -595,9 +595,8 @@ struct StackLds {
   T xi[NW * R + 2 * FL_H][64];
 };
 
-template <typename T, int ORDER, int NW, int R, bool INNER>
-__device__ __forceinline__ void stack_wave(const FilterArgs<T> &a, int q, int s, int j0, int w,
-                                           StackLds<T, NW, R> &L) {
+template <typename T, int ORDER, int NW, int R, bool INNER, typename AT>
+__device__ __forceinline__ void stack_wave(const AT &a, int q, int s, int j0, int w, StackLds<T, NW, R> &L) {
   constexpr int NB = ORDER / 2, HL = March<1>::HL, NH = 2 * FL_H / NW;  // halo xi rows per wave
   static_assert(NW * NH == 2 * FL_H && R >= FL_H, "stack shape");
   const int lane = threadIdx.x;
@@ -680,12 +679,10 @@ __device__ __forceinline__ void stack_wave(const FilterArgs<T> &a, int q, int s,
   }
 }
 
-// Stacked launch: a.ntx strips, a.nty stacks of NW R rows per strip,
-// a.ntiles = nvar * ntx * nty workgroups, a.chunk workgroups per XCD.
-template <typename T, int ORDER, int NW, int R>
-__device__ __forceinline__ void filter_stack(const FilterArgs<T> &a, StackLds<T, NW, R> &L) {
-  const int t = xcd_tile(blockIdx.x, a.chunk);
-  if (t >= a.ntiles) return;
+// Stack t of a launch: a.ntx strips, a.nty stacks of NW R rows per strip,
+// a.ntiles = nvar * ntx * nty stacks.
+template <typename T, int ORDER, int NW, int R, typename AT>
+__device__ __forceinline__ void stack_item(const AT &a, int t, StackLds<T, NW, R> &L) {
   const int c = t % a.nty, s = (t / a.nty) % a.ntx, q = t / (a.nty * a.ntx);
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   const int j0 = c * NW * R, i_s = s * March<1>::STRIDE - FL_H;
@@ -693,6 +690,14 @@ __device__ __forceinline__ void filter_stack(const FilterArgs<T> &a, StackLds<T,
     stack_wave<T, ORDER, NW, R, true>(a, q, s, j0, w, L);
   else
     stack_wave<T, ORDER, NW, R, false>(a, q, s, j0, w, L);
+}
+
+// One stack per workgroup, a.chunk workgroups per XCD.
+template <typename T, int ORDER, int NW, int R>
+__device__ __forceinline__ void filter_stack(const FilterArgs<T> &a, StackLds<T, NW, R> &L) {
+  const int t = xcd_tile(blockIdx.x, a.chunk);
+  if (t >= a.ntiles) return;
+  stack_item<T, ORDER, NW, R>(a, t, L);
 }
 
 // The wave's share from its index within the launch: four consecutive row

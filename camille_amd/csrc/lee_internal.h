@@ -171,6 +171,7 @@ struct BatchPlan {
   bool tuned = false;
   bool tuning = false;         // tables address scratch buffers only (autotuner)
   int R = 1, TY = 4;           // stage rows per thread, filter tile height
+  int stack = 0;               // filter: rows per wave of four-wave column-march stacks (0: tiles)
   int grid_stage = 0, grid_filter = 0;  // persistent grid sizes (0: one workgroup per tile)
   void *stage_args = nullptr;  // device StageArgs<T>[stages][nsolve]
   void *filter_args = nullptr; // device FilterArgs<T>[nsolve]

@@ -820,6 +820,24 @@ __global__ void __launch_bounds__(TPB)
 }
 
 
+// Batched stacked column march (solves without holes): the list entries are
+// (solve, stack) pairs; four waves of R rows per stack.
+template <typename T, int ORDER, int R>
+__global__ void __launch_bounds__(TPB)
+    k_batch_filter_s(BatchFilterArgs<T> ba, const LEE_CONST_AS FilterArgs<T> *args, const LEE_CONST_AS int *map) {
+  __shared__ StackLds<T, 4, R> L;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    const int c = __hip_atomic_load(ba.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(ba.counter, c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // vector store
+  }
+  for (int v = blockIdx.x; v < 8 * ba.chunk; v += gridDim.x) {
+    const int t = (v & 7) * ba.chunk + (v >> 3);
+    if (t >= ba.total) continue;
+    stack_item<T, ORDER, 4, R>(args[map[2 * t]], map[2 * t + 1], L);
+    __syncthreads();  // the LDS rows are rewritten by the next stack
+  }
+}
+
 // Passive w' in 2-D: only the sponge acts on it (the velocity loop of
 // spatial:2077-2103 stops at num_dim), then the RK update.
 template <typename T>
@@ -1327,6 +1345,11 @@ static hipError_t batch_prepare_t(hipStream_t s, lee_handle *h) {
     for (int st = 0; st < S; st++)
       sa[(size_t)st * ns + k] = make_stage_args<T>(h, b, p.tuning ? -1 : st, Y, rho, p.R, zero);
     fa[k] = make_filter_args<T>(h, b, Y, 1, p.tuning ? 2 : 0, p.TY);
+    if (p.stack) {  // stacks of four waves of p.stack rows, four variables
+      fa[k].ntx = (b.ni + March<1>::STRIDE - 1) / March<1>::STRIDE;
+      fa[k].nty = (b.nj + 4 * p.stack - 1) / (4 * p.stack);
+      fa[k].ntiles = 4 * fa[k].ntx * fa[k].nty;
+    }
     for (int t = 0; t < sa[k].ntiles; t++) {
       ts.push_back(k);
       ts.push_back(t);
@@ -1430,6 +1453,20 @@ static hipError_t batch_filter_t(hipStream_t s, lee_handle *h) {
   const dim3 grid(batch_grid(p.total_filter, p.grid_filter)), block(64, 4);
   const bool o8 = h->cfg.filter_order == 8;
   const int TY = p.TY;
+  if (p.stack) {
+#define LEE_BS(OR, RR) hipLaunchKernelGGL((k_batch_filter_s<T, OR, RR>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map))
+#define LEE_BS_O(OR)                        \
+  do {                                      \
+    if (p.stack == 4) LEE_BS(OR, 4);        \
+    else if (p.stack == 8) LEE_BS(OR, 8);   \
+    else LEE_BS(OR, 16);                    \
+  } while (0)
+    if (o8) LEE_BS_O(8);
+    else LEE_BS_O(4);
+#undef LEE_BS_O
+#undef LEE_BS
+    return hipGetLastError();
+  }
 #define LEE_BF(OR)                                                                                   \
   do {                                                                                               \
     if (TY == 4) hipLaunchKernelGGL((k_batch_filter<T, OR, 4>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map)); \

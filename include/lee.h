@@ -209,10 +209,10 @@ int lee_get_filter_grid(lee_handle *h, int iblock, int *workgroups);
  * lanes and keeping the eta window in registers.  waves 1: each wave filters
  * `rows` rows alone (8 or 16); waves 4 or 8: a workgroup stacks that many
  * waves of `rows` rows each (4, 8 or 16 for 4 waves; 8 for 8) and shares
- * their xi rows through LDS.  (0, 0) returns to the tile forms above.
- * LEE_EINVAL for a block with holes or a batch.  The autotuner tries these
- * forms on blocks without holes; results are bit-identical to the tile
- * forms. */
+ * their xi rows through LDS.  (0, 0) returns to the tile forms above.  A
+ * batch (one launch for every solve) takes stacks of 4 waves only.
+ * LEE_EINVAL for a block with holes.  The autotuners try these forms on
+ * blocks without holes; results are bit-identical to the tile forms. */
 int lee_set_filter_march(lee_handle *h, int iblock, int waves, int rows);
 int lee_get_filter_march(lee_handle *h, int iblock, int *waves, int *rows);
 

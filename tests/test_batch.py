@@ -133,23 +133,28 @@ def test_batch_equals_single_handles_bitwise(prec):
 
 @pytest.mark.gpu
 def test_batch_launch_modes_bit_identical():
-    """Graph replay or direct launches, any persistent grid and tile shape:
-    the same bits; also two solve calls equal one."""
+    """Graph replay or direct launches, any persistent grid, tile shape or
+    column-march stack: the same bits; also two solve calls equal one."""
     solves = _ensemble(seed=11, n=4)
     ref = None
-    for use_graph, gs, gf, shape in [(True, 0, 0, (1, 4)), (False, 0, 0, (1, 4)), (True, 256, 512, (2, 8)),
-                                     (True, 1024, 0, (4, 16)), (False, 8, 8, (1, 8)), (True, 0, 64, (2, 12))]:
+    for use_graph, gs, gf, shape, march in [
+            (True, 0, 0, (1, 4), None), (False, 0, 0, (1, 4), None), (True, 256, 512, (2, 8), None),
+            (True, 1024, 0, (4, 16), None), (False, 8, 8, (1, 8), None), (True, 0, 64, (2, 12), None),
+            (True, 0, 0, (1, 4), (4, 8)), (False, 0, 24, (2, 8), (4, 4)), (True, 0, 512, (1, 4), (4, 16))]:
         b = _batch(solves, FP32)
         b.set_tile_shape(*shape)
+        if march:
+            b.set_filter_march(*march)
+            assert b.filter_march() == march
         b.set_launch(use_graph, gs, gf)
         b.solve(7)
         b.solve(13)
         sols = [b.solution(k) for k in range(len(solves))]
-        print(f"graph {use_graph} grids {gs},{gf} tile {shape}: {b.launch()}", flush=True)
+        print(f"graph {use_graph} grids {gs},{gf} tile {shape} march {march}: {b.launch()}", flush=True)
         if ref is None:
             ref = sols
         for k in range(len(solves)):
-            assert np.array_equal(sols[k], ref[k]), (use_graph, gs, gf, shape, k)
+            assert np.array_equal(sols[k], ref[k]), (use_graph, gs, gf, shape, march, k)
         b.close()
     b = _batch(solves, FP32)
     b.solve(20)
