@@ -466,9 +466,9 @@ int batch_solve(lee_handle *h, int nsteps) {
 // Y3, output Y1; filter: Y1 -> Y2); the fastest is kept and cached per (ni,
 // nj, precision, mean kind, holes) for the life of the process.
 struct TuneKey {
-  int ni, nj, prec, meanf, holes;
+  int ni, nj, nk, prec, meanf, holes;
   bool operator==(const TuneKey &o) const {
-    return ni == o.ni && nj == o.nj && prec == o.prec && meanf == o.meanf && holes == o.holes;
+    return ni == o.ni && nj == o.nj && nk == o.nk && prec == o.prec && meanf == o.meanf && holes == o.holes;
   }
 };
 struct TuneVal {
@@ -477,7 +477,7 @@ struct TuneVal {
 std::vector<std::pair<TuneKey, TuneVal>> g_tune_cache;
 
 int autotune_block(lee_handle *h, Block &b) {
-  const TuneKey key{b.ni, b.nj, h->cfg.precision, b.uniform_mean ? 0 : 1, b.has_holes ? 1 : 0};
+  const TuneKey key{b.ni, b.nj, 1, h->cfg.precision, b.uniform_mean ? 0 : 1, b.has_holes ? 1 : 0};
   for (const auto &e : g_tune_cache)
     if (e.first == key) {
       b.tune_r = e.second.r;
@@ -574,11 +574,57 @@ int autotune_block(lee_handle *h, Block &b) {
   return LEE_OK;
 }
 
+// 3-D blocks: the stage kernel has one shape; the plane pass of the filter
+// is timed in the tile form and as stacks of four waves of 4, 8 or 16 rows
+// (Y1 -> Y2 through Y3, buffers that carry no state between steps), cached
+// per (ni, nj, nk, precision).
+int autotune_block3d(lee_handle *h, Block &b) {
+  const TuneKey key{b.ni, b.nj, b.nk, h->cfg.precision, 0, 0};
+  for (const auto &e : g_tune_cache)
+    if (e.first == key) {
+      b.tune_mc = e.second.mc;
+      b.tune_ml = e.second.ml;
+      b.tuned = true;
+      return LEE_OK;
+    }
+  int best_mc = 0, best_ml = 0;
+  if (h->cfg.filter_order > 0) {
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best = 1e30f;
+    static const int forms[][2] = {{0, 0}, {4, 4}, {4, 8}, {4, 16}};
+    for (const auto &fm : forms) {
+      b.tune_mc = fm[0];
+      b.tune_ml = fm[1];
+      for (int k = 0; k < 2; k++) HIPCHK(fused_filter3d(h->stream, h, b, 1, 2, 3));
+      HIPCHK(hipEventRecord(e0, h->stream));
+      for (int k = 0; k < 5; k++) HIPCHK(fused_filter3d(h->stream, h, b, 1, 2, 3));
+      HIPCHK(hipEventRecord(e1, h->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_mc = fm[0];
+        best_ml = fm[1];
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  b.tune_mc = best_mc;
+  b.tune_ml = best_ml;
+  b.tuned = true;
+  g_tune_cache.push_back({key, {b.tune_r, b.tune_fty, b.tune_fgrid, b.tune_mc, b.tune_ml}});
+  return LEE_OK;
+}
+
 int autotune(lee_handle *h) {
-  if (is_plain(h) || h->cfg.num_dim == 3) return LEE_OK;  // the 3-D kernels have one tile shape
+  if (is_plain(h)) return LEE_OK;
   for (int ib = 0; ib < h->nblk; ib++) {
     if (h->b[ib].tuned) continue;
-    int rc = autotune_block(h, h->b[ib]);
+    int rc = h->cfg.num_dim == 3 ? autotune_block3d(h, h->b[ib]) : autotune_block(h, h->b[ib]);
     if (rc) return rc;
   }
   return LEE_OK;
@@ -1362,6 +1408,8 @@ int lee_set_filter_march(lee_handle *h, int ib, int waves, int rows) {
                 "column-march filter: 1 wave of 8 or 16 rows, stacks of 4 waves of 4, 8 or 16 rows or of 8 waves of "
                 "8 rows, or (0, 0)");
   if (!off && h->b[ib].has_holes) return fail(h, LEE_EINVAL, "the column-march filter serves blocks without holes");
+  if (!off && h->cfg.num_dim == 3 && waves != 4)
+    return fail(h, LEE_EINVAL, "the 3-D plane pass takes stacks of 4 waves of 4, 8 or 16 rows, or (0, 0)");
   h->b[ib].tune_mc = off ? 0 : waves;
   h->b[ib].tune_ml = off ? 0 : rows;
   h->b[ib].tuned = true;

@@ -328,6 +328,19 @@ __global__ void __launch_bounds__(TPB) k_filter3d_plane(FilterArgs<T> a, size_t 
   filter_tile<T, false, ORDER, 5, TY>(b, tile, L);
 }
 
+// Pass 1 in the stacked column-march form (stack_item): stacks of four waves
+// of R rows on plane blockIdx.y.
+template <typename T, int ORDER, int R>
+__global__ void __launch_bounds__(TPB) k_filter3d_plane_s(FilterArgs<T> a, size_t plane) {
+  __shared__ StackLds<T, 4, R> L;
+  const int t = xcd_tile(blockIdx.x, a.chunk);
+  if (t >= a.ntiles) return;
+  FilterArgs<T> b = a;
+  b.src = a.src + (size_t)blockIdx.y * plane;
+  b.dst = a.dst + (size_t)blockIdx.y * plane;
+  stack_item<T, ORDER, 4, R>(b, t, L);
+}
+
 // Pass 2: the zeta sweep of tmp along each (i, j) column through a chunk of
 // planes (9-plane register window), blend with y, into dst.
 template <typename T, int ORDER>
@@ -515,10 +528,28 @@ static hipError_t filter3d_t(hipStream_t s, lee_handle *h, Block &b, T *const *Y
   a.rem = (T)0;
   for (int v = 0; v < 5; v++) a.vars[v] = v;
   const size_t plane = (size_t)b.ni * b.nj;
-  const dim3 grid(a.chunk * 8, b.nk), block(64, 4);
   const bool o8 = h->cfg.filter_order == 8;
-  if (o8) hipLaunchKernelGGL((k_filter3d_plane<T, 8, TYF>), grid, block, 0, s, a, plane);
-  else hipLaunchKernelGGL((k_filter3d_plane<T, 4, TYF>), grid, block, 0, s, a, plane);
+  if (b.tune_mc == 4) {  // stacks of four waves of tune_ml rows
+    const int R = b.tune_ml;
+    a.ntx = (b.ni + March<1>::STRIDE - 1) / March<1>::STRIDE;
+    a.nty = (b.nj + 4 * R - 1) / (4 * R);
+    a.ntiles = 5 * a.ntx * a.nty;
+    a.chunk = (a.ntiles + 7) / 8;
+    const dim3 grid(a.chunk * 8, b.nk), block(64, 4);
+#define LEE_F3S(OR)                                                                                   \
+  do {                                                                                                \
+    if (R == 4) hipLaunchKernelGGL((k_filter3d_plane_s<T, OR, 4>), grid, block, 0, s, a, plane);      \
+    else if (R == 8) hipLaunchKernelGGL((k_filter3d_plane_s<T, OR, 8>), grid, block, 0, s, a, plane); \
+    else hipLaunchKernelGGL((k_filter3d_plane_s<T, OR, 16>), grid, block, 0, s, a, plane);            \
+  } while (0)
+    if (o8) LEE_F3S(8);
+    else LEE_F3S(4);
+#undef LEE_F3S
+  } else {
+    const dim3 grid(a.chunk * 8, b.nk), block(64, 4);
+    if (o8) hipLaunchKernelGGL((k_filter3d_plane<T, 8, TYF>), grid, block, 0, s, a, plane);
+    else hipLaunchKernelGGL((k_filter3d_plane<T, 4, TYF>), grid, block, 0, s, a, plane);
+  }
   const dim3 gz(nb3(plane), (b.nk + Z_CHUNK - 1) / Z_CHUNK);
   const T blend = (T)h->cfg.filter_blend, rem = (T)(1.0 - h->cfg.filter_blend);
   if (o8) hipLaunchKernelGGL((k_filter3d_z<T, 8>), gz, dim3(TPB), 0, s, Y[tmp], Y[src], Y[dst], b.ni, b.nj, b.nk, rem, blend);

@@ -105,6 +105,36 @@ def test_axis_exchange_symmetry():
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_3d_plane_filter_forms_bit_identical(prec):
+    """The plane pass of the 3-D filter in the stacked column-march form
+    (stacks of 4 waves of 4, 8 or 16 rows) gives the tile form's bits; the
+    130 x 80 planes have interior stacks besides edge ones."""
+    ni, nj, nk = 130, 80, 12
+    x, y, z = _box(ni, nj, nk)
+    yc, zc = y.mean(), z.mean()
+    mean = np.zeros((5, nk, nj, ni))
+    mean[1] = 0.3 * 0.5 * (1 - np.tanh((np.abs(y - yc) - 1.2) / 0.4))
+    mean[4] = 1.0 / GAMMA
+    ic = np.zeros((5, nk, nj, ni))
+    ic[4] = 1e-3 * np.exp(-((x - 9.0) ** 2 + (y - yc - 0.4) ** 2 + (z - zc + 0.3) ** 2) / 2.0)
+    ic[1] = 2e-4 * np.exp(-((x - 20.0) ** 2 + (y - yc) ** 2 + (z - zc) ** 2) / 3.0)
+    ref = None
+    for form in ((0, 0), (4, 4), (4, 8), (4, 16)):
+        s = Solver([dict(x=x, y=y, z=z, mean=mean)], boundaries=(), precision=prec, mode="FIX_DT", value=0.04,
+                   num_dim=3)
+        s.set_filter_march(*form)
+        s.set_solution(ic)
+        s.solve(12)
+        q = s.solution()
+        s.close()
+        if ref is None:
+            ref = q
+            assert np.max(np.abs(q[4])) > 1e-5
+        assert np.array_equal(q, ref), (prec, form)
+    print(f"3-D plane filter forms prec {prec}: bit-identical", flush=True)
+
+
 def test_derivative_exact_on_quadratics():
     ni, nj, nk = 20, 18, 16
     x, y, z = _box(ni, nj, nk)
