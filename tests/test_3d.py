@@ -175,9 +175,12 @@ def _shear_box(ni=22, nj=20, nk=18, h=0.25):
     return x, y, z, mean, bc, r
 
 
+@pytest.mark.parametrize("dims", [(22, 20, 18), (14, 12, 70)])
 @pytest.mark.parametrize("prec,tol", [(FP64, 1e-12), (FP32, 1e-5)])
-def test_3d_transposes_dot_products(prec, tol):
-    x, y, z, mean, bc, _ = _shear_box()
+def test_3d_transposes_dot_products(prec, tol, dims):
+    """<L a, k> = <a, L^T k> for the right-hand side and the filter; the
+    70-plane box spans three 32-plane chunks of the z passes."""
+    x, y, z, mean, bc, _ = _shear_box(*dims)
     s = Solver([dict(x=x, y=y, z=z, mean=mean)], boundaries=bc, precision=prec, num_dim=3)
     rng = np.random.default_rng(5)
     a, k = rng.standard_normal((2, 5) + x.shape)
@@ -192,10 +195,12 @@ def test_3d_transposes_dot_products(prec, tol):
     s.close()
 
 
-def test_3d_gradient_matches_central_differences():
+@pytest.mark.parametrize("dims", [(22, 20, 18), (16, 14, 70)])
+def test_3d_gradient_matches_central_differences(dims):
     """Observer energy on a cylindrical surface around the shear layer,
-    gradient with respect to a field source, checkpoints every 8 steps."""
-    x, y, z, mean, bc, r = _shear_box()
+    gradient with respect to a field source, checkpoints every 8 steps; the
+    70-plane box spans three 32-plane chunks of the stage kernel."""
+    x, y, z, mean, bc, r = _shear_box(*dims)
     h = 0.25
     s = Solver([dict(x=x, y=y, z=z, mean=mean)], boundaries=bc, precision=FP64, num_dim=3,
                simulation="SOURCE_FIELD", source_params=(1.0, 0.5 * math.pi))
