@@ -540,13 +540,13 @@ int autotune_block(lee_handle *h, Block &b) {
   }
   b.tune_fty = best_f;
   b.tune_fgrid = best_g;
-  // column-march forms (blocks without holes): single waves of 8 or 16
-  // rows, stacks of four waves of 4, 8 or 16 rows or of eight of 8 rows
+  // column-march forms: single waves of 8 or 16 rows (blocks without
+  // holes), stacks of four waves of 4, 8 or 16 rows or of eight of 8 rows
   int best_mc = 0, best_ml = 0;
-  if (h->cfg.filter_order > 0 && !b.has_holes) {
+  if (h->cfg.filter_order > 0) {
     static const int forms[][2] = {{1, 8}, {1, 16}, {4, 4}, {4, 8}, {4, 16}, {8, 8}};
     for (const auto &fm : forms) {
-      {
+      if (fm[0] > 1 || !b.has_holes) {
         const int mc = fm[0], ml = fm[1];
         b.tune_mc = mc;
         b.tune_ml = ml;
@@ -1407,7 +1407,8 @@ int lee_set_filter_march(lee_handle *h, int ib, int waves, int rows) {
     return fail(h, LEE_EINVAL,
                 "column-march filter: 1 wave of 8 or 16 rows, stacks of 4 waves of 4, 8 or 16 rows or of 8 waves of "
                 "8 rows, or (0, 0)");
-  if (!off && h->b[ib].has_holes) return fail(h, LEE_EINVAL, "the column-march filter serves blocks without holes");
+  if (!off && waves == 1 && h->b[ib].has_holes)
+    return fail(h, LEE_EINVAL, "single-wave column-march filter: blocks without holes only (stacks take holes)");
   if (!off && h->cfg.num_dim == 3 && waves != 4)
     return fail(h, LEE_EINVAL, "the 3-D plane pass takes stacks of 4 waves of 4, 8 or 16 rows, or (0, 0)");
   h->b[ib].tune_mc = off ? 0 : waves;

@@ -595,8 +595,12 @@ struct StackLds {
   T xi[NW * R + 2 * FL_H][64];
 };
 
-template <typename T, int ORDER, int NW, int R, bool INNER, typename AT>
+// HOLES: per-point filter codes (xi at code[p], eta at code[n + p]) and the
+// hole reset, as filter_tile's; INNER (never with HOLES): every load inside
+// the block and every point of the stack on the interior rows.
+template <typename T, int ORDER, int NW, int R, bool HOLES, bool INNER, typename AT>
 __device__ __forceinline__ void stack_wave(const AT &a, int q, int s, int j0, int w, StackLds<T, NW, R> &L) {
+  static_assert(!(HOLES && INNER), "an interior stack has no special points");
   constexpr int NB = ORDER / 2, HL = March<1>::HL, NH = 2 * FL_H / NW;  // halo xi rows per wave
   static_assert(NW * NH == 2 * FL_H && R >= FL_H, "stack shape");
   const int lane = threadIdx.x;
@@ -620,13 +624,22 @@ __device__ __forceinline__ void stack_wave(const AT &a, int q, int s, int j0, in
     tr[R + k] = h < FL_H ? h : NW * R + h;
   }
   T raw[R + NH];
+  int8_t cxr[HOLES ? R + NH : 1], cyr[HOLES ? R : 1], hbr[HOLES ? R : 1];  // codes and iblank (HOLES)
 #pragma unroll
   for (int k = 0; k < R + NH; k++) {
     const int gj = j0 - FL_H + tr[k];
     if constexpr (INNER) {
       raw[k] = src[(size_t)gj * ni + gi];
     } else {
-      raw[k] = src[(size_t)min(max(gj, 0), nj - 1) * ni + ci];
+      const size_t pc = (size_t)min(max(gj, 0), nj - 1) * ni + ci;
+      raw[k] = src[pc];
+      if constexpr (HOLES) {
+        cxr[k] = a.code[pc];
+        if (k < R) {
+          cyr[k] = a.code[(size_t)ni * nj + pc];
+          hbr[k] = a.iblank[pc];
+        }
+      }
     }
   }
   // xi rows (values outside the block are 0, selected here at the use)
@@ -649,7 +662,8 @@ __device__ __forceinline__ void stack_wave(const AT &a, int q, int s, int j0, in
     if constexpr (INNER) {
       xo[k] = filter_row<T, ORDER>(&wv[FL_H], 1, CODE_INTERIOR);
     } else {
-      xo[k] = (rok && cx != CODE_ZERO) ? filter_row<T, ORDER>(&wv[FL_H], 1, cx) : (T)0;
+      const int code = HOLES ? (cok ? (int)cxr[k] : CODE_ZERO) : cx;
+      xo[k] = (rok && code != CODE_ZERO) ? filter_row<T, ORDER>(&wv[FL_H], 1, code) : (T)0;
     }
   }
   // neighbours need the first and last 4 own rows and the halo rows
@@ -673,31 +687,43 @@ __device__ __forceinline__ void stack_wave(const AT &a, int q, int s, int j0, in
       const int r = k - FL_H + m;  // own-row index of the window row
       col[m] = r < 0 ? above[r + FL_H] : (r >= R ? below[r - R] : xo[r]);
     }
-    const int cy = INNER ? CODE_INTERIOR : default_code(go, nj, NB);
-    const T out = filter_blend(a.rem, raw[k], a.blend, filter_row<T, ORDER>(&col[FL_H], 1, cy));
+    const int cy = INNER ? CODE_INTERIOR : (HOLES ? (int)cyr[k] : default_code(go, nj, NB));
+    T out = filter_blend(a.rem, raw[k], a.blend, filter_row<T, ORDER>(&col[FL_H], 1, cy));
+    if (HOLES && hbr[k] == BLANKED) out = 0;
     if (out_lane && (INNER || cok)) dst[(size_t)go * ni + (INNER ? gi : ci)] = out;
   }
 }
 
 // Stack t of a launch: a.ntx strips, a.nty stacks of NW R rows per strip,
 // a.ntiles = nvar * ntx * nty stacks.
-template <typename T, int ORDER, int NW, int R, typename AT>
+template <typename T, int ORDER, int NW, int R, bool HOLES = false, typename AT>
 __device__ __forceinline__ void stack_item(const AT &a, int t, StackLds<T, NW, R> &L) {
   const int c = t % a.nty, s = (t / a.nty) % a.ntx, q = t / (a.nty * a.ntx);
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   const int j0 = c * NW * R, i_s = s * March<1>::STRIDE - FL_H;
-  if (i_s >= 0 && i_s + 64 <= a.ni && j0 - FL_H >= 0 && j0 + NW * R + FL_H <= a.nj)
-    stack_wave<T, ORDER, NW, R, true>(a, q, s, j0, w, L);
-  else
-    stack_wave<T, ORDER, NW, R, false>(a, q, s, j0, w, L);
+  const bool inside = i_s >= 0 && i_s + 64 <= a.ni && j0 - FL_H >= 0 && j0 + NW * R + FL_H <= a.nj;
+  if constexpr (HOLES) {
+    // a stack whose output columns and xi rows hold no point off the
+    // interior filter (no hole, no closure row) runs the interior instance
+    if (inside && a.clean &&
+        box_clean(a.clean, a.ni, a.nj, i_s + FL_H, i_s + 64 - FL_H, j0 - FL_H, j0 + NW * R + FL_H))
+      stack_wave<T, ORDER, NW, R, false, true>(a, q, s, j0, w, L);
+    else
+      stack_wave<T, ORDER, NW, R, true, false>(a, q, s, j0, w, L);
+  } else {
+    if (inside)
+      stack_wave<T, ORDER, NW, R, false, true>(a, q, s, j0, w, L);
+    else
+      stack_wave<T, ORDER, NW, R, false, false>(a, q, s, j0, w, L);
+  }
 }
 
 // One stack per workgroup, a.chunk workgroups per XCD.
-template <typename T, int ORDER, int NW, int R>
+template <typename T, int ORDER, int NW, int R, bool HOLES>
 __device__ __forceinline__ void filter_stack(const FilterArgs<T> &a, StackLds<T, NW, R> &L) {
   const int t = xcd_tile(blockIdx.x, a.chunk);
   if (t >= a.ntiles) return;
-  stack_item<T, ORDER, NW, R>(a, t, L);
+  stack_item<T, ORDER, NW, R, HOLES>(a, t, L);
 }
 
 // The wave's share from its index within the launch: four consecutive row

@@ -786,10 +786,10 @@ __global__ void __launch_bounds__(TPB) k_fused_filter_m(FilterArgs<T> a) {
 
 // Stacked column march (blocks without holes): NW waves per (variable,
 // strip, stack of NW R rows), xi rows shared through LDS (filter_stack).
-template <typename T, int ORDER, int NW, int R>
+template <typename T, int ORDER, int NW, int R, bool HOLES>
 __global__ void __launch_bounds__(64 * NW) k_fused_filter_s(FilterArgs<T> a) {
   __shared__ StackLds<T, NW, R> L;
-  filter_stack<T, ORDER, NW, R>(a, L);
+  filter_stack<T, ORDER, NW, R, HOLES>(a, L);
 }
 
 template <typename T>
@@ -1237,7 +1237,7 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
   const bool pipe = b.tune_fgrid > 0;
   const dim3 pgrid(std::min(a.chunk * 8, (b.tune_fgrid + 7) / 8 * 8));
   const bool o8 = h->cfg.filter_order == 8;
-  if (b.tune_mc > 0 && !b.has_holes) {
+  if (b.tune_mc > 0 && (!b.has_holes || b.tune_mc > 1)) {
     // column-march forms: every carried variable in one launch
     const int nv = b.carry_w ? 5 : 4;
     a.vars[4] = IUZ;
@@ -1248,7 +1248,11 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
     a.chunk = NW == 1 ? ((a.ntiles + 3) / 4 + 7) / 8 : (a.ntiles + 7) / 8;
     const dim3 mgrid(a.chunk * 8), sblock(64, NW == 1 ? 4 : NW);
 #define LEE_FLM(OR, LL) hipLaunchKernelGGL((k_fused_filter_m<T, OR, 1, LL>), mgrid, sblock, 0, s, a)
-#define LEE_FLS(OR, NWV, RR) hipLaunchKernelGGL((k_fused_filter_s<T, OR, NWV, RR>), mgrid, sblock, 0, s, a)
+#define LEE_FLS(OR, NWV, RR)                                                                              \
+  do {                                                                                                    \
+    if (b.has_holes) hipLaunchKernelGGL((k_fused_filter_s<T, OR, NWV, RR, true>), mgrid, sblock, 0, s, a); \
+    else hipLaunchKernelGGL((k_fused_filter_s<T, OR, NWV, RR, false>), mgrid, sblock, 0, s, a);            \
+  } while (0)
 #define LEE_FLM_O(OR)                              \
   do {                                             \
     if (NW == 1 && L == 8) LEE_FLM(OR, 8);         \
