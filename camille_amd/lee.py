@@ -423,8 +423,8 @@ class Solver:
         return g.value
 
     def set_filter_march(self, waves, rows, ib=0):
-        """Column-march filter forms (blocks without holes): waves 1 with
-        rows 8 or 16, stacks of 4 waves with rows 4, 8 or 16 or of 8 waves
+        """Column-march filter forms: waves 1 with rows 8 or 16 (blocks
+        without holes), stacks of 4 waves with rows 4, 8 or 16 or of 8 waves
         with rows 8; (0, 0) returns to the tile forms."""
         self._check(self.L.lee_set_filter_march(self.h, ib, int(waves), int(rows)))
 

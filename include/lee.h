@@ -203,16 +203,16 @@ int lee_get_tile_shape(lee_handle *h, int iblock, int *stage_rows, int *filter_r
  * the tile shape or the grid selects the tile forms. */
 int lee_set_filter_grid(lee_handle *h, int iblock, int workgroups);
 int lee_get_filter_grid(lee_handle *h, int iblock, int *workgroups);
-/* Column-march forms of the fused filter, for blocks without holes (no
- * reference counterpart).  A wave filters one variable on a strip of 64
- * grid columns (56 of them output), taking the xi neighbours from adjacent
- * lanes and keeping the eta window in registers.  waves 1: each wave filters
- * `rows` rows alone (8 or 16); waves 4 or 8: a workgroup stacks that many
+/* Column-march forms of the fused filter (no reference counterpart).  A
+ * wave filters one variable on a strip of 64 grid columns (56 of them
+ * output), taking the xi neighbours from adjacent lanes and keeping the eta
+ * window in registers.  waves 1: each wave filters `rows` rows alone (8 or
+ * 16; blocks without holes); waves 4 or 8: a workgroup stacks that many
  * waves of `rows` rows each (4, 8 or 16 for 4 waves; 8 for 8) and shares
- * their xi rows through LDS.  (0, 0) returns to the tile forms above.  A
- * batch (one launch for every solve) takes stacks of 4 waves only.
- * LEE_EINVAL for a block with holes.  The autotuners try these forms on
- * blocks without holes; results are bit-identical to the tile forms. */
+ * their xi rows through LDS (blocks with holes too).  (0, 0) returns to the
+ * tile forms above.  A batch (one launch for every solve) and the 3-D plane
+ * pass take stacks of 4 waves only.  The autotuners try these forms;
+ * results are bit-identical to the tile forms. */
 int lee_set_filter_march(lee_handle *h, int iblock, int waves, int rows);
 int lee_get_filter_march(lee_handle *h, int iblock, int *waves, int *rows);
 
