@@ -493,17 +493,29 @@ int autotune_block(lee_handle *h, Block &b) {
   HIPCHK(hipEventCreate(&e1));
   const int reps = 10;
   const StageScalars sc{1.0, 1.0, 0.0, 0.0, 0.0};
+  // best of three timings of `reps` launches (after two untimed ones):
+  // candidates a few per cent apart are then told apart reliably
+  auto time_ms = [&](bool stage, float &ms) -> int {
+    for (int k = 0; k < 2; k++) HIPCHK(stage ? fused_stage(h->stream, h, b, -1, sc) : fused_filter(h->stream, h, b, 1, 2));
+    ms = 1e30f;
+    for (int trial = 0; trial < 3; trial++) {
+      HIPCHK(hipEventRecord(e0, h->stream));
+      for (int k = 0; k < reps; k++)
+        HIPCHK(stage ? fused_stage(h->stream, h, b, -1, sc) : fused_filter(h->stream, h, b, 1, 2));
+      HIPCHK(hipEventRecord(e1, h->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, e0, e1));
+      ms = std::min(ms, t);
+    }
+    return LEE_OK;
+  };
   float best = 1e30f;
   int best_r = b.tune_r;
   for (int r : {1, 2, 4}) {
     b.tune_r = r;
-    for (int k = 0; k < 2; k++) HIPCHK(fused_stage(h->stream, h, b, -1, sc));
-    HIPCHK(hipEventRecord(e0, h->stream));
-    for (int k = 0; k < reps; k++) HIPCHK(fused_stage(h->stream, h, b, -1, sc));
-    HIPCHK(hipEventRecord(e1, h->stream));
-    HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (int rc = time_ms(true, ms)) return rc;
     if (ms < best) {
       best = ms;
       best_r = r;
@@ -523,13 +535,8 @@ int autotune_block(lee_handle *h, Block &b) {
         if (pw > 0 && b.cus <= 0) continue;
         b.tune_fty = ty;
         b.tune_fgrid = pw * b.cus;
-        for (int k = 0; k < 2; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
-        HIPCHK(hipEventRecord(e0, h->stream));
-        for (int k = 0; k < reps; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
-        HIPCHK(hipEventRecord(e1, h->stream));
-        HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (int rc = time_ms(false, ms)) return rc;
         if (ms < best) {
           best = ms;
           best_f = ty;
@@ -550,13 +557,8 @@ int autotune_block(lee_handle *h, Block &b) {
         const int mc = fm[0], ml = fm[1];
         b.tune_mc = mc;
         b.tune_ml = ml;
-        for (int k = 0; k < 2; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
-        HIPCHK(hipEventRecord(e0, h->stream));
-        for (int k = 0; k < reps; k++) HIPCHK(fused_filter(h->stream, h, b, 1, 2));
-        HIPCHK(hipEventRecord(e1, h->stream));
-        HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (int rc = time_ms(false, ms)) return rc;
         if (ms < best) {
           best = ms;
           best_mc = mc;
