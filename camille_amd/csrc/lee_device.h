@@ -422,16 +422,14 @@ __device__ __forceinline__ void filter_tile(const AT &a, int tile, FilterLds<T, 
 
 // ---------------------------------------------------------------------------
 // Column-march form of the filter (blocks without holes).  One wave filters
-// one variable on a strip of 64 C columns over L rows (the launcher uses
-// C = 1: two columns per lane halve the waves and were measured slower).  Lane l holds the C
-// consecutive columns i_s + C l; the outer 4 columns on each side are halo,
-// so strips advance by 64 C - 8 columns and overlap by 8.  The wave requests
-// its L + 8 raw rows at once, then per row takes the xi neighbours of its
-// columns from the adjacent lanes (DPP wave shifts: no LDS, no barrier),
-// keeps the xi-filtered rows in registers and runs the eta sweep and the
-// blend as soon as the nine rows of an output row are there.  Each point
-// gets the operations of filter_tile in the same order, so both forms give
-// the same bits.
+// one variable on a strip of 64 columns over L rows.  Lane l holds grid
+// column i_s + l; the outer 4 lanes on each side are halo, so strips advance
+// by 56 columns and overlap by 8.  The wave requests its L + 8 raw rows at
+// once, then per row takes the xi neighbours from the adjacent lanes (DPP
+// wave shifts: no LDS, no barrier), keeps the xi-filtered rows in registers
+// and runs the eta sweep and the blend as soon as the nine rows of an output
+// row are there.  Each point gets the operations of filter_tile in the same
+// order, so both forms give the same bits.
 
 // Whole-wave lane shifts by one lane (DPP wave_shr:1, control 0x138, and
 // wave_shl:1, 0x130): UP gives lane l the value of lane l - 1, !UP that of
@@ -451,135 +449,72 @@ __device__ __forceinline__ double wave_shift1(double x) {
   return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// C consecutive values at p (aligned to one element only) as one vector
-// access.
-template <typename T, int C>
-__device__ __forceinline__ void load_cols(const T *p, T (&v)[C]) {
-  if constexpr (C == 1) {
-    v[0] = *p;
-  } else {
-    typedef T VT __attribute__((ext_vector_type(C)));
-    VT x;
-    __builtin_memcpy(&x, p, sizeof(VT));
-#pragma unroll
-    for (int e = 0; e < C; e++) v[e] = x[e];
-  }
-}
-template <typename T, int C>
-__device__ __forceinline__ void store_cols(T *p, const T (&v)[C]) {
-  if constexpr (C == 1) {
-    *p = v[0];
-  } else {
-    typedef T VT __attribute__((ext_vector_type(C)));
-    VT x;
-#pragma unroll
-    for (int e = 0; e < C; e++) x[e] = v[e];
-    __builtin_memcpy(p, &x, sizeof(VT));
-  }
-}
-
-// Strip geometry of the column-march form.
-template <int C>
+// Strip geometry of the column-march forms: one grid column per lane, the
+// outer 4 lanes on each side are halo.
 struct March {
-  static constexpr int HL = FL_H / C;              // halo lanes on each side
-  static constexpr int STRIDE = (64 - 2 * HL) * C;  // output columns per strip
+  static constexpr int HL = FL_H;               // halo lanes on each side
+  static constexpr int STRIDE = 64 - 2 * HL;    // output columns per strip
 };
 
 // One wave's share: variable slot q, strip s, row chunk c.  INNER: every
 // load of the wave lies inside the block, so every output point takes the
 // interior rows and no bounds are tested.
-template <typename T, int ORDER, int C, int L, bool INNER>
+template <typename T, int ORDER, int L, bool INNER>
 __device__ __forceinline__ void march_wave(const FilterArgs<T> &a, int q, int s, int c) {
-  static_assert(FL_H % C == 0, "the halo is a whole number of lanes");
-  constexpr int NB = ORDER / 2, HL = March<C>::HL, NR = L + 2 * FL_H, W = C + 2 * FL_H;
+  constexpr int NB = ORDER / 2, HL = March::HL, NR = L + 2 * FL_H;
   const int lane = threadIdx.x;
   const int ni = a.ni, nj = a.nj;
   const size_t vs = a.vs ? a.vs : (size_t)ni * nj;
   const T *src = a.src + a.vars[q] * vs;
   T *dst = a.dst + a.vars[q] * vs;
-  const int gi0 = s * March<C>::STRIDE - FL_H + C * lane;
+  const int gi = s * March::STRIDE - FL_H + lane;
   const int j0 = c * L;
   const bool out_lane = lane >= HL && lane < 64 - HL;
+  const bool cok = gi >= 0 && gi < ni;
+  const int ci = min(max(gi, 0), ni - 1);
   // raw rows j0 - 4 .. j0 + L + 3 (outside the block: 0), all requested here
   // from clamped addresses so the sequence has no branch
-  T xr[NR][C];
+  T xr[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     const int gj = j0 - FL_H + r;
     if constexpr (INNER) {
-      load_cols<T, C>(src + (size_t)gj * ni + gi0, xr[r]);
+      xr[r] = src[(size_t)gj * ni + gi];
     } else {
-      const int cj = min(max(gj, 0), nj - 1);
-#pragma unroll
-      for (int e = 0; e < C; e++) {
-        const int gi = gi0 + e;
-        const T v = src[(size_t)cj * ni + min(max(gi, 0), ni - 1)];
-        xr[r][e] = (gj >= 0 && gj < nj && gi >= 0 && gi < ni) ? v : (T)0;
-      }
+      const T v = src[(size_t)min(max(gj, 0), nj - 1) * ni + ci];
+      xr[r] = (gj >= 0 && gj < nj && cok) ? v : (T)0;
     }
   }
-  int cx[C];
-#pragma unroll
-  for (int e = 0; e < C; e++) {
-    const int gi = gi0 + e;
-    cx[e] = (gi >= 0 && gi < ni) ? default_code(gi, ni, NB) : CODE_ZERO;
-  }
-  T xi[NR][C];
+  const int cx = cok ? default_code(gi, ni, NB) : CODE_ZERO;
+  T xi[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) {
-    // w: columns gi0 - 4 .. gi0 + C + 3 of row r; lane l - k holds columns
-    // gi0 - k C .. gi0 - k C + C - 1
-    T w[W];
-    T tl[C], tr[C];
+    // w: columns gi - 4 .. gi + 4 of row r from lanes l - 4 .. l + 4
+    T w[2 * FL_H + 1];
+    T tl = xr[r], tr = xr[r];
+    w[FL_H] = xr[r];
 #pragma unroll
-    for (int e = 0; e < C; e++) {
-      w[FL_H + e] = xr[r][e];
-      tl[e] = xr[r][e];
-      tr[e] = xr[r][e];
-    }
-#pragma unroll
-    for (int k = 1; k <= HL; k++) {
-#pragma unroll
-      for (int e = 0; e < C; e++) {
-        tl[e] = wave_shift1<true>(tl[e]);
-        tr[e] = wave_shift1<false>(tr[e]);
-        w[FL_H - k * C + e] = tl[e];
-        w[FL_H + k * C + e] = tr[e];
-      }
+    for (int k = 1; k <= FL_H; k++) {
+      tl = wave_shift1<true>(tl);
+      tr = wave_shift1<false>(tr);
+      w[FL_H - k] = tl;
+      w[FL_H + k] = tr;
     }
     const int gj = j0 - FL_H + r;
-#pragma unroll
-    for (int e = 0; e < C; e++) {
-      if constexpr (INNER) {
-        xi[r][e] = filter_row<T, ORDER>(&w[FL_H + e], 1, CODE_INTERIOR);
-      } else {
-        const bool ok = gj >= 0 && gj < nj && cx[e] != CODE_ZERO;
-        xi[r][e] = ok ? filter_row<T, ORDER>(&w[FL_H + e], 1, cx[e]) : (T)0;
-      }
+    if constexpr (INNER) {
+      xi[r] = filter_row<T, ORDER>(&w[FL_H], 1, CODE_INTERIOR);
+    } else {
+      const bool ok = gj >= 0 && gj < nj && cx != CODE_ZERO;
+      xi[r] = ok ? filter_row<T, ORDER>(&w[FL_H], 1, cx) : (T)0;
     }
     if (r < 2 * FL_H) continue;
     // eta sweep, blend and store of output row o
     const int o = r - 2 * FL_H, go = j0 + o;
     if (!INNER && go >= nj) continue;
     const int cy = INNER ? CODE_INTERIOR : default_code(go, nj, NB);
-    T out[C];
-#pragma unroll
-    for (int e = 0; e < C; e++) {
-      T col[2 * FL_H + 1];
-#pragma unroll
-      for (int k = 0; k <= 2 * FL_H; k++) col[k] = xi[o + k][e];
-      const T f = filter_row<T, ORDER>(&col[FL_H], 1, cy);
-      out[e] = filter_blend(a.rem, xr[o + FL_H][e], a.blend, f);
-    }
-    if constexpr (INNER) {
-      if (out_lane) store_cols<T, C>(dst + (size_t)go * ni + gi0, out);
-    } else {
-#pragma unroll
-      for (int e = 0; e < C; e++) {
-        const int gi = gi0 + e;
-        if (out_lane && gi >= 0 && gi < ni) dst[(size_t)go * ni + gi] = out[e];
-      }
-    }
+    const T f = filter_row<T, ORDER>(&xi[o + FL_H], 1, cy);
+    const T out = filter_blend(a.rem, xr[o + FL_H], a.blend, f);
+    if (out_lane && (INNER || cok)) dst[(size_t)go * ni + (INNER ? gi : ci)] = out;
   }
 }
 
@@ -601,14 +536,14 @@ struct StackLds {
 template <typename T, int ORDER, int NW, int R, bool HOLES, bool INNER, typename AT>
 __device__ __forceinline__ void stack_wave(const AT &a, int q, int s, int j0, int w, StackLds<T, NW, R> &L) {
   static_assert(!(HOLES && INNER), "an interior stack has no special points");
-  constexpr int NB = ORDER / 2, HL = March<1>::HL, NH = 2 * FL_H / NW;  // halo xi rows per wave
+  constexpr int NB = ORDER / 2, HL = March::HL, NH = 2 * FL_H / NW;  // halo xi rows per wave
   static_assert(NW * NH == 2 * FL_H && R >= FL_H, "stack shape");
   const int lane = threadIdx.x;
   const int ni = a.ni, nj = a.nj;
   const size_t vs = a.vs ? a.vs : (size_t)ni * nj;
   const T *src = a.src + a.vars[q] * vs;
   T *dst = a.dst + a.vars[q] * vs;
-  const int gi = s * March<1>::STRIDE - FL_H + lane;
+  const int gi = s * March::STRIDE - FL_H + lane;
   const bool out_lane = lane >= HL && lane < 64 - HL;
   const bool cok = gi >= 0 && gi < ni;
   const int ci = min(max(gi, 0), ni - 1);
@@ -700,7 +635,7 @@ template <typename T, int ORDER, int NW, int R, bool HOLES = false, typename AT>
 __device__ __forceinline__ void stack_item(const AT &a, int t, StackLds<T, NW, R> &L) {
   const int c = t % a.nty, s = (t / a.nty) % a.ntx, q = t / (a.nty * a.ntx);
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-  const int j0 = c * NW * R, i_s = s * March<1>::STRIDE - FL_H;
+  const int j0 = c * NW * R, i_s = s * March::STRIDE - FL_H;
   const bool inside = i_s >= 0 && i_s + 64 <= a.ni && j0 - FL_H >= 0 && j0 + NW * R + FL_H <= a.nj;
   if constexpr (HOLES) {
     // a stack whose output columns and xi rows hold no point off the
@@ -731,17 +666,17 @@ __device__ __forceinline__ void filter_stack(const FilterArgs<T> &a, StackLds<T,
 // xcd_tile so that each XCD walks whole strips (adjacent chunks share their
 // halo rows in its L2).  a.ntx strips, a.nty chunks, a.ntiles = nvar * ntx *
 // nty waves, a.chunk = workgroups per XCD.
-template <typename T, int ORDER, int C, int L>
+template <typename T, int ORDER, int L>
 __device__ __forceinline__ void filter_march(const FilterArgs<T> &a) {
   // the wave index is uniform: keep it (and every row index) in SGPRs
   const int wv = 4 * xcd_tile(blockIdx.x, a.chunk) + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   if (wv >= a.ntiles) return;
   const int c = wv % a.nty, s = (wv / a.nty) % a.ntx, q = wv / (a.nty * a.ntx);
-  const int i_s = s * March<C>::STRIDE - FL_H, j_s = c * L - FL_H;
-  if (i_s >= 0 && i_s + 64 * C <= a.ni && j_s >= 0 && j_s + L + 2 * FL_H <= a.nj)
-    march_wave<T, ORDER, C, L, true>(a, q, s, c);
+  const int i_s = s * March::STRIDE - FL_H, j_s = c * L - FL_H;
+  if (i_s >= 0 && i_s + 64 <= a.ni && j_s >= 0 && j_s + L + 2 * FL_H <= a.nj)
+    march_wave<T, ORDER, L, true>(a, q, s, c);
   else
-    march_wave<T, ORDER, C, L, false>(a, q, s, c);
+    march_wave<T, ORDER, L, false>(a, q, s, c);
 }
 
 }  // namespace

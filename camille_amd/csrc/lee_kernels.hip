@@ -779,9 +779,9 @@ k_fused_filter_p(FilterArgs<T> a) {
 
 // Column-march form (blocks without holes): one wave per (variable, strip,
 // row chunk), four waves per workgroup, no LDS (filter_march).
-template <typename T, int ORDER, int C, int L>
+template <typename T, int ORDER, int L>
 __global__ void __launch_bounds__(TPB) k_fused_filter_m(FilterArgs<T> a) {
-  filter_march<T, ORDER, C, L>(a);
+  filter_march<T, ORDER, L>(a);
 }
 
 // Stacked column march (blocks without holes): NW waves per (variable,
@@ -1242,12 +1242,12 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
     const int nv = b.carry_w ? 5 : 4;
     a.vars[4] = IUZ;
     const int NW = b.tune_mc, L = b.tune_ml;
-    a.ntx = (b.ni + March<1>::STRIDE - 1) / March<1>::STRIDE;
+    a.ntx = (b.ni + March::STRIDE - 1) / March::STRIDE;
     a.nty = (b.nj + NW * L - 1) / (NW * L);
     a.ntiles = nv * a.ntx * a.nty;  // waves (single-wave form) or stacks
     a.chunk = NW == 1 ? ((a.ntiles + 3) / 4 + 7) / 8 : (a.ntiles + 7) / 8;
     const dim3 mgrid(a.chunk * 8), sblock(64, NW == 1 ? 4 : NW);
-#define LEE_FLM(OR, LL) hipLaunchKernelGGL((k_fused_filter_m<T, OR, 1, LL>), mgrid, sblock, 0, s, a)
+#define LEE_FLM(OR, LL) hipLaunchKernelGGL((k_fused_filter_m<T, OR, LL>), mgrid, sblock, 0, s, a)
 #define LEE_FLS(OR, NWV, RR)                                                                              \
   do {                                                                                                    \
     if (b.has_holes) hipLaunchKernelGGL((k_fused_filter_s<T, OR, NWV, RR, true>), mgrid, sblock, 0, s, a); \
@@ -1350,7 +1350,7 @@ static hipError_t batch_prepare_t(hipStream_t s, lee_handle *h) {
       sa[(size_t)st * ns + k] = make_stage_args<T>(h, b, p.tuning ? -1 : st, Y, rho, p.R, zero);
     fa[k] = make_filter_args<T>(h, b, Y, 1, p.tuning ? 2 : 0, p.TY);
     if (p.stack) {  // stacks of four waves of p.stack rows, four variables
-      fa[k].ntx = (b.ni + March<1>::STRIDE - 1) / March<1>::STRIDE;
+      fa[k].ntx = (b.ni + March::STRIDE - 1) / March::STRIDE;
       fa[k].nty = (b.nj + 4 * p.stack - 1) / (4 * p.stack);
       fa[k].ntiles = 4 * fa[k].ntx * fa[k].nty;
     }

@@ -531,7 +531,7 @@ static hipError_t filter3d_t(hipStream_t s, lee_handle *h, Block &b, T *const *Y
   const bool o8 = h->cfg.filter_order == 8;
   if (b.tune_mc == 4) {  // stacks of four waves of tune_ml rows
     const int R = b.tune_ml;
-    a.ntx = (b.ni + March<1>::STRIDE - 1) / March<1>::STRIDE;
+    a.ntx = (b.ni + March::STRIDE - 1) / March::STRIDE;
     a.nty = (b.nj + 4 * R - 1) / (4 * R);
     a.ntiles = 5 * a.ntx * a.nty;
     a.chunk = (a.ntiles + 7) / 8;
