@@ -133,6 +133,11 @@ def test_3d_plane_filter_forms_bit_identical(prec):
             assert np.max(np.abs(q[4])) > 1e-5
         assert np.array_equal(q, ref), (prec, form)
     print(f"3-D plane filter forms prec {prec}: bit-identical", flush=True)
+    s = Solver([dict(x=x, y=y, z=z, mean=mean)], boundaries=(), precision=prec, mode="FIX_DT", value=0.04,
+               num_dim=3)
+    with pytest.raises(LeeError):
+        s.set_filter_march(1, 8)  # the plane pass takes stacks of 4 waves only
+    s.close()
 
 
 def test_derivative_exact_on_quadratics():

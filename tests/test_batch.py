@@ -157,6 +157,8 @@ def test_batch_launch_modes_bit_identical():
             assert np.array_equal(sols[k], ref[k]), (use_graph, gs, gf, shape, march, k)
         b.close()
     b = _batch(solves, FP32)
+    with pytest.raises(Exception):
+        b.set_filter_march(1, 8)  # one launch covers every solve: stacks of 4 waves only
     b.solve(20)
     for k in range(len(solves)):
         assert np.array_equal(b.solution(k), ref[k])

@@ -285,6 +285,29 @@ def _overset_solver(prec, path, name="overset_small"):
     return c, s
 
 
+def test_filter_march_rejections():
+    """Shapes outside the forms, single waves on blocks with holes, and
+    single waves on 3-D blocks are refused with an error; a tile shape
+    returns the block to the tile forms."""
+    from camille_amd import LeeError
+    c = cases.case_curv()
+    s = Solver.from_case(c, precision=FP32, path=PATH_FUSED)
+    for bad in ((3, 8), (1, 4), (1, 32), (4, 32), (8, 4), (0, 8)):
+        with pytest.raises(LeeError):
+            s.set_filter_march(*bad)
+    s.set_filter_march(4, 8)
+    assert s.filter_march() == (4, 8)
+    s.set_tile_shape(1, 8)
+    assert s.filter_march() == (0, 0)
+    s.close()
+    c, s = _overset_solver(FP32, PATH_FUSED, "cylinder_small")
+    with pytest.raises(LeeError):
+        s.set_filter_march(1, 8, ib=0)   # block 0 has holes
+    s.set_filter_march(4, 4, ib=0)
+    assert s.filter_march(0) == (4, 4)
+    s.close()
+
+
 @pytest.mark.parametrize("prec", [FP64, FP32])
 @pytest.mark.parametrize("name", ["overset_small", "cylinder_small"])
 def test_stacked_filter_with_holes_bit_identical(prec, name):
