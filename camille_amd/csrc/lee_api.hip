@@ -600,12 +600,16 @@ int autotune_block3d(lee_handle *h, Block &b) {
       b.tune_mc = fm[0];
       b.tune_ml = fm[1];
       for (int k = 0; k < 2; k++) HIPCHK(fused_filter3d(h->stream, h, b, 1, 2, 3));
-      HIPCHK(hipEventRecord(e0, h->stream));
-      for (int k = 0; k < 5; k++) HIPCHK(fused_filter3d(h->stream, h, b, 1, 2, 3));
-      HIPCHK(hipEventRecord(e1, h->stream));
-      HIPCHK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      float ms = 1e30f;  // best of three timings of 5 launches
+      for (int trial = 0; trial < 3; trial++) {
+        HIPCHK(hipEventRecord(e0, h->stream));
+        for (int k = 0; k < 5; k++) HIPCHK(fused_filter3d(h->stream, h, b, 1, 2, 3));
+        HIPCHK(hipEventRecord(e1, h->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        ms = std::min(ms, t);
+      }
       if (ms < best) {
         best = ms;
         best_mc = fm[0];
