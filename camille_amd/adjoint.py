@@ -17,14 +17,32 @@ import torch
 from .lee import Solver  # noqa: F401  (type of the solver argument)
 
 
+def _run_forward(solver, q, y0, clock, nsteps, window_start, ckpt_every):
+    solver.set_source_field(q)
+    solver.set_solution(y0)
+    solver.set_time(*clock)
+    return solver.objective_forward(nsteps, window_start, ckpt_every)
+
+
 class ObserverEnergy(torch.autograd.Function):
+    """The solver keeps the checkpoints of its latest forward run only.  The
+    context therefore records the run's inputs, observers and the solver's
+    objective generation; when another forward run (or a change of
+    observers) has happened on the same solver since, backward first re-runs
+    this forward from the recorded inputs, so J1 + J2 on one solver, or a
+    finite-difference probe between forward and backward, still get the
+    gradient of their own run."""
+
     @staticmethod
     def forward(ctx, q, y0, solver, nsteps, window_start, ckpt_every, clock):
-        solver.set_source_field(q.detach().to("cpu", torch.float64).numpy())
-        solver.set_solution(y0.detach().to("cpu", torch.float64).numpy())
-        solver.set_time(*clock)
-        J = solver.objective_forward(nsteps, window_start, ckpt_every)
+        qn = q.detach().to("cpu", torch.float64).numpy().copy()
+        yn = y0.detach().to("cpu", torch.float64).numpy().copy()
+        args = (tuple(clock), int(nsteps), int(window_start), int(ckpt_every))
+        J = _run_forward(solver, qn, yn, *args)
         ctx.solver = solver
+        ctx.inputs = (qn, yn) + args
+        ctx.observers = getattr(solver, "observers", None)
+        ctx.generation = solver.objective_generation
         ctx.q_meta = (q.device, q.dtype)
         ctx.y_meta = (y0.device, y0.dtype)
         return torch.tensor(J, dtype=torch.float64, device=q.device)
@@ -32,6 +50,10 @@ class ObserverEnergy(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gJ):
         s = ctx.solver
+        if s.objective_generation != ctx.generation:
+            if ctx.observers is not None and getattr(s, "observers", None) is not ctx.observers:
+                s.set_observer_points(*ctx.observers)
+            _run_forward(s, *ctx.inputs)
         s.objective_backward(float(gJ))
         gq = torch.from_numpy(s.source_gradient()).to(*ctx.q_meta)
         gy = torch.from_numpy(s.state_gradient()).to(*ctx.y_meta)

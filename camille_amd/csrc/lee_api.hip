@@ -8,8 +8,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "lee_internal.h"
@@ -464,20 +466,32 @@ int batch_solve(lee_handle *h, int nsteps) {
 // or 8 workgroups per CU) is timed with HIP events on the block itself, in
 // buffers that carry no state between steps (stage: input Y2, accumulator
 // Y3, output Y1; filter: Y1 -> Y2); the fastest is kept and cached per (ni,
-// nj, precision, mean kind, holes) for the life of the process.
+// nj, nk, device, precision, mean kind, holes, filter order, carried w')
+// for the life of the process.  The cache is shared by every handle of the
+// process and guarded by a mutex, so handles on different host threads may
+// tune concurrently (two threads may both time one key; the first result
+// stored wins).
 struct TuneKey {
-  int ni, nj, nk, prec, meanf, holes;
+  int ni, nj, nk, device, prec, meanf, holes, filter_order, carry_w;
   bool operator==(const TuneKey &o) const {
-    return ni == o.ni && nj == o.nj && nk == o.nk && prec == o.prec && meanf == o.meanf && holes == o.holes;
+    return ni == o.ni && nj == o.nj && nk == o.nk && device == o.device && prec == o.prec && meanf == o.meanf &&
+           holes == o.holes && filter_order == o.filter_order && carry_w == o.carry_w;
   }
 };
 struct TuneVal {
   int r, fty, fgrid, mc, ml;
 };
 std::vector<std::pair<TuneKey, TuneVal>> g_tune_cache;
+std::mutex g_tune_mutex;
 
-int autotune_block(lee_handle *h, Block &b) {
-  const TuneKey key{b.ni, b.nj, 1, h->cfg.precision, b.uniform_mean ? 0 : 1, b.has_holes ? 1 : 0};
+TuneKey tune_key(const lee_handle *h, const Block &b) {
+  const bool d3 = h->cfg.num_dim == 3;
+  return TuneKey{b.ni, b.nj, b.nk, h->device, h->cfg.precision, d3 ? 0 : (b.uniform_mean ? 0 : 1),
+                 d3 ? 0 : (b.has_holes ? 1 : 0), h->cfg.filter_order, b.carry_w ? 1 : 0};
+}
+
+bool tune_lookup(const TuneKey &key, Block &b) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
   for (const auto &e : g_tune_cache)
     if (e.first == key) {
       b.tune_r = e.second.r;
@@ -486,8 +500,21 @@ int autotune_block(lee_handle *h, Block &b) {
       b.tune_mc = e.second.mc;
       b.tune_ml = e.second.ml;
       b.tuned = true;
-      return LEE_OK;
+      return true;
     }
+  return false;
+}
+
+void tune_store(const TuneKey &key, const Block &b) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  for (const auto &e : g_tune_cache)
+    if (e.first == key) return;
+  g_tune_cache.push_back({key, {b.tune_r, b.tune_fty, b.tune_fgrid, b.tune_mc, b.tune_ml}});
+}
+
+int autotune_block(lee_handle *h, Block &b) {
+  const TuneKey key = tune_key(h, b);
+  if (tune_lookup(key, b)) return LEE_OK;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -572,7 +599,7 @@ int autotune_block(lee_handle *h, Block &b) {
   b.tuned = true;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  g_tune_cache.push_back({key, {b.tune_r, b.tune_fty, b.tune_fgrid, b.tune_mc, b.tune_ml}});
+  tune_store(key, b);
   return LEE_OK;
 }
 
@@ -581,14 +608,8 @@ int autotune_block(lee_handle *h, Block &b) {
 // (Y1 -> Y2 through Y3, buffers that carry no state between steps), cached
 // per (ni, nj, nk, precision).
 int autotune_block3d(lee_handle *h, Block &b) {
-  const TuneKey key{b.ni, b.nj, b.nk, h->cfg.precision, 0, 0};
-  for (const auto &e : g_tune_cache)
-    if (e.first == key) {
-      b.tune_mc = e.second.mc;
-      b.tune_ml = e.second.ml;
-      b.tuned = true;
-      return LEE_OK;
-    }
+  const TuneKey key = tune_key(h, b);
+  if (tune_lookup(key, b)) return LEE_OK;
   int best_mc = 0, best_ml = 0;
   if (h->cfg.filter_order > 0) {
     hipEvent_t e0, e1;
@@ -622,7 +643,7 @@ int autotune_block3d(lee_handle *h, Block &b) {
   b.tune_mc = best_mc;
   b.tune_ml = best_ml;
   b.tuned = true;
-  g_tune_cache.push_back({key, {b.tune_r, b.tune_fty, b.tune_fgrid, b.tune_mc, b.tune_ml}});
+  tune_store(key, b);
   return LEE_OK;
 }
 
@@ -1347,7 +1368,10 @@ int lee_synchronize(lee_handle *h) {
 int lee_autotune(lee_handle *h) {
   if (!h) return LEE_EINVAL;
   for (int ib = 0; ib < h->nblk; ib++) h->b[ib].tuned = false;
-  g_tune_cache.clear();
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune_cache.clear();
+  }
   if (h->batch) {
     HIPCHK(hipStreamSynchronize(h->stream));
     h->bp.tuned = false;
@@ -1618,14 +1642,22 @@ int lee_set_observer(lee_handle *h, int nobs, const int *block, const int *index
     Block &b = h->b[ib];
     std::vector<int> l;
     std::vector<double> w;
+    std::unordered_map<int, int> slot_of;
     b.obs_pos.clear();
+    b.obs_slot.clear();
     for (int q = 0; q < nobs; q++) {
       if (block[q] < 0 || block[q] >= h->nblk) return fail(h, LEE_EINVAL, "observer in a bad block");
       if (block[q] != ib) continue;
       if (index[q] < 0 || index[q] >= b.n) return fail(h, LEE_EINVAL, "observer index outside its block");
-      l.push_back(index[q]);
-      w.push_back(weight[q]);
+      auto it = slot_of.find(index[q]);
+      if (it == slot_of.end()) {
+        it = slot_of.emplace(index[q], (int)l.size()).first;
+        l.push_back(index[q]);
+        w.push_back(0.0);
+      }
+      w[it->second] += weight[q];
       b.obs_pos.push_back(q);
+      b.obs_slot.push_back(it->second);
     }
     dfree(b.obs_l);
     dfree(b.obs_w);
@@ -1652,7 +1684,7 @@ int lee_sample_observers(lee_handle *h, double *out) {
     tmp.resize(b.nobs);
     HIPCHK(hipMemcpyAsync(tmp.data(), b.obs_buf, sizeof(double) * b.nobs, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    for (int q = 0; q < b.nobs; q++) out[b.obs_pos[q]] = tmp[q];
+    for (size_t e = 0; e < b.obs_pos.size(); e++) out[b.obs_pos[e]] = tmp[b.obs_slot[e]];
   }
   return LEE_OK;
 }

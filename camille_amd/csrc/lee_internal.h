@@ -124,7 +124,12 @@ struct Block {
   int nobs = 0;
   int *obs_l = nullptr;
   double *obs_w = nullptr;
-  std::vector<int> obs_pos;    // position of each observer in the caller's list
+  // Observer points of this block, duplicates merged: obs_l/obs_w hold each
+  // distinct point once with the sum of its weights (J and the adjoint seed
+  // then update every point from one thread).  The caller's entry obs_pos[e]
+  // samples the distinct point obs_slot[e].
+  std::vector<int> obs_pos;
+  std::vector<int> obs_slot;
   double *obs_buf = nullptr;   // device samples of the current state
   // checkpoints of an objective run: slots of (state 5n + aux density n)
   // in the bulk type; the last slot holds the state at the end of the run
@@ -274,6 +279,9 @@ hipError_t aux_density(hipStream_t s, const Block &b, double gamma, const double
 hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, float *y32, bool fp32,
                      double *rho64, float *rho32);
 hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_index, bool last_stage);
+// Overset interpolation of caller-owned FP64 block arrays ys[b] (device,
+// 5n each), receiver values staged in buf (nrecv * NV doubles, device).
+hipError_t overset_interpolate_arrays(hipStream_t s, lee_handle *h, double *const *ys, double *buf);
 hipError_t metrics2d(hipStream_t s, Block &b, const OpTable &fd);
 
 hipError_t fused_stage(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc);

@@ -1037,7 +1037,7 @@ hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, 
 }
 
 template <typename T>
-static hipError_t overset_t(hipStream_t s, lee_handle *h, T *const *ys, T *const *rhos, bool with_rho) {
+static hipError_t overset_t(hipStream_t s, lee_handle *h, T *const *ys, T *const *rhos, bool with_rho, double *buf) {
   OvTables<T> t{};
   for (int ib = 0; ib < h->nblk; ib++) {
     t.y[ib] = ys[ib];
@@ -1050,11 +1050,11 @@ static hipError_t overset_t(hipStream_t s, lee_handle *h, T *const *ys, T *const
   const Overset &o = h->ov;
   if (o.coef5)
     hipLaunchKernelGGL(k_ov_gather5<T>, dim3(nblocks((size_t)NV * o.nrecv)), dim3(TPB), 0, s, t, o.db, o.dl, o.coef5,
-                       o.buf, o.nrecv);
+                       buf, o.nrecv);
   else
     hipLaunchKernelGGL(k_ov_gather<T>, dim3(nblocks(o.nrecv)), dim3(TPB), 0, s, t, o.db, o.dl, o.w, o.coef, o.stride,
-                       o.buf, o.nrecv);
-  hipLaunchKernelGGL(k_ov_scatter<T>, dim3(nblocks(o.nrecv)), dim3(TPB), 0, s, t, o.rb, o.rl, o.buf, o.nrecv,
+                       buf, o.nrecv);
+  hipLaunchKernelGGL(k_ov_scatter<T>, dim3(nblocks(o.nrecv)), dim3(TPB), 0, s, t, o.rb, o.rl, buf, o.nrecv,
                      with_rho ? 1 : 0, 1.0 / h->cfg.gamma);
   return hipGetLastError();
 }
@@ -1071,7 +1071,7 @@ hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_
       rh[ib] = h->b[ib].uniform_mean ? nullptr : h->b[ib].rho32;
       any_rho |= rh[ib] != nullptr;
     }
-    return overset_t<float>(s, h, ys, rh, last_stage && any_rho);
+    return overset_t<float>(s, h, ys, rh, last_stage && any_rho, h->ov.buf);
   }
   double *ys[MAX_OV_BLOCKS], *rh[MAX_OV_BLOCKS];
   bool any_rho = false;
@@ -1081,7 +1081,12 @@ hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_
     rh[ib] = (buf_index >= 0 && !h->b[ib].uniform_mean) ? h->b[ib].rho64 : nullptr;
     any_rho |= rh[ib] != nullptr;
   }
-  return overset_t<double>(s, h, ys, rh, last_stage && any_rho);
+  return overset_t<double>(s, h, ys, rh, last_stage && any_rho, h->ov.buf);
+}
+
+hipError_t overset_interpolate_arrays(hipStream_t s, lee_handle *h, double *const *ys, double *buf) {
+  if (h->ov.nrecv == 0) return hipSuccess;
+  return overset_t<double>(s, h, ys, nullptr, false, buf);
 }
 
 template <typename T>

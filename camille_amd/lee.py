@@ -493,9 +493,7 @@ class Solver:
             return (k * nj + int(p[2])) * ni + int(p[1])
         idx = np.array([lin(p) for p in pts], np.int32)
         w = np.array([p[-1] for p in pts], np.float64)
-        self._keep += [blk, idx, w]
-        self._check(self.L.lee_set_observer(self.h, len(pts), _ip(blk), _ip(idx), _dp(w)))
-        self._nobs = len(pts)
+        self.set_observer_points(blk, idx, w)
 
     def sample_observers(self):
         """p' at the observer points of set_observer, in its order."""
@@ -511,11 +509,17 @@ class Solver:
         self._keep += [blk, idx, w]
         self._check(self.L.lee_set_observer(self.h, len(blk), _ip(blk), _ip(idx), _dp(w)))
         self._nobs = len(blk)
+        self.observers = (blk, idx, w)
+        self.objective_generation = getattr(self, "objective_generation", 0) + 1
 
     def objective_forward(self, nsteps, window_start, ckpt_every):
+        """J of a run of nsteps from the current state and clock.  Each call
+        (and each set_observer) advances `objective_generation`: the reverse
+        pass always belongs to the latest forward run."""
         J = ctypes.c_double()
         self._check(self.L.lee_objective_forward(self.h, int(nsteps), int(window_start), int(ckpt_every),
                                                  ctypes.byref(J)))
+        self.objective_generation = getattr(self, "objective_generation", 0) + 1
         return J.value
 
     def objective_backward(self, scale=1.0):

@@ -55,7 +55,9 @@ static char g_err[512];
  *  1 fabs instead of int abs in the CFL      2 sources at t_n instead of t_{n+1}
  *  3 aux density refreshed after the filter  4 metrics transposed in d/dx_i
  *  5 mean-gradient terms dropped             6 filter sweeps eta before xi
- *  7 right hole closures after any non-hole cell (`!=` for `/=`) */
+ *  7 right hole closures after any non-hole cell (`!=` for `/=`)
+ *  8 monopole with cos(omega t) for sin(omega t)
+ *  9 monopole Gaussian exp(-|x-x0|^2/w^2), the ln 2 of the half-width dropped */
 static int g_plant = 0;
 void orc_set_plant(int code) { g_plant = code; }
 static void set_err(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); }
@@ -581,8 +583,8 @@ static void physical_source(orc_t *o, blk_t *b, double **rhs, double time) {
     /* src = {amplitude, omega, x0, y0, z0, halfwidth}; evaluated at the same
      * (t_{n+1} + c_i dt) times as the reference's sources */
     double A = o->src[0], om = o->src[1], hw = o->src[5];
-    double k = -log(2.0) / (hw * hw);
-    double a_sin = A * sin(om * time);
+    double k = (g_plant == 9 ? -1.0 : -log(2.0)) / (hw * hw);
+    double a_sin = A * (g_plant == 8 ? cos(om * time) : sin(om * time));
     for (int p = 0; p < n; p++) {
       double dx = b->xyz[0][p] - o->src[2], dy = b->xyz[1][p] - o->src[3], dz = b->xyz[2][p] - o->src[4];
       rhs[IP][p] += exp(k * (dx * dx + dy * dy + dz * dz)) * a_sin;

@@ -215,3 +215,64 @@ def test_autograd_function_matches_central_differences():
         print(f"autograd direction {trial}: fd {fd:.12e} adjoint {ad:.12e} rel {err:.2e}", flush=True)
         assert err <= 1e-6
     s.close()
+
+
+def test_autograd_two_forwards_then_one_backward():
+    """J1 + J2 from two forward runs on one solver (different controls), then
+    one backward: each term gets the gradient of its own run, as separate
+    backward passes on fresh forward runs give them."""
+    import torch
+
+    from camille_amd.adjoint import embed_control, observer_energy
+    p = _problem()
+    s = _solver(p, FP64)
+    shape = p["x"].shape
+    rng = np.random.default_rng(11)
+    a1 = torch.tensor(rng.uniform(0.5, 1.5, (8, 8)), dtype=torch.float64, requires_grad=True)
+    a2 = torch.tensor(rng.uniform(-1.0, 1.0, (8, 8)), dtype=torch.float64, requires_grad=True)
+    y0 = torch.zeros((5,) + shape, dtype=torch.float64)
+    nsteps, window, every = 40, 20, 10
+
+    def J_of(av):
+        return observer_energy(s, embed_control(av, shape), y0, nsteps, window, every)
+
+    sep = []
+    for a in (a1, a2):
+        J_of(a).backward()
+        sep.append(a.grad.detach().numpy().copy())
+        a.grad = None
+    (J_of(a1) + J_of(a2)).backward()
+    for k, a in enumerate((a1, a2)):
+        g = a.grad.detach().numpy()
+        err = float(np.max(np.abs(g - sep[k])) / np.max(np.abs(sep[k])))
+        print(f"two forwards, control {k}: gradient rel diff {err:.2e}", flush=True)
+        assert err <= 1e-13
+    s.close()
+
+
+def test_repeated_observer_points_add_their_weights():
+    """An observer list that names a point twice with weights w1, w2 gives the
+    J and gradient of the point named once with w1 + w2 (the duplicates are
+    merged when the observers are set), and sampling still returns p' for
+    every entry of the caller's list."""
+    p = _problem()
+    y0 = np.zeros((5,) + p["x"].shape)
+    obs = p["obs"][:40]
+    dup = [obs[3][:3] + (0.25 * obs[3][3],), obs[17][:3] + (0.5 * obs[17][3],)]
+    once = list(obs)
+    once[3] = obs[3][:3] + (1.25 * obs[3][3],)
+    once[17] = obs[17][:3] + (1.5 * obs[17][3],)
+    out = []
+    for pts in (obs + dup, once):
+        s = _solver(p, FP64)
+        s.set_observer(pts)
+        J, gq, gy = _grads(s, p["q"], y0, 40, 20, 10)
+        samples = s.sample_observers()
+        out.append((J, gq, samples))
+        s.close()
+    (Ja, ga, sa), (Jb, gb, sb) = out
+    assert abs(Ja - Jb) <= 1e-14 * abs(Jb)
+    assert float(np.max(np.abs(ga - gb))) <= 1e-14 * float(np.max(np.abs(gb)))
+    assert len(sa) == len(obs) + 2 and len(sb) == len(obs)
+    assert sa[len(obs)] == sa[3] and sa[len(obs) + 1] == sa[17]
+    assert np.array_equal(sa[:len(obs)], sb)

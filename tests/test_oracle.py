@@ -281,3 +281,97 @@ def test_cylinder_golden_bitwise():
     assert o.time == float(g["tau"][3])
     for b in range(2):
         assert np.array_equal(o.solution(b)[:, 0], g[f"sol{b}"]), b
+
+
+# ---------------------------------------------------------------------------
+# The monopole source of the curvilinear and ensemble workloads has no
+# counterpart in the reference (source.cpp:22-107 holds only case-specific
+# sources), so no fixture covers it.  It is pinned against its closed form,
+#   S(x, t) = A exp(-ln2 |x - x_s|^2 / w^2) sin(omega t),
+# evaluated, like every reference source, at t_{n+1} + c_i dt
+# (solver.cpp:72-75 advances time_sol before temporal:351 hands it over).
+
+_MONO = (1e-3, 0.2 * math.pi, 3.1, -2.3, 0.0, 1.7)  # A, omega, x_s, y_s, z_s, w
+
+
+def _monopole_oracle(mean_u, scheme, dt=1e-4, curvilinear=True):
+    ni, nj, L = 48, 40, 20.0
+    x = np.empty((nj, ni))
+    y = np.empty((nj, ni))
+    for j in range(nj):
+        for i in range(ni):
+            xi, eta = -10.0 + 0.5 * i, -10.0 + 0.5 * j
+            a = 0.05 * L if curvilinear else 0.0
+            x[j, i] = xi + a * math.sin(2 * math.pi * eta / L)
+            y[j, i] = eta + a * math.sin(2 * math.pi * xi / L)
+    mean = np.zeros((5, nj, ni))
+    mean[1] = mean_u
+    mean[4] = 1.0 / cases.GAMMA
+    ic = np.zeros((5, nj, ni))
+    o = Oracle([dict(x=x, y=y, mean=mean, ic=ic)], scheme, (), ("CASE_MONOPOLE", _MONO), "FIX_DT", dt)
+    return o, x, y
+
+
+def _monopole_closed_form(x, y, t):
+    A, om, xs, ys, _, w = _MONO
+    return A * np.exp(-np.log(2.0) * ((x - xs) ** 2 + (y - ys) ** 2) / w ** 2) * np.sin(om * t)
+
+
+def _rhs_source_error(o, x, y, t):
+    r = o.rhs(np.zeros((5, 1) + x.shape), t)[:, 0]
+    exact = _monopole_closed_form(x, y, t)
+    others = max(float(np.max(np.abs(r[v]))) for v in range(4))
+    return float(np.max(np.abs(r[4] - exact)) / np.max(np.abs(exact))), others
+
+
+def test_monopole_rhs_closed_form():
+    """With a zero state and a uniform mean (M = 0.5) every LEE term
+    vanishes, so the p'-RHS is the source itself at the physical (x, y) of a
+    curvilinear grid, and the other equations get nothing."""
+    o, x, y = _monopole_oracle(0.5, dict(cases.BASE_SCHEME))
+    for t in (0.37, 2.0, 7.9):
+        err, others = _rhs_source_error(o, x, y, t)
+        assert err < 1e-14, (t, err)
+        assert others == 0.0
+
+
+def _one_step_from_rest(t0, dt):
+    """p' after one RK4 step from rest, filter off.  The first-order part is
+    dt * sum_i b_i S(t0 + dt + c_i dt); the LEE operator enters only at
+    O(dt^3) (the source's p' drives u' through grad p', which feeds back into
+    p' through div u' two stages later)."""
+    o, x, y = _monopole_oracle(0.0, dict(cases.BASE_SCHEME, filter_order=0), dt=dt, curvilinear=False)
+    o.set_time(0, t0, dt)
+    o.step()
+    p = o.solution()[4, 0]
+    t1 = t0 + dt
+    expect = dt * (_monopole_closed_form(x, y, t1) + 4.0 * _monopole_closed_form(x, y, t1 + 0.5 * dt)
+                   + _monopole_closed_form(x, y, t1 + dt)) / 6.0
+    return o, float(np.max(np.abs(p - expect)) / np.max(np.abs(expect)))
+
+
+def test_monopole_time_convention():
+    """One step from rest samples the source at t_{n+1} + c_i dt: the
+    residual is the O(dt^2) relative operator term (~1e-8 at dt = 1e-4),
+    while a source taken at t_n + c_i dt is off by omega dt cot(omega t)
+    (~2e-5 here)."""
+    t0, dt = 2.0, 1e-4
+    o, err = _one_step_from_rest(t0, dt)
+    assert o.time == t0 + dt
+    assert err < 2e-7, err
+
+
+@pytest.mark.parametrize("plant", [2, 8, 9])
+def test_monopole_planted_mistakes_are_caught(plant):
+    """Sources at t_n (2), cos for sin (8) and the half-width without its
+    ln 2 (9) each fail one of the two closed-form checks above."""
+    L = lib()
+    L.orc_set_plant.argtypes = [ctypes.c_int]
+    L.orc_set_plant(plant)
+    try:
+        o, x, y = _monopole_oracle(0.5, dict(cases.BASE_SCHEME))
+        rhs_err = max(_rhs_source_error(o, x, y, t)[0] for t in (0.37, 2.0, 7.9))
+        _, step_err = _one_step_from_rest(2.0, 1e-4)
+        assert rhs_err > 1e-3 or step_err > 2e-6, (rhs_err, step_err)
+    finally:
+        L.orc_set_plant(0)
