@@ -925,6 +925,10 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
     HIPCHK(hipStreamCreate(&h->stream));
     HIPCHK(hipEventCreate(&h->ev0));
     HIPCHK(hipEventCreate(&h->ev1));
+    // both recorded once, so lee_synchronize can time a stream that has
+    // not run lee_solve yet
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
     make_fd_table(h->fd, cfg->fd_order);
     if (cfg->filter_order > 0) make_filter_table(h->flt, cfg->filter_order, cfg->filter_strength);
     // Butcher tableau (RungeKutta::initialize, temporal:33-85)

@@ -246,8 +246,45 @@ def case_axisym():
                 dt=None, nsteps=40, simulation="CASE_TANNA_TPN49", axisym=1, **BASE_SCHEME)
 
 
+def case_dirichlet():
+    """dirichlet_allzero on the +xi and -eta faces of a wavy 81x65 grid with
+    a mean flow (u, v) = (0.3, 0.1), sponges on the other two faces, a pulse
+    next to the +xi wall; RK4 CFL 0.5, 50 steps."""
+    ni, nj = 81, 65
+    x, y = _wavy(ni, nj, 20.0, 16.0, 0.03)
+    ic = np.zeros((5, nj, ni))
+    ic[4] = _field(x, y, _pulse(1e-3, -6.5, -2.0, 1.0))
+    ic[0] = _field(x, y, _pulse(2e-4, -5.0, 4.0, 1.0))
+    bc = [(0, "dirichlet_allzero", "+xi", "0", "0", "0", "-0", "0", "0"),
+          (0, "dirichlet_allzero", "-eta", "0", "-0", "-0", "-0", "0", "0")]
+    bc += _sponges(10, sides=("-xi", "+eta"))
+    return dict(name="dirichlet", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x, u=0.3, v=0.1), ic=ic,
+                bc=bc, mode="FIX_CFL", cfl=0.5, dt=None, nsteps=50,
+                simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
+
+
+def case_centerline():
+    """Quarter domain x, y in [0, 18] x [0, 15] (73x61) of a quiescent
+    problem symmetric about both axes: centerline_cart_norm2x on the x = 0
+    face (u' = 0, Neumann for the rest), centerline_cart_norm2y on y = 0
+    (v' = 0), sponges on the far faces; pressure pulse at (2, 0) and an
+    entropy spot at (0, 3); RK4 CFL 0.5, 50 steps."""
+    ni, nj = 73, 61
+    x, y = _grid(ni, nj, lambda i, j: 0.25 * i, lambda i, j: 0.25 * j)
+    ic = np.zeros((5, nj, ni))
+    ic[4] = _field(x, y, _pulse(1e-3, 2.0, 0.0, 1.0))
+    ic[0] = _field(x, y, _pulse(2e-4, 0.0, 3.0, 1.0))
+    bc = [(0, "centerline_cart_norm2x", "+xi", "0", "0", "0", "-0", "0", "0"),
+          (0, "centerline_cart_norm2y", "+eta", "0", "-0", "0", "0", "0", "0")]
+    bc += _sponges(10, sides=("-xi", "-eta"))
+    return dict(name="centerline", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x), ic=ic,
+                bc=bc, mode="FIX_CFL", cfl=0.5, dt=None, nsteps=50,
+                simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
+
+
 ALL = {f.__name__[5:]: f for f in (case_c1, case_curv, case_filt, case_shear, case_cylsrc, case_wall,
-                                    case_variants, case_jet, case_opt11, case_axisym)}
+                                    case_variants, case_jet, case_opt11, case_axisym, case_dirichlet,
+                                    case_centerline)}
 
 # cases whose schemes run on the plain path only (FDO11P / SFO11P)
 PLAIN_ONLY = {"opt11"}

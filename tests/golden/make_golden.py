@@ -68,8 +68,10 @@ def run_case(case, nranks=1):
         return res
 
 
-def run_overset_case(case, nranks=2):
-    """Two-block overset problem: one MPI rank per block (decomp.cpp:29-30)."""
+def run_overset_case(case, nranks=2, report_every=None):
+    """Two-block overset problem: one MPI rank per block (decomp.cpp:29-30).
+    With report_every, the reference's min/max report file (io.cpp:440-497,
+    ./report/solution/solution) is returned as a fourth value."""
     import overset_cases
     with tempfile.TemporaryDirectory(prefix=f"golden_{case['name']}_") as d:
         blocks = case["blocks"]
@@ -87,6 +89,8 @@ def run_overset_case(case, nranks=2):
                                  case["holes"])
         items = cases.deck_items(case)
         items = [(k, v) for k, v in items if k != "OVERSET"]
+        if report_every:
+            items = [(k, str(report_every) if k == "REPORT_TIMESTEP" else v) for k, v in items]
         items.insert(12, ("OVERSET", f"OVERTURE, TYPE = EXPLICIT, ORDER_ACCURACY = {case['order']}"))
         items.append(("OVERSET_FILE", "overset.bin"))
         refio.write_deck(os.path.join(d, "camille.in"), items)
@@ -99,11 +103,44 @@ def run_overset_case(case, nranks=2):
             raise RuntimeError(f"reference wrote no final solution:\n{p.stdout[-3000:]}\n{p.stderr[-2000:]}")
         sols, taus = refio.read_solution(files[-1])
         log = "\n".join(ln for ln in p.stdout.splitlines() if ln.startswith("Time step:"))
+        if report_every:
+            return sols, taus, log, open(os.path.join(d, "report", "solution", "solution")).read()
         return sols, taus, log
+
+
+def pressure_series(report):
+    """(step, min p', max p') per line of the reference's min/max report."""
+    out = []
+    for ln in report.splitlines():
+        if not ln.startswith("Time step:"):
+            continue
+        step = int(ln.split(",")[0].split(":")[1])
+        seg = ln.split("P': ")[1].split(",")[0]
+        lo, hi = (float(t) for t in seg.split("~"))
+        out.append((step, lo, hi))
+    return out
+
+
+def cylinder_long(nsteps=1500, every=100):
+    """The reference's own long run of the small cylinder case: min/max p'
+    every `every` steps (the wall mode's growth), stored as
+    tests/golden/cylinder_long.json."""
+    import overset_cases
+    case = overset_cases.case_cylinder_small()
+    case["nsteps"] = nsteps
+    _, taus, log, report = run_overset_case(case, report_every=every)
+    series = pressure_series(report)
+    out = dict(case="cylinder_small", nsteps=nsteps, report_every=every, time=float(taus[0][3]), ranks=2,
+               p_min_max=[[st, lo, hi] for st, lo, hi in series])
+    json.dump(out, open(os.path.join(HERE, "cylinder_long.json"), "w"), indent=1)
+    print("cylinder_long:", series[-1])
 
 
 def main(names):
     import overset_cases
+    if names and names[0] == "cylinder_long":
+        cylinder_long()
+        return
     overset = {"overset_small": overset_cases.case_overset_small,
                "cylinder_small": overset_cases.case_cylinder_small}
     if names and names[0] in overset:

@@ -353,13 +353,15 @@ def test_overset_plain_bit_identical_to_reference(name):
 def test_overset_fused_matches_reference(label, prec, name):
     """Fused path against the reference's fields after the full run.
 
-    overset_small holds a fluid/hole corner (a fluid row meeting the hole
-    under a hole row, above receivers) where the reference scheme has a
-    growing local mode: the FP32-FP64 difference there grows about 1.35x
-    per step over the last ten steps (6.9e-7 at step 30, 1.45e-5 at 40)
-    while the field elsewhere stays at ~1e-6.  Its FP32 run is therefore
-    held to 1e-5 against the FP64 fused path at step 30, and to 5e-5
-    against the reference after all 40 steps."""
+    overset_small holds a fluid point at a concave hole corner of block 0,
+    (i, j) = (44, 45), where the reference's scheme has a local growing
+    mode (about 1.4x per step from step ~35) that amplifies any
+    perturbation: an FP64 run with its initial condition perturbed at FP32
+    round-off differs there by 1.6e-6 at step 40 and by 4e-8 elsewhere
+    (tests/test_oracle.py, the corner-mode test).  The FP32 run is held to
+    1e-5 against FP64 over the whole field at step 30, to 1e-5 against the
+    reference at step 40 everywhere outside the 5 x 5 box around that point,
+    and to 5e-5 inside it."""
     g = _golden(name)
     c, s = _overset_solver(prec, PATH_FUSED, name)
     corner = name == "overset_small" and label == "fused32"
@@ -375,10 +377,19 @@ def test_overset_fused_matches_reference(label, prec, name):
     else:
         s.solve(c["nsteps"])
     assert s.time == float(g["tau"][3])
-    errs = []
+    errs, box = [], []
     for ib in range(2):
-        sol = s.solution(ib)
-        errs += [_rel(sol[v], g[f"sol{ib}"][v]) for v in (0, 1, 2, 4)]
-    print(f"{name} {label}: rel errors {['%.2e' % e for e in errs]}", flush=True)
-    assert max(errs) <= (5e-5 if corner else TOL[label])
+        sol, ref = s.solution(ib), g[f"sol{ib}"]
+        for v in (0, 1, 2, 4):
+            scale = max(float(np.max(np.abs(ref[v]))), 1e-300)
+            d = np.abs(sol[v] - ref[v]) / scale
+            if corner and ib == 0:
+                box.append(float(np.max(d[43:48, 42:47])))
+                d[43:48, 42:47] = 0.0
+            errs.append(float(np.max(d)))
+    print(f"{name} {label}: rel errors {['%.2e' % e for e in errs]}"
+          + (f", corner box {max(box):.2e}" if box else ""), flush=True)
+    assert max(errs) <= TOL[label]
+    if box:
+        assert max(box) <= 5e-5
     s.close()

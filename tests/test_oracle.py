@@ -375,3 +375,65 @@ def test_monopole_planted_mistakes_are_caught(plant):
         assert rhs_err > 1e-3 or step_err > 2e-6, (rhs_err, step_err)
     finally:
         L.orc_set_plant(0)
+
+
+def test_cylinder_long_run_follows_the_reference():
+    """The small cylinder case run 1500 steps by the reference binary (two
+    MPI ranks): its min/max p' report every 100 steps shows a wall mode
+    growing about 2.7x per 100 steps from step ~400 on.  The oracle follows
+    the reference's series to the report's 7 significant digits, so the
+    growth is the reference's discretization of this configuration, not a
+    property of this build."""
+    import overset_cases
+    ref = json.load(open(os.path.join(GOLD, "cylinder_long.json")))
+    c = overset_cases.case_cylinder_small()
+    o = overset_cases.oracle_for_cylinder(c, Oracle)
+    every = ref["report_every"]
+    for step, lo, hi in ref["p_min_max"]:
+        o.run(every)
+        assert o.time_step == step
+        p = [o.solution(b)[4] for b in range(2)]
+        mn, mx = min(float(q.min()) for q in p), max(float(q.max()) for q in p)
+        assert abs(mn - lo) <= 1e-6 * abs(lo) and abs(mx - hi) <= 1e-6 * abs(hi), (step, mn, lo, mx, hi)
+    assert o.time == ref["time"]
+    amp = [max(-lo, hi) for _, lo, hi in ref["p_min_max"]]
+    assert amp[-1] > 1e4 * amp[2]
+
+
+# The two-block overset fixture holds a fluid point at a concave hole corner
+# of block 0, (i, j) = (44, 45): BLANKED at (45, 45) and (44, 44), no
+# INTERPOLATED cell between.  Both of its derivative directions take
+# one-sided closure rows there, and the reference's scheme has a local
+# growing mode at that point (about 1.4x per step from step ~35).  It
+# amplifies any perturbation of the state, which is why an FP32 run differs
+# from the FP64 reference there by more than elsewhere.
+OVS_CORNER = (0, 45, 44)  # block, j, i
+
+
+def test_overset_small_corner_mode_is_the_reference_scheme():
+    import overset_cases
+    c = overset_cases.case_overset_small()
+    o = overset_cases.oracle_for(c, Oracle)
+    b, jc, ic = OVS_CORNER
+    ib = o.iblank(b)[0]
+    assert ib[jc, ic] == 0 and ib[jc, ic + 1] == 1 and ib[jc - 1, ic] == 1
+    # FP64 with the initial condition perturbed at FP32 round-off (6e-8)
+    c2 = overset_cases.case_overset_small()
+    rng = np.random.default_rng(3)
+    for blk in c2["blocks"]:
+        blk["ic"] = blk["ic"] * (1.0 + 6e-8 * rng.standard_normal(blk["ic"].shape))
+    o2 = overset_cases.oracle_for(c2, Oracle)
+    o.run(40)
+    o2.run(40)
+    p = [o.solution(k)[4, 0] for k in range(2)]
+    q = [o2.solution(k)[4, 0] for k in range(2)]
+    scale = max(float(np.max(np.abs(a))) for a in p)
+    d = [np.abs(p[k] - q[k]) / scale for k in range(2)]
+    at_corner = float(d[b][jc, ic])
+    d[b][jc - 2:jc + 3, ic - 2:ic + 3] = 0.0
+    elsewhere = max(float(np.max(a)) for a in d)
+    assert at_corner > 1e-6 and elsewhere < 1e-7, (at_corner, elsewhere)
+    # the mode itself: |p'| at the corner grows > 1000x over the next 20 steps
+    p40 = abs(float(p[b][jc, ic]))
+    o.run(20)
+    assert abs(float(o.solution(b)[4, 0][jc, ic])) > 1e3 * p40
