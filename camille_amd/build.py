@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblee_amd.so")
-SOURCES = ["lee_api.hip", "lee_kernels.hip", "lee_kernels3d.hip", "lee_adjoint.hip", "lee_adjoint3d.hip", "lee_plan.cpp",
+SOURCES = ["lee_api.hip", "lee_kernels.hip", "lee_kernels3d.hip", "lee_adjoint.hip", "lee_adjoint3d.hip", "lee_noalloc.hip", "lee_plan.cpp",
            "lee_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 OBJDIR = os.path.join(HERE, "_build")

@@ -918,6 +918,9 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
   }
   int rc = [&]() -> int {
     HIPCHK(hipGetDevice(&h->device));
+    // the allocation-free entry point's staging area on this device; a
+    // failed look-up is kept and reported by that entry point only
+    (void)noalloc_prepare_device(h->device);
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
     for (auto &blk : h->b) blk.cus = cus;

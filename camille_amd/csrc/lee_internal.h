@@ -282,6 +282,9 @@ hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_
 // Overset interpolation of caller-owned FP64 block arrays ys[b] (device,
 // 5n each), receiver values staged in buf (nrecv * NV doubles, device).
 hipError_t overset_interpolate_arrays(hipStream_t s, lee_handle *h, double *const *ys, double *buf);
+// Looks up the current device's static staging area of
+// lee_update_boundary_noalloc (once per device; the device must be current).
+hipError_t noalloc_prepare_device(int device);
 hipError_t metrics2d(hipStream_t s, Block &b, const OpTable &fd);
 
 hipError_t fused_stage(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc);

@@ -95,7 +95,7 @@ EXPORTS = [
     "lee_batch_create", "lee_get_solve_time", "lee_set_batch_launch", "lee_get_batch_launch",
     "lee_set_source_field", "lee_set_observer", "lee_objective_forward", "lee_objective_backward",
     "lee_get_source_gradient", "lee_get_state_gradient", "lee_compute_rhs_adjoint", "lee_apply_filter_adjoint",
-    "lee_sample_observers",
+    "lee_sample_observers", "lee_update_boundary_noalloc",
 ]
 
 _lib = None
@@ -130,6 +130,7 @@ def library():
     L.lee_apply_filter.argtypes = [vp, i, dp]
     L.lee_take_derivative_xyz.argtypes = [vp, i, dp, i, dp]
     L.lee_update_boundary.argtypes = [vp, ctypes.POINTER(dp)]
+    L.lee_update_boundary_noalloc.argtypes = [vp, ctypes.POINTER(vp)]
     L.lee_get_metrics.argtypes = [vp, i, dp, dp]
     L.lee_get_iblank.argtypes = [vp, i, ip]
     L.lee_get_mean_gradient.argtypes = [vp, i, dp]

@@ -167,6 +167,20 @@ int lee_take_derivative_xyz(lee_handle *h, int iblock, const double *f, int dir_
  * interpolation, in place on all blocks (ys[b] = 5n doubles of block b). */
 int lee_update_boundary(lee_handle *h, double *const *ys);
 
+/* lee_update_boundary without any allocation.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).
+ * ys_dev[b] = 5n doubles of block b in DEVICE memory, updated in place:
+ * BCs, then overset interpolation, enqueued on the handle's stream
+ * (lee_synchronize waits; the caller orders its own streams around it).
+ * The receiver values are staged in a static device array of
+ * LEE_NOALLOC_MAX_RECV * 5 doubles per device (about 5 MiB), held until the
+ * process ends; a handle with more overset receivers gets LEE_EINVAL.  All
+ * calls on one device share that array: calls that overlap in time on one
+ * device race.  No hipMalloc, hipFree or host copy is made. */
+#define LEE_NOALLOC_MAX_RECV 131072
+int lee_update_boundary_noalloc(lee_handle *h, double *const *ys_dev);
+
 /* Init products, for checks: metrics[l][i] (9n), J (n), iblank (n),
  * mean gradients (5*num_dim*n, ivar*num_dim+dir), summed sponge strength (n). */
 int lee_get_metrics(lee_handle *h, int iblock, double *metrics, double *jac);
