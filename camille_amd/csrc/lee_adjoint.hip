@@ -73,6 +73,21 @@ __device__ __forceinline__ T fd_row_t(const T *f, int st, int c, int len, bool i
   return acc;
 }
 
+// Transposed FD row at block point p (coordinate c of a line of length
+// len, stride gs in the block, st in LDS) with the per-point codes of a
+// block with holes: the rows of the neighbours c + o, o in [-2, 2].
+template <typename T>
+__device__ __forceinline__ T fd_row_t_codes(const T *f, int st, int c, int len, const int8_t *code, int p, int gs) {
+  T acc = 0;
+#pragma unroll
+  for (int o = -2; o <= 2; o++) {
+    const int j = c + o;
+    if (j < 0 || j >= len) continue;
+    acc += fd_wt<T>((int)code[p + o * gs], -o) * f[o * st];
+  }
+  return acc;
+}
+
 // Weight of the filter row with code `code` on offset d (rows are centred
 // and symmetric; filter_row in lee_device.h).
 template <typename T, int ORDER>
@@ -119,6 +134,19 @@ __device__ __forceinline__ T flt_row_t(const T *f, int st, int c, int len, bool 
   return acc;
 }
 
+template <typename T, int ORDER>
+__device__ __forceinline__ T flt_row_t_codes(const T *f, int st, int c, int len, const int8_t *code, int p, int gs) {
+  constexpr int H = ORDER / 2;
+  T acc = 0;
+#pragma unroll
+  for (int o = -H; o <= H; o++) {
+    const int j = c + o;
+    if (j < 0 || j >= len) continue;
+    acc += flt_wt<T, ORDER>((int)code[p + o * gs], -o) * f[o * st];
+  }
+  return acc;
+}
+
 template <typename T>
 struct AdjArgs {
   int ni, nj, ntx, nty, ntiles, chunk;
@@ -135,6 +163,8 @@ struct AdjArgs {
   ZoneT<T> z[MAX_ZONES];
   double hb, ha;   // b_i h and a_i h
   double src;      // s1 s2 of this stage
+  const int8_t *iblank, *code;  // holes only: iblank, FD codes (xi then eta)
+  const int *clean;             // holes only: integral image of points off the interior FD rows
 };
 
 template <typename T, int R>
@@ -144,7 +174,10 @@ using AdjLds = T[8][4 * R + 2 * ST_H][ST_TX + 2 * ST_H];
 // tile and its 2-point halo go to LDS, then the transposed stencils, the
 // point-local transposed terms and the accumulations of the running
 // adjoints for the R points of each thread.
-template <typename T, bool MEANF, int KIND, int R>
+// Blocks with holes (HOLES): the forward stage writes 0 at BLANKED points
+// whatever its inputs, so the adjoints arriving there (lacc, lin) count as 0
+// and kappa vanishes there; the transposed rows take the per-point codes.
+template <typename T, bool MEANF, int KIND, int R, bool HOLES>
 __global__ void __launch_bounds__(TPB) k_adj_stage(AdjArgs<T> a) {
   constexpr int TY = 4 * R, NR = TY + 2 * ST_H, RS = ST_TX + 2 * ST_H;
   __shared__ AdjLds<T, R> F;  // F[v] xi-flux, F[4 + v] eta-flux
@@ -154,7 +187,9 @@ __global__ void __launch_bounds__(TPB) k_adj_stage(AdjArgs<T> a) {
   const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
   const int VOFF[4] = {IS * n, IUX * n, IUY * n, IP * n};
-  const bool inner = i0 >= 2 * ST_H && i0 + ST_TX + 2 * ST_H <= ni && j0 >= 2 * ST_H && j0 + TY + 2 * ST_H <= nj;
+  const bool inner = HOLES ? box_clean(a.clean, ni, nj, i0 - ST_H, i0 + ST_TX + ST_H, j0 - ST_H, j0 + TY + ST_H)
+                           : (i0 >= 2 * ST_H && i0 + ST_TX + 2 * ST_H <= ni && j0 >= 2 * ST_H &&
+                              j0 + TY + 2 * ST_H <= nj);
   const T hb = (T)a.hb, ha = (T)a.ha;
 
   // (1) kappa and fluxes over the tile plus halo; row r = ty + 4k
@@ -168,7 +203,7 @@ __global__ void __launch_bounds__(TPB) k_adj_stage(AdjArgs<T> a) {
       const int c = half == 0 ? tx : ST_TX + tx;
       if (half == 1 && tx >= 2 * ST_H) continue;
       const int gi = i0 - ST_H + c;
-      const bool ok = gj >= 0 && gj < nj && gi >= 0 && gi < ni;
+      const bool ok = gj >= 0 && gj < nj && gi >= 0 && gi < ni && !(HOLES && a.iblank[gj * ni + gi] == BLANKED);
       double fx[4] = {0.0, 0.0, 0.0, 0.0}, fy[4] = {0.0, 0.0, 0.0, 0.0};
       if (ok) {
         const int p = gj * ni + gi;
@@ -218,17 +253,26 @@ __global__ void __launch_bounds__(TPB) k_adj_stage(AdjArgs<T> a) {
     const int p = gj * ni + gi;
     const int lj = rr + ST_H, li = tx + ST_H;
     T lam[4];
+    if (HOLES && !inner) {
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      lam[v] = fd_row_t<T>(&F[v][lj][li], 1, gi, ni, inner);
-      lam[v] += fd_row_t<T>(&F[4 + v][lj][li], RS, gj, nj, inner);
+      for (int v = 0; v < 4; v++) {
+        lam[v] = fd_row_t_codes<T>(&F[v][lj][li], 1, gi, ni, a.code, p, 1);
+        lam[v] += fd_row_t_codes<T>(&F[4 + v][lj][li], RS, gj, nj, a.code + n, p, ni);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        lam[v] = fd_row_t<T>(&F[v][lj][li], 1, gi, ni, inner);
+        lam[v] += fd_row_t<T>(&F[4 + v][lj][li], RS, gj, nj, inner);
+      }
     }
+    const bool hole = HOLES && a.iblank[p] == BLANKED;
     // kappa at the point (recomputed from the same operands)
     T kap[4], lacc[4], lin[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      lacc[v] = a.lacc[VOFF[v] + p];
-      lin[v] = (KIND != STAGE_LAST && a.lin) ? a.lin[VOFF[v] + p] : (T)0;
+      lacc[v] = hole ? (T)0 : a.lacc[VOFF[v] + p];
+      lin[v] = (KIND != STAGE_LAST && a.lin && !hole) ? a.lin[VOFF[v] + p] : (T)0;
       T kv = hb * lacc[v];
       if (KIND != STAGE_LAST && a.lin) kv += ha * lin[v];
       kap[v] = kv;
@@ -296,11 +340,16 @@ struct AdjFilterArgs {
   const double *mean, *rhob;
   double gi;
   T rem, blend;
+  const int8_t *iblank, *code;  // holes only: iblank, filter codes (xi then eta)
+  const int *clean;             // holes only: integral image of points off the interior filter rows
 };
 
 // Transposed filter on a 64 x TY tile: eta^T sweep of the (TY + 8) x 72
 // region into LDS, then the xi^T sweep, blend, and R^T mu.
-template <typename T, int ORDER, int TY>
+// Blocks with holes (HOLES): the forward filter is followed by the hole
+// reset, so the adjoint arriving at a BLANKED point counts as 0, and the
+// transposed rows take the per-point filter codes.
+template <typename T, int ORDER, int TY, bool HOLES>
 __global__ void __launch_bounds__(TPB) k_adj_filter(AdjFilterArgs<T> a) {
   constexpr int NR = TY + 2 * FL_H, NC = FL_TX + 2 * FL_H, KR = NR / 4;
   __shared__ T in[4][NR][NC];
@@ -311,7 +360,12 @@ __global__ void __launch_bounds__(TPB) k_adj_filter(AdjFilterArgs<T> a) {
   const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
   const int VOFF[4] = {IS * n, IUX * n, IUY * n, IP * n};
-  const bool inner = i0 >= 2 * FL_H && i0 + FL_TX + 2 * FL_H <= ni && j0 >= 2 * FL_H && j0 + TY + 2 * FL_H <= nj;
+  // (no filter codes when the filter is off: blend is 0 and only the
+  // masked identity part counts)
+  const bool inner = HOLES ? (a.clean && box_clean(a.clean, ni, nj, i0 - FL_H, i0 + FL_TX + FL_H, j0 - FL_H,
+                                                   j0 + TY + FL_H))
+                           : (i0 >= 2 * FL_H && i0 + FL_TX + 2 * FL_H <= ni && j0 >= 2 * FL_H &&
+                              j0 + TY + 2 * FL_H <= nj);
 #pragma unroll
   for (int k = 0; k < KR; k++) {
     const int r = ty + 4 * k;
@@ -321,7 +375,7 @@ __global__ void __launch_bounds__(TPB) k_adj_filter(AdjFilterArgs<T> a) {
       const int c = half == 0 ? tx : FL_TX + tx;
       if (half == 1 && tx >= 2 * FL_H) continue;
       const int gi = i0 - FL_H + c;
-      const bool ok = gj >= 0 && gj < nj && gi >= 0 && gi < ni;
+      const bool ok = gj >= 0 && gj < nj && gi >= 0 && gi < ni && !(HOLES && a.iblank[gj * ni + gi] == BLANKED);
 #pragma unroll
       for (int v = 0; v < 4; v++) in[v][r][c] = ok ? a.src[VOFF[v] + gj * ni + gi] : (T)0;
     }
@@ -338,9 +392,17 @@ __global__ void __launch_bounds__(TPB) k_adj_filter(AdjFilterArgs<T> a) {
       if (half == 1 && tx >= 2 * FL_H) continue;
       const int gi = i0 - FL_H + c;
       const bool ok = gj < nj && gi >= 0 && gi < ni;
+      if (HOLES && !inner) {
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        mid[v][r][c] = ok ? flt_row_t<T, ORDER>(&in[v][r + FL_H][c], NC, gj, nj, inner) : (T)0;
+        for (int v = 0; v < 4; v++)
+          mid[v][r][c] = ok && a.code
+                             ? flt_row_t_codes<T, ORDER>(&in[v][r + FL_H][c], NC, gj, nj, a.code + n, gj * ni + gi, ni)
+                             : (T)0;
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          mid[v][r][c] = ok ? flt_row_t<T, ORDER>(&in[v][r + FL_H][c], NC, gj, nj, inner) : (T)0;
+      }
     }
   }
   __syncthreads();
@@ -354,7 +416,9 @@ __global__ void __launch_bounds__(TPB) k_adj_filter(AdjFilterArgs<T> a) {
     T out[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      const T f = flt_row_t<T, ORDER>(&mid[v][r][tx + FL_H], 1, gi, ni, inner);
+      const T f = (HOLES && !inner) ? (a.code ? flt_row_t_codes<T, ORDER>(&mid[v][r][tx + FL_H], 1, gi, ni, a.code, p, 1)
+                                              : (T)0)
+                                    : flt_row_t<T, ORDER>(&mid[v][r][tx + FL_H], 1, gi, ni, inner);
       out[v] = a.rem * in[v][r + FL_H][tx + FL_H] + a.blend * f;
     }
     if (a.mu) {
@@ -429,6 +493,137 @@ __global__ void __launch_bounds__(TPB) k_fill_t(T *__restrict__ a, size_t n) {
 
 inline unsigned nblk(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+// ------------------------------------------- transposed boundary update
+//
+// The forward boundary update after a stage (spatial::update_boundary,
+// spatial:2162-2187) is linear: each BC face replaces the values at its
+// points, then the overset interpolation replaces every receiver by the
+// weighted sum of its donors, all donor sums formed first.  Its transpose
+// runs backwards: the overset part, then the faces of each block in reverse
+// order.
+
+template <typename T>
+struct AdjOvTables {
+  T *lam[MAX_OV_BLOCKS];
+  int ni[MAX_OV_BLOCKS], nn[MAX_OV_BLOCKS];
+};
+
+// Overset transpose, part 1: each receiver's adjoint is kept in buf and
+// zeroed (the forward overwrote the receiver's value).
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_adj_ov_take(AdjOvTables<T> t, const int *__restrict__ rb,
+                                                     const int *__restrict__ rl, double *__restrict__ buf,
+                                                     int nrecv) {
+  const int gid = blockIdx.x * TPB + threadIdx.x;
+  if (gid >= NV * nrecv) return;
+  const int v = gid / nrecv, r = gid - v * nrecv;
+  const int b = rb[r], p = rl[r];
+  if (p < 0 || p >= t.nn[b]) {
+    buf[gid] = 0.0;
+    return;
+  }
+  T *lam = t.lam[b] + (size_t)v * t.nn[b] + p;
+  buf[gid] = (double)*lam;
+  *lam = (T)0;
+}
+
+// Overset transpose, part 2: each receiver's adjoint goes to its donors
+// with the normalised weights (the forward's donors outside the block were
+// skipped, and are here).  Receivers sharing a donor add atomically.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_adj_ov_give(AdjOvTables<T> t, const int *__restrict__ db,
+                                                     const int *__restrict__ dl, const int *__restrict__ w,
+                                                     const double *__restrict__ coef, int stride,
+                                                     const double *__restrict__ buf, int nrecv) {
+  const int gid = blockIdx.x * TPB + threadIdx.x;
+  if (gid >= NV * nrecv) return;
+  const int v = gid / nrecv, r = gid - v * nrecv;
+  const double lr = buf[gid];
+  if (lr == 0.0) return;
+  const int b = db[r];
+  const int ni = t.ni[b], n = t.nn[b];
+  const int w1 = w[3 * b], w2 = w[3 * b + 1];
+  const double *c = coef + (size_t)r * stride;
+  T *lam = t.lam[b] + (size_t)v * n;
+  int q = 0;
+  for (int b2 = 0; b2 < w2; b2++)
+    for (int b1 = 0; b1 < w1; b1++, q++) {
+      const int l1 = dl[r] + b1 + b2 * ni;
+      if (l1 >= 0 && l1 < n) atomicAdd(&lam[l1], (T)(c[q] * lr));
+    }
+}
+
+struct AdjBcArgs {
+  int type, dir, end;
+  int i0, i1, j0, j1;   // face box (collapsed along dir)
+  int ni, nj;
+  double neu[9];        // closure row 0 of the FD operator
+  int sb;
+};
+
+// Transpose of one BC face (bc::apply_BC, boundary_condition.cpp:46-209):
+// at a face point that is not a hole, the variables a Neumann closure
+// (bc:348-444) sets hand their adjoint to the points along the normal
+// (weights -neu[s] / neu[0]); wall-slip velocities (bc:448-494) take the
+// transposed 2 x 2 map J Minv Z M; dirichlet_allzero and a centerline's
+// normal velocity (bc:498-545) pass nothing on.  Points of one face reach
+// distinct points along their normals, so no two threads touch one value.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_adj_bc_face(T *__restrict__ lam, const double *__restrict__ met,
+                                                     const double *__restrict__ jac,
+                                                     const int8_t *__restrict__ iblank, AdjBcArgs a) {
+  const int fw = a.i1 - a.i0 + 1, fh = a.j1 - a.j0 + 1;
+  const int q = blockIdx.x * TPB + threadIdx.x;
+  if (q >= fw * fh) return;
+  const int i = a.i0 + q % fw, j = a.j0 + q / fw;
+  const int n = a.ni * a.nj;
+  const int p = j * a.ni + i;
+  if (iblank && iblank[p] == BLANKED) return;
+  const int mf = a.dir == 0 ? 1 : a.ni;
+  const int c = a.dir == 0 ? i : j, len = a.dir == 0 ? a.ni : a.nj;
+  const int sg = a.end == 0 ? 1 : -1;
+  double lv[NV];
+  for (int v = 0; v < NV; v++) lv[v] = (double)lam[(size_t)v * n + p];
+  bool neu[NV] = {false, false, false, false, false};
+  double keep[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};  // adjoint left at the face point
+  switch (a.type) {
+    case LEE_BC_NEUMANN:
+      for (int v = 0; v < NV; v++) neu[v] = true;
+      break;
+    case LEE_BC_WALL_SLIP_KINEMATIC: {
+      neu[IS] = neu[IP] = true;
+      const double m00 = met[p], m01 = met[(size_t)n + p], m10 = met[(size_t)2 * n + p], m11 = met[(size_t)3 * n + p];
+      const double J = jac[p];
+      // forward: k = M u, k[dir] = 0, u' = J Minv k with Minv = [[m11, -m01], [-m10, m00]]
+      double g0 = J * (m11 * lv[IUX] - m10 * lv[IUY]);
+      double g1 = J * (-m01 * lv[IUX] + m00 * lv[IUY]);
+      if (a.dir == 0) g0 = 0.0; else g1 = 0.0;
+      keep[IUX] = m00 * g0 + m10 * g1;
+      keep[IUY] = m01 * g0 + m11 * g1;
+      keep[IUZ] = lv[IUZ];
+      break;
+    }
+    case LEE_BC_CENTERLINE_CART_NORM2X:
+    case LEE_BC_CENTERLINE_CART_NORM2Y:
+    case LEE_BC_CENTERLINE_CART_NORM2Z:
+      for (int v = 0; v < NV; v++) neu[v] = true;
+      neu[IUX + (a.type - LEE_BC_CENTERLINE_CART_NORM2X)] = false;
+      break;
+    default:  // dirichlet_allzero
+      break;
+  }
+  for (int v = 0; v < NV; v++) lam[(size_t)v * n + p] = (T)keep[v];
+  for (int v = 0; v < NV; v++) {
+    if (!neu[v] || lv[v] == 0.0) continue;
+    for (int s = 1; s < a.sb && s <= 8; s++) {
+      const int cs = c + sg * s;
+      if (cs < 0 || cs >= len) continue;
+      T *t = &lam[(size_t)v * n + p + sg * s * mf];
+      *t = (T)((double)*t - a.neu[s] / a.neu[0] * lv[v]);
+    }
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- launchers
@@ -477,21 +672,26 @@ static hipError_t adj_stage_t(hipStream_t s, lee_handle *h, Block &b, const AdjS
   a.hb = st.hb;
   a.ha = st.ha;
   a.src = st.src;
+  a.iblank = b.iblank;
+  a.code = b.code_fd;
+  a.clean = b.clean_fd;
   const dim3 grid(a.chunk * 8), block(64, 4);
-  const bool mf = !b.uniform_mean;
-#define LEE_ADJ(MF, K)                                                                       \
-  do {                                                                                       \
-    if (R == 1) hipLaunchKernelGGL((k_adj_stage<T, MF, K, 1>), grid, block, 0, s, a);        \
-    else hipLaunchKernelGGL((k_adj_stage<T, MF, K, 2>), grid, block, 0, s, a);               \
+  const bool mf = !b.uniform_mean, ho = b.has_holes;
+#define LEE_ADJ(MF, K, HO)                                                                     \
+  do {                                                                                         \
+    if (R == 1) hipLaunchKernelGGL((k_adj_stage<T, MF, K, 1, HO>), grid, block, 0, s, a);      \
+    else hipLaunchKernelGGL((k_adj_stage<T, MF, K, 2, HO>), grid, block, 0, s, a);             \
   } while (0)
-#define LEE_ADJK(MF)                                          \
-  do {                                                        \
-    if (st.kind == STAGE_FIRST) LEE_ADJ(MF, STAGE_FIRST);     \
-    else if (st.kind == STAGE_MID) LEE_ADJ(MF, STAGE_MID);    \
-    else LEE_ADJ(MF, STAGE_LAST);                             \
+#define LEE_ADJK(MF, HO)                                          \
+  do {                                                            \
+    if (st.kind == STAGE_FIRST) LEE_ADJ(MF, STAGE_FIRST, HO);     \
+    else if (st.kind == STAGE_MID) LEE_ADJ(MF, STAGE_MID, HO);    \
+    else LEE_ADJ(MF, STAGE_LAST, HO);                             \
   } while (0)
-  if (mf) LEE_ADJK(true);
-  else LEE_ADJK(false);
+  if (mf && ho) LEE_ADJK(true, true);
+  else if (mf) LEE_ADJK(true, false);
+  else if (ho) LEE_ADJK(false, true);
+  else LEE_ADJK(false, false);
 #undef LEE_ADJK
 #undef LEE_ADJ
   return hipGetLastError();
@@ -523,9 +723,20 @@ static hipError_t adj_filter_t(hipStream_t s, lee_handle *h, Block &b, int src, 
   const double blend = h->cfg.filter_order > 0 ? h->cfg.filter_blend : 0.0;
   a.blend = (T)blend;
   a.rem = (T)(1.0 - blend);
+  a.iblank = b.iblank;
+  a.code = b.code_flt;
+  a.clean = b.clean_flt;
   const dim3 grid(a.chunk * 8), block(64, 4);
-  if (h->cfg.filter_order == 8) hipLaunchKernelGGL((k_adj_filter<T, 8, TY>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((k_adj_filter<T, 4, TY>), grid, block, 0, s, a);
+  // without a filter there are no filter codes: the hole reset alone
+  // (blend 0 takes the identity part only)
+  const bool ho = b.has_holes;
+  if (h->cfg.filter_order == 8) {
+    if (ho) hipLaunchKernelGGL((k_adj_filter<T, 8, TY, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_adj_filter<T, 8, TY, false>), grid, block, 0, s, a);
+  } else {
+    if (ho) hipLaunchKernelGGL((k_adj_filter<T, 4, TY, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_adj_filter<T, 4, TY, false>), grid, block, 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -559,6 +770,51 @@ hipError_t adjoint_add_rt_mu(hipStream_t s, lee_handle *h, Block &b, int buf) {
 hipError_t adjoint_zero_grad(hipStream_t s, Block &b) {
   hipLaunchKernelGGL(k_fill_t<double>, dim3(nblk(b.n)), dim3(TPB), 0, s, b.agrad, (size_t)b.n);
   return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t adj_boundary_t(hipStream_t s, lee_handle *h, int buf) {
+  const Overset &o = h->ov;
+  if (o.nrecv > 0) {
+    AdjOvTables<T> t{};
+    for (int ib = 0; ib < h->nblk; ib++) {
+      t.lam[ib] = sizeof(T) == 8 ? (T *)h->b[ib].A64[buf] : (T *)h->b[ib].A32[buf];
+      t.ni[ib] = h->b[ib].ni;
+      t.nn[ib] = h->b[ib].n;
+    }
+    const unsigned g = nblk((size_t)NV * o.nrecv);
+    hipLaunchKernelGGL(k_adj_ov_take<T>, dim3(g), dim3(TPB), 0, s, t, o.rb, o.rl, o.buf, o.nrecv);
+    hipLaunchKernelGGL(k_adj_ov_give<T>, dim3(g), dim3(TPB), 0, s, t, o.db, o.dl, o.w, o.coef, o.stride, o.buf,
+                       o.nrecv);
+  }
+  for (int ib = 0; ib < h->nblk; ib++) {
+    Block &b = h->b[ib];
+    T *lam = sizeof(T) == 8 ? (T *)b.A64[buf] : (T *)b.A32[buf];
+    for (int k = (int)b.faces.size() - 1; k >= 0; k--) {
+      const BcFace &f = b.faces[k];
+      AdjBcArgs a{};
+      a.type = f.type;
+      a.dir = f.dir;
+      a.end = f.end;
+      a.i0 = f.is[0];
+      a.i1 = f.ie[0];
+      a.j0 = f.is[1];
+      a.j1 = f.ie[1];
+      a.ni = b.ni;
+      a.nj = b.nj;
+      a.sb = h->fd.sb;
+      for (int q = 0; q < 9; q++) a.neu[q] = h->fd.bnd[0][q];
+      const size_t cnt = (size_t)(a.i1 - a.i0 + 1) * (a.j1 - a.j0 + 1);
+      hipLaunchKernelGGL(k_adj_bc_face<T>, dim3(nblk(cnt)), dim3(TPB), 0, s, lam, b.met, b.jac,
+                         b.has_holes ? b.iblank : nullptr, a);
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t adjoint_boundary(hipStream_t s, lee_handle *h, int buf) {
+  if (h->cfg.precision == LEE_FP32) return adj_boundary_t<float>(s, h, buf);
+  return adj_boundary_t<double>(s, h, buf);
 }
 
 hipError_t observer_energy(hipStream_t s, lee_handle *h, Block &b, double *out) {

@@ -77,7 +77,9 @@ void free_block(Block &b) {
   for (auto &z : b.zones) {
     dfree(z.sigma64);
     dfree(z.sigma32);
+    dfree(z.hg);
   }
+  for (auto &f : b.faces) dfree(f.hg);
   dfree(b.src.g);
   for (int k = 0; k < 5; k++) {
     dfree(b.A64[k]);
@@ -128,6 +130,13 @@ int enqueue_fused_step(lee_handle *h) {
     sc.b = h->b_i[st];
     sc.a = h->a_next[st];
     source_factors(h->cfg, tn + h->c_i[st] * dt, sc.s1, sc.s2);
+    // boundary data after the stage: t + c_{min(i+1, last)} h (temporal:236-246)
+    const double t_bc = tn + h->c_i[std::min(st + 1, last)] * dt;
+    double he_bc = 0.0;
+    if (h->cfg.harmonic_wave == LEE_WAVE_ENTROPY) {
+      sc.he = harmonic_entropy(h->cfg, tn + h->c_i[st] * dt);
+      he_bc = harmonic_entropy(h->cfg, t_bc);
+    }
     if (h->ktiming) HIPCHK(mark(h));
     for (int ib = 0; ib < h->nblk; ib++)
       HIPCHK(h->cfg.num_dim == 3 ? fused_stage3d(h->stream, h, h->b[ib], st, sc)
@@ -145,7 +154,7 @@ int enqueue_fused_step(lee_handle *h) {
       if (b.faces.empty()) continue;
       const bool rho = st == last && !b.uniform_mean;
       HIPCHK(apply_bcs(h->stream, h, b, b.Y64[tgt], b.Y32[tgt], f32, rho ? b.rho64 : nullptr,
-                       rho ? b.rho32 : nullptr));
+                       rho ? b.rho32 : nullptr, he_bc));
     }
     HIPCHK(overset_interpolate(h->stream, h, f32, tgt, st == last));
     if (h->ktiming && any_bnd) {
@@ -182,12 +191,15 @@ int enqueue_plain_step(lee_handle *h) {
     HIPCHK(hipMemcpyAsync(b.Y64[1], b.Y64[0], bytes, hipMemcpyDeviceToDevice, s));  // ynp1 = yn
     HIPCHK(hipMemcpyAsync(b.Y64[2], b.Y64[0], bytes, hipMemcpyDeviceToDevice, s));  // interim = yn
   }
+  const bool harm = h->cfg.harmonic_wave == LEE_WAVE_ENTROPY;
   for (int st = 0; st < S; st++) {
     double s1, s2;
     source_factors(h->cfg, tn + h->c_i[st] * dt, s1, s2);
+    const double he = harm ? harmonic_entropy(h->cfg, tn + h->c_i[st] * dt) : 0.0;
+    const double he_bc = harm ? harmonic_entropy(h->cfg, tn + h->c_i[std::min(st + 1, last)] * dt) : 0.0;
     for (int ib = 0; ib < h->nblk; ib++) {
       Block &b = h->b[ib];
-      HIPCHK(plain_rhs(s, h, b, b.Y64[2], b.Y64[3], s1, s2, b.rho64));
+      HIPCHK(plain_rhs(s, h, b, b.Y64[2], b.Y64[3], s1, s2, b.rho64, he));
     }
     for (int ib = 0; ib < h->nblk; ib++) {
       Block &b = h->b[ib];
@@ -197,7 +209,7 @@ int enqueue_plain_step(lee_handle *h) {
     const int tgt = st < last ? 2 : 1;
     for (int ib = 0; ib < h->nblk; ib++) {
       Block &b = h->b[ib];
-      if (!b.faces.empty()) HIPCHK(apply_bcs(s, h, b, b.Y64[tgt], nullptr, false, nullptr, nullptr));
+      if (!b.faces.empty()) HIPCHK(apply_bcs(s, h, b, b.Y64[tgt], nullptr, false, nullptr, nullptr, he_bc));
     }
     HIPCHK(overset_interpolate(s, h, false, tgt, false));
     for (int ib = 0; ib < h->nblk; ib++) {
@@ -616,7 +628,7 @@ int autotune_block3d(lee_handle *h, Block &b) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     float best = 1e30f;
-    static const int forms[][2] = {{0, 0}, {4, 4}, {4, 8}, {4, 16}};
+    static const int forms[][2] = {{1, 8}, {1, 16}, {0, 0}, {4, 4}, {4, 8}, {4, 16}};
     for (const auto &fm : forms) {
       b.tune_mc = fm[0];
       b.tune_ml = fm[1];
@@ -676,12 +688,14 @@ void advance_clock(lee_handle *h) {
 int adjoint_check(lee_handle *h) {
   if (h->batch) return fail(h, LEE_EINVAL, "the adjoint runs on single handles, not batches");
   if (is_plain(h)) return fail(h, LEE_EINVAL, "the adjoint runs on the fused path");
-  if (h->ov.nrecv > 0) return fail(h, LEE_EINVAL, "the adjoint does not cover overset interpolation");
   if (h->cfg.axisym) return fail(h, LEE_EINVAL, "the adjoint does not cover the axisymmetric term");
   for (int ib = 0; ib < h->nblk; ib++) {
-    if (!h->b[ib].faces.empty())
-      return fail(h, LEE_EINVAL, "the adjoint covers sponges only, not other boundary conditions");
+    for (const auto &f : h->b[ib].faces)
+      if (f.type == LEE_BC_CENTERLINE_AXISYM || f.type == LEE_BC_DIRICHLET_HARMONICWAVE)
+        return fail(h, LEE_EINVAL, "the adjoint does not cover axisymmetric or harmonic-wave boundaries");
     if (h->b[ib].carry_w) return fail(h, LEE_EINVAL, "the adjoint needs w' = 0 in 2-D");
+    for (const auto &z : h->b[ib].zones)
+      if (z.harmonic) return fail(h, LEE_EINVAL, "the adjoint covers ambient sponges only");
   }
   return LEE_OK;
 }
@@ -735,34 +749,49 @@ int checkpoint_copy(lee_handle *h, int slot, bool save) {
 // One reversed step on every block: step k of the last objective run
 // (dt and time_sol of that step), adjoint state in A[0] and the aux-density
 // adjoint in amu on entry and on exit.
+// One reversed step.  The boundary update after each forward stage (BCs,
+// overset interpolation) couples the blocks, so the reversed stages run
+// block by block between the transposed boundary updates: the filter's
+// transpose, then the last stage's boundary update (on the accumulator,
+// A[1]), the last stage, the boundary update that produced its input, the
+// stage before it, and so on.
 int adjoint_step(lee_handle *h, int k) {
   const int S = h->cfg.rk_stages;
   const double dt = h->obj.hist_dt[k], t = h->obj.hist_t[k];
+  const bool d3 = h->cfg.num_dim == 3;
+  bool any_bnd = h->ov.nrecv > 0;
+  for (int ib = 0; ib < h->nblk; ib++) any_bnd |= !h->b[ib].faces.empty();
   for (int ib = 0; ib < h->nblk; ib++) {
     Block &b = h->b[ib];
-    const bool field = b.cfg.source_kind == LEE_SOURCE_FIELD;
-    const bool d3 = h->cfg.num_dim == 3;
     HIPCHK(d3 ? adjoint_filter3d(h->stream, h, b, 0, 1, 2, !b.uniform_mean)
               : adjoint_filter(h->stream, h, b, 0, 1, !b.uniform_mean));
-    auto stage = [&](const AdjStage &a) { return d3 ? adjoint_stage3d(h->stream, h, b, a) : adjoint_stage(h->stream, h, b, a); };
-    auto src = [&](int i) {
-      double s1 = 0.0, s2 = 0.0;
-      source_factors(b.cfg, t + h->c_i[i] * dt, s1, s2);
-      return s1 * s2;
-    };
-    int prev = 2;
-    AdjStage st{STAGE_LAST, 1, -1, -1, 2, -1, h->b_i[S - 1] * dt, 0.0, src(S - 1), field};
-    HIPCHK(stage(st));
-    for (int i = S - 2; i >= 1; i--) {
-      const int out = prev == 2 ? 3 : 2;
-      AdjStage m{STAGE_MID, 1, prev, i == S - 2 ? -1 : 4, out, 4, h->b_i[i] * dt, h->a_next[i] * dt, src(i), field};
-      HIPCHK(stage(m));
-      prev = out;
-    }
-    AdjStage f{STAGE_FIRST, 1, prev, S > 2 ? 4 : -1, 0, -1, h->b_i[0] * dt, h->a_next[0] * dt, src(0), field};
-    HIPCHK(stage(f));
   }
-  return LEE_OK;
+  auto stage_all = [&](const AdjStage &proto) -> int {
+    for (int ib = 0; ib < h->nblk; ib++) {
+      Block &b = h->b[ib];
+      AdjStage a = proto;
+      a.grad = b.cfg.source_kind == LEE_SOURCE_FIELD;
+      double s1 = 0.0, s2 = 0.0;
+      source_factors(b.cfg, t + h->c_i[proto.src_stage] * dt, s1, s2);
+      a.src = s1 * s2;
+      HIPCHK(d3 ? adjoint_stage3d(h->stream, h, b, a) : adjoint_stage(h->stream, h, b, a));
+    }
+    return LEE_OK;
+  };
+  if (any_bnd) HIPCHK(adjoint_boundary(h->stream, h, 1));
+  int prev = 2;
+  AdjStage st{STAGE_LAST, 1, -1, -1, 2, -1, h->b_i[S - 1] * dt, 0.0, 0.0, false, S - 1};
+  if (int rc = stage_all(st)) return rc;
+  for (int i = S - 2; i >= 1; i--) {
+    if (any_bnd) HIPCHK(adjoint_boundary(h->stream, h, prev));
+    const int out = prev == 2 ? 3 : 2;
+    AdjStage m{STAGE_MID, 1, prev, i == S - 2 ? -1 : 4, out, 4, h->b_i[i] * dt, h->a_next[i] * dt, 0.0, false, i};
+    if (int rc = stage_all(m)) return rc;
+    prev = out;
+  }
+  if (any_bnd) HIPCHK(adjoint_boundary(h->stream, h, prev));
+  AdjStage f{STAGE_FIRST, 1, prev, S > 2 ? 4 : -1, 0, -1, h->b_i[0] * dt, h->a_next[0] * dt, 0.0, false, 0};
+  return stage_all(f);
 }
 
 int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
@@ -791,6 +820,26 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
     return fail(nullptr, LEE_EINVAL, "unknown source");
   if (c->source_kind == LEE_SOURCE_MONOPOLE && !(c->source_params[5] > 0.0))
     return fail(nullptr, LEE_EINVAL, "monopole half-width must be positive");
+  // UserInput::check_consistency_wave (input.cpp:630-655) as the reference
+  // runs it: its `else` binds to the inner `if`, so an acoustic wave of a
+  // known shape ends in "Unknown wave." and an entropy wave of any shape
+  // passes to the amplitude check
+  if (c->harmonic_wave < LEE_WAVE_NONE || c->harmonic_wave > LEE_WAVE_ENTROPY)
+    return fail(nullptr, LEE_EINVAL, "unknown HARMONIC_WAVE type");
+  if (c->harmonic_wave == LEE_WAVE_ACOUSTIC)
+    return fail(nullptr, LEE_EINVAL,
+                c->harmonic_shape == LEE_WAVEFORM_PLANE || c->harmonic_shape == LEE_WAVEFORM_HOMOGENEOUS
+                    ? "Unknown wave."
+                    : "Unknown spatial distribution of acoustic waves.");
+  if (c->harmonic_wave == LEE_WAVE_ENTROPY) {
+    if (c->harmonic_amplitude < 0.0 || c->harmonic_amplitude > 1.0)
+      return fail(nullptr, LEE_EINVAL, "The wave amplitude should be between 0 and 1.");
+    if (!(c->harmonic_wavelength > 0.0)) return fail(nullptr, LEE_EINVAL, "HARMONIC_WAVE WAVELENGTH must be positive");
+    if (c->harmonic_direction < 0 || c->harmonic_direction > 2)
+      return fail(nullptr, LEE_EINVAL, "HARMONIC_WAVE DIRECTION must be X, Y or Z");
+    if (c->harmonic_shape == LEE_WAVEFORM_GAUSSIAN_ROUNDJET && !(c->harmonic_halfwidth > 0.0))
+      return fail(nullptr, LEE_EINVAL, "HARMONIC_WAVE HALFWIDTH must be positive");
+  }
   if (nblocks < 1 || nblocks > MAX_BLOCKS) return fail(nullptr, LEE_EINVAL, "block count must be 1..4096");
   // an interior row must fit: 2 half + 1 points for each operator
   const int fd_len = c->fd_order == 11 ? 11 : 5;
@@ -821,11 +870,25 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
         case LEE_BC_CENTERLINE_AXISYM:
         case LEE_SPONGE_FREUND_AMBIENT:
           break;
+        case LEE_BC_DIRICHLET_HARMONICWAVE:
+        case LEE_SPONGE_FREUND_HARMONICWAVE: {
+          // bc_dirichlet_harmonicwave's own checks (boundary_condition.cpp:266-322)
+          const int sh = c->harmonic_shape;
+          if (sh != LEE_WAVEFORM_PLANE && sh != LEE_WAVEFORM_HOMOGENEOUS && sh != LEE_WAVEFORM_GAUSSIAN_ROUNDJET)
+            return fail(nullptr, LEE_EINVAL, "Unknown type of wave form.");
+          if (c->harmonic_wave != LEE_WAVE_ENTROPY || sh == LEE_WAVEFORM_PLANE)
+            return fail(nullptr, LEE_EINVAL, "Unknown type of wave for this wave form.");
+          break;
+        }
+        case LEE_BC_DIRICHLET:
+        case LEE_SPONGE_FREUND_DIRICHLET:
+          // State::prescribe_on_boundary_solution for LEE (state.cpp:746-749)
+          return fail(nullptr, LEE_EINVAL, "Dirichlet boundary for the linearized Euler model is not implemented yet.");
         default:
           return fail(nullptr, LEE_EINVAL, "unsupported boundary type " + std::to_string(r.type));
       }
       if (c->num_dim == 3 && r.type != LEE_SPONGE_FREUND_AMBIENT)
-        return fail(nullptr, LEE_EINVAL, "3-D blocks take sponges only");
+        return fail(nullptr, LEE_EINVAL, "3-D blocks take ambient sponges only");
     }
   }
   return LEE_OK;
@@ -891,6 +954,11 @@ int lee_batch_create(const lee_config *cfgs, const lee_block *blocks, int nsolve
         c.path != a.path)
       return fail(nullptr, LEE_EINVAL, "batched solves must share the scheme settings (solve " + std::to_string(k) + ")");
   }
+  for (int k = 0; k < nsolve; k++)
+    for (int q = 0; q < blocks[k].nboundary; q++)
+      if (blocks[k].boundary[q].type == LEE_BC_DIRICHLET_HARMONICWAVE ||
+          blocks[k].boundary[q].type == LEE_SPONGE_FREUND_HARMONICWAVE)
+        return fail(nullptr, LEE_EINVAL, "batched solves take no harmonic-wave boundaries");
   if (cfgs[0].path != LEE_PATH_FUSED) return fail(nullptr, LEE_EINVAL, "batched solves run on the fused path");
   if (cfgs[0].filter_order == 0) return fail(nullptr, LEE_EINVAL, "batched solves need the filter");
   return create_impl(cfgs, 1, blocks, nsolve, nullptr, true, out);
@@ -1113,13 +1181,30 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
         }
       }
       // sponges and other boundaries (io.cpp:957-1369)
+      const bool roundjet = cfg->harmonic_shape == LEE_WAVEFORM_GAUSSIAN_ROUNDJET;
+      // the round-jet Gaussian at the points of a box, box-local (k, j, i) order
+      auto gaussian_box = [&](const int *lo, const int *hi, std::vector<double> &out) {
+        out.clear();
+        for (int k = lo[2]; k <= hi[2]; k++)
+          for (int j = lo[1]; j <= hi[1]; j++)
+            for (int i = lo[0]; i <= hi[0]; i++) {
+              const size_t p = ((size_t)k * b.nj + j) * b.ni + i;
+              out.push_back(harmonic_gaussian(*cfg, in.xyz[p], in.xyz[(size_t)n + p], in.xyz[(size_t)2 * n + p]));
+            }
+      };
       for (int q = 0; q < in.nboundary; q++) {
         const lee_boundary &r = in.boundary[q];
-        if (r.type == LEE_SPONGE_FREUND_AMBIENT) {
+        if (r.type == LEE_SPONGE_FREUND_AMBIENT || r.type == LEE_SPONGE_FREUND_HARMONICWAVE) {
           SpongeZone z;
           for (int d = 0; d < 3; d++) {
             z.is[d] = r.is[d];
             z.ie[d] = r.ie[d];
+          }
+          z.harmonic = r.type == LEE_SPONGE_FREUND_HARMONICWAVE;
+          if (z.harmonic && roundjet) {
+            gaussian_box(z.is, z.ie, z.host_hg);
+            HIPCHK(dalloc(&z.hg, z.host_hg.size()));
+            HIPCHK(hipMemcpy(z.hg, z.host_hg.data(), sizeof(double) * z.host_hg.size(), hipMemcpyHostToDevice));
           }
           compute_sponge_zone(b, in.xyz, z, r.dir, r.end, cfg->sponge_order, cfg->sponge_constant);
           const size_t cnt = z.host_sigma.size();
@@ -1143,7 +1228,12 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
           }
           const int face = r.end == 0 ? r.is[r.dir] : r.ie[r.dir];
           f.is[r.dir] = f.ie[r.dir] = face;
-          b.faces.push_back(f);
+          if (r.type == LEE_BC_DIRICHLET_HARMONICWAVE && roundjet) {
+            gaussian_box(f.is, f.ie, f.host_hg);
+            HIPCHK(dalloc(&f.hg, f.host_hg.size()));
+            HIPCHK(hipMemcpy(f.hg, f.host_hg.data(), sizeof(double) * f.host_hg.size(), hipMemcpyHostToDevice));
+          }
+          b.faces.push_back(std::move(f));
         }
       }
       if ((int)b.zones.size() > MAX_ZONES) return fail(h, LEE_EINVAL, "more than 8 sponge zones in a block");
@@ -1446,8 +1536,10 @@ int lee_set_filter_march(lee_handle *h, int ib, int waves, int rows) {
                 "8 rows, or (0, 0)");
   if (!off && waves == 1 && h->b[ib].has_holes)
     return fail(h, LEE_EINVAL, "single-wave column-march filter: blocks without holes only (stacks take holes)");
-  if (!off && h->cfg.num_dim == 3 && waves != 4)
-    return fail(h, LEE_EINVAL, "the 3-D plane pass takes stacks of 4 waves of 4, 8 or 16 rows, or (0, 0)");
+  if (!off && h->cfg.num_dim == 3 && !(waves == 4 || (waves == 1 && (rows == 8 || rows == 16))))
+    return fail(h, LEE_EINVAL,
+                "a 3-D block takes the single-pass filter (1 wave, tiles of 8 or 16 rows) or the two-pass "
+                "filter with its plane pass as stacks of 4 waves of 4, 8 or 16 rows, or (0, 0)");
   h->b[ib].tune_mc = off ? 0 : waves;
   h->b[ib].tune_ml = off ? 0 : rows;
   h->b[ib].tuned = true;
@@ -1514,12 +1606,13 @@ int lee_compute_rhs(lee_handle *h, int ib, const double *y, double time, double 
   hipStream_t s = h->stream;
   double s1, s2;
   source_factors(b.cfg, time, s1, s2);
+  const double he = h->cfg.harmonic_wave == LEE_WAVE_ENTROPY ? harmonic_entropy(h->cfg, time) : 0.0;
   HIPCHK(hipMemcpy(b.Y64[2], y, sizeof(double) * cnt, hipMemcpyHostToDevice));
   if (is_plain(h)) {
-    HIPCHK(plain_rhs(s, h, b, b.Y64[2], b.Y64[3], s1, s2, b.dsum));
+    HIPCHK(plain_rhs(s, h, b, b.Y64[2], b.Y64[3], s1, s2, b.dsum, he));
     HIPCHK(hipMemcpyAsync(rhs, b.Y64[3], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
   } else {
-    StageScalars sc{1.0, 1.0, 0.0, s1, s2};
+    StageScalars sc{1.0, 1.0, 0.0, s1, s2, he};
     if (is_fp32(h)) {
       HIPCHK(cast_state(s, b, NV, b.Y64[2], b.Y32[2], cnt));
       HIPCHK(hipMemsetAsync(b.Y32[3], 0, sizeof(float) * cnt, s));
@@ -1578,8 +1671,10 @@ int lee_update_boundary(lee_handle *h, double *const *ys) {
   hipStream_t s = h->stream;
   for (int ib = 0; ib < h->nblk; ib++)
     HIPCHK(hipMemcpy(h->b[ib].Y64[2], ys[ib], sizeof(double) * NV * h->b[ib].n, hipMemcpyHostToDevice));
+  const double he = h->cfg.harmonic_wave == LEE_WAVE_ENTROPY ? harmonic_entropy(h->cfg, h->time_sol) : 0.0;
   for (int ib = 0; ib < h->nblk; ib++)
-    if (!h->b[ib].faces.empty()) HIPCHK(apply_bcs(s, h, h->b[ib], h->b[ib].Y64[2], nullptr, false, nullptr, nullptr));
+    if (!h->b[ib].faces.empty())
+      HIPCHK(apply_bcs(s, h, h->b[ib], h->b[ib].Y64[2], nullptr, false, nullptr, nullptr, he));
   HIPCHK(overset_interpolate(s, h, false, 2, false));
   for (int ib = 0; ib < h->nblk; ib++)
     HIPCHK(hipMemcpyAsync(ys[ib], h->b[ib].Y64[2], sizeof(double) * NV * h->b[ib].n, hipMemcpyDeviceToHost, s));
@@ -1846,6 +1941,32 @@ int lee_get_state_gradient(lee_handle *h, int ib, double *grad_y0) {
     HIPCHK(hipMemcpyAsync(grad_y0, b.A64[0], sizeof(double) * cnt, hipMemcpyDeviceToHost, h->stream));
   }
   HIPCHK(hipStreamSynchronize(h->stream));
+  return LEE_OK;
+}
+
+int lee_update_boundary_adjoint(lee_handle *h, double *const *ys) {
+  if (!h || !ys) return fail(h, LEE_EINVAL, "bad argument");
+  int rc = adjoint_check(h);
+  if (rc) return rc;
+  hipStream_t s = h->stream;
+  for (int ib = 0; ib < h->nblk; ib++) {
+    Block &b = h->b[ib];
+    rc = adjoint_alloc(h, b);
+    if (rc) return rc;
+    const size_t cnt = (size_t)NV * b.n;
+    HIPCHK(hipMemcpy(b.Y64[3], ys[ib], sizeof(double) * cnt, hipMemcpyHostToDevice));
+    if (is_fp32(h)) HIPCHK(cast_state(s, b, NV, b.Y64[3], b.A32[0], cnt));
+    else HIPCHK(hipMemcpyAsync(b.A64[0], b.Y64[3], sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
+  }
+  HIPCHK(adjoint_boundary(s, h, 0));
+  for (int ib = 0; ib < h->nblk; ib++) {
+    Block &b = h->b[ib];
+    const size_t cnt = (size_t)NV * b.n;
+    if (is_fp32(h)) HIPCHK(cast_back(s, b, NV, b.A32[0], b.Y64[3], cnt));
+    else HIPCHK(hipMemcpyAsync(b.Y64[3], b.A64[0], sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(ys[ib], b.Y64[3], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
   return LEE_OK;
 }
 

@@ -91,6 +91,8 @@ template <typename T>
 struct ZoneT {
   int i0, i1, j0, j1, nzi;
   const T *sig;
+  int harm;            // sponge_freund_harmonicwave: s' relaxes to he * hg (hg null: he)
+  const double *hg;
 };
 
 template <typename T>
@@ -114,6 +116,7 @@ struct StageArgs {
   int si0, si1, sj0, sj1;
   const double *g;
   double s1, s2;
+  double he;           // harmonic-wave entropy factor at the stage time
   T h, b, a;
 };
 

@@ -44,17 +44,25 @@ struct OpTable {
   double bnd[5][9];    // closure rows
 };
 
-struct SpongeZone {    // one Freund ambient buffer zone
+struct SpongeZone {    // one Freund buffer zone (ambient or harmonic-wave target)
   int is[3], ie[3];
   int nz[3];
   double *sigma64 = nullptr;  // device, zone-local (k, j, i) order
   float *sigma32 = nullptr;
   std::vector<double> host_sigma;
+  // sponge_freund_harmonicwave: the zone relaxes s' to e(t) g(x) (and the
+  // other variables to 0); g is 1 for a homogeneous wave (hg null) or the
+  // round-jet Gaussian per zone point
+  bool harmonic = false;
+  double *hg = nullptr;
+  std::vector<double> host_hg;
 };
 
 struct BcFace {        // one boundary-condition face (BCs other than sponges)
   int type, dir, end;
   int is[3], ie[3];    // face box (dir index collapsed to the face)
+  double *hg = nullptr;  // dirichlet_harmonicwave: g per face point (null: 1)
+  std::vector<double> host_hg;
 };
 
 struct SourceBox {     // spatial factor of a physical source over its support
@@ -150,6 +158,7 @@ struct AdjStage {
   int lacc, lin, lyin, lout, lyout;
   double hb, ha, src;          // b_i h, a_i h, source time factor s1 s2
   bool grad;                   // accumulate the source-field gradient
+  int src_stage = 0;           // RK stage index whose time gives src
 };
 
 // Bookkeeping of the last lee_objective_forward.
@@ -264,20 +273,25 @@ struct StageScalars {
   double b;        // b_i
   double a;        // a_{i+1,i}
   double s1, s2;   // source time factors
+  double he = 0.0; // harmonic-wave entropy factor e(t) at the stage time
 };
 hipError_t plain_apply_op(hipStream_t s, const Block &b, const OpTable &op, const int8_t *codes,
                           const double *f, double *df, int dir);
 hipError_t plain_take_derivative_xyz(hipStream_t s, const Block &b, const OpTable &op, const double *f,
                                      int dir_xyz);
 hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double *y, double *rhs, double s1,
-                     double s2, double *rho);
+                     double s2, double *rho, double he = 0.0);
 hipError_t plain_rk_update(hipStream_t s, const Block &b, double *rhs, double *ynext, double *interim,
                            const double *ycur, double hdt, double bi, double a, bool last);
 hipError_t plain_filter(hipStream_t s, lee_handle *h, const Block &b, double *y);
 hipError_t plain_iblank_reset(hipStream_t s, const Block &b, double *y);
 hipError_t aux_density(hipStream_t s, const Block &b, double gamma, const double *y, double *rho);
+// HARMONIC_WAVE entropy factor e(t) = A (1 + sin(k c0 t)), k = 2 pi / wavelength
+// (boundary_condition.cpp:232-344), and the round-jet Gaussian of a point.
+double harmonic_entropy(const lee_config &c, double t);
+double harmonic_gaussian(const lee_config &c, double x, double y, double z);
 hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, float *y32, bool fp32,
-                     double *rho64, float *rho32);
+                     double *rho64, float *rho32, double he = 0.0);
 hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_index, bool last_stage);
 // Overset interpolation of caller-owned FP64 block arrays ys[b] (device,
 // 5n each), receiver values staged in buf (nrecv * NV doubles, device).
@@ -300,6 +314,9 @@ hipError_t batch_filter(hipStream_t s, lee_handle *h);
 void batch_release(lee_handle *h);
 hipError_t adjoint_stage(hipStream_t s, lee_handle *h, Block &b, const AdjStage &st);
 hipError_t adjoint_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst, bool with_mu);
+// Transposed boundary update (overset interpolation, then the BC faces of
+// every block in reverse order) on the adjoint buffer A[buf] of all blocks.
+hipError_t adjoint_boundary(hipStream_t s, lee_handle *h, int buf);
 hipError_t adjoint_zero(hipStream_t s, lee_handle *h, Block &b, int buf);  // buf < 0: the aux-density adjoint
 hipError_t adjoint_zero_grad(hipStream_t s, Block &b);
 hipError_t adjoint_add_rt_mu(hipStream_t s, lee_handle *h, Block &b, int buf);

@@ -274,7 +274,27 @@ extern "C" int lee_deck_settings(const lee_deck *d, lee_run_settings *s) {
   const std::string interp = r.str("INTERPOLATE_SOLUTION");
   if (r.rc) return r.rc;
   if (fmt != "PLOT3D") return io_fail("Unknown type of data files.");
-  if (hw != "NONE") return io_fail("HARMONIC_WAVE = %s is not supported by the GPU advance.", hw.c_str());
+  if (hw != "NONE") {  // input.cpp:221-231, in its order
+    const double amp = r.f("HARMONIC_WAVE", "AMPLITUDE");
+    const std::string shape = r.str("HARMONIC_WAVE", "SHAPE");
+    std::string dir = r.str("HARMONIC_WAVE", "DIRECTION");
+    const double wl = r.f("HARMONIC_WAVE", "WAVELENGTH");
+    double half = 0.0;
+    if (shape == "WAVEFORM_GAUSSIAN_ROUNDJET") half = r.f("HARMONIC_WAVE", "HALFWIDTH");
+    if (r.rc) return r.rc;
+    for (char &ch : dir) ch = (char)toupper((unsigned char)ch);  // xyz_2int (input.cpp:820-838)
+    if (dir != "X" && dir != "Y" && dir != "Z") return io_fail("Unknown direction %s.", dir.c_str());
+    c.harmonic_wave = hw == "WAVE_ACOUSTIC" ? LEE_WAVE_ACOUSTIC : hw == "WAVE_ENTROPY" ? LEE_WAVE_ENTROPY : -1;
+    if (c.harmonic_wave < 0) return io_fail("Unknown wave.");
+    c.harmonic_shape = shape == "WAVEFORM_PLANE"               ? LEE_WAVEFORM_PLANE
+                       : shape == "WAVEFORM_HOMOGENEOUS"       ? LEE_WAVEFORM_HOMOGENEOUS
+                       : shape == "WAVEFORM_GAUSSIAN_ROUNDJET" ? LEE_WAVEFORM_GAUSSIAN_ROUNDJET
+                                                               : 0;
+    c.harmonic_direction = dir[0] - 'X';
+    c.harmonic_amplitude = amp;
+    c.harmonic_wavelength = wl;
+    c.harmonic_halfwidth = half;
+  }
   if (interp != "NONE") return io_fail("INTERPOLATE_SOLUTION = %s is not supported by the GPU advance.", interp.c_str());
   if (!copy_name(s->grid_file, grid) || !copy_name(s->mean_file, mean) || !copy_name(s->bc_file, bc) ||
       !copy_name(s->overset_file, ovf))
@@ -392,7 +412,9 @@ namespace {
 // implements
 int bc_type(std::string s) {
   for (char &ch : s) ch = (char)tolower((unsigned char)ch);
+  if (s == "dirichlet") return LEE_BC_DIRICHLET;
   if (s == "dirichlet_allzero") return LEE_BC_DIRICHLET_ALLZERO;
+  if (s == "dirichlet_harmonicwave") return LEE_BC_DIRICHLET_HARMONICWAVE;
   if (s == "neumann") return LEE_BC_NEUMANN;
   if (s == "wall_slip_kinematic") return LEE_BC_WALL_SLIP_KINEMATIC;
   if (s == "centerline_cart_norm2x") return LEE_BC_CENTERLINE_CART_NORM2X;
@@ -400,6 +422,8 @@ int bc_type(std::string s) {
   if (s == "centerline_cart_norm2z") return LEE_BC_CENTERLINE_CART_NORM2Z;
   if (s == "centerline_axisym") return LEE_BC_CENTERLINE_AXISYM;
   if (s == "sponge_freund_ambient") return LEE_SPONGE_FREUND_AMBIENT;
+  if (s == "sponge_freund_dirichlet") return LEE_SPONGE_FREUND_DIRICHLET;
+  if (s == "sponge_freund_harmonicwave") return LEE_SPONGE_FREUND_HARMONICWAVE;
   return -1;
 }
 

@@ -175,17 +175,22 @@ __global__ void __launch_bounds__(TPB) k_plain_source(double *__restrict__ rp, c
   rp[(size_t)j * ni + i] += s1 * g[q] * s2;
 }
 
-// apply_bufferZone_Freund_ambient (source.cpp:174-201), one zone, all 5 vars
+// apply_bufferZone_Freund_ambient (source.cpp:174-201), one zone, all 5
+// vars; a harmonic zone (apply_bufferZone_Freund_harmonicWave, source.cpp:
+// 264-320) relaxes s' to the entropy wave he * hg (hg null: he) and the
+// other variables to 0, the data of bc_dirichlet_harmonicwave for LEE.
 __global__ void __launch_bounds__(TPB) k_plain_sponge(double *__restrict__ rhs, const double *__restrict__ y,
                                                       const double *__restrict__ sig, int ni, int n, int i0,
-                                                      int nzi, int j0, int nzj) {
+                                                      int nzi, int j0, int nzj, int harm, double he,
+                                                      const double *__restrict__ hg) {
 #pragma clang fp contract(off)
   const int q = blockIdx.x * TPB + threadIdx.x;
   if (q >= nzi * nzj) return;
   const int p = (j0 + q / nzi) * ni + (i0 + q % nzi);
   const double s = sig[q];
   for (int v = 0; v < NV; v++) {
-    const double relax = -s * (y[(size_t)v * n + p] - 0.0);
+    const double target = harm && v == IS ? (hg ? he * hg[q] : he) : 0.0;
+    const double relax = -s * (y[(size_t)v * n + p] - target);
     rhs[(size_t)v * n + p] += relax;
   }
 }
@@ -253,6 +258,8 @@ struct BcArgs {
   int ni, nj;
   double neu[9];        // closure row 0 of the FD operator (sb entries)
   int sb;
+  double he;            // dirichlet_harmonicwave: entropy factor e(t) at the BC time
+  const double *hg;     // and g per face point (null: 1)
 };
 
 // bc::apply_BC one face (boundary_condition.cpp:46-209) with bc_neumann
@@ -278,6 +285,10 @@ __global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double
   for (int v = 0; v < NV; v++) data[v] = 1.0e9 * (double)(a.dir + 1);
   if (a.type == LEE_BC_DIRICHLET_ALLZERO)
     for (int v = 0; v < NV; v++) data[v] = 0.0;
+  if (a.type == LEE_BC_DIRICHLET_HARMONICWAVE) {  // WAVE_ENTROPY: s' = e(t) g, the rest 0 (bc:250-334)
+    for (int v = 0; v < NV; v++) data[v] = 0.0;
+    data[IS] = a.hg ? a.he * a.hg[q] : a.he;
+  }
   const bool neumann = a.type == LEE_BC_NEUMANN || a.type == LEE_BC_WALL_SLIP_KINEMATIC ||
                        (a.type >= LEE_BC_CENTERLINE_CART_NORM2X && a.type <= LEE_BC_CENTERLINE_AXISYM);
   if (neumann) {
@@ -636,8 +647,11 @@ __device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, in
     for (int z = 0; z < a.nz; z++) {
       const auto &zz = a.z[z];
       if (gi >= zz.i0 && gi <= zz.i1 && gj >= zz.j0 && gj <= zz.j1) {
-        const T sg = zz.sig[(gj - zz.j0) * zz.nzi + (gi - zz.i0)];
-        rs = fmx(-sg, sv, rs);
+        const int zp = (gj - zz.j0) * zz.nzi + (gi - zz.i0);
+        const T sg = zz.sig[zp];
+        // a harmonic-wave zone relaxes s' to e(t) g (source.cpp:264-320)
+        const T ds = zz.harm ? sv - (T)(zz.hg ? a.he * zz.hg[zp] : a.he) : sv;
+        rs = fmx(-sg, ds, rs);
         ru = fmx(-sg, uv, ru);
         rv = fmx(-sg, vv, rv);
         rp = fmx(-sg, pv, rp);
@@ -921,7 +935,7 @@ hipError_t aux_density(hipStream_t s, const Block &b, double gamma, const double
 // operation of the reference.  rho: the aux density to use; anything other
 // than the block's carried array is first filled from y.
 hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double *y, double *rhs, double s1,
-                     double s2, double *rho) {
+                     double s2, double *rho, double he) {
   const int n = b.n;
   const dim3 g(nblocks(n)), t(TPB);
   const double gam = h->cfg.gamma;
@@ -972,7 +986,7 @@ hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double 
   for (const auto &z : b.zones) {
     const size_t cnt = (size_t)z.nz[0] * z.nz[1];
     hipLaunchKernelGGL(k_plain_sponge, dim3(nblocks(cnt)), t, 0, s, rhs, y, z.sigma64, b.ni, n, z.is[0], z.nz[0],
-                       z.is[1], z.nz[1]);
+                       z.is[1], z.nz[1], z.harmonic ? 1 : 0, he, z.hg);
   }
   const hipError_t last = hipGetLastError();
   return err != hipSuccess ? err : last;
@@ -1010,9 +1024,11 @@ hipError_t plain_iblank_reset(hipStream_t s, const Block &b, double *y) {
 }
 
 hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, float *y32, bool fp32,
-                     double *rho64, float *rho32) {
+                     double *rho64, float *rho32, double he) {
   for (const auto &f : b.faces) {
     BcArgs a{};
+    a.he = he;
+    a.hg = f.hg;
     a.type = f.type;
     a.dir = f.dir;
     a.end = f.end;
@@ -1102,6 +1118,8 @@ static void fill_zones(StageArgs<T> &a, const Block &b) {
     q.nzi = z.nz[0];
     if constexpr (sizeof(T) == 8) q.sig = (const T *)z.sigma64;
     else q.sig = (const T *)z.sigma32;
+    q.harm = z.harmonic ? 1 : 0;
+    q.hg = z.hg;
   }
 }
 
@@ -1167,6 +1185,7 @@ static StageArgs<T> make_stage_args(const lee_handle *h, const Block &b, int sta
   a.g = b.src.g;
   a.s1 = sc.s1;
   a.s2 = sc.s2;
+  a.he = sc.he;
   a.h = (T)sc.h;
   a.b = (T)sc.b;
   a.a = (T)sc.a;

@@ -30,6 +30,7 @@ constexpr int S3_TY = 4;      // tile rows in j
 #define LEE_Z_CHUNK 32
 #endif
 constexpr int Z_CHUNK = LEE_Z_CHUNK;  // planes per z chunk
+constexpr int FUSED_Z_CHUNK = 64;     // planes per chunk of the single-pass filter (4 halo planes a side)
 
 inline unsigned nb3(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
@@ -341,6 +342,165 @@ __global__ void __launch_bounds__(TPB) k_filter3d_plane_s(FilterArgs<T> a, size_
   stack_item<T, ORDER, 4, R>(b, t, L);
 }
 
+// Single pass: spatial::apply_filter's three sweeps (spatial:2210-2237,
+// xi, then eta, then zeta) for one variable on a 64 x TY column tile
+// through a chunk of planes.  Each plane of the tile plus its 4-point (i, j)
+// halo goes through LDS for the xi and eta sweeps (the next plane's loads
+// are in flight meanwhile); the thread keeps the xi-eta results of its TY/4
+// points for the last 9 planes in registers, and when plane k + 4 has
+// entered the window, sweeps zeta at plane k and blends with the unfiltered
+// value of plane k, kept in a 5-plane register ring since its load.  A chunk
+// reads its 4 halo planes on each side again (they are the neighbouring
+// chunks' planes); nothing else is read twice and no intermediate goes to
+// memory.  Each point takes the operations of the two-pass form in the
+// same order, so the results are bit-identical to it.
+template <typename T, int TY>
+using PlaneRegs = FilterRegs<T, 1, TY>;
+
+template <typename T, int ORDER, int TY, bool INNER>
+__device__ __forceinline__ void plane_xieta(const FilterArgs<T> &a, int tile, FilterLds<T, 1, TY> &L,
+                                            const PlaneRegs<T, TY> &R, T (&f)[TY / 4], T (&orig)[TY / 4]) {
+  constexpr int NR = TY + 2 * FL_H, KR = NR / 4, NB = ORDER / 2;
+  auto &in = L.in;
+  auto &mid = L.mid;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
+  const int ni = a.ni, nj = a.nj;
+  const bool xtra = tx < 2 * FL_H;
+  const int colb = xtra ? FL_TX + tx : tx;
+  const int gia = i0 - FL_H + tx, gib = i0 - FL_H + colb;
+  const bool cola = INNER || (gia >= 0 && gia < ni), colok_b = INNER || (gib >= 0 && gib < ni);
+#pragma unroll
+  for (int k = 0; k < KR; k++) {
+    const int gj = j0 - FL_H + ty + 4 * k;
+    const bool rowok = INNER || (gj >= 0 && gj < nj);
+    in[0][ty + 4 * k][tx] = rowok && cola ? R.ra[0][k] : (T)0;
+    in[0][ty + 4 * k][colb] = rowok && colok_b ? R.rb[0][k] : (T)0;
+  }
+  __syncthreads();
+  constexpr int NG = FL_TX / 4;
+  constexpr int NXI = NR * NG;
+  constexpr int NS = (NXI + TPB - 1) / TPB;
+  const int tid = ty * FL_TX + tx;
+  Vec4<T> xo[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int it = tid + s * TPB;
+    if (NXI % TPB != 0 && it >= NXI) break;
+    const int g = it % NG, r = it / NG;
+    T v[12];
+#pragma unroll
+    for (int h = 0; h < 3; h++) {
+      const Vec4<T> x = *reinterpret_cast<const Vec4<T> *>(&in[0][r][4 * g + 4 * h]);
+      v[4 * h] = x.x;
+      v[4 * h + 1] = x.y;
+      v[4 * h + 2] = x.z;
+      v[4 * h + 3] = x.w;
+    }
+    const int gj = j0 - FL_H + r;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const int gi = i0 + 4 * g + m;
+      int code = CODE_INTERIOR;
+      bool ok = true;
+      if (!INNER) {
+        ok = gj >= 0 && gj < nj && gi < ni;
+        if (ok) code = default_code(gi, ni, NB);
+      }
+      xo[s][m] = ok ? filter_row<T, ORDER>(&v[m + FL_H], 1, code) : (T)0;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int it = tid + s * TPB;
+    if (NXI % TPB != 0 && it >= NXI) break;
+    const int g = it % NG, r = it / NG;
+    *reinterpret_cast<Vec4<T> *>(&mid[0][r][4 * g]) = xo[s];
+  }
+  __syncthreads();
+  const int gi = i0 + tx;
+  const int src_lane = tx < FL_TX - FL_H ? tx + FL_H : tx - (FL_TX - FL_H);
+#pragma unroll
+  for (int k = 0; k < TY / 4; k++) {
+    const T va = __shfl(R.ra[0][k + 1], src_lane, FL_TX);
+    const T vb = __shfl(R.rb[0][k + 1], src_lane, FL_TX);
+    orig[k] = tx < FL_TX - FL_H ? va : vb;
+    const int r = ty + 4 * k;
+    const int gj = j0 + r;
+    const int code = INNER ? CODE_INTERIOR : ((gj < nj && gi < ni) ? default_code(gj, nj, NB) : CODE_ZERO);
+    f[k] = filter_row<T, ORDER>(&mid[0][r + FL_H][tx], FL_TX, code);
+  }
+}
+
+template <typename T, int ORDER, int TY>
+__global__ void __launch_bounds__(TPB) k_filter3d_fused(FilterArgs<T> a, int nk, int zchunk) {
+  constexpr int NP = TY / 4, H = FL_H, NB = ORDER / 2;
+  __shared__ FilterLds<T, 1, TY> L;
+  const int t = xcd_tile(blockIdx.x, a.chunk);
+  if (t >= a.ntiles) return;
+  const int per_var = a.ntx * a.nty;
+  const int v = t / per_var, tile = t - v * per_var;
+  const size_t plane = (size_t)a.ni * a.nj;
+  const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
+  const bool inner = i0 >= FL_H && i0 + FL_TX + FL_H <= a.ni && j0 >= FL_H && j0 + TY + FL_H <= a.nj;
+  const int k0 = blockIdx.y * zchunk, k1 = min(k0 + zchunk, nk);
+  FilterArgs<T> b = a;
+  b.vars[0] = 0;
+  b.vs = 0;
+  const T *src = a.src + (size_t)v * a.vs;
+  T *dst = a.dst + (size_t)v * a.vs;
+  T w[NP][2 * H + 1], yo[NP][H + 1];
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+#pragma unroll
+    for (int q = 0; q <= 2 * H; q++) w[p][q] = (T)0;
+#pragma unroll
+    for (int q = 0; q <= H; q++) yo[p][q] = (T)0;
+  }
+  const int kfirst = max(k0 - H, 0), klast = min(k1 + H, nk) - 1;
+  PlaneRegs<T, TY> R, Rn;
+  b.src = src + (size_t)kfirst * plane;
+  filter_load<T, false, 1, TY>(b, tile, R);
+  const int gi = i0 + threadIdx.x;
+  for (int kk = k0 - H; kk < k1 + H; kk++) {
+    T f[NP], orig[NP];
+    if (kk >= kfirst && kk <= klast) {
+      if (kk < klast) {  // the next plane's loads stay in flight through this plane's sweeps
+        b.src = src + (size_t)(kk + 1) * plane;
+        filter_load<T, false, 1, TY>(b, tile, Rn);
+      }
+      if (inner) plane_xieta<T, ORDER, TY, true>(b, tile, L, R, f, orig);
+      else plane_xieta<T, ORDER, TY, false>(b, tile, L, R, f, orig);
+      __syncthreads();  // the next plane's tile overwrites the LDS
+      if (kk < klast) R = Rn;
+    } else {
+#pragma unroll
+      for (int p = 0; p < NP; p++) f[p] = orig[p] = (T)0;
+    }
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+#pragma unroll
+      for (int q = 0; q < 2 * H; q++) w[p][q] = w[p][q + 1];
+      w[p][2 * H] = f[p];
+#pragma unroll
+      for (int q = 0; q < H; q++) yo[p][q] = yo[p][q + 1];
+      yo[p][H] = orig[p];
+    }
+    const int k = kk - H;
+    if (k >= k0) {
+      const int code = default_code(k, nk, NB);
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        const int gj = j0 + threadIdx.y + 4 * p;
+        if (!inner && (gj >= a.nj || gi >= a.ni)) continue;
+        const T fz = filter_row<T, ORDER>(&w[p][H], 1, code);
+        dst[(size_t)k * plane + (size_t)gj * a.ni + gi] = filter_blend(a.rem, yo[p][0], a.blend, fz);
+      }
+    }
+  }
+}
+
 // Pass 2: the zeta sweep of tmp along each (i, j) column through a chunk of
 // planes (9-plane register window), blend with y, into dst.
 template <typename T, int ORDER>
@@ -370,7 +530,7 @@ __global__ void __launch_bounds__(TPB) k_filter3d_z(const T *__restrict__ tmp, c
 #pragma unroll
     for (int v = 0; v < 5; v++) {
       const T f = filter_row<T, ORDER>(&w[v][H], 1, code);
-      dst[v * n + p] = rem * yk[v] + blend * f;
+      dst[v * n + p] = filter_blend(rem, yk[v], blend, f);
     }
 #pragma unroll
     for (int v = 0; v < 5; v++)
@@ -529,6 +689,26 @@ static hipError_t filter3d_t(hipStream_t s, lee_handle *h, Block &b, T *const *Y
   for (int v = 0; v < 5; v++) a.vars[v] = v;
   const size_t plane = (size_t)b.ni * b.nj;
   const bool o8 = h->cfg.filter_order == 8;
+  if (b.tune_mc == 1) {  // single pass, 64 x tune_ml column tiles, one variable each
+    const int TY = b.tune_ml;
+    a.dst = Y[dst];
+    a.blend = (T)h->cfg.filter_blend;
+    a.rem = (T)(1.0 - h->cfg.filter_blend);
+    a.nty = (b.nj + TY - 1) / TY;
+    a.ntiles = 5 * a.ntx * a.nty;
+    a.chunk = (a.ntiles + 7) / 8;
+    const int zc = FUSED_Z_CHUNK;
+    const dim3 grid(a.chunk * 8, (b.nk + zc - 1) / zc), block(64, 4);
+#define LEE_F3F(OR)                                                                                      \
+  do {                                                                                                   \
+    if (TY == 8) hipLaunchKernelGGL((k_filter3d_fused<T, OR, 8>), grid, block, 0, s, a, b.nk, zc);     \
+    else hipLaunchKernelGGL((k_filter3d_fused<T, OR, 16>), grid, block, 0, s, a, b.nk, zc);            \
+  } while (0)
+    if (o8) LEE_F3F(8);
+    else LEE_F3F(4);
+#undef LEE_F3F
+    return hipGetLastError();
+  }
   if (b.tune_mc == 4) {  // stacks of four waves of tune_ml rows
     const int R = b.tune_ml;
     a.ntx = (b.ni + March::STRIDE - 1) / March::STRIDE;

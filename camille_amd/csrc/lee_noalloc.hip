@@ -95,8 +95,9 @@ int lee_update_boundary_noalloc(lee_handle *h, double *const *ys_dev) {
   }
   hipStream_t s = h->stream;
   hipError_t e = hipSuccess;
+  const double he = h->cfg.harmonic_wave == LEE_WAVE_ENTROPY ? harmonic_entropy(h->cfg, h->time_sol) : 0.0;
   for (int ib = 0; ib < h->nblk && e == hipSuccess; ib++)
-    if (!h->b[ib].faces.empty()) e = apply_bcs(s, h, h->b[ib], ys_dev[ib], nullptr, false, nullptr, nullptr);
+    if (!h->b[ib].faces.empty()) e = apply_bcs(s, h, h->b[ib], ys_dev[ib], nullptr, false, nullptr, nullptr, he);
   if (e == hipSuccess) e = overset_interpolate_arrays(s, h, ys_dev, buf);
   if (e != hipSuccess) {
     h->err = std::string("HIP error: ") + hipGetErrorString(e);

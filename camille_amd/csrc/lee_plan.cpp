@@ -419,6 +419,29 @@ void plan_source(const lee_config &c, Block &b, const double *xyz) {
       for (int i = i0; i <= i1; i++) b.src.host_g.push_back(g[(k * b.nj + j) * b.ni + i]);
 }
 
+// bc_dirichlet_harmonicwave (boundary_condition.cpp:232-344) for LEE with
+// WAVE_ENTROPY, the one wave type the reference's deck check lets through
+// (input.cpp:636-648): s' = A (1 + sin(k c0 t)) with k = twopi / wavelength
+// (twopi = 2 acos(-1), math.cpp:7-8, c0 = 1), times, for the round-jet
+// shape, exp(-ln2 / halfwidth^2 (t1^2 + t2^2)) over the two coordinates
+// transverse to the propagation direction (param.h:133).  Evaluated in the
+// reference's operation order, on the host, so the device only multiplies.
+double harmonic_entropy(const lee_config &c, double t) {
+  const double twopi = 2.0 * std::acos(-1.0);
+  const double wavenumber = twopi / c.harmonic_wavelength;
+  const double c_0 = 1.0;
+  return c.harmonic_amplitude * (1.0 + std::sin(wavenumber * c_0 * t));
+}
+
+double harmonic_gaussian(const lee_config &c, double x, double y, double z) {
+  static const int other[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+  const double xyz[3] = {x, y, z};
+  const int d = c.harmonic_direction;
+  const double t1 = xyz[other[d][0]], t2 = xyz[other[d][1]];
+  const double fac = -std::log(2.0) / std::pow(c.harmonic_halfwidth, 2);
+  return std::exp(fac * (std::pow(t1, 2) + std::pow(t2, 2)));
+}
+
 void source_factors(const lee_config &c, double t, double &s1, double &s2) {
   const double pi = std::acos(-1.0), twopi = 2.0 * pi;
   s1 = 1.0;

@@ -18,7 +18,9 @@ PATH_FUSED, PATH_PLAIN = 0, 1
 
 # BC-file type names (src/io/io.cpp:1273-1320) -> src/core/param.h codes
 BC_TYPES = {
+    "dirichlet": 0,
     "dirichlet_allzero": 1,
+    "dirichlet_harmonicwave": 2,
     "neumann": 100,
     "wall_slip_kinematic": 201,
     "centerline_cart_norm2x": 900,
@@ -26,7 +28,12 @@ BC_TYPES = {
     "centerline_cart_norm2z": 902,
     "centerline_axisym": 903,
     "sponge_freund_ambient": 1000,
+    "sponge_freund_dirichlet": 1001,
+    "sponge_freund_harmonicwave": 1002,
 }
+# HARMONIC_WAVE type and SHAPE names (input.cpp:219-231)
+WAVES = {"NONE": 0, "WAVE_ACOUSTIC": 1, "WAVE_ENTROPY": 2}
+WAVEFORMS = {"WAVEFORM_PLANE": 1, "WAVEFORM_HOMOGENEOUS": 2, "WAVEFORM_GAUSSIAN_ROUNDJET": 3}
 # SIMULATION key -> physical source
 SOURCES = {
     "SOURCE_FIELD": 4,
@@ -62,6 +69,12 @@ class _Config(ctypes.Structure):
         ("path", ctypes.c_int),
         ("filter_strength", ctypes.c_double),
         ("axisym", ctypes.c_int),
+        ("harmonic_wave", ctypes.c_int),
+        ("harmonic_shape", ctypes.c_int),
+        ("harmonic_direction", ctypes.c_int),
+        ("harmonic_amplitude", ctypes.c_double),
+        ("harmonic_wavelength", ctypes.c_double),
+        ("harmonic_halfwidth", ctypes.c_double),
     ]
 
 
@@ -95,7 +108,7 @@ EXPORTS = [
     "lee_batch_create", "lee_get_solve_time", "lee_set_batch_launch", "lee_get_batch_launch",
     "lee_set_source_field", "lee_set_observer", "lee_objective_forward", "lee_objective_backward",
     "lee_get_source_gradient", "lee_get_state_gradient", "lee_compute_rhs_adjoint", "lee_apply_filter_adjoint",
-    "lee_sample_observers", "lee_update_boundary_noalloc",
+    "lee_sample_observers", "lee_update_boundary_noalloc", "lee_update_boundary_adjoint",
 ]
 
 _lib = None
@@ -131,6 +144,7 @@ def library():
     L.lee_take_derivative_xyz.argtypes = [vp, i, dp, i, dp]
     L.lee_update_boundary.argtypes = [vp, ctypes.POINTER(dp)]
     L.lee_update_boundary_noalloc.argtypes = [vp, ctypes.POINTER(vp)]
+    L.lee_update_boundary_adjoint.argtypes = [vp, ctypes.POINTER(dp)]
     L.lee_get_metrics.argtypes = [vp, i, dp, dp]
     L.lee_get_iblank.argtypes = [vp, i, ip]
     L.lee_get_mean_gradient.argtypes = [vp, i, dp]
@@ -216,9 +230,19 @@ def read_bc_file(path, dims_per_block):
 def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, filter_strength=1.0, rk=4,
                  mode="FIX_CFL", value=0.5,
                  sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49", source_params=(),
-                 precision=FP64, path=PATH_FUSED, num_dim=2, axisym=0):
-    """lee_config from the deck settings (src/core/input.cpp:99-284 keys)."""
+                 precision=FP64, path=PATH_FUSED, num_dim=2, axisym=0, harmonic=None):
+    """lee_config from the deck settings (src/core/input.cpp:99-284 keys).
+    harmonic: None or dict(wave, shape, direction, amplitude, wavelength[,
+    halfwidth]) with the HARMONIC_WAVE names (e.g. wave="WAVE_ENTROPY",
+    shape="WAVEFORM_HOMOGENEOUS", direction="X")."""
     cfg = _Config()
+    if harmonic:
+        cfg.harmonic_wave = WAVES[harmonic["wave"]]
+        cfg.harmonic_shape = WAVEFORMS.get(harmonic.get("shape", ""), 0)
+        cfg.harmonic_direction = "XYZ".index(str(harmonic.get("direction", "X")).upper())
+        cfg.harmonic_amplitude = float(harmonic.get("amplitude", 0.0))
+        cfg.harmonic_wavelength = float(harmonic.get("wavelength", 1.0))
+        cfg.harmonic_halfwidth = float(harmonic.get("halfwidth", 1.0))
     cfg.num_dim = num_dim
     cfg.gamma = gamma
     cfg.fd_order = fd_order
@@ -250,7 +274,7 @@ class Solver:
     def __init__(self, blocks, *, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4,
                  mode="FIX_CFL", value=0.5, sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49",
                  source_params=(), precision=FP64, path=PATH_FUSED, boundaries=(), overset=None, num_dim=2,
-                 filter_strength=1.0, axisym=0):
+                 filter_strength=1.0, axisym=0, harmonic=None):
         L = library()
         self.L = L
         self.num_dim = num_dim
@@ -259,7 +283,7 @@ class Solver:
                                 value=value,
                                 sponge_order=sponge_order, sponge_constant=sponge_constant,
                                 simulation=simulation, source_params=source_params, precision=precision, path=path,
-                                num_dim=num_dim, axisym=axisym)
+                                num_dim=num_dim, axisym=axisym, harmonic=harmonic)
         self._keep = []
         barr = self._blocks(blocks, boundaries)
         ovp = None
@@ -332,7 +356,7 @@ class Solver:
                 mode=case["mode"], value=value, sponge_order=case["sponge_order"],
                 sponge_constant=case["sponge_constant"], simulation=case["simulation"],
                 source_params=case.get("source_params", ()), precision=precision, path=path,
-                boundaries=case["bc"])
+                boundaries=case["bc"], harmonic=case.get("harmonic"))
         s.set_solution(case["ic"])
         return s
 
@@ -535,6 +559,13 @@ class Solver:
         out = np.empty((NV,) + self.shapes[ib])
         self._check(self.L.lee_get_state_gradient(self.h, ib, _dp(out)))
         return out
+
+    def update_boundary_adjoint(self, ys):
+        """B^T of update_boundary on every block (ys: one 5-field array per block)."""
+        arrs = [np.array(y, np.float64, copy=True, order="C") for y in ys]
+        ptrs = (ctypes.POINTER(ctypes.c_double) * len(arrs))(*[_dp(a) for a in arrs])
+        self._check(self.L.lee_update_boundary_adjoint(self.h, ptrs))
+        return arrs
 
     def compute_rhs_adjoint(self, kappa, ib=0):
         k = np.ascontiguousarray(np.asarray(kappa, np.float64))

@@ -43,7 +43,9 @@ extern "C" {
                              bit-identical arithmetic order */
 
 /* boundary / buffer-zone types: src/core/param.h:104-125 numbering */
+#define LEE_BC_DIRICHLET 0                   /* rejected: the reference exits for LEE (state.cpp:746-749) */
 #define LEE_BC_DIRICHLET_ALLZERO 1
+#define LEE_BC_DIRICHLET_HARMONICWAVE 2      /* the HARMONIC_WAVE state (boundary_condition.cpp:232-344) */
 #define LEE_BC_NEUMANN 100
 #define LEE_BC_WALL_SLIP_KINEMATIC 201
 #define LEE_BC_CENTERLINE_CART_NORM2X 900
@@ -51,6 +53,16 @@ extern "C" {
 #define LEE_BC_CENTERLINE_CART_NORM2Z 902
 #define LEE_BC_CENTERLINE_AXISYM 903         /* the axis r = 0 of an axisymmetric run */
 #define LEE_SPONGE_FREUND_AMBIENT 1000
+#define LEE_SPONGE_FREUND_DIRICHLET 1001     /* rejected: the reference exits for LEE (state.cpp:746-749) */
+#define LEE_SPONGE_FREUND_HARMONICWAVE 1002  /* relaxes to the HARMONIC_WAVE state (source.cpp:264-320) */
+
+/* HARMONIC_WAVE settings (input.cpp:219-231): wave type and shape */
+#define LEE_WAVE_NONE 0
+#define LEE_WAVE_ACOUSTIC 1        /* refused like the reference does ("Unknown wave.", input.cpp:636-648) */
+#define LEE_WAVE_ENTROPY 2
+#define LEE_WAVEFORM_PLANE 1
+#define LEE_WAVEFORM_HOMOGENEOUS 2
+#define LEE_WAVEFORM_GAUSSIAN_ROUNDJET 3
 
 /* physical sources (SIMULATION key; src/solver/source.cpp:22-107) */
 #define LEE_SOURCE_NONE 0         /* CASE_TANNA_TPN49 and other source-free cases */
@@ -85,6 +97,14 @@ typedef struct {
                               standard filter, which runs at strength 1) */
   int axisym;              /* AXISYMMETRY TRUE (axis x, r = y; 2-D): the pressure
                               equation gains -gamma (pbar u_r' + p' ubar_r) / r */
+  int harmonic_wave;       /* HARMONIC_WAVE: LEE_WAVE_*; the state that
+                              dirichlet_harmonicwave faces impose and
+                              sponge_freund_harmonicwave zones relax to */
+  int harmonic_shape;      /* HARMONIC_WAVE SHAPE: LEE_WAVEFORM_* */
+  int harmonic_direction;  /* HARMONIC_WAVE DIRECTION: 0 x, 1 y, 2 z */
+  double harmonic_amplitude;   /* AMPLITUDE, in [0, 1] */
+  double harmonic_wavelength;  /* WAVELENGTH */
+  double harmonic_halfwidth;   /* HALFWIDTH (WAVEFORM_GAUSSIAN_ROUNDJET) */
 } lee_config;
 
 /* One BC-file region (src/io/io.cpp:1158-1268) with negative indices
@@ -164,7 +184,8 @@ int lee_apply_filter(lee_handle *h, int iblock, double *y);
 /* StandardCentral::take_derivative_xyz (spatial:1802-1826) of one field. */
 int lee_take_derivative_xyz(lee_handle *h, int iblock, const double *f, int dir_xyz, double *out);
 /* spatial::update_boundary (spatial:2162-2187): BCs then overset
- * interpolation, in place on all blocks (ys[b] = 5n doubles of block b). */
+ * interpolation, in place on all blocks (ys[b] = 5n doubles of block b);
+ * time-dependent boundary data (a harmonic wave) at the handle's time_sol. */
 int lee_update_boundary(lee_handle *h, double *const *ys);
 
 /* lee_update_boundary without any allocation.  No counterpart in the
@@ -251,8 +272,10 @@ int lee_set_batch_launch(lee_handle *h, int use_graph, int grid_stage, int grid_
 int lee_get_batch_launch(lee_handle *h, int *use_graph, int *grid_stage, int *grid_filter);
 
 /* Discrete adjoint (no reference counterpart: the reference has no
- * adjoint).  Fused path, 2-D blocks with sponges and sources; no other
- * boundary conditions and no overset.
+ * adjoint).  Fused path; 2-D blocks with sponges, sources, holes, overset
+ * interpolation and the dirichlet_allzero, Neumann, wall-slip and Cartesian
+ * centerline faces; 3-D blocks with sponges and sources.  Not covered:
+ * axisymmetric runs and harmonic-wave boundaries.
  *
  * Source field of a LEE_SOURCE_FIELD block: p-equation source q(x) A sin(omega t). */
 int lee_set_source_field(lee_handle *h, int iblock, const double *q);
@@ -280,6 +303,10 @@ int lee_get_state_gradient(lee_handle *h, int iblock, double *grad_y0);
  * lee_compute_rhs (without its source term) and lee_apply_filter:
  * out = L^T kappa with the aux density of the argument, and y <- F^T y. */
 int lee_compute_rhs_adjoint(lee_handle *h, int iblock, const double *kappa, double *out);
+/* Transposed boundary update, for dot-product checks against
+ * lee_update_boundary: ys <- B^T ys on every block (overset interpolation
+ * transposed, then each block's BC faces in reverse order). */
+int lee_update_boundary_adjoint(lee_handle *h, double *const *ys);
 int lee_apply_filter_adjoint(lee_handle *h, int iblock, double *y);
 
 /* Library identification: returns a static string (build info). */

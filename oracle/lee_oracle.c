@@ -42,7 +42,12 @@
 #define BC_WALL_SLIP 201
 #define BC_CL_X 900
 #define BC_CL_Z 902
+#define BC_DIRICHLET_HARMONIC 2
 #define SPONGE_AMBIENT 1000
+#define SPONGE_HARMONIC 1002
+#define WAVE_ENTROPY 2
+#define WAVEFORM_HOMOGENEOUS 2
+#define WAVEFORM_GAUSSIAN_ROUNDJET 3
 
 #define SRC_NONE 0
 #define SRC_TWOCYL 1 /* CASE_SCATTERING_TWOCYLINDER, source.cpp:60-92 */
@@ -57,7 +62,8 @@ static char g_err[512];
  *  5 mean-gradient terms dropped             6 filter sweeps eta before xi
  *  7 right hole closures after any non-hole cell (`!=` for `/=`)
  *  8 monopole with cos(omega t) for sin(omega t)
- *  9 monopole Gaussian exp(-|x-x0|^2/w^2), the ln 2 of the half-width dropped */
+ *  9 monopole Gaussian exp(-|x-x0|^2/w^2), the ln 2 of the half-width dropped
+ * 10 entropy wave with cos(k c0 t) for sin(k c0 t) */
 static int g_plant = 0;
 void orc_set_plant(int code) { g_plant = code; }
 static void set_err(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); }
@@ -121,6 +127,8 @@ typedef struct {
   double src[8];
   int do_overset;
   int axisym;
+  int hw_wave, hw_shape, hw_dir;      /* HARMONIC_WAVE (input.cpp:219-231) */
+  double hw_amp, hw_wavelength, hw_halfwidth;
   ovs_t ov;
   double *dflux, *dflux_sum, *flux, *func0, *func; /* scratch of max n */
   int nmax;
@@ -592,11 +600,35 @@ static void physical_source(orc_t *o, blk_t *b, double **rhs, double time) {
   }
 }
 
-/* apply_bufferZone_Freund_ambient, source.cpp:174-201 (sol_ref = 0) */
-static void sponges(orc_t *o, blk_t *b, double **y, double **rhs) {
+/* bc_dirichlet_harmonicwave, boundary_condition.cpp:232-344, for LEE and
+ * the one wave type that passes the reference's deck check, WAVE_ENTROPY
+ * (HOMOGENEOUS or GAUSSIAN_ROUNDJET): data = {s', 0, 0, 0, 0} at a point
+ * that is not a hole, all 0 at a hole. */
+static void harmonic_data(const orc_t *o, const blk_t *b, int l0, double time, double *data) {
+  static const int dir_other[3][2] = {{1, 2}, {0, 2}, {0, 1}}; /* param.h:133 */
+  double rho_0 = 1.0, c_0 = 1.0;
+  double wavenumber = o->twopi / o->hw_wavelength;
+  for (int v = 0; v < NV; v++) data[v] = 0.0;
+  if (b->iblank[l0] == BLANKED) return;
+  double t1 = b->xyz[dir_other[o->hw_dir][0]][l0], t2 = b->xyz[dir_other[o->hw_dir][1]][l0];
+  double pressure = 0.0, velocity = 0.0;
+  double entropy = o->hw_amp * (g_plant == 10 ? 1.0 + cos(wavenumber * c_0 * time)
+                                              : 1.0 + sin(wavenumber * c_0 * time));
+  if (o->hw_shape == WAVEFORM_GAUSSIAN_ROUNDJET) {
+    double fac_Gaussian = -log(2.0) / pow(o->hw_halfwidth, 2);
+    entropy *= exp(fac_Gaussian * (pow(t1, 2) + pow(t2, 2)));
+  }
+  data[IS_] = entropy;
+  data[IUX + o->hw_dir] = velocity / (rho_0 * c_0);
+  data[IP] = pressure;
+}
+
+/* apply_bufferZone_Freund_ambient, source.cpp:174-201 (sol_ref = 0), and
+ * apply_bufferZone_Freund_harmonicWave, source.cpp:264-320 */
+static void sponges(orc_t *o, blk_t *b, double **y, double **rhs, double time) {
   for (int z = 0; z < b->nbnd; z++) {
     bnd_t *zb = &b->bnd[z];
-    if (zb->type != SPONGE_AMBIENT) continue;
+    if (zb->type != SPONGE_AMBIENT && zb->type != SPONGE_HARMONIC) continue;
     int nz0 = zb->ie[0] - zb->is[0] + 1, nz1 = zb->ie[1] - zb->is[1] + 1;
     for (int k = zb->is[2]; k <= zb->ie[2]; k++)
       for (int j = zb->is[1]; j <= zb->ie[1]; j++)
@@ -604,8 +636,10 @@ static void sponges(orc_t *o, blk_t *b, double **y, double **rhs) {
           int lb = ((k - zb->is[2]) * nz1 + (j - zb->is[1])) * nz0 + (i - zb->is[0]);
           int l0 = idx(b, i, j, k);
           double s = zb->sigma[lb];
+          double ref[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+          if (zb->type == SPONGE_HARMONIC) harmonic_data(o, b, l0, time, ref);
           for (int v = 0; v < NV; v++) {
-            double relax = -s * (y[v][l0] - 0.0);
+            double relax = -s * (y[v][l0] - ref[v]);
             rhs[v][l0] += relax;
           }
         }
@@ -618,14 +652,14 @@ static int compute_rhs_all(orc_t *o, double ***y, double ***rhs, double time) {
     blk_t *b = &o->b[ib];
     if (rhs_lee(o, b, y[ib], rhs[ib])) return -1;
     physical_source(o, b, rhs[ib], time);
-    sponges(o, b, y[ib], rhs[ib]);
+    sponges(o, b, y[ib], rhs[ib], time);
   }
   return 0;
 }
 
 /* bc::apply_BC, boundary_condition.cpp:46-209 with bc_neumann :348-444,
  * bc_wall_slip_kinematic :448-494, bc_dirichlet_allzero, bc_centerline */
-static void apply_bc(orc_t *o, blk_t *b, double **y) {
+static void apply_bc(orc_t *o, blk_t *b, double **y, double time) {
   for (int ib = 0; ib < b->nbnd; ib++) {
     bnd_t *bc = &b->bnd[ib];
     if (bc->type >= SPONGE_AMBIENT) continue;
@@ -647,6 +681,7 @@ static void apply_bc(orc_t *o, blk_t *b, double **y) {
           if (bc->type == BC_DIRICHLET_ALLZERO) {
             for (int v = 0; v < NV; v++) data[v] = 0.0;
           }
+          if (bc->type == BC_DIRICHLET_HARMONIC) harmonic_data(o, b, l0, time, data);
           if (neumann) {
             const op_t *op = &o->fd;
             for (int q = 0; q < nvu; q++) {
@@ -708,8 +743,8 @@ static void overset_interp(orc_t *o, double ***y) {
 }
 
 /* spatial::update_boundary, spatial:2162-2187 */
-static void update_boundary(orc_t *o, double ***y) {
-  for (int ib = 0; ib < o->nblk; ib++) apply_bc(o, &o->b[ib], y[ib]);
+static void update_boundary(orc_t *o, double ***y, double time) {
+  for (int ib = 0; ib < o->nblk; ib++) apply_bc(o, &o->b[ib], y[ib], time);
   if (o->do_overset) overset_interp(o, y);
 }
 
@@ -758,12 +793,12 @@ static int rk_integrate(orc_t *o, double tn, double h) {
       }
     }
     int nx = st + 1 < last ? st + 1 : last;
-    (void)nx; /* boundary time tn + c_{min(i+1,last)} h: no BC here depends on time */
+    double tb = tn + o->c_i[nx] * h; /* boundary time tn + c_{min(i+1,last)} h (temporal:236-246) */
     if (st < last) {
-      update_boundary(o, yint);
+      update_boundary(o, yint, tb);
       for (int ib = 0; ib < o->nblk; ib++) compute_aux(o, &o->b[ib], o->b[ib].interim);
     } else {
-      update_boundary(o, ynext);
+      update_boundary(o, ynext, tb);
       for (int ib = 0; ib < o->nblk; ib++) compute_aux(o, &o->b[ib], o->b[ib].ynp1);
     }
   }
@@ -920,6 +955,14 @@ void orc_set_source(void *h, int kind, const double *params) {
   if (params) memcpy(o->src, params, sizeof o->src);
 }
 
+/* HARMONIC_WAVE type, SHAPE, DIRECTION (0 x, 1 y, 2 z), AMPLITUDE,
+ * WAVELENGTH, HALFWIDTH */
+void orc_set_harmonic(void *h, int wave, int shape, int dir, double amp, double wavelength, double halfwidth) {
+  orc_t *o = h;
+  o->hw_wave = wave; o->hw_shape = shape; o->hw_dir = dir;
+  o->hw_amp = amp; o->hw_wavelength = wavelength; o->hw_halfwidth = halfwidth;
+}
+
 void orc_set_timestep(void *h, int fix_cfl, double value) {
   orc_t *o = h;
   o->fix_cfl = fix_cfl;
@@ -974,7 +1017,7 @@ int orc_init(void *h, const double *sol0_all, int step0, double time0) {
     off += (size_t)NV * b->n;
     compute_aux(o, b, b->sol);
     for (int z = 0; z < b->nbnd; z++)
-      if (b->bnd[z].type == SPONGE_AMBIENT) compute_sigma(o, b, &b->bnd[z]);
+      if (b->bnd[z].type == SPONGE_AMBIENT || b->bnd[z].type == SPONGE_HARMONIC) compute_sigma(o, b, &b->bnd[z]);
     /* metrics are computed with all points fluid: stash iblank */
     int *ibk = b->iblank;
     int *zero = calloc(b->n, sizeof(int));
@@ -1076,7 +1119,7 @@ void orc_get_sigma(void *h, int ib, double *sig) {
   for (int p = 0; p < b->n; p++) sig[p] = 0.0;
   for (int z = 0; z < b->nbnd; z++) {
     bnd_t *zb = &b->bnd[z];
-    if (zb->type != SPONGE_AMBIENT || !zb->sigma) continue;
+    if ((zb->type != SPONGE_AMBIENT && zb->type != SPONGE_HARMONIC) || !zb->sigma) continue;
     int nz0 = zb->ie[0] - zb->is[0] + 1, nz1 = zb->ie[1] - zb->is[1] + 1;
     for (int k = zb->is[2]; k <= zb->ie[2]; k++)
       for (int j = zb->is[1]; j <= zb->ie[1]; j++)
@@ -1094,7 +1137,7 @@ int orc_rhs(void *h, int ib, const double *y, double *rhs, double time, int with
   if (with_aux_from_y) compute_aux(o, b, yy);
   if (rhs_lee(o, b, yy, rr)) return -1;
   physical_source(o, b, rr, time);
-  sponges(o, b, yy, rr);
+  sponges(o, b, yy, rr, time);
   return 0;
 }
 

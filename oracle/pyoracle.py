@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "liblee_oracle.so")
 
 BC_TYPES = {
     "dirichlet_allzero": 1,
+    "dirichlet_harmonicwave": 2,
     "neumann": 100,
     "wall_slip_kinematic": 201,
     "centerline_cart_norm2x": 900,
@@ -20,7 +21,10 @@ BC_TYPES = {
     "centerline_cart_norm2z": 902,
     "centerline_axisym": 903,
     "sponge_freund_ambient": 1000,
+    "sponge_freund_harmonicwave": 1002,
 }
+WAVES = {"NONE": 0, "WAVE_ACOUSTIC": 1, "WAVE_ENTROPY": 2}
+WAVEFORMS = {"WAVEFORM_PLANE": 1, "WAVEFORM_HOMOGENEOUS": 2, "WAVEFORM_GAUSSIAN_ROUNDJET": 3}
 SRC = {"CASE_TANNA_TPN49": 0, "CASE_SCATTERING_TWOCYLINDER": 1,
        "CASE_2DJET_WITH_A_HARMONIC_SOURCE": 2, "CASE_MONOPOLE": 3}
 
@@ -41,6 +45,7 @@ def lib():
         L.orc_set_source.argtypes = [vp, i, dp]
         L.orc_set_axisym.argtypes = [vp, i]
         L.orc_set_timestep.argtypes = [vp, i, d]
+        L.orc_set_harmonic.argtypes = [vp, i, i, i, d, d, d]
         L.orc_set_overset.argtypes = [vp, i, ip, ip, ip, ip, ip, dp, i, i, ip, ip]
         L.orc_init.argtypes = [vp, dp, i, d]
         L.orc_step.argtypes = [vp]
@@ -99,7 +104,7 @@ class Oracle:
     """One multi-block LEE problem on the CPU, FP64, reference semantics."""
 
     def __init__(self, blocks, scheme, bc_lines=(), source=None, mode="FIX_CFL", value=0.5,
-                 overset=None, ndim=2, gamma=1.4):
+                 overset=None, ndim=2, gamma=1.4, harmonic=None):
         """blocks: list of dict(x, y[, z], mean (5,...), ic (5,...)) arrays shaped (nk?, nj, ni)
         scheme: dict(fd_order, filter_order, filter_blend, rk, sponge_order, sponge_constant
         [, filter_strength, axisym]); order 11 selects FDO11P / SFO11P
@@ -138,6 +143,11 @@ class Oracle:
             p[:len(params)] = params
             L.orc_set_source(self.h, SRC[kind], _dp(p))
         L.orc_set_timestep(self.h, 1 if mode == "FIX_CFL" else 0, float(value))
+        if harmonic:
+            L.orc_set_harmonic(self.h, WAVES[harmonic["wave"]], WAVEFORMS.get(harmonic.get("shape", ""), 0),
+                               "XYZ".index(str(harmonic.get("direction", "X")).upper()),
+                               float(harmonic.get("amplitude", 0.0)), float(harmonic.get("wavelength", 1.0)),
+                               float(harmonic.get("halfwidth", 1.0)))
         if overset is not None:
             self._set_overset(overset)
         ic = np.concatenate([np.asarray(b["ic"], np.float64).reshape(-1) for b in blocks])
@@ -168,7 +178,7 @@ class Oracle:
                                        "sponge_constant")}
         scheme["filter_strength"] = case.get("filter_strength", 1.0)
         scheme["axisym"] = case.get("axisym", 0)
-        return cls(blocks, scheme, case["bc"], src, case["mode"], value, **kw)
+        return cls(blocks, scheme, case["bc"], src, case["mode"], value, harmonic=case.get("harmonic"), **kw)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -282,12 +282,66 @@ def case_centerline():
                 simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
 
 
+def case_entropy():
+    """An entropy wave (HARMONIC_WAVE = WAVE_ENTROPY, WAVEFORM_HOMOGENEOUS,
+    amplitude 0.01, wavelength 6) imposed by dirichlet_harmonicwave on the
+    +xi face of an 81x57 box and relaxed to by a 10-point
+    sponge_freund_harmonicwave zone behind it, convected by a mean flow
+    u = 0.4; ambient sponges on the other three faces, a pressure pulse;
+    RK4 CFL 0.5, 50 steps."""
+    ni, nj = 81, 57
+    x, y = _grid(ni, nj, lambda i, j: -10.0 + 0.25 * i, lambda i, j: -7.0 + 0.25 * j)
+    ic = np.zeros((5, nj, ni))
+    ic[4] = _field(x, y, _pulse(1e-3, 2.0, 1.0, 1.0))
+    bc = [(0, "dirichlet_harmonicwave", "+xi", "0", "0", "0", "-0", "0", "0"),
+          (0, "sponge_freund_harmonicwave", "+xi", "0", "9", "0", "-0", "0", "0")]
+    bc += _sponges(10, sides=("-xi", "+eta", "-eta"))
+    harmonic = dict(wave="WAVE_ENTROPY", shape="WAVEFORM_HOMOGENEOUS", direction="X", amplitude=0.01,
+                    wavelength=6.0)
+    return dict(name="entropy", ni=ni, nj=nj, x=x, y=y, mean=_uniform_mean(x, u=0.4), ic=ic,
+                bc=bc, mode="FIX_CFL", cfl=0.5, dt=None, nsteps=50, harmonic=harmonic,
+                simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
+
+
+def case_roundjet():
+    """A round-jet entropy wave (WAVE_ENTROPY, WAVEFORM_GAUSSIAN_ROUNDJET,
+    half-width 1.5, amplitude 0.02, wavelength 5, direction x) through the
+    +xi face of a wavy 89x65 grid with a jet-like mean flow
+    u = 0.3 (1 - tanh(|y| - 2)) / 2: dirichlet_harmonicwave on the face and a
+    12-point sponge_freund_harmonicwave zone, ambient sponges elsewhere;
+    RK4 CFL 0.5, 50 steps."""
+    ni, nj = 89, 65
+    x, y = _wavy(ni, nj, 22.0, 16.0, 0.02)
+    mean = _uniform_mean(x)
+    mean[1] = _field(x, y, lambda a, b: 0.15 * (1.0 - math.tanh(abs(b) - 2.0)))
+    ic = np.zeros((5, nj, ni))
+    bc = [(0, "sponge_freund_harmonicwave", "+xi", "0", "11", "0", "-0", "0", "0"),
+          (0, "dirichlet_harmonicwave", "+xi", "0", "0", "0", "-0", "0", "0")]
+    bc += _sponges(10, sides=("-xi", "+eta", "-eta"))
+    harmonic = dict(wave="WAVE_ENTROPY", shape="WAVEFORM_GAUSSIAN_ROUNDJET", direction="X", amplitude=0.02,
+                    wavelength=5.0, halfwidth=1.5)
+    return dict(name="roundjet", ni=ni, nj=nj, x=x, y=y, mean=mean, ic=ic,
+                bc=bc, mode="FIX_CFL", cfl=0.5, dt=None, nsteps=50, harmonic=harmonic,
+                simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
+
+
 ALL = {f.__name__[5:]: f for f in (case_c1, case_curv, case_filt, case_shear, case_cylsrc, case_wall,
                                     case_variants, case_jet, case_opt11, case_axisym, case_dirichlet,
-                                    case_centerline)}
+                                    case_centerline, case_entropy, case_roundjet)}
 
 # cases whose schemes run on the plain path only (FDO11P / SFO11P)
 PLAIN_ONLY = {"opt11"}
+
+
+def _harmonic_deck(h):
+    """HARMONIC_WAVE deck value (input.cpp:219-231)."""
+    if not h:
+        return "NONE"
+    out = f"{h['wave']}, AMPLITUDE = {h['amplitude']!r}, SHAPE = {h['shape']}, DIRECTION = {h['direction']}, " \
+          f"WAVELENGTH = {h['wavelength']!r}"
+    if "halfwidth" in h:
+        out += f", HALFWIDTH = {h['halfwidth']!r}"
+    return out
 
 
 def deck_items(case, grid_file="grid.xyz", mean_file="mean.q", bc_file="bc.in"):
@@ -330,7 +384,7 @@ def deck_items(case, grid_file="grid.xyz", mean_file="mean.q", bc_file="bc.in"):
         ("BASESTATE_FILE", mean_file),
         ("BC_FILE", bc_file),
         ("SOLUTION_WRITING_TIME", "1.0E+06"),
-        ("HARMONIC_WAVE", "NONE"),
+        ("HARMONIC_WAVE", _harmonic_deck(case.get("harmonic"))),
         ("INTERPOLATE_SOLUTION", "NONE"),
     ]
     return items

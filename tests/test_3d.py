@@ -106,11 +106,14 @@ def test_axis_exchange_symmetry():
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-def test_3d_plane_filter_forms_bit_identical(prec):
-    """The plane pass of the 3-D filter in the stacked column-march form
-    (stacks of 4 waves of 4, 8 or 16 rows) gives the tile form's bits; the
-    130 x 80 planes have interior stacks besides edge ones."""
-    ni, nj, nk = 130, 80, 12
+@pytest.mark.parametrize("dims", [(130, 80, 12), (70, 22, 140)])
+def test_3d_plane_filter_forms_bit_identical(prec, dims):
+    """Every form of the 3-D filter gives the same bits: the two-pass form
+    with its plane pass as tiles or as stacks of 4 waves of 4, 8 or 16 rows,
+    and the single pass (tiles of 8 or 16 rows marching through chunks of
+    64 planes).  The 130 x 80 planes have interior tiles and stacks besides
+    edge ones; the 140-plane box spans three chunks of the single pass."""
+    ni, nj, nk = dims
     x, y, z = _box(ni, nj, nk)
     yc, zc = y.mean(), z.mean()
     mean = np.zeros((5, nk, nj, ni))
@@ -120,7 +123,7 @@ def test_3d_plane_filter_forms_bit_identical(prec):
     ic[4] = 1e-3 * np.exp(-((x - 9.0) ** 2 + (y - yc - 0.4) ** 2 + (z - zc + 0.3) ** 2) / 2.0)
     ic[1] = 2e-4 * np.exp(-((x - 20.0) ** 2 + (y - yc) ** 2 + (z - zc) ** 2) / 3.0)
     ref = None
-    for form in ((0, 0), (4, 4), (4, 8), (4, 16)):
+    for form in ((0, 0), (4, 4), (4, 8), (4, 16), (1, 8), (1, 16)):
         s = Solver([dict(x=x, y=y, z=z, mean=mean)], boundaries=(), precision=prec, mode="FIX_DT", value=0.04,
                    num_dim=3)
         s.set_filter_march(*form)
@@ -132,11 +135,12 @@ def test_3d_plane_filter_forms_bit_identical(prec):
             ref = q
             assert np.max(np.abs(q[4])) > 1e-5
         assert np.array_equal(q, ref), (prec, form)
-    print(f"3-D plane filter forms prec {prec}: bit-identical", flush=True)
+    print(f"3-D filter forms {dims} prec {prec}: bit-identical", flush=True)
     s = Solver([dict(x=x, y=y, z=z, mean=mean)], boundaries=(), precision=prec, mode="FIX_DT", value=0.04,
                num_dim=3)
-    with pytest.raises(LeeError):
-        s.set_filter_march(1, 8)  # the plane pass takes stacks of 4 waves only
+    for bad in ((1, 4), (8, 8)):
+        with pytest.raises(LeeError):
+            s.set_filter_march(*bad)
     s.close()
 
 

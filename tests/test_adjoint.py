@@ -67,12 +67,36 @@ def test_rhs_transpose_dot_product(name, prec, tol):
     s.close()
 
 
-def test_adjoint_rejects_other_boundaries():
-    c = cases.case_wall()
+@pytest.mark.parametrize("name", ["axisym", "entropy"])
+def test_adjoint_rejects_uncovered_terms(name):
+    """The adjoint covers sponges, sources, holes, overset interpolation and
+    the dirichlet_allzero, Neumann, wall-slip and Cartesian centerline faces;
+    the axisymmetric term and harmonic-wave boundaries are refused."""
+    c = cases.ALL[name]()
     s = Solver.from_case(c)
     with pytest.raises(LeeError):
         s.objective_forward(5, 0, 5)
     s.close()
+
+
+@pytest.mark.parametrize("name", ["wall", "dirichlet", "centerline", "opt11"])
+def test_boundary_face_transposes_dot_products(name):
+    """<B y, z> = <y, B^T z> for the BC faces of the fixtures (wall-slip and
+    Neumann on curvilinear faces, dirichlet_allzero, Cartesian centerlines);
+    the opt11 case runs its faces with the standard operator's closure row
+    here (the fused path takes OA4)."""
+    c = cases.ALL[name]()
+    kw = dict(fd_order=4, filter_order=8, filter_strength=1.0) if name == "opt11" else {}
+    c = dict(c, **kw)
+    s = Solver.from_case(c)
+    rng = np.random.default_rng(14)
+    y, z = _rand_state(c["x"].shape, rng), _rand_state(c["x"].shape, rng)
+    By = s.update_boundary([y])[0]
+    Btz = s.update_boundary_adjoint([z])[0]
+    err = abs(_dot(By, z) - _dot(y, Btz)) / abs(_dot(By, z))
+    print(f"face transposes {name}: rel {err:.2e}", flush=True)
+    s.close()
+    assert err <= 1e-14
 
 
 def _problem(ni=97, nj=81, shear=True):
@@ -276,3 +300,90 @@ def test_repeated_observer_points_add_their_weights():
     assert len(sa) == len(obs) + 2 and len(sb) == len(obs)
     assert sa[len(obs)] == sa[3] and sa[len(obs) + 1] == sa[17]
     assert np.array_equal(sa[:len(obs)], sb)
+
+
+# ----------------------------------------------------------- overset, BCs
+# The two overset fixtures: holes (per-point closure codes), interpolated
+# fringes, for the cylinder an O-grid seam interpolated from its own block,
+# a wall-slip face and a potential-flow mean.
+
+def _overset_adjoint_solver(name, prec):
+    import overset_cases
+    if name == "overset_small":
+        c = overset_cases.case_overset_small()
+        tables = overset_cases.abi_tables(c)
+    else:
+        c = overset_cases.case_cylinder_small()
+        tables = overset_cases.cylinder_abi_tables(c)
+    s = Solver([dict(x=b["x"], y=b["y"], mean=b["mean"]) for b in c["blocks"]], mode=c["mode"], value=c["cfl"],
+               precision=prec, boundaries=c["bc"], overset=tables, simulation="SOURCE_FIELD",
+               source_params=(1.0, 0.6 * math.pi))
+    return c, s
+
+
+@pytest.mark.parametrize("name", ["overset_small", "cylinder_small"])
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-13), (FP32, 1e-5)])
+def test_overset_transposes_dot_products(name, prec, tol):
+    """<B y, z> = <y, B^T z> for the boundary update (BC faces, then overset
+    interpolation, on every block), and the right-hand side and filter
+    transposes on blocks with holes."""
+    c, s = _overset_adjoint_solver(name, prec)
+    rng = np.random.default_rng(12)
+    shapes = [b["x"].shape for b in c["blocks"]]
+    ys = [_rand_state(sh, rng) for sh in shapes]
+    zs = [_rand_state(sh, rng) for sh in shapes]
+    By = s.update_boundary(ys)
+    Btz = s.update_boundary_adjoint(zs)
+    lhs = sum(_dot(By[k], zs[k]) for k in range(len(ys)))
+    rhs = sum(_dot(ys[k], Btz[k]) for k in range(len(ys)))
+    eb = abs(lhs - rhs) / abs(lhs)
+    errs = [eb]
+    for ib, sh in enumerate(shapes):
+        a, k = _rand_state(sh, rng), _rand_state(sh, rng)
+        La = s.compute_rhs(a, 0.3, ib) - s.compute_rhs(np.zeros_like(a), 0.3, ib)
+        l1, r1 = _dot(La, k), _dot(a, s.compute_rhs_adjoint(k, ib))
+        l2, r2 = _dot(s.apply_filter(a, ib), k), _dot(a, s.apply_filter_adjoint(k, ib))
+        errs += [abs(l1 - r1) / abs(l1), abs(l2 - r2) / abs(l2)]
+    print(f"{name} prec {prec}: boundary {eb:.2e}, rhs/filter per block {['%.2e' % e for e in errs[1:]]}",
+          flush=True)
+    s.close()
+    assert max(errs) <= tol
+
+
+@pytest.mark.parametrize("name,nsteps,window,every", [("overset_small", 30, 15, 10), ("cylinder_small", 40, 20, 10)])
+def test_overset_gradient_matches_central_differences(name, nsteps, window, every):
+    """Observer energy on fluid points of the background block, gradient
+    with respect to a source field on both blocks, through holes, fringes,
+    the O-grid seam and the slip wall; central differences in FP64."""
+    c, s = _overset_adjoint_solver(name, FP64)
+    rng = np.random.default_rng(13)
+    ib0 = s.iblank(0)
+    x0, y0_ = c["blocks"][0]["x"], c["blocks"][0]["y"]
+    cand = np.flatnonzero((ib0 == 0).reshape(-1))
+    pick = rng.choice(cand, 40, replace=False)
+    s.set_observer_points(np.zeros(40), pick, np.full(40, 1.0))
+    qs = [np.exp(-((b["x"] - 0.5) ** 2 + (b["y"] - 0.3) ** 2) / 0.5) * (s.iblank(k) != 1)
+          for k, b in enumerate(c["blocks"])]
+    ics = [b["ic"] for b in c["blocks"]]
+
+    def J(qq):
+        for k in range(2):
+            s.set_source_field(qq[k], k)
+            s.set_solution(ics[k], k)
+        s.set_time(0, 0.0, 1e-6)
+        return s.objective_forward(nsteps, window, every)
+
+    J0 = J(qs)
+    s.objective_backward(1.0)
+    g = [s.source_gradient(k) for k in range(2)]
+    assert J0 > 0 and all(np.any(gk != 0) for gk in g)
+    for trial in range(2):
+        e = [rng.standard_normal(q.shape) * (s.iblank(k) != 1) for k, q in enumerate(qs)]
+        eps = 1e-3
+        fd = (J([qs[k] + eps * e[k] for k in range(2)]) - J([qs[k] - eps * e[k] for k in range(2)])) / (2 * eps)
+        ad = sum(_dot(g[k], e[k]) for k in range(2))
+        err = abs(fd - ad) / abs(fd)
+        print(f"{name} gradient direction {trial}: J {J0:.6e} fd {fd:.12e} adjoint {ad:.12e} rel {err:.2e}",
+              flush=True)
+        assert err <= 1e-6
+    s.close()

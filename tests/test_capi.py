@@ -103,9 +103,37 @@ def test_create_rejects_bad_boundaries():
     rc, err, _ = _create(_cfg(), boundary=b)
     assert rc == 1 and "outside" in err
     b.ie[0] = 5
-    b.type = 0  # Dirichlet: not implemented for LEE in the reference either
+    for t in (0, 1001):  # Dirichlet face and sponge: the reference exits for LEE (state.cpp:746-749)
+        b.type = t
+        rc, err, _ = _create(_cfg(), boundary=b)
+        assert rc == 1 and "not implemented yet" in err
+    b.type = 7
     rc, err, _ = _create(_cfg(), boundary=b)
     assert rc == 1 and "unsupported boundary" in err
+
+
+def test_create_rejects_harmonic_waves_like_the_reference():
+    """The reference's deck check (input.cpp:630-655) ends every acoustic
+    wave in "Unknown wave." (its else binds to the inner if) and checks an
+    entropy wave's amplitude; bc_dirichlet_harmonicwave refuses an entropy
+    plane wave and an unset shape."""
+    b = lee._Boundary()
+    b.type, b.dir, b.end = 2, 0, 0
+    for q in range(3):
+        b.is_[q], b.ie[q] = 0, 0
+    b.ie[1] = 15
+    cases_ = [
+        (dict(harmonic_wave=1, harmonic_shape=1, harmonic_wavelength=4.0), "Unknown wave."),
+        (dict(harmonic_wave=1, harmonic_shape=3, harmonic_wavelength=4.0), "Unknown spatial distribution"),
+        (dict(harmonic_wave=2, harmonic_shape=2, harmonic_amplitude=1.5, harmonic_wavelength=4.0),
+         "between 0 and 1"),
+        (dict(harmonic_wave=2, harmonic_shape=1, harmonic_amplitude=0.1, harmonic_wavelength=4.0),
+         "Unknown type of wave for this wave form."),
+        (dict(harmonic_wave=0), "Unknown type of wave form."),
+    ]
+    for kw, msg in cases_:
+        rc, err, _ = _create(_cfg(**kw), boundary=b)
+        assert rc == 1 and msg in err, (kw, err)
 
 
 def test_bc_line_parsing_matches_reference_convention():

@@ -87,6 +87,19 @@ def test_deck_settings_map_the_case(tmp_path):
     assert fileio.Deck(_deck_for(cases.case_axisym(), tmp_path)).settings()["cfg"]["axisym"] == 1
 
 
+def test_deck_settings_harmonic_wave(tmp_path):
+    """HARMONIC_WAVE and its sub-keys (input.cpp:219-231) map onto the
+    harmonic_* settings; an acoustic wave is refused at create time the way
+    the reference's deck check refuses it."""
+    st = fileio.Deck(_deck_for(cases.case_roundjet(), tmp_path)).settings()
+    cfg = st["cfg"]
+    assert (cfg["harmonic_wave"], cfg["harmonic_shape"], cfg["harmonic_direction"]) == (2, 3, 0)
+    assert (cfg["harmonic_amplitude"], cfg["harmonic_wavelength"], cfg["harmonic_halfwidth"]) == (0.02, 5.0, 1.5)
+    st = fileio.Deck(_deck_for(cases.case_entropy(), tmp_path)).settings()
+    assert (st["cfg"]["harmonic_wave"], st["cfg"]["harmonic_shape"]) == (2, 2)
+    assert fileio.Deck(_deck_for(cases.case_c1(), tmp_path)).settings()["cfg"]["harmonic_wave"] == 0
+
+
 def test_deck_settings_optimized_schemes(tmp_path):
     st = fileio.Deck(_deck_for(cases.case_opt11(), tmp_path)).settings()
     cfg = st["cfg"]
@@ -149,7 +162,7 @@ def test_bc_file_matches_the_python_reader(tmp_path):
                      ("0 sponge_freund_ambient +chi 0 9 0 -0 0 0", "Unknown direction"),
                      ("0 sponge_freund_ambient +xi 0 9 0 -0 0 #0", "comment sign"),
                      ("5 sponge_freund_ambient +xi 0 9 0 -0 0 0", "names block 5"),
-                     ("0 dirichlet_harmonicwave +xi 0 9 0 -0 0 0", "unsupported boundary type")):
+                     ("0 characteristic_outflow +xi 0 9 0 -0 0 0", "unsupported boundary type")):
         with pytest.raises(LeeError, match=msg):
             fileio.bc_file_read(_write(tmp_path / "bad.in", bad + "\n"), dims)
 
@@ -212,8 +225,8 @@ def _case_folder(name, d):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["c1", "shear", "wall", "jet", "opt11", "axisym", "overset_small",
-                                  "cylinder_small"])
+@pytest.mark.parametrize("name", ["c1", "shear", "wall", "jet", "opt11", "axisym", "dirichlet", "centerline",
+                                  "entropy", "roundjet", "overset_small", "cylinder_small"])
 def test_run_deck_writes_the_reference_solution_file(tmp_path, name):
     """A golden case folder through lee_run_deck (plain FP64): the final
     camille.000001.q holds the reference's bits and tau; the set-up files

@@ -359,9 +359,9 @@ def test_overset_fused_matches_reference(label, prec, name):
     perturbation: an FP64 run with its initial condition perturbed at FP32
     round-off differs there by 1.6e-6 at step 40 and by 4e-8 elsewhere
     (tests/test_oracle.py, the corner-mode test).  The FP32 run is held to
-    1e-5 against FP64 over the whole field at step 30, to 1e-5 against the
-    reference at step 40 everywhere outside the 5 x 5 box around that point,
-    and to 5e-5 inside it."""
+    1e-5 against FP64 at step 30 and to 1e-5 against the reference at step
+    40, over the whole field; the 5 x 5 box around that point is reported
+    separately."""
     g = _golden(name)
     c, s = _overset_solver(prec, PATH_FUSED, name)
     corner = name == "overset_small" and label == "fused32"
@@ -385,11 +385,8 @@ def test_overset_fused_matches_reference(label, prec, name):
             d = np.abs(sol[v] - ref[v]) / scale
             if corner and ib == 0:
                 box.append(float(np.max(d[43:48, 42:47])))
-                d[43:48, 42:47] = 0.0
             errs.append(float(np.max(d)))
     print(f"{name} {label}: rel errors {['%.2e' % e for e in errs]}"
           + (f", corner box {max(box):.2e}" if box else ""), flush=True)
     assert max(errs) <= TOL[label]
-    if box:
-        assert max(box) <= 5e-5
     s.close()

@@ -61,6 +61,7 @@ def test_metrics_match_reference_file(name):
     (4, "curv"),    # metrics transposed in the chain rule
     (5, "shear"),   # mean-gradient terms dropped
     (6, "filt"),    # filter sweeps in the wrong order
+    (10, "entropy"),  # entropy wave with cos for sin
 ])
 def test_planted_mistakes_are_caught(plant, expect):
     L = lib()
