@@ -551,7 +551,9 @@ int autotune_block(lee_handle *h, Block &b) {
   };
   float best = 1e30f;
   int best_r = b.tune_r;
+  const bool wide_fd = h->cfg.fd_order == 11, wide_flt = h->cfg.filter_order == 11;
   for (int r : {1, 2, 4}) {
+    if (wide_fd && r != 1) continue;  // the 11-point stage runs on tiles of 4 rows
     b.tune_r = r;
     float ms = 0.f;
     if (int rc = time_ms(true, ms)) return rc;
@@ -572,6 +574,7 @@ int autotune_block(lee_handle *h, Block &b) {
     for (int ty : {4, 8, 12, 16}) {
       for (int pw : {0, 2, 4, 8}) {
         if (pw > 0 && b.cus <= 0) continue;
+        if (wide_flt && (ty > 8 || pw > 0)) continue;  // SFO11P: tiles of 4 or 8 rows, one workgroup each
         b.tune_fty = ty;
         b.tune_fgrid = pw * b.cus;
         float ms = 0.f;
@@ -589,7 +592,7 @@ int autotune_block(lee_handle *h, Block &b) {
   // column-march forms: single waves of 8 or 16 rows (blocks without
   // holes), stacks of four waves of 4, 8 or 16 rows or of eight of 8 rows
   int best_mc = 0, best_ml = 0;
-  if (h->cfg.filter_order > 0) {
+  if (h->cfg.filter_order > 0 && !wide_flt) {
     static const int forms[][2] = {{1, 8}, {1, 16}, {4, 4}, {4, 8}, {4, 16}, {8, 8}};
     for (const auto &fm : forms) {
       if (fm[0] > 1 || !b.has_holes) {
@@ -689,6 +692,8 @@ int adjoint_check(lee_handle *h) {
   if (h->batch) return fail(h, LEE_EINVAL, "the adjoint runs on single handles, not batches");
   if (is_plain(h)) return fail(h, LEE_EINVAL, "the adjoint runs on the fused path");
   if (h->cfg.axisym) return fail(h, LEE_EINVAL, "the adjoint does not cover the axisymmetric term");
+  if (h->cfg.fd_order == 11 || h->cfg.filter_order == 11)
+    return fail(h, LEE_EINVAL, "the adjoint covers the standard schemes (OA4 derivative, 4th- or 8th-order filter)");
   for (int ib = 0; ib < h->nblk; ib++) {
     for (const auto &f : h->b[ib].faces)
       if (f.type == LEE_BC_CENTERLINE_AXISYM || f.type == LEE_BC_DIRICHLET_HARMONICWAVE)
@@ -803,8 +808,8 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
     return fail(nullptr, LEE_EINVAL, "SPATIAL_SCHEME must be STANDARD_CENTRAL ORDER_ACCURACY 4 or FDO11P (11)");
   if (c->filter_order != 0 && c->filter_order != 4 && c->filter_order != 8 && c->filter_order != 11)
     return fail(nullptr, LEE_EINVAL, "filter order must be 0 (NONE), 4, 8 or 11 (SFO11P)");
-  if ((c->fd_order == 11 || c->filter_order == 11) && c->path != LEE_PATH_PLAIN)
-    return fail(nullptr, LEE_EINVAL, "the FDO11P and SFO11P schemes run on the plain path");
+  if ((c->fd_order == 11 || c->filter_order == 11) && c->num_dim == 3)
+    return fail(nullptr, LEE_EINVAL, "the FDO11P and SFO11P schemes run in 2-D");
   if (c->axisym != 0 && c->axisym != 1) return fail(nullptr, LEE_EINVAL, "AXISYMMETRY must be 0 or 1");
   if (c->axisym && c->num_dim != 2) return fail(nullptr, LEE_EINVAL, "Axisymmetry requires 2-D.");
   if (c->filter_order == 11 && !(c->filter_strength >= 0.0 && c->filter_strength <= 1.0))
@@ -922,6 +927,8 @@ void lee_destroy(lee_handle *h) {
   dfree(h->ov.coef);
   dfree(h->ov.coef5);
   dfree(h->ov.buf);
+  if (h->fdw) (void)hipFree(h->fdw);
+  if (h->flw) (void)hipFree(h->flw);
   for (auto e : h->evpool) (void)hipEventDestroy(e);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -959,6 +966,8 @@ int lee_batch_create(const lee_config *cfgs, const lee_block *blocks, int nsolve
       if (blocks[k].boundary[q].type == LEE_BC_DIRICHLET_HARMONICWAVE ||
           blocks[k].boundary[q].type == LEE_SPONGE_FREUND_HARMONICWAVE)
         return fail(nullptr, LEE_EINVAL, "batched solves take no harmonic-wave boundaries");
+  if (cfgs[0].fd_order == 11 || cfgs[0].filter_order == 11)
+    return fail(nullptr, LEE_EINVAL, "batched solves take the standard schemes");
   if (cfgs[0].path != LEE_PATH_FUSED) return fail(nullptr, LEE_EINVAL, "batched solves run on the fused path");
   if (cfgs[0].filter_order == 0) return fail(nullptr, LEE_EINVAL, "batched solves need the filter");
   return create_impl(cfgs, 1, blocks, nsolve, nullptr, true, out);
@@ -1002,6 +1011,17 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     make_fd_table(h->fd, cfg->fd_order);
     if (cfg->filter_order > 0) make_filter_table(h->flt, cfg->filter_order, cfg->filter_strength);
+    // the fused kernels read the 11-point rows from device memory
+    auto upload_wide = [&](const OpTable &t, WideOp **dst) -> int {
+      const WideOp w = make_wide_op(t);
+      HIPCHK(hipMalloc((void **)dst, sizeof(WideOp)));
+      HIPCHK(hipMemcpy(*dst, &w, sizeof(WideOp), hipMemcpyHostToDevice));
+      return LEE_OK;
+    };
+    if (!is_plain(h) && cfg->fd_order == 11)
+      if (int e = upload_wide(h->fd, &h->fdw)) return e;
+    if (!is_plain(h) && cfg->filter_order == 11)
+      if (int e = upload_wide(h->flt, &h->flw)) return e;
     // Butcher tableau (RungeKutta::initialize, temporal:33-85)
     if (cfg->rk_stages == 4) {
       h->b_i[0] = 1.0 / 6.0; h->b_i[1] = 1.0 / 3.0; h->b_i[2] = 1.0 / 3.0; h->b_i[3] = 1.0 / 6.0;
@@ -1489,6 +1509,8 @@ int lee_set_tile_shape(lee_handle *h, int ib, int stage_rows, int filter_rows) {
   if (stage_rows != 1 && stage_rows != 2 && stage_rows != 4) return fail(h, LEE_EINVAL, "stage rows must be 1, 2 or 4");
   if (filter_rows != 4 && filter_rows != 8 && filter_rows != 12 && filter_rows != 16)
     return fail(h, LEE_EINVAL, "filter tile height must be 4, 8, 12 or 16");
+  if (h->cfg.filter_order == 11 && filter_rows > 8)
+    return fail(h, LEE_EINVAL, "the SFO11P filter takes tiles of 4 or 8 rows");
   if (h->batch) {  // one launch covers every solve: the shape is the batch's
     HIPCHK(hipStreamSynchronize(h->stream));
     h->bp.R = stage_rows;
@@ -1508,6 +1530,8 @@ int lee_set_tile_shape(lee_handle *h, int ib, int stage_rows, int filter_rows) {
 int lee_set_filter_grid(lee_handle *h, int ib, int workgroups) {
   if (!h || ib < 0 || ib >= h->nblk || workgroups < 0) return fail(h, LEE_EINVAL, "bad argument");
   if (h->batch) return fail(h, LEE_EINVAL, "a batch launches its filter through the batch launch settings");
+  if (h->cfg.filter_order == 11 && workgroups > 0)
+    return fail(h, LEE_EINVAL, "the SFO11P filter runs one workgroup per tile");
   h->b[ib].tune_fgrid = (workgroups + 7) / 8 * 8;
   h->b[ib].tune_mc = h->b[ib].tune_ml = 0;
   h->b[ib].tuned = true;
@@ -1517,6 +1541,7 @@ int lee_set_filter_grid(lee_handle *h, int ib, int workgroups) {
 int lee_set_filter_march(lee_handle *h, int ib, int waves, int rows) {
   if (!h || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
   const bool off = waves == 0 && rows == 0;
+  if (h->cfg.filter_order == 11 && !off) return fail(h, LEE_EINVAL, "the SFO11P filter runs in the tile form");
   if (h->batch) {  // one launch covers every solve: stacks of four waves only
     if (!off && !(waves == 4 && (rows == 4 || rows == 8 || rows == 16)))
       return fail(h, LEE_EINVAL, "a batch takes stacks of 4 waves of 4, 8 or 16 rows, or (0, 0)");

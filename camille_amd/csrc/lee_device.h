@@ -30,6 +30,19 @@ __device__ __forceinline__ int default_code(int c, int len, int nb) {
 constexpr int ST_TX = 64, ST_H = 2;   // stage tile width and FD halo
 constexpr int FL_TX = 64, FL_H = 4;   // filter tile width and halo (height: autotuned)
 
+// The optimized 11-point schemes (FDO11P derivative, SFO11P filter) in the
+// fused kernels: an OpTable in device memory, each closure row with its own
+// length.  Their tiles take a halo of 6: the interior rows reach 5 points,
+// FDO11P's closure rows 0 and 1 (anchored at the boundary, 7 points) reach
+// 6 from the point, which a closure next to a hole inside a tile needs.
+constexpr int WIDE_H = 6;
+
+// Halo and closure-row count of the stage's derivative operator: OA4 (W 4)
+// or FDO11P (W 11).
+template <int W> struct StageOp;
+template <> struct StageOp<4> { static constexpr int H = ST_H, NB = 2; };
+template <> struct StageOp<11> { static constexpr int H = WIDE_H, NB = 5; };
+
 // OA4 finite-difference rows of StandardCentral::initialize (spatial:64-105)
 // as compile-time constants: interior {1/12, -8/12, 0, 8/12, -1/12}; left
 // rows 0 {-3/2, 2, -1/2} (anchored at the point) and 1 {-1/2, 0, 1/2}
@@ -47,6 +60,30 @@ __device__ __forceinline__ bool box_clean(const int *S, int ni, int nj, int i0, 
 // that their instances (interior tiles, edge tiles, batched) round alike.
 __device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// One row of a WideOp at the point s (stride st along the sweep): the
+// interior row, zero (hole after hole), a left closure row anchored k
+// points to the left, or a right closure row mirrored with sign_right.
+template <typename T>
+__device__ __forceinline__ T wide_row(const T *s, int st, int code, const WideOp *op) {
+#pragma clang fp contract(off)
+  if (code == CODE_INTERIOR) {
+    T acc = (T)op->st[0] * s[-5 * st];
+#pragma unroll
+    for (int k = 1; k < 11; k++) acc = fmx((T)op->st[k], s[(k - 5) * st], acc);
+    return acc;
+  }
+  if (code == CODE_ZERO) return (T)0;
+  int k = code - CODE_LEFT;
+  T acc = 0;
+  if (k < op->nb) {
+    for (int q = 0; q < op->len[k]; q++) acc = fmx((T)op->bnd[k][q], s[(q - k) * st], acc);
+    return acc;
+  }
+  k -= op->nb;
+  for (int q = 0; q < op->len[k]; q++) acc = fmx((T)op->bnd[k][q], s[(k - q) * st], acc);
+  return (T)op->sign_right * acc;
+}
 
 template <typename T>
 __device__ __forceinline__ T fd_row(const T *s, int st, int code) {
@@ -118,6 +155,7 @@ struct StageArgs {
   double s1, s2;
   double he;           // harmonic-wave entropy factor at the stage time
   T h, b, a;
+  const WideOp *fdw;   // FDO11P rows (the 11-point stage instances only)
 };
 
 // Maps the launch index to a tile so that each XCD (dispatch is round-robin
@@ -138,6 +176,8 @@ struct StepT {       // per-stage scalars: dt, b_i, a_{i+1,i}, source time facto
 
 template <typename T, int R>
 using StageLds = T[4][4 * R + 2 * ST_H][ST_TX + 2 * ST_H];
+template <typename T, int R, int W>
+using StageLdsW = T[4][4 * R + 2 * StageOp<W>::H][ST_TX + 2 * StageOp<W>::H];
 
 
 template <typename T>
@@ -150,6 +190,7 @@ struct FilterArgs {
   int vars[5];       // variables of this launch
   size_t vs;         // variable stride (0: ni nj)
   T rem, blend;
+  const WideOp *flw; // SFO11P rows (the 11-point filter instances only)
 };
 
 // Filter rows with the reference's coefficients as compile-time constants
@@ -226,11 +267,22 @@ __device__ __forceinline__ T filter_blend(T rem, T u, T blend, T f) {
   return fmx(rem, u, blend * f);
 }
 
-template <typename T, int NVAR, int TY>
+template <typename T, int NVAR, int TY, int H = FL_H>
 union alignas(32) FilterLds {  // the xi-sweep overwrites its input
-  T in[NVAR][TY + 2 * FL_H][FL_TX + 2 * FL_H];
-  T mid[NVAR][TY + 2 * FL_H][FL_TX];
+  T in[NVAR][TY + 2 * H][FL_TX + 2 * H];
+  T mid[NVAR][TY + 2 * H][FL_TX];
 };
+
+// Halo of the filter tiles: 4 for the standard filters, WIDE_H for SFO11P.
+template <int ORDER>
+constexpr int filter_halo() { return ORDER == 11 ? WIDE_H : FL_H; }
+
+// A filter row of ORDER 4 or 8 (compile-time rows) or of SFO11P (a.flw).
+template <typename T, int ORDER, typename AT>
+__device__ __forceinline__ T filter_row_of(const AT &a, const T *s, int st, int code) {
+  if constexpr (ORDER == 11) return wide_row<T>(s, st, code, a.flw);
+  else return filter_row<T, ORDER>(s, st, code);
+}
 
 // Four consecutive LDS values as one vector access (rows of the filter
 // tiles are padded to multiples of four values).
@@ -248,32 +300,32 @@ using Vec4 = T __attribute__((ext_vector_type(4)));
 // Global loads of one filter tile (tile plus its 4-point halo) into the
 // thread's registers: region rows ty + 4k, columns tx and, for tx < 8,
 // 64 + tx.
-template <typename T, int NVAR, int TY>
+template <typename T, int NVAR, int TY, int H = FL_H>
 struct FilterRegs {
-  T ra[NVAR][(TY + 2 * FL_H) / 4], rb[NVAR][(TY + 2 * FL_H) / 4];
+  T ra[NVAR][(TY + 2 * H) / 4], rb[NVAR][(TY + 2 * H) / 4];
 };
 
-template <typename T, bool HOLES, int NVAR, int TY, typename AT>
-__device__ __forceinline__ void filter_load(const AT &a, int tile, FilterRegs<T, NVAR, TY> &R) {
-  constexpr int NR = TY + 2 * FL_H, KR = NR / 4;
-  static_assert(NR == 4 * KR && FL_H == 4, "tile rows and the halo come in fours");
+template <typename T, bool HOLES, int NVAR, int TY, int H = FL_H, typename AT>
+__device__ __forceinline__ void filter_load(const AT &a, int tile, FilterRegs<T, NVAR, TY, H> &R) {
+  constexpr int NR = TY + 2 * H, KR = NR / 4;
+  static_assert(NR == 4 * KR && H % 2 == 0, "tile rows and the halo come in fours");
   auto &ra = R.ra;
   auto &rb = R.rb;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
   const size_t vs = a.vs ? a.vs : (size_t)n;
-  const bool xtra = tx < 2 * FL_H;
+  const bool xtra = tx < 2 * H;
   // one straight-line sequence of loads for every tile, from clamped
   // addresses (filter_compute puts zeros outside the block): the wait
   // counters then let a caller keep the next tile's loads in flight while it
   // filters the current one.  Lanes without a right-halo column repeat
   // their own column.
-  const int gia = i0 - FL_H + tx, gib = xtra ? i0 - FL_H + FL_TX + tx : gia;
+  const int gia = i0 - H + tx, gib = xtra ? i0 - H + FL_TX + tx : gia;
   const int cia = min(max(gia, 0), ni - 1), cib = min(max(gib, 0), ni - 1);
 #pragma unroll
   for (int k = 0; k < KR; k++) {
-    const int gj = j0 - FL_H + ty + 4 * k;
+    const int gj = j0 - H + ty + 4 * k;
     const int cj = min(max(gj, 0), nj - 1);
     const int pa = cj * ni + cia, pb = cj * ni + cib;
 #pragma unroll
@@ -289,10 +341,13 @@ __device__ __forceinline__ void filter_load(const AT &a, int tile, FilterRegs<T,
 // INNER: the tile and its halo lie inside a block without holes, so every
 // point takes the interior stencil and no bounds are tested.
 template <typename T, bool HOLES, int ORDER, int NVAR, int TY, bool INNER, typename AT>
-__device__ __forceinline__ void filter_compute_k(const AT &a, int tile, FilterLds<T, NVAR, TY> &L,
-                                                 const FilterRegs<T, NVAR, TY> &R) {
-  constexpr int NR = TY + 2 * FL_H, KR = NR / 4;
-  constexpr int NB = ORDER / 2;
+__device__ __forceinline__ void filter_compute_k(const AT &a, int tile,
+                                                 FilterLds<T, NVAR, TY, filter_halo<ORDER>()> &L,
+                                                 const FilterRegs<T, NVAR, TY, filter_halo<ORDER>()> &R) {
+  constexpr int H = filter_halo<ORDER>();
+  constexpr int NR = TY + 2 * H, KR = NR / 4;
+  constexpr int NB = ORDER == 11 ? 5 : ORDER / 2;
+  constexpr int NV4 = (4 + 2 * H) / 4;  // 4-vectors of LDS behind 4 xi outputs
   auto &in = L.in;
   auto &mid = L.mid;
   const auto &ra = R.ra;
@@ -301,16 +356,16 @@ __device__ __forceinline__ void filter_compute_k(const AT &a, int tile, FilterLd
   const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
   const size_t vs = a.vs ? a.vs : (size_t)n;
-  const bool xtra = tx < 2 * FL_H;
+  const bool xtra = tx < 2 * H;
   constexpr bool inner = INNER;
   // rb of a lane without a right-halo column holds its own column again
   // (filter_load): writing it back there keeps the stores unconditional
   const int colb = xtra ? FL_TX + tx : tx;
-  const int gia = i0 - FL_H + tx, gib = i0 - FL_H + colb;
+  const int gia = i0 - H + tx, gib = i0 - H + colb;
   const bool cola = inner || (gia >= 0 && gia < ni), colok_b = inner || (gib >= 0 && gib < ni);
 #pragma unroll
   for (int k = 0; k < KR; k++) {
-    const int gj = j0 - FL_H + ty + 4 * k;
+    const int gj = j0 - H + ty + 4 * k;
     const bool rowok = inner || (gj >= 0 && gj < nj);
     const bool oka = rowok && cola, okb = rowok && colok_b;
 #pragma unroll
@@ -332,16 +387,16 @@ __device__ __forceinline__ void filter_compute_k(const AT &a, int tile, FilterLd
     const int it = tid + s * TPB;
     if (NXI % TPB != 0 && it >= NXI) break;
     const int g = it % NG, r = (it / NG) % NR, q = it / (NG * NR);
-    T v[12];
+    T v[4 * NV4];
 #pragma unroll
-    for (int h = 0; h < 3; h++) {
+    for (int h = 0; h < NV4; h++) {
       const Vec4<T> x = *reinterpret_cast<const Vec4<T> *>(&in[q][r][4 * g + 4 * h]);
       v[4 * h] = x.x;
       v[4 * h + 1] = x.y;
       v[4 * h + 2] = x.z;
       v[4 * h + 3] = x.w;
     }
-    const int gj = j0 - FL_H + r;
+    const int gj = j0 - H + r;
 #pragma unroll
     for (int m = 0; m < 4; m++) {
       const int gi = i0 + 4 * g + m;
@@ -351,7 +406,7 @@ __device__ __forceinline__ void filter_compute_k(const AT &a, int tile, FilterLd
         ok = gj >= 0 && gj < nj && gi < ni;
         if (ok) code = HOLES ? (int)a.code[gj * ni + gi] : default_code(gi, ni, NB);
       }
-      xo[s][m] = ok ? filter_row<T, ORDER>(&v[m + FL_H], 1, code) : (T)0;
+      xo[s][m] = ok ? filter_row_of<T, ORDER>(a, &v[m + H], 1, code) : (T)0;
     }
   }
   __syncthreads();  // every read of the input tile is done: mid overwrites it
@@ -369,25 +424,31 @@ __device__ __forceinline__ void filter_compute_k(const AT &a, int tile, FilterLd
   // in ra[.][k] and column 64 + tx in rb[.][k]: the value at block column
   // i0 + tx comes from lane tx + 4, or for tx >= 60 from rb of lane tx - 60.
   const int gi = i0 + tx;
-  const int src_lane = tx < FL_TX - FL_H ? tx + FL_H : tx - (FL_TX - FL_H);
+  const int src_lane = tx < FL_TX - H ? tx + H : tx - (FL_TX - H);
 #pragma unroll
   for (int k = 0; k < TY / 4; k++) {
     T orig[NVAR];
+    if constexpr (H % 4 == 0) {  // the loaded row ty + 4 (k + H/4) is tile row ty + 4k
 #pragma unroll
-    for (int q = 0; q < NVAR; q++) {
-      const T va = __shfl(ra[q][k + 1], src_lane, FL_TX);
-      const T vb = __shfl(rb[q][k + 1], src_lane, FL_TX);
-      orig[q] = tx < FL_TX - FL_H ? va : vb;
+      for (int q = 0; q < NVAR; q++) {
+        const T va = __shfl(ra[q][k + H / 4], src_lane, FL_TX);
+        const T vb = __shfl(rb[q][k + H / 4], src_lane, FL_TX);
+        orig[q] = tx < FL_TX - H ? va : vb;
+      }
     }
     const int r = ty + 4 * k;
     const int gj = j0 + r;
     if (!inner && (gj >= nj || gi >= ni)) continue;
     const int p = gj * ni + gi;
+    if constexpr (H % 4 != 0) {  // the halo of 6 shifts the loaded rows: the value again
+#pragma unroll
+      for (int q = 0; q < NVAR; q++) orig[q] = a.src[a.vars[q] * vs + p];
+    }
     const int code = inner ? CODE_INTERIOR : (HOLES ? (int)a.code[n + p] : default_code(gj, nj, NB));
     const bool hole = HOLES && a.iblank[p] == BLANKED;
 #pragma unroll
     for (int q = 0; q < NVAR; q++) {
-      const T f = filter_row<T, ORDER>(&mid[q][r + FL_H][tx], FL_TX, code);
+      const T f = filter_row_of<T, ORDER>(a, &mid[q][r + H][tx], FL_TX, code);
       T out = filter_blend(a.rem, orig[q], a.blend, f);
       if (hole) out = 0;
       a.dst[a.vars[q] * vs + p] = out;
@@ -396,18 +457,20 @@ __device__ __forceinline__ void filter_compute_k(const AT &a, int tile, FilterLd
 }
 
 template <typename T, bool HOLES, int ORDER, int NVAR, int TY, typename AT>
-__device__ __forceinline__ void filter_compute(const AT &a, int tile, FilterLds<T, NVAR, TY> &L,
-                                               const FilterRegs<T, NVAR, TY> &R) {
+__device__ __forceinline__ void filter_compute(const AT &a, int tile,
+                                               FilterLds<T, NVAR, TY, filter_halo<ORDER>()> &L,
+                                               const FilterRegs<T, NVAR, TY, filter_halo<ORDER>()> &R) {
+  constexpr int H = filter_halo<ORDER>();
   const int i0 = (tile % a.ntx) * FL_TX, j0 = (tile / a.ntx) * TY;
   if constexpr (!HOLES) {
-    if (i0 >= FL_H && i0 + FL_TX + FL_H <= a.ni && j0 >= FL_H && j0 + TY + FL_H <= a.nj) {
+    if (i0 >= H && i0 + FL_TX + H <= a.ni && j0 >= H && j0 + TY + H <= a.nj) {
       filter_compute_k<T, HOLES, ORDER, NVAR, TY, true>(a, tile, L, R);
       return;
     }
   } else {
     // a tile of a block with holes whose points and halo rows all take the
     // interior filter (no hole, no closure row) runs the interior instance
-    if (a.clean && box_clean(a.clean, a.ni, a.nj, i0, i0 + FL_TX, j0 - FL_H, j0 + TY + FL_H)) {
+    if (a.clean && box_clean(a.clean, a.ni, a.nj, i0, i0 + FL_TX, j0 - H, j0 + TY + H)) {
       filter_compute_k<T, false, ORDER, NVAR, TY, true>(a, tile, L, R);
       return;
     }
@@ -417,9 +480,10 @@ __device__ __forceinline__ void filter_compute(const AT &a, int tile, FilterLds<
 
 
 template <typename T, bool HOLES, int ORDER, int NVAR, int TY, typename AT>
-__device__ __forceinline__ void filter_tile(const AT &a, int tile, FilterLds<T, NVAR, TY> &L) {
-  FilterRegs<T, NVAR, TY> R;
-  filter_load<T, HOLES, NVAR, TY>(a, tile, R);
+__device__ __forceinline__ void filter_tile(const AT &a, int tile,
+                                            FilterLds<T, NVAR, TY, filter_halo<ORDER>()> &L) {
+  FilterRegs<T, NVAR, TY, filter_halo<ORDER>()> R;
+  filter_load<T, HOLES, NVAR, TY, filter_halo<ORDER>()>(a, tile, R);
   filter_compute<T, HOLES, ORDER, NVAR, TY>(a, tile, L, R);
 }
 

@@ -152,6 +152,18 @@ struct Block {
   double clk_time = 0.0, clk_dt = 1e-6;
 };
 
+// The optimized 11-point operators (FDO11P, SFO11P) as the fused kernels
+// read them from device memory: the OpTable rows with each closure row's
+// own length.
+struct WideOp {
+  double st[11];       // interior row, index s + 5
+  double bnd[5][9];    // closure row k, anchored k points to the left
+  int len[5];          // entries of closure row k
+  int nb;              // closure rows
+  double sign_right;   // -1 derivative, +1 filter
+};
+WideOp make_wide_op(const OpTable &t);
+
 // One reversed RK stage: buffer indices into Block::A (-1: none).
 struct AdjStage {
   int kind;                    // 0 first, 1 middle, 2 last stage
@@ -231,6 +243,7 @@ struct lee_handle {
   int bp_zero[2] = {0, 0};
   lee::Overset ov;
   lee::OpTable fd{}, flt{};
+  lee::WideOp *fdw = nullptr, *flw = nullptr;  // device copies of FDO11P / SFO11P rows (fused path)
   int nb_fd = 2, nb_flt = 4;
   double b_i[4] = {0}, c_i[4] = {0}, a_next[4] = {0};  // a_next[i] = a_{i+1,i}
   int time_step = 0;

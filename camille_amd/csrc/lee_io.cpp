@@ -304,8 +304,7 @@ extern "C" int lee_deck_settings(const lee_deck *d, lee_run_settings *s) {
                   : sim == "CASE_2DJET_WITH_A_HARMONIC_SOURCE" ? LEE_SOURCE_JET2D
                                                                : LEE_SOURCE_NONE;
   c.precision = LEE_FP64;
-  // the optimized schemes run on the plain path
-  c.path = (c.fd_order == 11 || c.filter_order == 11) ? LEE_PATH_PLAIN : LEE_PATH_FUSED;
+  c.path = LEE_PATH_FUSED;
   if (s->report_freq <= 0) return io_fail("REPORT_TIMESTEP must be positive.");
   return LEE_OK;
 }

@@ -480,13 +480,18 @@ __global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__
 
 // INNER: the tile and its halo lie inside a block without holes (interior
 // stencils only, no bounds tests); stage_tile below picks the instance.
-template <typename T, bool MEANF, bool HOLES, int KIND, int R, bool INNER, typename AT>
-__device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, int tile, StageLds<T, R> &s) {
+// W: the derivative operator, OA4 (4, compile-time rows, halo 2) or FDO11P
+// (11, rows from a.fdw, halo 6).
+template <typename T, bool MEANF, bool HOLES, int KIND, int R, bool INNER, int W = 4, typename AT>
+__device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, int tile, StageLdsW<T, R, W> &s) {
   // every fusion is written out (fmx): the interior and edge instances of
   // this tile, and the batched kernels, round alike
 #pragma clang fp contract(off)
   constexpr int TY = 4 * R;
-  constexpr int RS = ST_TX + 2 * ST_H;
+  constexpr int H = StageOp<W>::H, NBC = StageOp<W>::NB;
+  constexpr int KR = (TY + 2 * H) / 4;  // region rows per thread
+  static_assert(TY + 2 * H == 4 * KR, "tile rows and the halo come in fours");
+  constexpr int RS = ST_TX + 2 * H;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
@@ -495,14 +500,14 @@ __device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, in
   // a tile whose halo lies inside the block needs no bounds checks and has
   // interior stencils only
   constexpr bool inner = INNER;
-  // (1) tile values: rows ty + 4k (k = 0..R), columns tx and, for tx < 4, 64 + tx
-  T ta[4][R + 1], tb[4][R + 1];
-  const bool xtra = tx < 2 * ST_H;
+  // (1) tile values: rows ty + 4k (k < KR), columns tx and, for tx < 2H, 64 + tx
+  T ta[4][KR], tb[4][KR];
+  const bool xtra = tx < 2 * H;
 #pragma unroll
-  for (int k = 0; k <= R; k++) {
-    const int gj = j0 - ST_H + ty + 4 * k;
+  for (int k = 0; k < KR; k++) {
+    const int gj = j0 - H + ty + 4 * k;
     const bool rowok = inner || (gj >= 0 && gj < nj);
-    const int gia = i0 - ST_H + tx, gib = i0 - ST_H + ST_TX + tx;
+    const int gia = i0 - H + tx, gib = i0 - H + ST_TX + tx;
     const bool oka = inner || (rowok && gia >= 0 && gia < ni), okb = xtra && (inner || (rowok && gib < ni));
     const int qa = gj * ni + gia, qb = gj * ni + gib;
 #pragma unroll
@@ -531,7 +536,7 @@ __device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, in
   }
   // (3) tile to LDS
 #pragma unroll
-  for (int k = 0; k <= R; k++) {
+  for (int k = 0; k < KR; k++) {
     const int r = ty + 4 * k;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -548,22 +553,27 @@ __device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, in
     const int gj = j0 + rr;
     if (!inner && (gi >= ni || gj >= nj)) continue;
     const int p = gj * ni + gi;
-    const int lj = rr + ST_H, li = tx + ST_H;
+    const int lj = rr + H, li = tx + H;
     int cx = CODE_INTERIOR, cy = CODE_INTERIOR;
     if (HOLES) {
       cx = a.code[p];
       cy = a.code[n + p];
     } else if (!inner) {
-      cx = default_code(gi, ni, 2);
-      cy = default_code(gj, nj, 2);
+      cx = default_code(gi, ni, NBC);
+      cy = default_code(gj, nj, NBC);
     }
     T dx[4], dy[4], yc[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const T *sp = &s[v][lj][li];
       yc[v] = sp[0];
-      dx[v] = fd_row<T>(sp, 1, cx);
-      dy[v] = fd_row<T>(sp, RS, cy);
+      if constexpr (W == 4) {
+        dx[v] = fd_row<T>(sp, 1, cx);
+        dy[v] = fd_row<T>(sp, RS, cy);
+      } else {
+        dx[v] = wide_row<T>(sp, 1, cx, a.fdw);
+        dy[v] = wide_row<T>(sp, RS, cy, a.fdw);
+      }
     }
     // chain rule with FP64 metrics (spatial:1810-1820)
     const double mxx = pm[k][0], mxy = pm[k][1], myx = pm[k][2], myy = pm[k][3];
@@ -681,33 +691,33 @@ __device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, in
   }
 }
 
-template <typename T, bool MEANF, bool HOLES, int KIND, int R, typename AT>
-__device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int tile, StageLds<T, R> &s) {
-  constexpr int TY = 4 * R;
+template <typename T, bool MEANF, bool HOLES, int KIND, int R, int W = 4, typename AT>
+__device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int tile, StageLdsW<T, R, W> &s) {
+  constexpr int TY = 4 * R, H = StageOp<W>::H;
   const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
   if constexpr (!HOLES) {
-    if (i0 >= ST_H && i0 + ST_TX + ST_H <= a.ni && j0 >= ST_H && j0 + TY + ST_H <= a.nj) {
-      stage_tile_k<T, MEANF, HOLES, KIND, R, true>(a, sc, tile, s);
+    if (i0 >= H && i0 + ST_TX + H <= a.ni && j0 >= H && j0 + TY + H <= a.nj) {
+      stage_tile_k<T, MEANF, HOLES, KIND, R, true, W>(a, sc, tile, s);
       return;
     }
   } else {
     // a tile of a block with holes whose points all take the interior
     // stencil (no hole, no closure row) runs the interior instance
     if (a.clean && box_clean(a.clean, a.ni, a.nj, i0, i0 + ST_TX, j0, j0 + TY)) {
-      stage_tile_k<T, MEANF, false, KIND, R, true>(a, sc, tile, s);
+      stage_tile_k<T, MEANF, false, KIND, R, true, W>(a, sc, tile, s);
       return;
     }
   }
-  stage_tile_k<T, MEANF, HOLES, KIND, R, false>(a, sc, tile, s);
+  stage_tile_k<T, MEANF, HOLES, KIND, R, false, W>(a, sc, tile, s);
 }
 
-template <typename T, bool MEANF, bool HOLES, int KIND, int R>
+template <typename T, bool MEANF, bool HOLES, int KIND, int R, int W = 4>
 __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
-  __shared__ StageLds<T, R> s;
+  __shared__ StageLdsW<T, R, W> s;
   const int tile = xcd_tile(blockIdx.x, a.chunk);
   if (tile >= a.ntiles) return;
   const StepT<T> sc{a.h, a.b, a.a, a.s1, a.s2};
-  stage_tile<T, MEANF, HOLES, KIND, R>(a, sc, tile, s);
+  stage_tile<T, MEANF, HOLES, KIND, R, W>(a, sc, tile, s);
 }
 
 // Batched ensemble: the tiles of all solves form one list; map[2t] is the
@@ -751,7 +761,7 @@ __global__ void __launch_bounds__(TPB)
 
 template <typename T, bool HOLES, int ORDER, int NVAR, int TY>
 __global__ void __launch_bounds__(TPB) k_fused_filter(FilterArgs<T> a) {
-  __shared__ FilterLds<T, NVAR, TY> L;
+  __shared__ FilterLds<T, NVAR, TY, filter_halo<ORDER>()> L;
   const int tile = xcd_tile(blockIdx.x, a.chunk);
   if (tile >= a.ntiles) return;
   filter_tile<T, HOLES, ORDER, NVAR, TY>(a, tile, L);
@@ -1189,6 +1199,7 @@ static StageArgs<T> make_stage_args(const lee_handle *h, const Block &b, int sta
   a.h = (T)sc.h;
   a.b = (T)sc.b;
   a.a = (T)sc.a;
+  a.fdw = h->fdw;
   return a;
 }
 
@@ -1196,13 +1207,15 @@ template <typename T>
 static hipError_t fused_stage_t(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc, T *const *Y,
                                 T *rho) {
   const int kind = stage_kind(h, stage);
-  const int R = b.tune_r;
+  const bool wide = h->cfg.fd_order == 11;
+  const int R = wide ? 1 : b.tune_r;  // the 11-point rows run on tiles of 4 rows
   const StageArgs<T> a = make_stage_args<T>(h, b, stage, Y, rho, R, sc);
   const dim3 grid(a.chunk * 8), block(64, 4);
   const bool mf = !b.uniform_mean, ho = b.has_holes;
 #define LEE_LAUNCH(MF, HO, K)                                                                     \
   do {                                                                                            \
-    if (R == 1) hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K, 1>), grid, block, 0, s, a);      \
+    if (wide) hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K, 1, 11>), grid, block, 0, s, a);    \
+    else if (R == 1) hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K, 1>), grid, block, 0, s, a); \
     else if (R == 2) hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K, 2>), grid, block, 0, s, a); \
     else hipLaunchKernelGGL((k_fused_stage<T, MF, HO, K, 4>), grid, block, 0, s, a);             \
   } while (0)
@@ -1249,6 +1262,7 @@ static FilterArgs<T> make_filter_args(const lee_handle *h, const Block &b, T *co
   a.vars[1] = IUX;
   a.vars[2] = IUY;
   a.vars[3] = IP;
+  a.flw = h->flw;
   return a;
 }
 
@@ -1261,6 +1275,26 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
   const bool pipe = b.tune_fgrid > 0;
   const dim3 pgrid(std::min(a.chunk * 8, (b.tune_fgrid + 7) / 8 * 8));
   const bool o8 = h->cfg.filter_order == 8;
+  if (h->cfg.filter_order == 11) {  // SFO11P: the tile form, tiles of 4 or 8 rows, halo 6
+    for (int pass = 0; pass < (b.carry_w ? 2 : 1); pass++) {
+      const bool four = pass == 0;
+      if (!four) a.vars[0] = IUZ;
+#define LEE_FLW(HO, NVV)                                                                                  \
+  do {                                                                                                    \
+    if (TYF == 4) hipLaunchKernelGGL((k_fused_filter<T, HO, 11, NVV, 4>), grid, block, 0, s, a);          \
+    else hipLaunchKernelGGL((k_fused_filter<T, HO, 11, NVV, 8>), grid, block, 0, s, a);                   \
+  } while (0)
+      if (b.has_holes) {
+        if (four) LEE_FLW(true, 4);
+        else LEE_FLW(true, 1);
+      } else {
+        if (four) LEE_FLW(false, 4);
+        else LEE_FLW(false, 1);
+      }
+#undef LEE_FLW
+    }
+    return hipGetLastError();
+  }
   if (b.tune_mc > 0 && (!b.has_holes || b.tune_mc > 1)) {
     // column-march forms: every carried variable in one launch
     const int nv = b.carry_w ? 5 : 4;

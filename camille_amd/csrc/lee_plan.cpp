@@ -47,6 +47,22 @@ static void make_fdo11p_table(OpTable &t) {
   for (int s = 0; s < 9; s++) t.bnd[4][s] = r4[s];
 }
 
+WideOp make_wide_op(const OpTable &t) {
+  WideOp w{};
+  for (int s = 0; s < 11; s++) w.st[s] = t.st[s];
+  for (int k = 0; k < 5; k++) {
+    int len = 0;
+    for (int s = 0; s < 9; s++) {
+      w.bnd[k][s] = t.bnd[k][s];
+      if (t.bnd[k][s] != 0.0) len = s + 1;
+    }
+    w.len[k] = len;
+  }
+  w.nb = t.nb;
+  w.sign_right = t.sign_right;
+  return w;
+}
+
 void make_fd_table(OpTable &t, int order) {
   if (order == 11) {
     make_fdo11p_table(t);

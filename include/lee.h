@@ -80,9 +80,9 @@ typedef struct {
   int num_dim;             /* DIMENSION: 2, or 3 (fused path, sponges only, no overset) */
   double gamma;            /* GAMMA_SPECIFICHEAT */
   int fd_order;            /* SPATIAL_SCHEME STANDARD_CENTRAL ORDER_ACCURACY: 4;
-                              11 = FDO11P (plain path) */
+                              11 = FDO11P (2-D; no batches, no adjoint) */
   int filter_order;        /* SPATIAL_FILTER STANDARD_CENTRAL ORDER_ACCURACY: 0 (NONE), 4, 8;
-                              11 = SFO11P (plain path) */
+                              11 = SFO11P (2-D; no batches, no adjoint) */
   double filter_blend;     /* SPATIAL_FILTER BLEND */
   int rk_stages;           /* TEMPORAL_SCHEME: 2 (RUNGE_KUTTA2) or 4 (RUNGE_KUTTA4) */
   int fix_cfl;             /* FIX_DT_OR_CFL: 1 = FIX_CFL, 0 = FIX_DT */

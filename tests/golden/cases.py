@@ -329,8 +329,9 @@ ALL = {f.__name__[5:]: f for f in (case_c1, case_curv, case_filt, case_shear, ca
                                     case_variants, case_jet, case_opt11, case_axisym, case_dirichlet,
                                     case_centerline, case_entropy, case_roundjet)}
 
-# cases whose schemes run on the plain path only (FDO11P / SFO11P)
-PLAIN_ONLY = {"opt11"}
+# cases whose filter (SFO11P) runs in the tile form only: the column-march
+# forms are written for the standard filters' 4-point halo
+TILE_FILTER_ONLY = {"opt11"}
 
 
 def _harmonic_deck(h):

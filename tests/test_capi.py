@@ -80,8 +80,8 @@ def test_create_rejects_bad_config(field, value, msg):
 def test_create_rejects_plain_fp32_and_small_blocks():
     rc, err, _ = _create(_cfg(precision=1, path=1))
     assert rc == 1 and "FP64 only" in err
-    rc, err, _ = _create(_cfg(fd_order=11))
-    assert rc == 1 and "plain path" in err
+    rc, err, _ = _create(_cfg(fd_order=11, num_dim=3))
+    assert rc == 1 and "run in 2-D" in err
     rc, err, _ = _create(_cfg(filter_order=11, filter_strength=1.5, path=1))
     assert rc == 1 and "STRENGTH" in err
     rc, err, _ = _create(_cfg(fd_order=11, path=1), ni=10, nj=20)

@@ -104,7 +104,7 @@ def test_deck_settings_optimized_schemes(tmp_path):
     st = fileio.Deck(_deck_for(cases.case_opt11(), tmp_path)).settings()
     cfg = st["cfg"]
     assert (cfg["fd_order"], cfg["filter_order"], cfg["filter_strength"]) == (11, 11, 0.6)
-    assert cfg["path"] == 1  # the optimized schemes select the plain path
+    assert cfg["path"] == 0  # the fused path takes the optimized schemes too
 
 
 @pytest.mark.parametrize("key,value,msg", [

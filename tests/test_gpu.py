@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import cases
-from camille_amd import FP32, FP64, PATH_FUSED, PATH_PLAIN, Solver
+from camille_amd import FP32, FP64, PATH_FUSED, PATH_PLAIN, LeeError, Solver
 from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -50,8 +50,6 @@ def test_plain_path_bit_identical_to_reference(name):
 @pytest.mark.parametrize("label,prec", [("fused64", FP64), ("fused32", FP32)])
 @pytest.mark.parametrize("name", sorted(cases.ALL))
 def test_fused_path_matches_reference(name, label, prec):
-    if name in cases.PLAIN_ONLY:
-        pytest.skip("the FDO11P / SFO11P schemes run on the plain path; the fused path rejects them")
     c = cases.ALL[name]()
     g = _golden(name)
     s = Solver.from_case(c, precision=prec, path=PATH_FUSED)
@@ -227,6 +225,29 @@ def test_tile_shapes_bit_identical(prec):
           flush=True)
 
 
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_optimized_schemes_fused_tile_shapes(prec):
+    """FDO11P and SFO11P on the fused path (halo-6 tiles, rows from device
+    memory): filter tiles of 4 and 8 rows give the same bits; the other
+    forms are refused."""
+    c = cases.case_opt11()
+    c["nsteps"] = 12
+    ref = None
+    for f in (4, 8):
+        s = Solver.from_case(c, precision=prec, path=PATH_FUSED)
+        s.set_tile_shape(1, f)
+        s.solve(c["nsteps"])
+        sol = s.solution()
+        if ref is None:
+            ref = sol
+            for bad in (lambda: s.set_tile_shape(1, 16), lambda: s.set_filter_grid(16),
+                        lambda: s.set_filter_march(4, 8)):
+                with pytest.raises(LeeError):
+                    bad()
+        s.close()
+        assert np.array_equal(sol, ref), f
+
+
 def _march_case():
     """301 x 150 wavy grid, uniform mean flow, off-centre pulse: wide and tall
     enough that every column-march filter form has interior waves or stacks
@@ -247,7 +268,7 @@ MARCH_FORMS = [(1, 8), (1, 16), (4, 4), (4, 8), (4, 16), (8, 8)]
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-@pytest.mark.parametrize("name", [n for n in sorted(cases.ALL) if n not in cases.PLAIN_ONLY] + ["march"])
+@pytest.mark.parametrize("name", [n for n in sorted(cases.ALL) if n not in cases.TILE_FILTER_ONLY] + ["march"])
 def test_filter_march_bit_identical(prec, name):
     """The column-march filter forms compute each point with the operations
     of the tile forms: every stack shape and row count gives the tile form's
