@@ -193,7 +193,7 @@ def c3_inputs():
     return dict(geo=geo, blocks=blocks, bc=bc, tables=og.abi_tables(geo))
 
 
-def c3_leg(steps, sample_every, window=1000):
+def c3_leg(steps, sample_every, window=1000, paths=("fp32", "fp64", "plain")):
     """Overset cylinder in FP32 bulk, FP64 fused and FP64 plain (the
     reference's arithmetic): throughput and probe time series.
 
@@ -205,7 +205,8 @@ def c3_leg(steps, sample_every, window=1000):
     state stays bounded; each window is checked (every probe sample finite
     and below 10x the pulse amplitude) and a violation fails the leg.  The
     fused paths run steps / window windows, the plain path one; errors are
-    over the first window's probe samples."""
+    over the first window's probe samples (when the plain path runs).
+    `paths` selects the legs (a profiling run may leave some out)."""
     from camille_amd import FP32, FP64, PATH_FUSED, PATH_PLAIN, Solver
     inp = c3_inputs()
     npts = sum(b["x"].size for b in inp["blocks"])
@@ -217,6 +218,8 @@ def c3_leg(steps, sample_every, window=1000):
     probes = None
     bound = 10.0 * 1e-3
     for label, prec, path in (("fp32", FP32, PATH_FUSED), ("fp64", FP64, PATH_FUSED), ("plain", FP64, PATH_PLAIN)):
+        if label not in paths:
+            continue
         runs = nwin if path == PATH_FUSED else 1
         print(f"[bench] C3 {label}: {npts} points, {runs} x {window} steps", file=sys.stderr, flush=True)
         s = Solver([dict(x=b["x"], y=b["y"], mean=b["mean"]) for b in inp["blocks"]], boundaries=inp["bc"],
@@ -253,6 +256,8 @@ def c3_leg(steps, sample_every, window=1000):
                           time=s.clock()[1], dt=s.clock()[2], windows=runs)
         s.close()
         print(f"[bench] C3 {label}: {res[label]['updates_per_s']:.3e} updates/s", file=sys.stderr, flush=True)
+    if "plain" not in series:
+        return res
     ref = series["plain"]
     scale = float(np.max(np.abs(ref)))
     res["probe_p_max"] = scale
@@ -492,6 +497,8 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=5000)
     ap.add_argument("--c3-window", type=int, default=1000)
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-paths", default="fp32,fp64,plain",
+                    help="with --only c3: the legs to run, comma-separated (fp32, fp64, plain)")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-steps", type=int, default=500,
                     help="3-D adjoint leg: steps (window: the last fifth; checkpoints every tenth)")
@@ -523,7 +530,8 @@ def main():
         print(json.dumps({"c5": c5_leg(args.steps, args.warmup, 10)}))
         return
     if args.only == "c3":
-        print(json.dumps({"c3": c3_leg(args.steps, 50, min(args.steps, args.c3_window))}))
+        print(json.dumps({"c3": c3_leg(args.steps, 50, min(args.steps, args.c3_window),
+                                       tuple(args.c3_paths.split(",")))}))
         return
     if args.only == "c4":
         n4 = args.c4_steps

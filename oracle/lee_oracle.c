@@ -27,7 +27,8 @@
 #include <string.h>
 
 #define MAXB 8
-#define NV 5
+#define NV 5   /* s', u'_x, u'_y, u'_z, p' */
+#define MAXV 9 /* NV + up to 4 passive scalars Z'_k (LEE_SCALAR, input.cpp:768-774) */
 #define IS_ 0 /* s'  (param.h IVAR_S)  */
 #define IUX 1
 #define IUY 2 /* u_y, also IVAR_UR of an axisymmetric run (param.h:74) */
@@ -63,7 +64,9 @@ static char g_err[512];
  *  7 right hole closures after any non-hole cell (`!=` for `/=`)
  *  8 monopole with cos(omega t) for sin(omega t)
  *  9 monopole Gaussian exp(-|x-x0|^2/w^2), the ln 2 of the half-width dropped
- * 10 entropy wave with cos(k c0 t) for sin(k c0 t) */
+ * 10 entropy wave with cos(k c0 t) for sin(k c0 t)
+ * 11 passive scalars left out of the filter
+ * 12 wall-slip leaving the scalars out of its Neumann update */
 static int g_plant = 0;
 void orc_set_plant(int code) { g_plant = code; }
 static void set_err(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); }
@@ -91,9 +94,9 @@ typedef struct {
   double *minv;  /* [9][n] metricsInverse[i][l] */
   double *jac;
   int *iblank;
-  double *mean[NV], *mgrad[NV * 3], *aux[4];
-  double *sol[NV], *sol_old[NV];
-  double *rhs[NV], *interim[NV], *yn[NV], *ynp1[NV];
+  double *mean[MAXV], *mgrad[MAXV * 3], *aux[4];
+  double *sol[MAXV], *sol_old[MAXV];
+  double *rhs[MAXV], *interim[MAXV], *yn[MAXV], *ynp1[MAXV];
   int nbnd;
   bnd_t bnd[32];
   /* operator extents per op: [0] derivative, [1] filter */
@@ -107,11 +110,12 @@ typedef struct {
   int w[MAXB][3];      /* stencil widths of the donor block */
   double *coef;        /* nrecv * maxstencil normalised coefficients */
   int maxst;
-  double *buf;         /* nrecv * NV */
+  double *buf;         /* nrecv * nv */
 } ovs_t;
 
 typedef struct {
   int nblk, ndim;
+  int nv;             /* NV + number of passive scalars */
   double gamma;
   op_t fd, flt;
   int do_filter;
@@ -525,7 +529,7 @@ static int rhs_lee(orc_t *o, blk_t *b, double **y, double **rhs) {
   int n = b->n, nd = o->ndim;
   double *df = o->dflux;
   double g = o->gamma;
-  for (int v = 0; v < NV; v++) for (int p = 0; p < n; p++) rhs[v][p] = 0.0;
+  for (int v = 0; v < o->nv; v++) for (int p = 0; p < n; p++) rhs[v][p] = 0.0;
   for (int d = 0; d < nd; d++) {
     if (deriv_xyz(o, b, y[IS_], d)) return -1;
     for (int p = 0; p < n; p++) rhs[IS_][p] -= b->mean[IUX + d][p] * df[p];
@@ -555,6 +559,15 @@ static int rhs_lee(orc_t *o, blk_t *b, double **y, double **rhs) {
   if (o->axisym)
     for (int p = 0; p < n; p++)
       rhs[IP][p] -= g * (b->mean[IP][p] * y[IUY][p] + y[IP][p] * b->mean[IUY][p]) / (b->xyz[1][p] + 1e-99);
+  /* passive scalars, compute_RHS_linearized_Euler_scalar (spatial:2134-2158),
+   * called after the LEE terms (spatial:1977-1979):
+   * rhs_Z -= ubar_i dZ'/dx_i, then rhs_Z -= u'_i dZbar/dx_i, direction by direction */
+  for (int v = NV; v < o->nv; v++)
+    for (int d = 0; d < nd; d++) {
+      if (deriv_xyz(o, b, y[v], d)) return -1;
+      for (int p = 0; p < n; p++) rhs[v][p] -= b->mean[IUX + d][p] * df[p];
+      for (int p = 0; p < n; p++) rhs[v][p] -= y[IUX + d][p] * b->mgrad[v * nd + d][p];
+    }
   return 0;
 }
 
@@ -608,7 +621,7 @@ static void harmonic_data(const orc_t *o, const blk_t *b, int l0, double time, d
   static const int dir_other[3][2] = {{1, 2}, {0, 2}, {0, 1}}; /* param.h:133 */
   double rho_0 = 1.0, c_0 = 1.0;
   double wavenumber = o->twopi / o->hw_wavelength;
-  for (int v = 0; v < NV; v++) data[v] = 0.0;
+  for (int v = 0; v < o->nv; v++) data[v] = 0.0;
   if (b->iblank[l0] == BLANKED) return;
   double t1 = b->xyz[dir_other[o->hw_dir][0]][l0], t2 = b->xyz[dir_other[o->hw_dir][1]][l0];
   double pressure = 0.0, velocity = 0.0;
@@ -636,9 +649,9 @@ static void sponges(orc_t *o, blk_t *b, double **y, double **rhs, double time) {
           int lb = ((k - zb->is[2]) * nz1 + (j - zb->is[1])) * nz0 + (i - zb->is[0]);
           int l0 = idx(b, i, j, k);
           double s = zb->sigma[lb];
-          double ref[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+          double ref[MAXV] = {0.0};
           if (zb->type == SPONGE_HARMONIC) harmonic_data(o, b, l0, time, ref);
-          for (int v = 0; v < NV; v++) {
+          for (int v = 0; v < o->nv; v++) {
             double relax = -s * (y[v][l0] - ref[v]);
             rhs[v][l0] += relax;
           }
@@ -667,19 +680,24 @@ static void apply_bc(orc_t *o, blk_t *b, double **y, double time) {
     for (int d = 0; d < 3; d++) { lo[d] = bc->is[d]; hi[d] = bc->ie[d]; }
     lo[bc->dir] = hi[bc->dir] = bc->end == 0 ? bc->is[bc->dir] : bc->ie[bc->dir];
     int mf = stride_of(b, bc->dir);
-    int vars[NV], nvu = 0;
+    int vars[MAXV], nvu = 0;
     int neumann = bc->type == BC_NEUMANN || bc->type == BC_WALL_SLIP || (bc->type >= BC_CL_X && bc->type <= 903);
-    if (bc->type == BC_WALL_SLIP) { vars[nvu++] = IS_; vars[nvu++] = IP; }
-    else for (int v = 0; v < NV; v++) vars[nvu++] = v;
+    /* wall-slip: the non-velocity variables (s', p' and the scalars) take
+     * the Neumann data (boundary_condition.cpp:124-136) */
+    if (bc->type == BC_WALL_SLIP) {
+      vars[nvu++] = IS_; vars[nvu++] = IP;
+      if (g_plant != 12)
+        for (int v = NV; v < o->nv; v++) vars[nvu++] = v;
+    } else for (int v = 0; v < o->nv; v++) vars[nvu++] = v;
     for (int k = lo[2]; k <= hi[2]; k++)
       for (int j = lo[1]; j <= hi[1]; j++)
         for (int i = lo[0]; i <= hi[0]; i++) {
           int l0 = idx(b, i, j, k);
           if (b->iblank[l0] == BLANKED) continue;
-          double data[NV];
-          for (int v = 0; v < NV; v++) data[v] = 1e9 * (double)(bc->dir + 1); /* -DUMMY_DOUBLE*(dir+1) */
+          double data[MAXV];
+          for (int v = 0; v < o->nv; v++) data[v] = 1e9 * (double)(bc->dir + 1); /* -DUMMY_DOUBLE*(dir+1) */
           if (bc->type == BC_DIRICHLET_ALLZERO) {
-            for (int v = 0; v < NV; v++) data[v] = 0.0;
+            for (int v = 0; v < o->nv; v++) data[v] = 0.0;
           }
           if (bc->type == BC_DIRICHLET_HARMONIC) harmonic_data(o, b, l0, time, data);
           if (neumann) {
@@ -714,7 +732,7 @@ static void apply_bc(orc_t *o, blk_t *b, double **y, double time) {
           if (bc->type >= BC_CL_X && bc->type <= BC_CL_Z) data[IUX + (bc->type - BC_CL_X)] = 0.0;
           /* BC_CENTERLINE_AXISYM: Neumann, then u_r = 0 (bc:150-156, 534-541) */
           if (bc->type == 903) data[IUY] = 0.0;
-          for (int v = 0; v < NV; v++) y[v][l0] = data[v];
+          for (int v = 0; v < o->nv; v++) y[v][l0] = data[v];
         }
   }
 }
@@ -727,19 +745,19 @@ static void overset_interp(orc_t *o, double ***y) {
     blk_t *db = &o->b[ov->db[r]];
     const int *w = ov->w[ov->db[r]];
     const double *c = ov->coef + (size_t)r * ov->maxst;
-    double acc[NV] = {0, 0, 0, 0, 0};
+    double acc[MAXV] = {0};
     int s = 0;
     for (int w3 = 0; w3 < w[2]; w3++)
       for (int w2 = 0; w2 < w[1]; w2++)
         for (int w1 = 0; w1 < w[0]; w1++) {
           int l1 = ov->dl[r] + (w1 + w2 * db->ni + w3 * db->ni * db->nj);
-          for (int v = 0; v < NV; v++) acc[v] += c[s] * y[ov->db[r]][v][l1];
+          for (int v = 0; v < o->nv; v++) acc[v] += c[s] * y[ov->db[r]][v][l1];
           s++;
         }
-    for (int v = 0; v < NV; v++) ov->buf[r * NV + v] = acc[v];
+    for (int v = 0; v < o->nv; v++) ov->buf[r * o->nv + v] = acc[v];
   }
   for (int r = 0; r < ov->nrecv; r++)
-    for (int v = 0; v < NV; v++) y[ov->rb[r]][v][ov->rl[r]] = ov->buf[r * NV + v];
+    for (int v = 0; v < o->nv; v++) y[ov->rb[r]][v][ov->rl[r]] = ov->buf[r * o->nv + v];
 }
 
 /* spatial::update_boundary, spatial:2162-2187 */
@@ -751,7 +769,7 @@ static void update_boundary(orc_t *o, double ***y, double time) {
 /* spatial::apply_filter, spatial:2191-2246 */
 static int apply_filter_block(orc_t *o, blk_t *b, double **y) {
   int n = b->n;
-  for (int v = 0; v < NV; v++) {
+  for (int v = 0; v < (g_plant == 11 ? NV : o->nv); v++) {
     for (int p = 0; p < n; p++) o->func0[p] = y[v][p];
     for (int p = 0; p < n; p++) o->func[p] = o->func0[p];
     for (int dd = 0; dd < o->ndim; dd++) {
@@ -772,7 +790,7 @@ static int rk_integrate(orc_t *o, double tn, double h) {
   for (int ib = 0; ib < o->nblk; ib++) {
     blk_t *b = &o->b[ib];
     ynext[ib] = b->ynp1; yint[ib] = b->interim; rhs[ib] = b->rhs;
-    for (int v = 0; v < NV; v++)
+    for (int v = 0; v < o->nv; v++)
       for (int p = 0; p < b->n; p++) { b->ynp1[v][p] = b->yn[v][p]; b->interim[v][p] = b->yn[v][p]; }
   }
   int last = o->rk_stages - 1;
@@ -782,13 +800,13 @@ static int rk_integrate(orc_t *o, double tn, double h) {
       blk_t *b = &o->b[ib];
       for (int p = 0; p < b->n; p++) {
         if (b->iblank[p] != BLANKED) {
-          for (int v = 0; v < NV; v++) {
+          for (int v = 0; v < o->nv; v++) {
             b->rhs[v][p] *= h;
             b->ynp1[v][p] += o->b_i[st] * b->rhs[v][p];
             if (st < last) b->interim[v][p] = b->yn[v][p] + o->a_ij[st + 1][st] * b->rhs[v][p];
           }
         } else {
-          for (int v = 0; v < NV; v++) { b->rhs[v][p] = 0.0; b->ynp1[v][p] = 0.0; b->interim[v][p] = 0.0; }
+          for (int v = 0; v < o->nv; v++) { b->rhs[v][p] = 0.0; b->ynp1[v][p] = 0.0; b->interim[v][p] = 0.0; }
         }
       }
     }
@@ -809,16 +827,16 @@ static int rk_integrate(orc_t *o, double tn, double h) {
 static int time_advance(orc_t *o) {
   for (int ib = 0; ib < o->nblk; ib++) {
     blk_t *b = &o->b[ib];
-    for (int v = 0; v < NV; v++)
+    for (int v = 0; v < o->nv; v++)
       for (int p = 0; p < b->n; p++) { b->yn[v][p] = b->sol_old[v][p]; b->ynp1[v][p] = -1e9; }
   }
   if (rk_integrate(o, g_plant == 2 ? o->time_sol - o->dt : o->time_sol, o->dt)) return -1;
   for (int ib = 0; ib < o->nblk; ib++) {
     blk_t *b = &o->b[ib];
     if (o->do_filter && apply_filter_block(o, b, b->ynp1)) return -1;
-    for (int v = 0; v < NV; v++)
+    for (int v = 0; v < o->nv; v++)
       for (int p = 0; p < b->n; p++) if (b->iblank[p] == BLANKED) b->ynp1[v][p] = 0.0;
-    for (int v = 0; v < NV; v++) memcpy(b->sol[v], b->ynp1[v], b->n * sizeof(double));
+    for (int v = 0; v < o->nv; v++) memcpy(b->sol[v], b->ynp1[v], b->n * sizeof(double));
     if (g_plant == 3) compute_aux(o, b, b->sol);
   }
   return 0;
@@ -867,6 +885,7 @@ void *orc_create(int nblk, int ndim, double gamma, int fd_order, int filter_orde
   if (nblk < 1 || nblk > MAXB) { set_err("bad block count"); return NULL; }
   orc_t *o = calloc(1, sizeof(orc_t));
   o->nblk = nblk;
+  o->nv = NV;
   o->ndim = ndim;
   o->gamma = gamma;
   o->pi = acos(-1.0);
@@ -911,13 +930,13 @@ int orc_set_block(void *h, int ib, int ni, int nj, int nk, const double *xyz, co
   b->minv = dalloc(9 * n);
   b->jac = dalloc(n);
   b->iblank = calloc(n, sizeof(int));
-  for (int v = 0; v < NV; v++) {
+  for (int v = 0; v < o->nv; v++) {
     b->mean[v] = dalloc(n);
     memcpy(b->mean[v], mean + (size_t)v * n, n * sizeof(double));
     b->sol[v] = dalloc(n); b->sol_old[v] = dalloc(n);
     b->rhs[v] = dalloc(n); b->interim[v] = dalloc(n); b->yn[v] = dalloc(n); b->ynp1[v] = dalloc(n);
   }
-  for (int v = 0; v < NV * 3; v++) b->mgrad[v] = dalloc(n);
+  for (int v = 0; v < o->nv * 3; v++) b->mgrad[v] = dalloc(n);
   for (int v = 0; v < 4; v++) b->aux[v] = dalloc(n);
   if (n > o->nmax) {
     o->nmax = n;
@@ -925,6 +944,16 @@ int orc_set_block(void *h, int ib, int ni, int nj, int nk, const double *xyz, co
     o->dflux = dalloc(n); o->dflux_sum = dalloc(n); o->flux = dalloc(n); o->func0 = dalloc(n); o->func = dalloc(n);
   }
   set_extents(o, b);
+  return 0;
+}
+
+/* PHYSICAL_MODEL = LEE_SCALAR with NUMBER_SCALAR = ns (input.cpp:113-119,
+ * 768-774): ns passive scalars Z'_k after p' (state.cpp:336-372); call
+ * before orc_set_block. */
+int orc_set_scalars(void *h, int ns) {
+  orc_t *o = h;
+  if (ns < 0 || NV + ns > MAXV) { set_err("NUMBER_SCALAR must be 0 to 4."); return -1; }
+  o->nv = NV + ns;
   return 0;
 }
 
@@ -985,7 +1014,7 @@ int orc_set_overset(void *h, int nrecv, const int *rb, const int *rl, const int 
   ov->rb = malloc(nrecv * sizeof(int)); ov->rl = malloc(nrecv * sizeof(int));
   ov->db = malloc(nrecv * sizeof(int)); ov->dl = malloc(nrecv * sizeof(int));
   ov->coef = malloc((size_t)nrecv * maxst * sizeof(double));
-  ov->buf = malloc((size_t)nrecv * NV * sizeof(double));
+  ov->buf = malloc((size_t)nrecv * o->nv * sizeof(double));
   memcpy(ov->rb, rb, nrecv * sizeof(int)); memcpy(ov->rl, rl, nrecv * sizeof(int));
   memcpy(ov->db, db, nrecv * sizeof(int)); memcpy(ov->dl, dl, nrecv * sizeof(int));
   for (int ib = 0; ib < o->nblk; ib++) for (int d = 0; d < 3; d++) ov->w[ib][d] = widths[ib * 3 + d];
@@ -1013,8 +1042,8 @@ int orc_init(void *h, const double *sol0_all, int step0, double time0) {
   size_t off = 0;
   for (int ib = 0; ib < o->nblk; ib++) {
     blk_t *b = &o->b[ib];
-    for (int v = 0; v < NV; v++) memcpy(b->sol[v], sol0_all + off + (size_t)v * b->n, b->n * sizeof(double));
-    off += (size_t)NV * b->n;
+    for (int v = 0; v < o->nv; v++) memcpy(b->sol[v], sol0_all + off + (size_t)v * b->n, b->n * sizeof(double));
+    off += (size_t)o->nv * b->n;
     compute_aux(o, b, b->sol);
     for (int z = 0; z < b->nbnd; z++)
       if (b->bnd[z].type == SPONGE_AMBIENT || b->bnd[z].type == SPONGE_HARMONIC) compute_sigma(o, b, &b->bnd[z]);
@@ -1028,7 +1057,7 @@ int orc_init(void *h, const double *sol0_all, int step0, double time0) {
   }
   for (int ib = 0; ib < o->nblk; ib++) {
     blk_t *b = &o->b[ib];
-    for (int v = 0; v < NV; v++)
+    for (int v = 0; v < o->nv; v++)
       for (int d = 0; d < o->ndim; d++) {
         if (deriv_xyz(o, b, b->mean[v], d)) return -1;
         memcpy(b->mgrad[v * o->ndim + d], o->dflux, b->n * sizeof(double));
@@ -1036,7 +1065,7 @@ int orc_init(void *h, const double *sol0_all, int step0, double time0) {
   }
   if (g_plant == 5)
     for (int ib = 0; ib < o->nblk; ib++)
-      for (int v = 0; v < NV * o->ndim; v++) memset(o->b[ib].mgrad[v], 0, o->b[ib].n * sizeof(double));
+      for (int v = 0; v < o->nv * o->ndim; v++) memset(o->b[ib].mgrad[v], 0, o->b[ib].n * sizeof(double));
   o->time_step = step0;
   o->time_sol = time0;
   return 0;
@@ -1046,7 +1075,7 @@ int orc_init(void *h, const double *sol0_all, int step0, double time0) {
 int orc_step(void *h) {
   orc_t *o = h;
   for (int ib = 0; ib < o->nblk; ib++)
-    for (int v = 0; v < NV; v++) memcpy(o->b[ib].sol_old[v], o->b[ib].sol[v], o->b[ib].n * sizeof(double));
+    for (int v = 0; v < o->nv; v++) memcpy(o->b[ib].sol_old[v], o->b[ib].sol[v], o->b[ib].n * sizeof(double));
   o->time_step++;
   double cmax = compute_cfl(o);
   double dt = o->dt;
@@ -1070,14 +1099,15 @@ int orc_time_step(void *h) { return ((orc_t *)h)->time_step; }
 double orc_cfl(void *h) { return compute_cfl((orc_t *)h); }
 
 void orc_get_solution(void *h, int ib, double *out) {
-  blk_t *b = &((orc_t *)h)->b[ib];
-  for (int v = 0; v < NV; v++) memcpy(out + (size_t)v * b->n, b->sol[v], b->n * sizeof(double));
+  orc_t *o = h;
+  blk_t *b = &o->b[ib];
+  for (int v = 0; v < o->nv; v++) memcpy(out + (size_t)v * b->n, b->sol[v], b->n * sizeof(double));
 }
 
 void orc_set_solution(void *h, int ib, const double *in) {
   orc_t *o = h;
   blk_t *b = &o->b[ib];
-  for (int v = 0; v < NV; v++) memcpy(b->sol[v], in + (size_t)v * b->n, b->n * sizeof(double));
+  for (int v = 0; v < o->nv; v++) memcpy(b->sol[v], in + (size_t)v * b->n, b->n * sizeof(double));
   compute_aux(o, b, b->sol);
 }
 
@@ -1099,7 +1129,7 @@ void orc_get_metrics(void *h, int ib, double *met, double *minv, double *jac) {
 void orc_get_mean_gradient(void *h, int ib, double *g) {
   orc_t *o = h;
   blk_t *b = &o->b[ib];
-  for (int v = 0; v < NV * o->ndim; v++) memcpy(g + (size_t)v * b->n, b->mgrad[v], b->n * sizeof(double));
+  for (int v = 0; v < o->nv * o->ndim; v++) memcpy(g + (size_t)v * b->n, b->mgrad[v], b->n * sizeof(double));
 }
 
 void orc_get_aux(void *h, int ib, double *aux) {
@@ -1132,8 +1162,8 @@ void orc_get_sigma(void *h, int ib, double *sig) {
 int orc_rhs(void *h, int ib, const double *y, double *rhs, double time, int with_aux_from_y) {
   orc_t *o = h;
   blk_t *b = &o->b[ib];
-  double *yy[NV], *rr[NV];
-  for (int v = 0; v < NV; v++) { yy[v] = (double *)y + (size_t)v * b->n; rr[v] = rhs + (size_t)v * b->n; }
+  double *yy[MAXV], *rr[MAXV];
+  for (int v = 0; v < o->nv; v++) { yy[v] = (double *)y + (size_t)v * b->n; rr[v] = rhs + (size_t)v * b->n; }
   if (with_aux_from_y) compute_aux(o, b, yy);
   if (rhs_lee(o, b, yy, rr)) return -1;
   physical_source(o, b, rr, time);
@@ -1144,8 +1174,8 @@ int orc_rhs(void *h, int ib, const double *y, double *rhs, double time, int with
 int orc_filter(void *h, int ib, double *y) {
   orc_t *o = h;
   blk_t *b = &o->b[ib];
-  double *yy[NV];
-  for (int v = 0; v < NV; v++) yy[v] = y + (size_t)v * b->n;
+  double *yy[MAXV];
+  for (int v = 0; v < o->nv; v++) yy[v] = y + (size_t)v * b->n;
   return apply_filter_block(o, b, yy);
 }
 
@@ -1177,11 +1207,11 @@ void orc_destroy(void *h) {
     blk_t *b = &o->b[ib];
     for (int d = 0; d < 3; d++) free(b->xyz[d]);
     free(b->met); free(b->minv); free(b->jac); free(b->iblank);
-    for (int v = 0; v < NV; v++) {
+    for (int v = 0; v < o->nv; v++) {
       free(b->mean[v]); free(b->sol[v]); free(b->sol_old[v]); free(b->rhs[v]);
       free(b->interim[v]); free(b->yn[v]); free(b->ynp1[v]);
     }
-    for (int v = 0; v < NV * 3; v++) free(b->mgrad[v]);
+    for (int v = 0; v < o->nv * 3; v++) free(b->mgrad[v]);
     for (int v = 0; v < 4; v++) free(b->aux[v]);
     for (int z = 0; z < b->nbnd; z++) free(b->bnd[z].sigma);
   }

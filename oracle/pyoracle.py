@@ -25,7 +25,9 @@ BC_TYPES = {
 }
 WAVES = {"NONE": 0, "WAVE_ACOUSTIC": 1, "WAVE_ENTROPY": 2}
 WAVEFORMS = {"WAVEFORM_PLANE": 1, "WAVEFORM_HOMOGENEOUS": 2, "WAVEFORM_GAUSSIAN_ROUNDJET": 3}
-SRC = {"CASE_TANNA_TPN49": 0, "CASE_SCATTERING_TWOCYLINDER": 1,
+# physical source per SIMULATION (apply_physicalSource, source.cpp:22-107);
+# the cases without a source term (TPN49's is empty, the nozzle has none)
+SRC = {"CASE_TANNA_TPN49": 0, "CASE_LINEAR_NOZZLE": 0, "CASE_SCATTERING_TWOCYLINDER": 1,
        "CASE_2DJET_WITH_A_HARMONIC_SOURCE": 2, "CASE_MONOPOLE": 3}
 
 _lib = None
@@ -44,6 +46,7 @@ def lib():
         L.orc_add_boundary.argtypes = [vp, i, i, i, i, ip, ip]
         L.orc_set_source.argtypes = [vp, i, dp]
         L.orc_set_axisym.argtypes = [vp, i]
+        L.orc_set_scalars.argtypes = [vp, i]
         L.orc_set_timestep.argtypes = [vp, i, d]
         L.orc_set_harmonic.argtypes = [vp, i, i, i, d, d, d]
         L.orc_set_overset.argtypes = [vp, i, ip, ip, ip, ip, ip, dp, i, i, ip, ip]
@@ -104,8 +107,9 @@ class Oracle:
     """One multi-block LEE problem on the CPU, FP64, reference semantics."""
 
     def __init__(self, blocks, scheme, bc_lines=(), source=None, mode="FIX_CFL", value=0.5,
-                 overset=None, ndim=2, gamma=1.4, harmonic=None):
-        """blocks: list of dict(x, y[, z], mean (5,...), ic (5,...)) arrays shaped (nk?, nj, ni)
+                 overset=None, ndim=2, gamma=1.4, harmonic=None, num_scalar=0):
+        """blocks: list of dict(x, y[, z], mean (nv,...), ic (nv,...)) arrays shaped (nk?, nj, ni),
+        nv = 5 + num_scalar (LEE_SCALAR: passive scalars after p')
         scheme: dict(fd_order, filter_order, filter_blend, rk, sponge_order, sponge_constant
         [, filter_strength, axisym]); order 11 selects FDO11P / SFO11P
         source: None or (kind_name, params[8])."""
@@ -117,6 +121,9 @@ class Oracle:
                               scheme["sponge_constant"])
         if not self.h:
             raise RuntimeError(L.orc_error().decode())
+        self.nv = 5 + int(num_scalar)
+        if num_scalar and L.orc_set_scalars(self.h, int(num_scalar)):
+            raise RuntimeError(L.orc_error().decode())
         if scheme.get("axisym") and L.orc_set_axisym(self.h, 1):
             raise RuntimeError(L.orc_error().decode())
         self.shapes = []
@@ -127,7 +134,7 @@ class Oracle:
             z = b.get("z")
             z = np.zeros(shp) if z is None else np.asarray(z, np.float64).reshape(shp)
             xyz = np.ascontiguousarray(np.stack([x.reshape(shp), np.asarray(b["y"], np.float64).reshape(shp), z]))
-            mean = np.ascontiguousarray(np.asarray(b["mean"], np.float64).reshape((5,) + shp))
+            mean = np.ascontiguousarray(np.asarray(b["mean"], np.float64).reshape((self.nv,) + shp))
             L.orc_set_block(self.h, ib, ni, nj, nk, _dp(xyz), _dp(mean))
             self.shapes.append(shp)
         for ln in bc_lines:
@@ -170,7 +177,7 @@ class Oracle:
     @classmethod
     def from_case(cls, case, **kw):
         src = None
-        if case["simulation"] != "CASE_TANNA_TPN49":
+        if SRC[case["simulation"]] != 0:
             src = (case["simulation"], case.get("source_params", ()))
         value = case["cfl"] if case["mode"] == "FIX_CFL" else case["dt"]
         blocks = [dict(x=case["x"], y=case["y"], mean=case["mean"], ic=case["ic"])]
@@ -178,7 +185,8 @@ class Oracle:
                                        "sponge_constant")}
         scheme["filter_strength"] = case.get("filter_strength", 1.0)
         scheme["axisym"] = case.get("axisym", 0)
-        return cls(blocks, scheme, case["bc"], src, case["mode"], value, harmonic=case.get("harmonic"), **kw)
+        return cls(blocks, scheme, case["bc"], src, case["mode"], value, harmonic=case.get("harmonic"),
+                   num_scalar=case.get("num_scalar", 0), **kw)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -213,7 +221,7 @@ class Oracle:
         return self.L.orc_cfl(self.h)
 
     def solution(self, ib=0):
-        out = np.empty((5,) + self.shapes[ib])
+        out = np.empty((self.nv,) + self.shapes[ib])
         self.L.orc_get_solution(self.h, ib, _dp(out))
         return out
 
@@ -232,7 +240,7 @@ class Oracle:
         return met.reshape((3, 3) + shp), minv.reshape((3, 3) + shp), jac.reshape(shp)
 
     def mean_gradient(self, ib=0, ndim=2):
-        out = np.empty((5 * ndim,) + self.shapes[ib])
+        out = np.empty((self.nv * ndim,) + self.shapes[ib])
         self.L.orc_get_mean_gradient(self.h, ib, _dp(out))
         return out
 

@@ -5,9 +5,10 @@ so that the same inputs come out bit-for-bit on any x86-64 host; the cases
 are small enough for that to take well under a second each.
 
 A case is a dict with:
-  name, ni, nj, x, y (float64 (nj, ni)), mean (5, nj, ni) [s, u, v, w, p],
-  ic (5, nj, ni), bc (BC-file lines), deck overrides, nsteps, and the
-  solver settings the deck carries.
+  name, ni, nj, x, y (float64 (nj, ni)), mean (nv, nj, ni) [s, u, v, w, p,
+  then the passive scalars of a LEE_SCALAR case], ic (nv, nj, ni), bc
+  (BC-file lines), deck overrides, nsteps, and the solver settings the deck
+  carries; nv = 5 + num_scalar (num_scalar absent: 0).
 """
 import math
 
@@ -325,9 +326,39 @@ def case_roundjet():
                 simulation="CASE_TANNA_TPN49", **BASE_SCHEME)
 
 
+def case_scalar():
+    """LEE_SCALAR with two passive scalars (NUMBER_SCALAR = 2) on a wavy 85x61
+    grid: shear mean u = 0.2 + 0.1 tanh(y), mean scalar fields
+    Zbar_0 = (1 + tanh(y / 1.5)) / 2 and Zbar_1 a Gaussian bump, so both the
+    ubar . grad Z' and the u' . grad Zbar terms act; a scalar blob
+    (amplitude 0.01, the size of CASE_LINEAR_NOZZLE's, state.cpp:630-639),
+    a pressure pulse and an entropy spot; wall_slip_kinematic on +eta
+    (scalars take the Neumann data), neumann on -eta, dirichlet_allzero on
+    +xi, a 10-point sponge on -xi; RK4 CFL 0.5, 50 steps.  The reference
+    admits LEE_SCALAR only with SIMULATION = CASE_LINEAR_NOZZLE
+    (input.cpp:596-606), whose initial state the restart file replaces."""
+    ni, nj = 85, 61
+    x, y = _wavy(ni, nj, 21.0, 15.0, 0.02)
+    mean = np.zeros((7, nj, ni))
+    mean[1] = _field(x, y, lambda a, b: 0.2 + 0.1 * math.tanh(b))
+    mean[4] = 1.0 / GAMMA
+    mean[5] = _field(x, y, lambda a, b: 0.5 * (1.0 + math.tanh(b / 1.5)))
+    mean[6] = _field(x, y, _pulse(1.0, -2.0, 0.0, 2.0))
+    ic = np.zeros((7, nj, ni))
+    ic[4] = _field(x, y, _pulse(1e-3, -3.0, 1.0, 1.0))
+    ic[0] = _field(x, y, _pulse(2e-4, 1.0, 2.0, 1.0))
+    ic[5] = _field(x, y, _pulse(1e-2, 2.0, -1.0, 1.0))
+    bc = [(0, "wall_slip_kinematic", "+eta", "0", "-0", "0", "0", "0", "0"),
+          (0, "neumann", "-eta", "0", "-0", "-0", "-0", "0", "0"),
+          (0, "dirichlet_allzero", "+xi", "0", "0", "0", "-0", "0", "0")]
+    bc += _sponges(10, sides=("-xi",))
+    return dict(name="scalar", ni=ni, nj=nj, x=x, y=y, mean=mean, ic=ic, bc=bc, mode="FIX_CFL", cfl=0.5,
+                dt=None, nsteps=50, num_scalar=2, simulation="CASE_LINEAR_NOZZLE", **BASE_SCHEME)
+
+
 ALL = {f.__name__[5:]: f for f in (case_c1, case_curv, case_filt, case_shear, case_cylsrc, case_wall,
                                     case_variants, case_jet, case_opt11, case_axisym, case_dirichlet,
-                                    case_centerline, case_entropy, case_roundjet)}
+                                    case_centerline, case_entropy, case_roundjet, case_scalar)}
 
 # cases whose filter (SFO11P) runs in the tile form only: the column-march
 # forms are written for the standard filters' 4-point halo
@@ -350,7 +381,8 @@ def deck_items(case, grid_file="grid.xyz", mean_file="mean.q", bc_file="bc.in"):
     items = [
         ("DIMENSION", "2"),
         ("AXISYMMETRY", "TRUE, AXIS_OF_SYMMETRY = X" if case.get("axisym") else "FALSE"),
-        ("PHYSICAL_MODEL", "LEE"),
+        ("PHYSICAL_MODEL", f"LEE_SCALAR, NUMBER_SCALAR = {case['num_scalar']}" if case.get("num_scalar")
+         else "LEE"),
         ("FLUID_MODEL", "IDEAL_GAS_CALORIC"),
         ("SIMULATION", case["simulation"]),
         ("GAMMA_SPECIFICHEAT", repr(GAMMA)),

@@ -11,7 +11,7 @@ files (deck, PLOT3D grid / base state / restart IC, BC file) into a scratch
 directory, runs `oracle/_ref/camille ic.q` there, and stores the outputs the
 parity tests need as tests/golden/<case>.npz:
 
-  sol      final solution (5, nj, ni) as written by the reference
+  sol      final solution (nv, nj, ni) as written by the reference
   tau      its PLOT3D tau header {0, 1, step, time}
   met      metrics file camille.met.q (11, nj, ni), when requested
   dts      dt printed at each report step (text, 6 digits; informative only)
@@ -59,8 +59,15 @@ def run_case(case, nranks=1):
         files = sorted(glob.glob(os.path.join(d, "camille.[0-9]*.q")))
         if len(files) < 2:
             raise RuntimeError(f"reference wrote no final solution:\n{out[-3000:]}\n{p.stderr[-2000:]}")
-        sols, taus = refio.read_solution(files[-1])
-        res = dict(sol=sols[0][:, 0], tau=taus[0])
+        if case.get("num_scalar"):
+            # more than 5 variables: the reference writes the solution in the
+            # PLOT3D function format, which has no tau (io.cpp:740-761); the
+            # time is kept as NaN and checked against the report line
+            sol = refio.read_function(files[-1])[0][:, 0]
+            res = dict(sol=sol, tau=np.array([0.0, 1.0, float(case["nsteps"]), np.nan]))
+        else:
+            sols, taus = refio.read_solution(files[-1])
+            res = dict(sol=sols[0][:, 0], tau=taus[0])
         if case["name"] in KEEP_METRICS:
             res["met"] = refio.read_function(os.path.join(d, "camille.met.q"))[0][:, 0]
         dts = [ln for ln in out.splitlines() if ln.startswith("Time step:")]
@@ -171,7 +178,7 @@ def main(names):
         arrs = dict(tau=res["tau"])
         sol = res["sol"]
         # keep only the variables that are not identically zero
-        nz = [v for v in range(5) if np.any(sol[v] != 0.0)]
+        nz = [v for v in range(sol.shape[0]) if np.any(sol[v] != 0.0)]
         arrs["vars"] = np.array(nz, dtype=np.int32)
         arrs["sol"] = sol[nz]
         if "met" in res:
@@ -179,7 +186,8 @@ def main(names):
         np.savez_compressed(path, **arrs)
         manifest[name] = dict(nsteps=case["nsteps"], ni=case["ni"], nj=case["nj"], mode=case["mode"],
                               cfl=case["cfl"], dt=case["dt"], simulation=case["simulation"],
-                              time=float(res["tau"][3]), step=int(res["tau"][2]), nonzero_vars=nz,
+                              time=None if np.isnan(res["tau"][3]) else float(res["tau"][3]),
+                              step=int(res["tau"][2]), nonzero_vars=nz, num_scalar=case.get("num_scalar", 0),
                               report=res["log"])
         print(name, "time", res["tau"][3], "vars", nz, "->", os.path.getsize(path), "bytes", flush=True)
     mpath = os.path.join(HERE, "manifest.json")

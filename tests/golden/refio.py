@@ -33,7 +33,7 @@ def write_function(path, blocks):
 
 
 def write_solution(path, blocks, step=0, time=0.0):
-    """blocks: list of arrays shaped (5, nk, nj, ni); tau = {0, 1, step, time}."""
+    """blocks: list of arrays shaped (nv, nk, nj, ni); tau = {0, 1, step, time}."""
     with open(path, "wb") as f:
         np.array([len(blocks)], dtype=np.int32).tofile(f)
         for q in blocks:
@@ -44,8 +44,8 @@ def write_solution(path, blocks, step=0, time=0.0):
             np.ascontiguousarray(q, dtype=np.float64).tofile(f)
 
 
-def read_solution(path):
-    """Returns (list of (5, nk, nj, ni) arrays, list of tau arrays)."""
+def read_solution(path, nv=5):
+    """Returns (list of (nv, nk, nj, ni) arrays, list of tau arrays)."""
     raw = open(path, "rb").read()
     nb = int(np.frombuffer(raw, np.int32, 1, 0)[0])
     dims = np.frombuffer(raw, np.int32, 3 * nb, 4).reshape(nb, 3)
@@ -55,8 +55,8 @@ def read_solution(path):
         taus.append(np.frombuffer(raw, np.float64, 4, off).copy())
         off += 32
         n = int(ni) * int(nj) * int(nk)
-        out.append(np.frombuffer(raw, np.float64, 5 * n, off).reshape(5, nk, nj, ni).copy())
-        off += 40 * n
+        out.append(np.frombuffer(raw, np.float64, nv * n, off).reshape(nv, nk, nj, ni).copy())
+        off += 8 * nv * n
     return out, taus
 
 
@@ -130,3 +130,14 @@ def write_overture(path, ndim, dims, recv, widths, holes):
             for h in holes:
                 if h[0] == b:
                     ints(list(h[1]) + [0] * (3 - len(h[1])))
+
+
+def time_matches(t, tau, report):
+    """The solver time t against a fixture: tau[3] bit for bit, or, when the
+    reference wrote a PLOT3D function file (a LEE_SCALAR run: no tau, so
+    tau[3] is NaN), against the last report line's time, which the reference
+    prints with std::scientific (6 digits)."""
+    if not np.isnan(tau[3]):
+        return t == float(tau[3])
+    last = report.strip().splitlines()[-1]
+    return f"{t:.6e}" == last.split("time:")[1].split(",")[0].strip()
