@@ -187,7 +187,7 @@ int enqueue_plain_step(lee_handle *h) {
   hipStream_t s = h->stream;
   for (int ib = 0; ib < h->nblk; ib++) {
     Block &b = h->b[ib];
-    const size_t bytes = sizeof(double) * NV * (size_t)b.n;
+    const size_t bytes = sizeof(double) * nvars(h) * (size_t)b.n;
     HIPCHK(hipMemcpyAsync(b.Y64[1], b.Y64[0], bytes, hipMemcpyDeviceToDevice, s));  // ynp1 = yn
     HIPCHK(hipMemcpyAsync(b.Y64[2], b.Y64[0], bytes, hipMemcpyDeviceToDevice, s));  // interim = yn
   }
@@ -204,7 +204,7 @@ int enqueue_plain_step(lee_handle *h) {
     for (int ib = 0; ib < h->nblk; ib++) {
       Block &b = h->b[ib];
       HIPCHK(plain_rk_update(s, b, b.Y64[3], b.Y64[1], b.Y64[2], b.Y64[0], dt, h->b_i[st], h->a_next[st],
-                             st == last));
+                             st == last, nvars(h)));
     }
     const int tgt = st < last ? 2 : 1;
     for (int ib = 0; ib < h->nblk; ib++) {
@@ -220,7 +220,7 @@ int enqueue_plain_step(lee_handle *h) {
   for (int ib = 0; ib < h->nblk; ib++) {
     Block &b = h->b[ib];
     if (h->cfg.filter_order > 0) HIPCHK(plain_filter(s, h, b, b.Y64[1]));
-    HIPCHK(plain_iblank_reset(s, b, b.Y64[1]));
+    HIPCHK(plain_iblank_reset(s, b, b.Y64[1], nvars(h)));
     std::swap(b.Y64[0], b.Y64[1]);
   }
   return LEE_OK;
@@ -692,6 +692,7 @@ int adjoint_check(lee_handle *h) {
   if (h->batch) return fail(h, LEE_EINVAL, "the adjoint runs on single handles, not batches");
   if (is_plain(h)) return fail(h, LEE_EINVAL, "the adjoint runs on the fused path");
   if (h->cfg.axisym) return fail(h, LEE_EINVAL, "the adjoint does not cover the axisymmetric term");
+  if (h->cfg.num_scalar > 0) return fail(h, LEE_EINVAL, "the adjoint does not cover passive scalars (LEE_SCALAR)");
   if (h->cfg.fd_order == 11 || h->cfg.filter_order == 11)
     return fail(h, LEE_EINVAL, "the adjoint covers the standard schemes (OA4 derivative, 4th- or 8th-order filter)");
   for (int ib = 0; ib < h->nblk; ib++) {
@@ -811,6 +812,12 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
   if ((c->fd_order == 11 || c->filter_order == 11) && c->num_dim == 3)
     return fail(nullptr, LEE_EINVAL, "the FDO11P and SFO11P schemes run in 2-D");
   if (c->axisym != 0 && c->axisym != 1) return fail(nullptr, LEE_EINVAL, "AXISYMMETRY must be 0 or 1");
+  // LEE_SCALAR (input.cpp:113-119): at least one scalar is the reference's
+  // own check, made by the deck reader; the arrays here hold at most four
+  if (c->num_scalar < 0 || c->num_scalar > LEE_MAX_SCALAR)
+    return fail(nullptr, LEE_EINVAL, "NUMBER_SCALAR must be 0 (LEE) to 4");
+  if (c->num_scalar > 0 && c->num_dim != 2)
+    return fail(nullptr, LEE_EINVAL, "passive scalars (LEE_SCALAR) run in 2-D");
   if (c->axisym && c->num_dim != 2) return fail(nullptr, LEE_EINVAL, "Axisymmetry requires 2-D.");
   if (c->filter_order == 11 && !(c->filter_strength >= 0.0 && c->filter_strength <= 1.0))
     return fail(nullptr, LEE_EINVAL, "SFO11P STRENGTH must lie in [0, 1]");
@@ -887,8 +894,11 @@ int validate(const lee_config *c, const lee_block *blocks, int nblocks) {
         }
         case LEE_BC_DIRICHLET:
         case LEE_SPONGE_FREUND_DIRICHLET:
-          // State::prescribe_on_boundary_solution for LEE (state.cpp:746-749)
-          return fail(nullptr, LEE_EINVAL, "Dirichlet boundary for the linearized Euler model is not implemented yet.");
+          // State::prescribe_on_boundary_solution for LEE and LEE_SCALAR (state.cpp:746-754)
+          return fail(nullptr, LEE_EINVAL,
+                      c->num_scalar > 0
+                          ? "Dirichlet boundary for the linearized Euler model with a scalar is not implemented yet."
+                          : "Dirichlet boundary for the linearized Euler model is not implemented yet.");
         default:
           return fail(nullptr, LEE_EINVAL, "unsupported boundary type " + std::to_string(r.type));
       }
@@ -968,6 +978,8 @@ int lee_batch_create(const lee_config *cfgs, const lee_block *blocks, int nsolve
         return fail(nullptr, LEE_EINVAL, "batched solves take no harmonic-wave boundaries");
   if (cfgs[0].fd_order == 11 || cfgs[0].filter_order == 11)
     return fail(nullptr, LEE_EINVAL, "batched solves take the standard schemes");
+  for (int k = 0; k < nsolve; k++)
+    if (cfgs[k].num_scalar > 0) return fail(nullptr, LEE_EINVAL, "batched solves carry no passive scalars");
   if (cfgs[0].path != LEE_PATH_FUSED) return fail(nullptr, LEE_EINVAL, "batched solves run on the fused path");
   if (cfgs[0].filter_order == 0) return fail(nullptr, LEE_EINVAL, "batched solves need the filter");
   return create_impl(cfgs, 1, blocks, nsolve, nullptr, true, out);
@@ -1075,26 +1087,27 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
       const int D = cfg->num_dim;
       HIPCHK(dalloc(&b.met, (size_t)D * D * n));
       HIPCHK(dalloc(&b.jac, (size_t)n));
-      HIPCHK(dalloc(&b.mean, NV * (size_t)n));
-      HIPCHK(dalloc(&b.mgrad, (size_t)D * NV * n));
+      const int nv = nvars(h);
+      HIPCHK(dalloc(&b.mean, nv * (size_t)n));
+      HIPCHK(dalloc(&b.mgrad, (size_t)D * nv * n));
       HIPCHK(dalloc(&b.rhobar, (size_t)n));
       HIPCHK(dalloc(&b.rho64, (size_t)n));
       HIPCHK(dalloc(&b.dflux, (size_t)n));
       HIPCHK(dalloc(&b.dsum, (size_t)n));
-      for (int k = 0; k < 4; k++) HIPCHK(dalloc(&b.Y64[k], NV * (size_t)n));
+      for (int k = 0; k < 4; k++) HIPCHK(dalloc(&b.Y64[k], nv * (size_t)n));
       if (is_fp32(h)) {
         HIPCHK(dalloc(&b.rho32, (size_t)n));
-        for (int k = 0; k < 4; k++) HIPCHK(dalloc(&b.Y32[k], NV * (size_t)n));
+        for (int k = 0; k < 4; k++) HIPCHK(dalloc(&b.Y32[k], nv * (size_t)n));
       }
       if (is_plain(h)) {
         HIPCHK(dalloc(&b.func0, (size_t)n));
         HIPCHK(dalloc(&b.func, (size_t)n));
       }
-      for (int k = 0; k < 4; k++) HIPCHK(hipMemsetAsync(b.Y64[k], 0, sizeof(double) * NV * n, s));
+      for (int k = 0; k < 4; k++) HIPCHK(hipMemsetAsync(b.Y64[k], 0, sizeof(double) * nv * n, s));
       if (is_fp32(h))
-        for (int k = 0; k < 4; k++) HIPCHK(hipMemsetAsync(b.Y32[k], 0, sizeof(float) * NV * n, s));
+        for (int k = 0; k < 4; k++) HIPCHK(hipMemsetAsync(b.Y32[k], 0, sizeof(float) * nv * n, s));
       HIPCHK(hipMemcpyAsync(b.xyz, in.xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
-      HIPCHK(hipMemcpyAsync(b.mean, in.mean, sizeof(double) * NV * n, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(b.mean, in.mean, sizeof(double) * nv * n, hipMemcpyHostToDevice, s));
       // rho_bar = (gamma p_bar)^(1/gamma) e^(-s_bar)  (state.cpp:713-716), host libm
       {
         std::vector<double> rb(n);
@@ -1151,8 +1164,9 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
             if (int e = integral(b.h_code_flt, &b.clean_flt)) return e;
         }
       }
-      // mean gradients after the holes exist (simulation.cpp:240, spatial:2250-2271)
-      for (int v = 0; v < NV; v++)
+      // mean gradients after the holes exist (simulation.cpp:240, spatial:2250-2271),
+      // the scalars' means included
+      for (int v = 0; v < nv; v++)
         for (int d = 0; d < D; d++) {
           if (D == 3) HIPCHK(deriv_xyz3(s, b, b.mean + (size_t)v * n, b.dflux, d));
           else HIPCHK(plain_take_derivative_xyz(s, b, h->fd, b.mean + (size_t)v * n, d));
@@ -1299,7 +1313,7 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
       HIPCHK(dalloc(&o.dl, o.nrecv));
       HIPCHK(dalloc(&o.w, 3 * (size_t)nblocks));
       HIPCHK(dalloc(&o.coef, (size_t)o.nrecv * o.stride));
-      HIPCHK(dalloc(&o.buf, (size_t)o.nrecv * NV));
+      HIPCHK(dalloc(&o.buf, (size_t)o.nrecv * nvars(h)));
       HIPCHK(hipMemcpy(o.rb, o.h_rb.data(), sizeof(int) * o.nrecv, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(o.rl, o.h_rl.data(), sizeof(int) * o.nrecv, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(o.db, o.h_db.data(), sizeof(int) * o.nrecv, hipMemcpyHostToDevice));
@@ -1362,11 +1376,12 @@ int lee_set_solution(lee_handle *h, int ib, const double *sol) {
     }
   if (h->batch && b.carry_w)
     return fail(h, LEE_EINVAL, "batched 2-D solves carry s', u', v', p' only: w' must be zero");
-  HIPCHK(hipMemcpy(b.Y64[0], sol, sizeof(double) * NV * n, hipMemcpyHostToDevice));
+  const int nv = nvars(h);
+  HIPCHK(hipMemcpy(b.Y64[0], sol, sizeof(double) * nv * n, hipMemcpyHostToDevice));
   // compute_dependent_variables after a restart (simulation.cpp:178-180)
   HIPCHK(aux_density(s, b, h->cfg.gamma, b.Y64[0], b.rho64));
   if (is_fp32(h)) {
-    HIPCHK(cast_state(s, b, NV, b.Y64[0], b.Y32[0], (size_t)NV * n));
+    HIPCHK(cast_state(s, b, nv, b.Y64[0], b.Y32[0], (size_t)nv * n));
     HIPCHK(cast_state(s, b, 1, b.rho64, b.rho32, (size_t)n));
   }
   HIPCHK(hipStreamSynchronize(s));
@@ -1378,9 +1393,9 @@ int lee_get_solution(lee_handle *h, int ib, double *sol) {
   if (!h || !sol || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
   Block &b = h->b[ib];
   hipStream_t s = h->stream;
-  const size_t cnt = (size_t)NV * b.n;
+  const size_t cnt = (size_t)nvars(h) * b.n;
   if (is_fp32(h)) {
-    HIPCHK(cast_back(s, b, NV, b.Y32[0], b.Y64[3], cnt));
+    HIPCHK(cast_back(s, b, nvars(h), b.Y32[0], b.Y64[3], cnt));
     HIPCHK(hipMemcpyAsync(sol, b.Y64[3], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
   } else {
     HIPCHK(hipMemcpyAsync(sol, b.Y64[0], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
@@ -1627,7 +1642,8 @@ int lee_last_solve_ms(lee_handle *h, float *ms) {
 int lee_compute_rhs(lee_handle *h, int ib, const double *y, double time, double *rhs) {
   if (!h || !y || !rhs || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
   Block &b = h->b[ib];
-  const size_t cnt = (size_t)NV * b.n;
+  const int nv = nvars(h);
+  const size_t cnt = (size_t)nv * b.n;
   hipStream_t s = h->stream;
   double s1, s2;
   source_factors(b.cfg, time, s1, s2);
@@ -1639,10 +1655,10 @@ int lee_compute_rhs(lee_handle *h, int ib, const double *y, double time, double 
   } else {
     StageScalars sc{1.0, 1.0, 0.0, s1, s2, he};
     if (is_fp32(h)) {
-      HIPCHK(cast_state(s, b, NV, b.Y64[2], b.Y32[2], cnt));
+      HIPCHK(cast_state(s, b, nv, b.Y64[2], b.Y32[2], cnt));
       HIPCHK(hipMemsetAsync(b.Y32[3], 0, sizeof(float) * cnt, s));
       HIPCHK(h->cfg.num_dim == 3 ? fused_stage3d(s, h, b, -1, sc) : fused_stage(s, h, b, -1, sc));
-      HIPCHK(cast_back(s, b, NV, b.Y32[3], b.Y64[3], cnt));
+      HIPCHK(cast_back(s, b, nv, b.Y32[3], b.Y64[3], cnt));
     } else {
       HIPCHK(hipMemsetAsync(b.Y64[3], 0, sizeof(double) * cnt, s));
       HIPCHK(h->cfg.num_dim == 3 ? fused_stage3d(s, h, b, -1, sc) : fused_stage(s, h, b, -1, sc));
@@ -1657,17 +1673,18 @@ int lee_apply_filter(lee_handle *h, int ib, double *y) {
   if (!h || !y || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
   if (h->cfg.filter_order == 0) return LEE_OK;
   Block &b = h->b[ib];
-  const size_t cnt = (size_t)NV * b.n;
+  const int nv = nvars(h);
+  const size_t cnt = (size_t)nv * b.n;
   hipStream_t s = h->stream;
   HIPCHK(hipMemcpy(b.Y64[1], y, sizeof(double) * cnt, hipMemcpyHostToDevice));
   if (is_plain(h)) {
     HIPCHK(plain_filter(s, h, b, b.Y64[1]));
     HIPCHK(hipMemcpyAsync(y, b.Y64[1], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
   } else if (is_fp32(h)) {
-    HIPCHK(cast_state(s, b, NV, b.Y64[1], b.Y32[1], cnt));
-    HIPCHK(cast_state(s, b, NV, b.Y64[1], b.Y32[2], cnt));
+    HIPCHK(cast_state(s, b, nv, b.Y64[1], b.Y32[1], cnt));
+    HIPCHK(cast_state(s, b, nv, b.Y64[1], b.Y32[2], cnt));
     HIPCHK(h->cfg.num_dim == 3 ? fused_filter3d(s, h, b, 1, 2, 3) : fused_filter(s, h, b, 1, 2));
-    HIPCHK(cast_back(s, b, NV, b.Y32[2], b.Y64[2], cnt));
+    HIPCHK(cast_back(s, b, nv, b.Y32[2], b.Y64[2], cnt));
     HIPCHK(hipMemcpyAsync(y, b.Y64[2], sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
   } else {
     HIPCHK(hipMemcpyAsync(b.Y64[2], b.Y64[1], sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
@@ -1695,14 +1712,15 @@ int lee_update_boundary(lee_handle *h, double *const *ys) {
   if (!h || !ys) return fail(h, LEE_EINVAL, "bad argument");
   hipStream_t s = h->stream;
   for (int ib = 0; ib < h->nblk; ib++)
-    HIPCHK(hipMemcpy(h->b[ib].Y64[2], ys[ib], sizeof(double) * NV * h->b[ib].n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->b[ib].Y64[2], ys[ib], sizeof(double) * nvars(h) * h->b[ib].n, hipMemcpyHostToDevice));
   const double he = h->cfg.harmonic_wave == LEE_WAVE_ENTROPY ? harmonic_entropy(h->cfg, h->time_sol) : 0.0;
   for (int ib = 0; ib < h->nblk; ib++)
     if (!h->b[ib].faces.empty())
       HIPCHK(apply_bcs(s, h, h->b[ib], h->b[ib].Y64[2], nullptr, false, nullptr, nullptr, he));
   HIPCHK(overset_interpolate(s, h, false, 2, false));
   for (int ib = 0; ib < h->nblk; ib++)
-    HIPCHK(hipMemcpyAsync(ys[ib], h->b[ib].Y64[2], sizeof(double) * NV * h->b[ib].n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ys[ib], h->b[ib].Y64[2], sizeof(double) * nvars(h) * h->b[ib].n, hipMemcpyDeviceToHost,
+                          s));
   HIPCHK(hipStreamSynchronize(s));
   return LEE_OK;
 }
@@ -1735,7 +1753,7 @@ int lee_get_iblank(lee_handle *h, int ib, int *iblank) {
 int lee_get_mean_gradient(lee_handle *h, int ib, double *grad) {
   if (!h || !grad || ib < 0 || ib >= h->nblk) return fail(h, LEE_EINVAL, "bad argument");
   Block &b = h->b[ib];
-  HIPCHK(hipMemcpy(grad, b.mgrad, sizeof(double) * h->cfg.num_dim * NV * b.n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(grad, b.mgrad, sizeof(double) * h->cfg.num_dim * nvars(h) * b.n, hipMemcpyDeviceToHost));
   return LEE_OK;
 }
 

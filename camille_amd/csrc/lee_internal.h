@@ -12,6 +12,7 @@
 namespace lee {
 
 constexpr int NV = 5;          // s', u', v', w', p' (src/core/param.h:64-71)
+constexpr int MAXV = NV + LEE_MAX_SCALAR;  // and the passive scalars of LEE_SCALAR (state.cpp:336-372)
 constexpr int IS = 0, IUX = 1, IUY = 2, IUZ = 3, IP = 4;
 constexpr int NOTBLANKED = 0, BLANKED = 1, INTERPOLATED = -1;  // param.h:126-128
 constexpr int MAX_ZONES = 8;   // sponge zones per block handled by the fused kernel
@@ -264,6 +265,9 @@ struct lee_handle {
 
 namespace lee {
 
+// Variables of a handle's state arrays: NV and the passive scalars.
+inline int nvars(const lee_handle *h) { return NV + h->cfg.num_scalar; }
+
 // ---- host-side planning (lee_plan.cpp) ----
 void make_fd_table(OpTable &t, int order);
 void make_filter_table(OpTable &t, int order, double strength);
@@ -295,9 +299,9 @@ hipError_t plain_take_derivative_xyz(hipStream_t s, const Block &b, const OpTabl
 hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double *y, double *rhs, double s1,
                      double s2, double *rho, double he = 0.0);
 hipError_t plain_rk_update(hipStream_t s, const Block &b, double *rhs, double *ynext, double *interim,
-                           const double *ycur, double hdt, double bi, double a, bool last);
+                           const double *ycur, double hdt, double bi, double a, bool last, int nv);
 hipError_t plain_filter(hipStream_t s, lee_handle *h, const Block &b, double *y);
-hipError_t plain_iblank_reset(hipStream_t s, const Block &b, double *y);
+hipError_t plain_iblank_reset(hipStream_t s, const Block &b, double *y, int nv);
 hipError_t aux_density(hipStream_t s, const Block &b, double gamma, const double *y, double *rho);
 // HARMONIC_WAVE entropy factor e(t) = A (1 + sin(k c0 t)), k = 2 pi / wavelength
 // (boundary_condition.cpp:232-344), and the round-jet Gaussian of a point.

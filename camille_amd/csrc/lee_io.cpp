@@ -215,7 +215,17 @@ extern "C" int lee_deck_settings(const lee_deck *d, lee_run_settings *s) {
   } else if (ax != "FALSE") {
     return io_fail("AXISYMMETRY must be TRUE or FALSE.");
   }
-  if (model != "LEE") return io_fail("PHYSICAL_MODEL = %s is not supported by the GPU advance.", model.c_str());
+  if (model == "LEE_SCALAR") {  // input.cpp:113-119, and the model/case check of :596-606, 620-621
+    c.num_scalar = r.i("PHYSICAL_MODEL", "NUMBER_SCALAR");
+    if (r.rc) return r.rc;
+    if (!(c.num_scalar >= 1))
+      return io_fail("For PHYSICAL_MODEL = LEE_SCALAR, at least one scalar should be solved for.");
+    if (c.num_scalar > LEE_MAX_SCALAR) return io_fail("NUMBER_SCALAR = %d: the GPU advance carries at most %d.",
+                                                      c.num_scalar, LEE_MAX_SCALAR);
+    if (sim != "CASE_LINEAR_NOZZLE") return io_fail("Unknown simulation for the current physical model.");
+  } else if (model != "LEE") {
+    return io_fail("PHYSICAL_MODEL = %s is not supported by the GPU advance.", model.c_str());
+  }
   if (!(c.gamma > 0.0)) return io_fail("GAMMA_SPECIFICHEAT must be positive.");
   if (fix == "FIX_DT") {
     c.fix_cfl = 0;
@@ -319,14 +329,25 @@ struct P3D {
   long data_start = 0;
 };
 
-int p3d_header(FILE *f, int kind, P3D &h) {
+// A file kind and, for solution files, their variable count: LEE_P3D_SOLUTION
+// holds 5, LEE_P3D_SOLUTION_VARS(nv) nv (false for an unknown kind).
+bool p3d_kind(int &kind, int &nsol) {
+  nsol = 5;
+  if (kind >= LEE_P3D_SOLUTION_VARS(1) && kind <= LEE_P3D_SOLUTION_VARS(64)) {
+    nsol = kind - LEE_P3D_SOLUTION_VARS(0);
+    kind = LEE_P3D_SOLUTION;
+  }
+  return kind >= 0 && kind <= 2;
+}
+
+int p3d_header(FILE *f, int kind, int nsol, P3D &h) {
   int nb = 0;
   if (fread(&nb, 4, 1, f) != 1 || nb <= 0 || nb > (1 << 20)) return io_fail("bad PLOT3D header");
   h.nblocks = nb;
   h.dims.assign(4 * (size_t)nb, 0);
   const int per = kind == LEE_P3D_FUNCTION ? 4 : 3;
   for (int b = 0; b < nb; b++) {
-    int v[4] = {0, 0, 0, kind == LEE_P3D_GRID ? 3 : 5};
+    int v[4] = {0, 0, 0, kind == LEE_P3D_GRID ? 3 : nsol};
     if (fread(v, 4, per, f) != (size_t)per) return io_fail("truncated PLOT3D header");
     for (int q = 0; q < 4; q++) {
       if (v[q] <= 0) return io_fail("bad PLOT3D block size");
@@ -345,11 +366,12 @@ long p3d_block_bytes(int kind, const int *d) {
 }  // namespace
 
 extern "C" int lee_plot3d_dims(const char *path, int kind, int *nblocks, int *dims, int cap) {
-  if (!path || !nblocks || kind < 0 || kind > 2) return io_fail("bad argument");
+  int nsol = 5;
+  if (!path || !nblocks || !p3d_kind(kind, nsol)) return io_fail("bad argument");
   FILE *f = fopen(path, "rb");
   if (!f) return io_fail("The PLOT3D file %s does not exist.", path);
   P3D h;
-  const int rc = p3d_header(f, kind, h);
+  const int rc = p3d_header(f, kind, nsol, h);
   fclose(f);
   if (rc) return rc;
   *nblocks = h.nblocks;
@@ -360,11 +382,12 @@ extern "C" int lee_plot3d_dims(const char *path, int kind, int *nblocks, int *di
 }
 
 extern "C" int lee_plot3d_read(const char *path, int kind, int iblock, double *data, double *tau) {
-  if (!path || !data || kind < 0 || kind > 2) return io_fail("bad argument");
+  int nsol = 5;
+  if (!path || !data || !p3d_kind(kind, nsol)) return io_fail("bad argument");
   FILE *f = fopen(path, "rb");
   if (!f) return io_fail("The PLOT3D file %s does not exist.", path);
   P3D h;
-  int rc = p3d_header(f, kind, h);
+  int rc = p3d_header(f, kind, nsol, h);
   if (!rc && (iblock < 0 || iblock >= h.nblocks)) rc = io_fail("block %d is not in %s", iblock, path);
   if (!rc) {
     long off = h.data_start;
@@ -383,11 +406,12 @@ extern "C" int lee_plot3d_read(const char *path, int kind, int iblock, double *d
 
 extern "C" int lee_plot3d_write(const char *path, int kind, int nblocks, const int *dims, const double *const *data,
                                 const double *tau) {
-  if (!path || !dims || !data || nblocks <= 0 || kind < 0 || kind > 2) return io_fail("bad argument");
+  int nsol = 5;
+  if (!path || !dims || !data || nblocks <= 0 || !p3d_kind(kind, nsol)) return io_fail("bad argument");
   if (kind == LEE_P3D_SOLUTION && !tau) return io_fail("a solution file needs tau");
   for (int b = 0; b < nblocks; b++)
     if (dims[4 * b] <= 0 || dims[4 * b + 1] <= 0 || dims[4 * b + 2] <= 0 || dims[4 * b + 3] <= 0 || !data[b] ||
-        (kind == LEE_P3D_GRID && dims[4 * b + 3] != 3) || (kind == LEE_P3D_SOLUTION && dims[4 * b + 3] != 5))
+        (kind == LEE_P3D_GRID && dims[4 * b + 3] != 3) || (kind == LEE_P3D_SOLUTION && dims[4 * b + 3] != nsol))
       return io_fail("bad block %d", b);
   FILE *f = fopen(path, "wb");
   if (!f) return io_fail("cannot write %s", path);
@@ -648,6 +672,8 @@ extern "C" int lee_run_deck(const char *deck_path, const char *restart_path, int
   const size_t sl = dir.rfind('/');
   dir = sl == std::string::npos ? std::string() : dir.substr(0, sl);
 
+  // variables: s', u', v', w', p' and the passive scalars (input.cpp:760-774)
+  const int nv = 5 + st.cfg.num_scalar;
   // grid and base state (io::read_grid, io::read_function)
   const std::string gfile = resolve(dir, st.grid_file), mfile = resolve(dir, st.mean_file);
   int nb = 0;
@@ -663,10 +689,11 @@ extern "C" int lee_run_deck(const char *deck_path, const char *restart_path, int
       if (md[4 * b + q] != gd[4 * b + q]) return io_fail("the base state and the grid differ in size (block %d)", b);
       d3[3 * b + q] = gd[4 * b + q];
     }
-    if (md[4 * b + 3] != 5) return io_fail("the base-state file must hold 5 variables");
+    // read_function_serialIO asserts num_vars_mean variables (plot3d.cpp:922)
+    if (md[4 * b + 3] != nv) return io_fail("the base-state file must hold %d variables", nv);
     const size_t n = (size_t)gd[4 * b] * gd[4 * b + 1] * gd[4 * b + 2];
     xyz[b].resize(3 * n);
-    mean[b].resize(5 * n);
+    mean[b].resize(nv * n);
     if ((rc = lee_plot3d_read(gfile.c_str(), LEE_P3D_GRID, b, xyz[b].data(), nullptr))) return rc;
     if ((rc = lee_plot3d_read(mfile.c_str(), LEE_P3D_FUNCTION, b, mean[b].data(), nullptr))) return rc;
   }
@@ -698,17 +725,19 @@ extern "C" int lee_run_deck(const char *deck_path, const char *restart_path, int
   int step = 0;
   double t = 0.0;
   std::vector<std::vector<double>> sol(nb);
-  for (int b = 0; b < nb; b++) sol[b].assign(5 * (size_t)d3[3 * b] * d3[3 * b + 1] * d3[3 * b + 2], 0.0);
+  for (int b = 0; b < nb; b++) sol[b].assign(nv * (size_t)d3[3 * b] * d3[3 * b + 1] * d3[3 * b + 2], 0.0);
+  // io::read_solution reads num_vars_sol variables after tau (plot3d.cpp:715-833)
+  const int rkind = nv == 5 ? LEE_P3D_SOLUTION : LEE_P3D_SOLUTION_VARS(nv);
   if (restart_path) {
     int nbr = 0;
     std::vector<int> rd(4 * nb);
-    if ((rc = lee_plot3d_dims(restart_path, LEE_P3D_SOLUTION, &nbr, rd.data(), nb))) return rc;
+    if ((rc = lee_plot3d_dims(restart_path, rkind, &nbr, rd.data(), nb))) return rc;
     if (nbr != nb) return io_fail("the restart file has %d blocks, the grid %d", nbr, nb);
     for (int b = 0; b < nb; b++) {
       for (int q = 0; q < 3; q++)
         if (rd[4 * b + q] != d3[3 * b + q]) return io_fail("the restart file and the grid differ in size (block %d)", b);
       double tau[4];
-      if ((rc = lee_plot3d_read(restart_path, LEE_P3D_SOLUTION, b, sol[b].data(), tau))) return rc;
+      if ((rc = lee_plot3d_read(restart_path, rkind, b, sol[b].data(), tau))) return rc;
       step = (int)tau[2];
       t = tau[3];
     }
@@ -727,11 +756,21 @@ extern "C" int lee_run_deck(const char *deck_path, const char *restart_path, int
   // State::initialize_state_linearizedEuler (state.cpp:284-312): radial and
   // azimuthal names in an axisymmetric run
   const bool ax = st.cfg.axisym != 0;
-  const char *names[5] = {"S'", "UX'", ax ? "UR'" : "UY'", ax ? "UTHETA'" : "UZ'", "P'"};
-  const char *mnames[5] = {"Sbar", "UXbar", ax ? "URbar" : "UYbar", ax ? "UTHETAbar" : "UZbar", "Pbar"};
-  auto write_names = [&](const std::string &fn, const char *const *nm) {
+  std::vector<std::string> names = {"S'", "UX'", ax ? "UR'" : "UY'", ax ? "UTHETA'" : "UZ'", "P'"};
+  std::vector<std::string> mnames = {"Sbar", "UXbar", ax ? "URbar" : "UYbar", ax ? "UTHETAbar" : "UZbar", "Pbar"};
+  {
+    // State::initialize_state_linearizedEuler_scalar (state.cpp:345-357)
+    // appends each counter to one stringstream: Z0', Z01', Z012', ...
+    std::string counter;
+    for (int k = 0; k < st.cfg.num_scalar; k++) {
+      counter += std::to_string(k);
+      names.push_back("Z" + counter + "'");
+      mnames.push_back("Z" + counter + "bar");
+    }
+  }
+  auto write_names = [&](const std::string &fn, const std::vector<std::string> &nm) {
     if (FILE *f = fopen(fn.c_str(), "w")) {
-      for (int v = 0; v < 5; v++) fprintf(f, "%s\n", nm[v]);
+      for (const auto &v : nm) fprintf(f, "%s\n", v.c_str());
       fclose(f);
     }
   };
@@ -746,11 +785,17 @@ extern "C" int lee_run_deck(const char *deck_path, const char *restart_path, int
     }
     return lee_plot3d_write(fn.c_str(), kind, nb, wd.data(), ptr.data(), tau);
   };
+  // io::write_solution (io.cpp:728-770): the PLOT3D solution format for 5
+  // variables, the function format (no tau) for more
+  auto write_solution = [&](const std::string &fn, const double *tau) {
+    return nv == 5 ? write_fields(fn, LEE_P3D_SOLUTION, 5, sol, tau)
+                   : write_fields(fn, LEE_P3D_FUNCTION, nv, sol, nullptr);
+  };
   {
     const double tau0[4] = {0.0, 1.0, (double)step, t};
     if ((rc = write_fields(od + "/camille.xyz", LEE_P3D_GRID, 3, xyz, nullptr)) ||
-        (rc = write_fields(od + "/camille.000000.q", LEE_P3D_SOLUTION, 5, sol, tau0)) ||
-        (rc = write_fields(od + "/mean_camille.q", LEE_P3D_FUNCTION, 5, mean, nullptr)))
+        (rc = write_solution(od + "/camille.000000.q", tau0)) ||
+        (rc = write_fields(od + "/mean_camille.q", LEE_P3D_FUNCTION, nv, mean, nullptr)))
       return rc;
     write_names(od + "/camille.nam", names);
     write_names(od + "/mean_camille.nam", mnames);
@@ -774,15 +819,11 @@ extern "C" int lee_run_deck(const char *deck_path, const char *restart_path, int
     if ((rc = lee_compute_cfl(H.h, &cfl))) return lee_fail(H.h, rc, "lee_compute_cfl");
     if (st.cfg.fix_cfl && dt > 0.0) cfl *= dt_prev / dt;
     if (!quiet) printf("Time step: %10d, time: %e, dt: %e, CFL: %e\n", step, t, dt, cfl);
-    double lo[5], hi[5];
-    for (int v = 0; v < 5; v++) {
-      lo[v] = INFINITY;
-      hi[v] = -INFINITY;
-    }
+    std::vector<double> lo(nv, INFINITY), hi(nv, -INFINITY);
     for (int b = 0; b < nb; b++) {
       if ((rc = lee_get_solution(H.h, b, sol[b].data()))) return lee_fail(H.h, rc, "lee_get_solution");
-      const size_t n = sol[b].size() / 5;
-      for (int v = 0; v < 5; v++)
+      const size_t n = sol[b].size() / nv;
+      for (int v = 0; v < nv; v++)
         for (size_t p = 0; p < n; p++) {
           lo[v] = std::min(lo[v], sol[b][v * n + p]);
           hi[v] = std::max(hi[v], sol[b][v * n + p]);
@@ -791,7 +832,7 @@ extern "C" int lee_run_deck(const char *deck_path, const char *restart_path, int
     const std::string rep = od + "/report/solution/solution";
     if (FILE *f = fopen(rep.c_str(), "a")) {
       fprintf(f, "Time step: %10d, time: %e", step, t);
-      for (int v = 0; v < 5; v++) fprintf(f, ", %s: %e ~ %e", names[v], lo[v], hi[v]);
+      for (int v = 0; v < nv; v++) fprintf(f, ", %s: %e ~ %e", names[v].c_str(), lo[v], hi[v]);
       fprintf(f, "\n");
       fclose(f);
     }
@@ -800,7 +841,7 @@ extern "C" int lee_run_deck(const char *deck_path, const char *restart_path, int
       char fn[64];
       snprintf(fn, sizeof fn, "/camille.%06d.q", counter);
       const double tau[4] = {0.0, 1.0, (double)step, t};
-      if ((rc = write_fields(od + fn, LEE_P3D_SOLUTION, 5, sol, tau))) return rc;
+      if ((rc = write_solution(od + fn, tau))) return rc;
       write_names(od + "/camille.nam", names);
     }
   }

@@ -175,20 +175,21 @@ __global__ void __launch_bounds__(TPB) k_plain_source(double *__restrict__ rp, c
   rp[(size_t)j * ni + i] += s1 * g[q] * s2;
 }
 
-// apply_bufferZone_Freund_ambient (source.cpp:174-201), one zone, all 5
-// vars; a harmonic zone (apply_bufferZone_Freund_harmonicWave, source.cpp:
-// 264-320) relaxes s' to the entropy wave he * hg (hg null: he) and the
-// other variables to 0, the data of bc_dirichlet_harmonicwave for LEE.
+// apply_bufferZone_Freund_ambient (source.cpp:174-201), one zone, all nv
+// vars (the passive scalars too); a harmonic zone
+// (apply_bufferZone_Freund_harmonicWave, source.cpp:264-320) relaxes s' to
+// the entropy wave he * hg (hg null: he) and the other variables to 0, the
+// data of bc_dirichlet_harmonicwave for LEE.
 __global__ void __launch_bounds__(TPB) k_plain_sponge(double *__restrict__ rhs, const double *__restrict__ y,
                                                       const double *__restrict__ sig, int ni, int n, int i0,
                                                       int nzi, int j0, int nzj, int harm, double he,
-                                                      const double *__restrict__ hg) {
+                                                      const double *__restrict__ hg, int nv) {
 #pragma clang fp contract(off)
   const int q = blockIdx.x * TPB + threadIdx.x;
   if (q >= nzi * nzj) return;
   const int p = (j0 + q / nzi) * ni + (i0 + q % nzi);
   const double s = sig[q];
-  for (int v = 0; v < NV; v++) {
+  for (int v = 0; v < nv; v++) {
     const double target = harm && v == IS ? (hg ? he * hg[q] : he) : 0.0;
     const double relax = -s * (y[(size_t)v * n + p] - target);
     rhs[(size_t)v * n + p] += relax;
@@ -199,19 +200,19 @@ __global__ void __launch_bounds__(TPB) k_plain_sponge(double *__restrict__ rhs, 
 __global__ void __launch_bounds__(TPB) k_plain_rk(double *__restrict__ rhs, double *__restrict__ ynext,
                                                   double *__restrict__ interim, const double *__restrict__ ycur,
                                                   const int8_t *__restrict__ iblank, int n, double h, double bi,
-                                                  double a, int last) {
+                                                  double a, int last, int nv) {
 #pragma clang fp contract(off)
   const int p = blockIdx.x * TPB + threadIdx.x;
   if (p >= n) return;
   if (!iblank || iblank[p] != BLANKED) {
-    for (int v = 0; v < NV; v++) {
+    for (int v = 0; v < nv; v++) {
       const size_t q = (size_t)v * n + p;
       rhs[q] *= h;
       ynext[q] += bi * rhs[q];
       if (!last) interim[q] = ycur[q] + a * rhs[q];
     }
   } else {
-    for (int v = 0; v < NV; v++) {
+    for (int v = 0; v < nv; v++) {
       const size_t q = (size_t)v * n + p;
       rhs[q] = 0.0;
       ynext[q] = 0.0;
@@ -244,10 +245,10 @@ __global__ void __launch_bounds__(TPB) k_blend(double *__restrict__ y, const dou
 
 // temporal::iblank_correction (temporal:380-393): holes -> sol_ref = 0
 __global__ void __launch_bounds__(TPB) k_iblank_reset(double *__restrict__ y, const int8_t *__restrict__ iblank,
-                                                      int n) {
+                                                      int n, int nv) {
   const int p = blockIdx.x * TPB + threadIdx.x;
   if (p >= n || iblank[p] != BLANKED) return;
-  for (int v = 0; v < NV; v++) y[(size_t)v * n + p] = 0.0;
+  for (int v = 0; v < nv; v++) y[(size_t)v * n + p] = 0.0;
 }
 
 // ------------------------------------------------------ boundary conditions
@@ -260,6 +261,7 @@ struct BcArgs {
   int sb;
   double he;            // dirichlet_harmonicwave: entropy factor e(t) at the BC time
   const double *hg;     // and g per face point (null: 1)
+  int nsc;              // passive scalars (variables NV .. NV + nsc - 1)
 };
 
 // bc::apply_BC one face (boundary_condition.cpp:46-209) with bc_neumann
@@ -344,6 +346,31 @@ __global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double
   if (a.type >= LEE_BC_CENTERLINE_CART_NORM2X && a.type <= LEE_BC_CENTERLINE_CART_NORM2Z)
     data[IUX + (a.type - LEE_BC_CENTERLINE_CART_NORM2X)] = 0.0;
   if (a.type == LEE_BC_CENTERLINE_AXISYM) data[IUY] = 0.0;  // u_r on the axis (bc:498-545, IVAR_UR)
+  // passive scalars: Neumann data at Neumann, wall-slip (its non-velocity
+  // variables, boundary_condition.cpp:122-136) and centerline faces; 0 at
+  // dirichlet_allzero and harmonic-wave faces, whose data start at 0
+  for (int z = 0; z < a.nsc; z++) {
+    const size_t vo = (size_t)(NV + z) * n;
+    double d = 0.0;
+    if (neumann) {
+      const int sg = a.end == 0 ? 1 : -1;
+      T nbz[8];
+#pragma unroll
+      for (int s = 1; s <= 8; s++) {
+        const int cs = c + sg * s;
+        nbz[s - 1] = y[vo + p + sg * mf * (cs >= 0 && cs < len ? s : 0)];
+      }
+#pragma unroll
+      for (int s = 1; s <= 8; s++) {
+        const int cs = c + sg * s;
+        const bool use = s < a.sb && cs >= 0 && cs < len;
+        const double t1 = a.end == 0 ? d - a.neu[s] * (double)nbz[s - 1] : d + a.neu[s] * (double)nbz[s - 1];
+        d = use ? t1 : d;
+      }
+      d /= a.end == 0 ? a.neu[0] : -a.neu[0];
+    }
+    y[vo + p] = (T)d;
+  }
   for (int v = 0; v < NV; v++) y[(size_t)v * n + p] = (T)data[v];
   if (rho) {
     const double pbi = 1.0 / mean[(size_t)IP * n + p];
@@ -357,6 +384,7 @@ __global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double
 
 template <typename T>
 struct OvTables {
+  int nv;  // variables per point (NV and the passive scalars)
   T *y[MAX_OV_BLOCKS];
   int ni[MAX_OV_BLOCKS], nn[MAX_OV_BLOCKS];
   T *rho[MAX_OV_BLOCKS];
@@ -395,8 +423,7 @@ __global__ void __launch_bounds__(TPB) k_ov_gather(OvTables<T> t, const int *__r
       li[q] = ok[q] ? l1 : 0;
       cc[q] = c[q];
     }
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
+    for (int v = 0; v < t.nv; v++) {
       T yv[25];
 #pragma unroll
       for (int q = 0; q < 25; q++) yv[q] = y[(size_t)v * n + li[q]];
@@ -406,20 +433,21 @@ __global__ void __launch_bounds__(TPB) k_ov_gather(OvTables<T> t, const int *__r
         const double t1 = a + cc[q] * (double)yv[q];
         a = ok[q] ? t1 : a;
       }
-      buf[(size_t)r * NV + v] = a;
+      buf[(size_t)r * t.nv + v] = a;
     }
     return;
   }
-  double acc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[MAXV];
+  for (int v = 0; v < MAXV; v++) acc[v] = 0.0;
   int s = 0;
   for (int b2 = 0; b2 < w2; b2++)
     for (int b1 = 0; b1 < w1; b1++) {
       const int l1 = dl[r] + b1 + b2 * ni;
       if (l1 >= 0 && l1 < n)
-        for (int v = 0; v < NV; v++) acc[v] += c[s] * (double)y[(size_t)v * n + l1];
+        for (int v = 0; v < t.nv; v++) acc[v] += c[s] * (double)y[(size_t)v * n + l1];
       s++;
     }
-  for (int v = 0; v < NV; v++) buf[(size_t)r * NV + v] = acc[v];
+  for (int v = 0; v < t.nv; v++) buf[(size_t)r * t.nv + v] = acc[v];
 }
 
 // compute_weighted_sum for 5 x 5 stencils, one thread per (receiver,
@@ -432,7 +460,7 @@ __global__ void __launch_bounds__(TPB) k_ov_gather5(OvTables<T> t, const int *__
                                                     double *__restrict__ buf, int nrecv) {
 #pragma clang fp contract(off)
   const int gid = blockIdx.x * TPB + threadIdx.x;
-  if (gid >= NV * nrecv) return;
+  if (gid >= t.nv * nrecv) return;
   const int v = gid / nrecv, r = gid - v * nrecv;
   const int b = db[r];
   const T *y = t.y[b] + (size_t)v * t.nn[b];
@@ -453,7 +481,7 @@ __global__ void __launch_bounds__(TPB) k_ov_gather5(OvTables<T> t, const int *__
     const double t1 = a + cc[q] * (double)yv[q];
     a = ok[q] ? t1 : a;
   }
-  buf[(size_t)r * NV + v] = a;
+  buf[(size_t)r * t.nv + v] = a;
 }
 
 // update_each_grid (overset.cpp:1241-1278); optionally refreshes rho'.
@@ -467,10 +495,10 @@ __global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__
   const int b = rb[r];
   const int n = t.nn[b], p = rl[r];
   if (p < 0 || p >= n) return;
-  for (int v = 0; v < NV; v++) t.y[b][(size_t)v * n + p] = (T)buf[(size_t)r * NV + v];
+  for (int v = 0; v < t.nv; v++) t.y[b][(size_t)v * n + p] = (T)buf[(size_t)r * t.nv + v];
   if (with_rho && t.rho[b]) {
     const double pbi = 1.0 / t.mean[b][(size_t)IP * n + p];
-    double q = buf[(size_t)r * NV + IP] * gi * pbi - buf[(size_t)r * NV + IS];
+    double q = buf[(size_t)r * t.nv + IP] * gi * pbi - buf[(size_t)r * t.nv + IS];
     q *= t.rhob[b][p];
     t.rho[b][p] = (T)q;
   }
@@ -892,6 +920,74 @@ __global__ void __launch_bounds__(TPB) k_passive_w(StageArgs<T> a, int kind) {
   }
 }
 
+// One RK stage of the passive scalars of a LEE_SCALAR run, one thread per
+// (scalar, point), after the stage kernel of the same buffers (it writes
+// only the five LEE variables of acc and yout; this kernel only the
+// scalars'): compute_RHS_linearized_Euler_scalar (spatial:2134-2158),
+//   rhs_Z = -ubar dZ'/dx - u' dZbar/dx - vbar dZ'/dy - v' dZbar/dy
+// in the reference's term order, the Freund sponges relaxing Z' to 0
+// (source.cpp:174-201, 264-320), the RK update and the hole reset as in
+// the stage kernel.  Derivative rows read the stage input in place (the
+// closure rows stay inside the block); metrics, J, the mean velocity and
+// the scalars' mean gradients are FP64.  W: OA4 (4) or FDO11P (11) rows.
+template <typename T, int KIND, int W>
+__global__ void __launch_bounds__(TPB) k_scalar_stage(StageArgs<T> a, int nsc) {
+#pragma clang fp contract(off)
+  const int ni = a.ni, nj = a.nj, n = ni * nj;
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= nsc * n) return;
+  const int z = t / n, p = t - z * n;
+  const int gi = p % ni, gj = p / ni;
+  const size_t q = (size_t)(NV + z) * n + p;
+  int cx, cy;
+  if (a.code) {
+    cx = a.code[p];
+    cy = a.code[n + p];
+  } else {
+    cx = default_code(gi, ni, StageOp<W>::NB);
+    cy = default_code(gj, nj, StageOp<W>::NB);
+  }
+  const T *sp = a.yin + q;
+  T dx, dy;
+  if constexpr (W == 4) {
+    dx = fd_row<T>(sp, 1, cx);
+    dy = fd_row<T>(sp, ni, cy);
+  } else {
+    dx = wide_row<T>(sp, 1, cx, a.fdw);
+    dy = wide_row<T>(sp, ni, cy, a.fdw);
+  }
+  const double mxx = a.met[p], mxy = a.met[(size_t)n + p], myx = a.met[(size_t)2 * n + p],
+               myy = a.met[(size_t)3 * n + p];
+  const double J = a.jac[p];
+  const T fx = (T)(fmx(mxx, (double)dx, myx * (double)dy) * J);
+  const T fy = (T)(fmx(mxy, (double)dx, myy * (double)dy) * J);
+  const T u0 = (T)a.mean[(size_t)IUX * n + p], v0 = (T)a.mean[(size_t)IUY * n + p];
+  const T up = a.yin[(size_t)IUX * n + p], vp = a.yin[(size_t)IUY * n + p];
+  const T gzx = (T)a.mgrad[(size_t)(2 * (NV + z)) * n + p], gzy = (T)a.mgrad[(size_t)(2 * (NV + z) + 1) * n + p];
+  const T zc = sp[0];
+  T r = -(u0 * fx);
+  r = fmx(-up, gzx, r);
+  r = fmx(-v0, fy, r);
+  r = fmx(-vp, gzy, r);
+  for (int k = 0; k < a.nz; k++) {
+    const ZoneT<T> &zz = a.z[k];
+    if (gi >= zz.i0 && gi <= zz.i1 && gj >= zz.j0 && gj <= zz.j1)
+      r = fmx(-zz.sig[(gj - zz.j0) * zz.nzi + (gi - zz.i0)], zc, r);
+  }
+  T accv = 0, outv = 0;
+  if (!(a.iblank && a.iblank[p] == BLANKED)) {
+    const T rh = r * a.h;
+    const T acc_in = KIND == STAGE_FIRST ? zc : a.acc[q];
+    accv = fmx(a.b, rh, acc_in);
+    if (KIND != STAGE_LAST) {
+      const T yn = KIND == STAGE_FIRST ? zc : a.yn[q];
+      outv = fmx(a.a, rh, yn);
+    }
+  }
+  a.acc[q] = accv;
+  if (KIND != STAGE_LAST) a.yout[q] = outv;
+}
+
 template <typename S, typename D>
 __global__ void __launch_bounds__(TPB) k_cast(const S *__restrict__ a, D *__restrict__ b, size_t n) {
   const size_t p = (size_t)blockIdx.x * TPB + threadIdx.x;
@@ -962,7 +1058,8 @@ hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double 
     const hipError_t e = plain_take_derivative_xyz(s, b, h->fd, f, d);
     if (err == hipSuccess) err = e;
   };
-  hipLaunchKernelGGL(k_fill, dim3(nblocks((size_t)NV * n)), t, 0, s, rhs, 0.0, (size_t)NV * n);
+  const int nv = nvars(h);
+  hipLaunchKernelGGL(k_fill, dim3(nblocks((size_t)nv * n)), t, 0, s, rhs, 0.0, (size_t)nv * n);
   for (int d = 0; d < 2; d++) {
     deriv(Y(IS), d);
     hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(IS), M(IUX + d), b.dflux, n);
@@ -988,6 +1085,14 @@ hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double 
   }
   if (h->cfg.axisym)
     hipLaunchKernelGGL(k_plain_axisym, g, t, 0, s, R(IP), gam, M(IP), Y(IUY), Y(IP), M(IUY), b.xyz + n, n);
+  // passive scalars, compute_RHS_linearized_Euler_scalar (spatial:2134-2158)
+  // after the LEE terms (spatial:1977-1979)
+  for (int v = NV; v < nv; v++)
+    for (int d = 0; d < 2; d++) {
+      deriv(Y(v), d);
+      hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(v), M(IUX + d), b.dflux, n);
+      hipLaunchKernelGGL(k_sub_ab, g, t, 0, s, R(v), Y(IUX + d), G(v, d), n);
+    }
   if (b.src.i1 >= b.src.i0) {
     const size_t cnt = (size_t)(b.src.i1 - b.src.i0 + 1) * (b.src.j1 - b.src.j0 + 1);
     hipLaunchKernelGGL(k_plain_source, dim3(nblocks(cnt)), t, 0, s, R(IP), b.src.g, b.ni, b.src.i0, b.src.i1,
@@ -996,16 +1101,16 @@ hipError_t plain_rhs(hipStream_t s, lee_handle *h, const Block &b, const double 
   for (const auto &z : b.zones) {
     const size_t cnt = (size_t)z.nz[0] * z.nz[1];
     hipLaunchKernelGGL(k_plain_sponge, dim3(nblocks(cnt)), t, 0, s, rhs, y, z.sigma64, b.ni, n, z.is[0], z.nz[0],
-                       z.is[1], z.nz[1], z.harmonic ? 1 : 0, he, z.hg);
+                       z.is[1], z.nz[1], z.harmonic ? 1 : 0, he, z.hg, nv);
   }
   const hipError_t last = hipGetLastError();
   return err != hipSuccess ? err : last;
 }
 
 hipError_t plain_rk_update(hipStream_t s, const Block &b, double *rhs, double *ynext, double *interim,
-                           const double *ycur, double hdt, double bi, double a, bool last) {
+                           const double *ycur, double hdt, double bi, double a, bool last, int nv) {
   hipLaunchKernelGGL(k_plain_rk, dim3(nblocks(b.n)), dim3(TPB), 0, s, rhs, ynext, interim, ycur,
-                     b.has_holes ? b.iblank : nullptr, b.n, hdt, bi, a, last ? 1 : 0);
+                     b.has_holes ? b.iblank : nullptr, b.n, hdt, bi, a, last ? 1 : 0, nv);
   return hipGetLastError();
 }
 
@@ -1013,7 +1118,7 @@ hipError_t plain_filter(hipStream_t s, lee_handle *h, const Block &b, double *y)
   const int n = b.n;
   const dim3 g(nblocks(n)), t(TPB);
   const double blend = h->cfg.filter_blend, rem = 1.0 - blend;
-  for (int v = 0; v < NV; v++) {
+  for (int v = 0; v < nvars(h); v++) {
     double *yv = y + (size_t)v * n;
     hipLaunchKernelGGL(k_copy, g, t, 0, s, yv, b.func0, (size_t)n);
     hipLaunchKernelGGL(k_copy, g, t, 0, s, b.func0, b.func, (size_t)n);
@@ -1027,9 +1132,9 @@ hipError_t plain_filter(hipStream_t s, lee_handle *h, const Block &b, double *y)
   return hipGetLastError();
 }
 
-hipError_t plain_iblank_reset(hipStream_t s, const Block &b, double *y) {
+hipError_t plain_iblank_reset(hipStream_t s, const Block &b, double *y, int nv) {
   if (!b.has_holes) return hipSuccess;
-  hipLaunchKernelGGL(k_iblank_reset, dim3(nblocks(b.n)), dim3(TPB), 0, s, y, b.iblank, b.n);
+  hipLaunchKernelGGL(k_iblank_reset, dim3(nblocks(b.n)), dim3(TPB), 0, s, y, b.iblank, b.n, nv);
   return hipGetLastError();
 }
 
@@ -1049,6 +1154,7 @@ hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, 
     a.ni = b.ni;
     a.nj = b.nj;
     a.sb = h->fd.sb;
+    a.nsc = h->cfg.num_scalar;
     for (int q = 0; q < 9; q++) a.neu[q] = h->fd.bnd[0][q];
     const size_t cnt = (size_t)(a.i1 - a.i0 + 1) * (a.j1 - a.j0 + 1);
     const int8_t *ib = b.has_holes ? b.iblank : nullptr;
@@ -1065,6 +1171,7 @@ hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, 
 template <typename T>
 static hipError_t overset_t(hipStream_t s, lee_handle *h, T *const *ys, T *const *rhos, bool with_rho, double *buf) {
   OvTables<T> t{};
+  t.nv = nvars(h);
   for (int ib = 0; ib < h->nblk; ib++) {
     t.y[ib] = ys[ib];
     t.ni[ib] = h->b[ib].ni;
@@ -1075,7 +1182,7 @@ static hipError_t overset_t(hipStream_t s, lee_handle *h, T *const *ys, T *const
   }
   const Overset &o = h->ov;
   if (o.coef5)
-    hipLaunchKernelGGL(k_ov_gather5<T>, dim3(nblocks((size_t)NV * o.nrecv)), dim3(TPB), 0, s, t, o.db, o.dl, o.coef5,
+    hipLaunchKernelGGL(k_ov_gather5<T>, dim3(nblocks((size_t)t.nv * o.nrecv)), dim3(TPB), 0, s, t, o.db, o.dl, o.coef5,
                        buf, o.nrecv);
   else
     hipLaunchKernelGGL(k_ov_gather<T>, dim3(nblocks(o.nrecv)), dim3(TPB), 0, s, t, o.db, o.dl, o.w, o.coef, o.stride,
@@ -1233,6 +1340,18 @@ static hipError_t fused_stage_t(hipStream_t s, lee_handle *h, Block &b, int stag
 #undef LEE_LAUNCH
   if (b.carry_w)
     hipLaunchKernelGGL(k_passive_w<T>, dim3(nblocks(b.n)), dim3(TPB), 0, s, a, kind);
+  if (const int nsc = h->cfg.num_scalar) {
+    const dim3 sg(nblocks((size_t)nsc * b.n));
+#define LEE_SCALARS(K)                                                                              \
+  do {                                                                                              \
+    if (wide) hipLaunchKernelGGL((k_scalar_stage<T, K, 11>), sg, dim3(TPB), 0, s, a, nsc);          \
+    else hipLaunchKernelGGL((k_scalar_stage<T, K, 4>), sg, dim3(TPB), 0, s, a, nsc);                \
+  } while (0)
+    if (kind == STAGE_FIRST) LEE_SCALARS(STAGE_FIRST);
+    else if (kind == STAGE_MID) LEE_SCALARS(STAGE_MID);
+    else LEE_SCALARS(STAGE_LAST);
+#undef LEE_SCALARS
+  }
   return hipGetLastError();
 }
 
@@ -1362,9 +1481,35 @@ static hipError_t fused_filter_t(hipStream_t s, lee_handle *h, Block &b, T *cons
   return hipGetLastError();
 }
 
+// The passive scalars of a LEE_SCALAR run (apply_filter filters every
+// solution variable, spatial:2191-2246): one tile-form launch of one
+// variable each, tiles of 8 rows (every filter form computes each point
+// with the same operations).
+template <typename T>
+static hipError_t scalar_filter_t(hipStream_t s, lee_handle *h, Block &b, T *const *Y, int src, int dst) {
+  FilterArgs<T> a = make_filter_args<T>(h, b, Y, src, dst, 8);
+  const dim3 grid(a.chunk * 8), block(64, 4);
+  const int o = h->cfg.filter_order;
+  for (int z = 0; z < h->cfg.num_scalar; z++) {
+    a.vars[0] = NV + z;
+#define LEE_FLZ(HO)                                                                       \
+  do {                                                                                    \
+    if (o == 11) hipLaunchKernelGGL((k_fused_filter<T, HO, 11, 1, 8>), grid, block, 0, s, a); \
+    else if (o == 8) hipLaunchKernelGGL((k_fused_filter<T, HO, 8, 1, 8>), grid, block, 0, s, a); \
+    else hipLaunchKernelGGL((k_fused_filter<T, HO, 4, 1, 8>), grid, block, 0, s, a);   \
+  } while (0)
+    if (b.has_holes) LEE_FLZ(true);
+    else LEE_FLZ(false);
+#undef LEE_FLZ
+  }
+  return hipGetLastError();
+}
+
 hipError_t fused_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst) {
-  if (h->cfg.precision == LEE_FP32) return fused_filter_t<float>(s, h, b, b.Y32, src, dst);
-  return fused_filter_t<double>(s, h, b, b.Y64, src, dst);
+  const bool f32 = h->cfg.precision == LEE_FP32;
+  hipError_t e = f32 ? fused_filter_t<float>(s, h, b, b.Y32, src, dst) : fused_filter_t<double>(s, h, b, b.Y64, src, dst);
+  if (e != hipSuccess || h->cfg.num_scalar == 0) return e;
+  return f32 ? scalar_filter_t<float>(s, h, b, b.Y32, src, dst) : scalar_filter_t<double>(s, h, b, b.Y64, src, dst);
 }
 
 // ------------------------------------------------------------ batched ensemble

@@ -81,9 +81,11 @@ int lee_update_boundary_noalloc(lee_handle *h, double *const *ys_dev) {
       h->err = "null device array for block " + std::to_string(ib);
       return LEE_EINVAL;
     }
-  if (h->ov.nrecv > LEE_NOALLOC_MAX_RECV) {
-    h->err = "lee_update_boundary_noalloc stages at most " + std::to_string(LEE_NOALLOC_MAX_RECV) +
-             " overset receivers; this handle has " + std::to_string(h->ov.nrecv);
+  // nrecv receivers stage nvars doubles each (5, plus the passive scalars)
+  if ((size_t)h->ov.nrecv * nvars(h) > (size_t)LEE_NOALLOC_MAX_RECV * NV) {
+    h->err = "lee_update_boundary_noalloc stages at most " + std::to_string((size_t)LEE_NOALLOC_MAX_RECV * NV) +
+             " receiver values; this handle has " + std::to_string(h->ov.nrecv) + " receivers of " +
+             std::to_string(nvars(h)) + " variables";
     return LEE_EINVAL;
   }
   hipError_t lookup = hipSuccess;

@@ -14,6 +14,12 @@ from .lee import LeeError, _Boundary, _Config, _Overset, library
 GRID, SOLUTION, FUNCTION = 0, 1, 2
 
 
+def SOLUTION_VARS(nv):
+    """Kind of a solution file whose blocks hold nv variables after tau (the
+    restart file of a LEE_SCALAR run)."""
+    return 100 + int(nv)
+
+
 class _Settings(ctypes.Structure):
     _fields_ = [("cfg", _Config), ("num_time_steps", ctypes.c_int), ("report_freq", ctypes.c_int),
                 ("time_writing_solutions", ctypes.c_double), ("overset", ctypes.c_int),
@@ -112,7 +118,8 @@ def plot3d_dims(path, kind):
 
 
 def plot3d_read(path, kind, iblock):
-    """(nvar, nk, nj, ni) array of block iblock, and tau for a solution file."""
+    """(nvar, nk, nj, ni) array of block iblock, and tau for a solution file
+    (SOLUTION or SOLUTION_VARS(nv))."""
     dims = plot3d_dims(path, kind)
     if not 0 <= iblock < len(dims):
         raise LeeError(f"block {iblock} is not in {path}")
@@ -120,7 +127,7 @@ def plot3d_read(path, kind, iblock):
     out = np.empty((nv, nk, nj, ni))
     tau = np.zeros(4)
     _check(_lib().lee_plot3d_read(_b(path), kind, iblock, _dp(out), _dp(tau)))
-    return (out, tau) if kind == SOLUTION else out
+    return (out, tau) if kind == SOLUTION or kind > 100 else out
 
 
 def plot3d_write(path, kind, blocks, tau=None):

@@ -39,6 +39,7 @@ SOURCES = {
     "SOURCE_FIELD": 4,
     "CASE_TANNA_TPN49": 0,
     "CASE_KBK_COMBUSTOR": 0,
+    "CASE_LINEAR_NOZZLE": 0,
     "CASE_SCATTERING_TWOCYLINDER": 1,
     "CASE_2DJET_WITH_A_HARMONIC_SOURCE": 2,
     "CASE_MONOPOLE": 3,
@@ -75,6 +76,7 @@ class _Config(ctypes.Structure):
         ("harmonic_amplitude", ctypes.c_double),
         ("harmonic_wavelength", ctypes.c_double),
         ("harmonic_halfwidth", ctypes.c_double),
+        ("num_scalar", ctypes.c_int),
     ]
 
 
@@ -230,12 +232,14 @@ def read_bc_file(path, dims_per_block):
 def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, filter_strength=1.0, rk=4,
                  mode="FIX_CFL", value=0.5,
                  sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49", source_params=(),
-                 precision=FP64, path=PATH_FUSED, num_dim=2, axisym=0, harmonic=None):
+                 precision=FP64, path=PATH_FUSED, num_dim=2, axisym=0, harmonic=None, num_scalar=0):
     """lee_config from the deck settings (src/core/input.cpp:99-284 keys).
     harmonic: None or dict(wave, shape, direction, amplitude, wavelength[,
     halfwidth]) with the HARMONIC_WAVE names (e.g. wave="WAVE_ENTROPY",
-    shape="WAVEFORM_HOMOGENEOUS", direction="X")."""
+    shape="WAVEFORM_HOMOGENEOUS", direction="X").  num_scalar: the passive
+    scalars of PHYSICAL_MODEL = LEE_SCALAR (0: LEE)."""
     cfg = _Config()
+    cfg.num_scalar = int(num_scalar)
     if harmonic:
         cfg.harmonic_wave = WAVES[harmonic["wave"]]
         cfg.harmonic_shape = WAVEFORMS.get(harmonic.get("shape", ""), 0)
@@ -266,24 +270,28 @@ def _make_config(*, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, fil
 class Solver:
     """One lee_handle: a set of blocks advanced on the current HIP device.
 
-    blocks: list of dict(x, y, [z], mean (5, nj, ni)) with x, y shaped (nj, ni)
+    blocks: list of dict(x, y, [z], mean (nv, nj, ni)) with x, y shaped (nj, ni)
+    and nv = 5 + num_scalar (the passive scalars of LEE_SCALAR after p')
     boundaries: list of parsed BC entries (parse_bc_line) or raw BC-file
     token tuples.  overset: dict of receiver/donor/hole tables or None.
     """
 
+    nv = NV  # variables per point; a LEE_SCALAR handle adds its scalars
+
     def __init__(self, blocks, *, gamma=1.4, fd_order=4, filter_order=8, filter_blend=1.0, rk=4,
                  mode="FIX_CFL", value=0.5, sponge_order=2, sponge_constant=5.0, simulation="CASE_TANNA_TPN49",
                  source_params=(), precision=FP64, path=PATH_FUSED, boundaries=(), overset=None, num_dim=2,
-                 filter_strength=1.0, axisym=0, harmonic=None):
+                 filter_strength=1.0, axisym=0, harmonic=None, num_scalar=0):
         L = library()
         self.L = L
         self.num_dim = num_dim
+        self.nv = NV + int(num_scalar)
         self.cfg = _make_config(gamma=gamma, fd_order=fd_order, filter_order=filter_order,
                                 filter_blend=filter_blend, filter_strength=filter_strength, rk=rk, mode=mode,
                                 value=value,
                                 sponge_order=sponge_order, sponge_constant=sponge_constant,
                                 simulation=simulation, source_params=source_params, precision=precision, path=path,
-                                num_dim=num_dim, axisym=axisym, harmonic=harmonic)
+                                num_dim=num_dim, axisym=axisym, harmonic=harmonic, num_scalar=num_scalar)
         self._keep = []
         barr = self._blocks(blocks, boundaries)
         ovp = None
@@ -312,7 +320,7 @@ class Solver:
             ni, nj, nk = dims[ib]
             z = np.zeros_like(x) if b.get("z") is None else np.asarray(b["z"], np.float64)
             xyz = np.ascontiguousarray(np.stack([x, np.asarray(b["y"], np.float64), z]))
-            mean = np.ascontiguousarray(np.asarray(b["mean"], np.float64).reshape((5,) + x.shape))
+            mean = np.ascontiguousarray(np.asarray(b["mean"], np.float64).reshape((self.nv,) + x.shape))
             mine = [p for p in parsed if p[0] == ib]
             bar = (_Boundary * max(1, len(mine)))()
             for k, (_, typ, dr, end, is3, ie3) in enumerate(mine):
@@ -356,7 +364,7 @@ class Solver:
                 mode=case["mode"], value=value, sponge_order=case["sponge_order"],
                 sponge_constant=case["sponge_constant"], simulation=case["simulation"],
                 source_params=case.get("source_params", ()), precision=precision, path=path,
-                boundaries=case["bc"], harmonic=case.get("harmonic"))
+                boundaries=case["bc"], harmonic=case.get("harmonic"), num_scalar=case.get("num_scalar", 0))
         s.set_solution(case["ic"])
         return s
 
@@ -378,12 +386,12 @@ class Solver:
     # state ---------------------------------------------------------------
     def set_solution(self, sol, ib=0):
         a = np.ascontiguousarray(np.asarray(sol, np.float64).reshape(-1))
-        if a.size != NV * self.n(ib):
+        if a.size != self.nv * self.n(ib):
             raise LeeError("solution size mismatch")
         self._check(self.L.lee_set_solution(self.h, ib, _dp(a)))
 
     def solution(self, ib=0):
-        out = np.empty((NV,) + self.shapes[ib])
+        out = np.empty((self.nv,) + self.shapes[ib])
         self._check(self.L.lee_get_solution(self.h, ib, _dp(out)))
         return out
 
@@ -591,7 +599,7 @@ class Solver:
         return out
 
     def mean_gradient(self, ib=0):
-        out = np.empty((5 * getattr(self, "num_dim", 2),) + self.shapes[ib])
+        out = np.empty((self.nv * getattr(self, "num_dim", 2),) + self.shapes[ib])
         self._check(self.L.lee_get_mean_gradient(self.h, ib, _dp(out)))
         return out
 

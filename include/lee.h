@@ -13,7 +13,9 @@
  * Arrays passed in or out are host memory in the reference layout: one
  * block = n = ni*nj*nk points, i fastest (Geometry::StructuredBlock::idx1D,
  * src/geometry/geometry.h:131-136); a field set is variable-major, e.g. a
- * solution is 5*n doubles ordered s', u', v', w', p' (src/core/param.h:64-71).
+ * solution is nv*n doubles ordered s', u', v', w', p' (src/core/param.h:64-71)
+ * and then the passive scalars Z'_0 .. Z'_{ns-1} of a LEE_SCALAR run
+ * (state.cpp:336-372), nv = 5 + lee_config.num_scalar.
  * The library copies what it needs; the caller keeps ownership.
  *
  * One handle = one set of blocks on the current HIP device, one stream;
@@ -105,7 +107,14 @@ typedef struct {
   double harmonic_amplitude;   /* AMPLITUDE, in [0, 1] */
   double harmonic_wavelength;  /* WAVELENGTH */
   double harmonic_halfwidth;   /* HALFWIDTH (WAVEFORM_GAUSSIAN_ROUNDJET) */
+  int num_scalar;          /* PHYSICAL_MODEL LEE_SCALAR NUMBER_SCALAR (input.cpp:113-119), 0 to
+                              LEE_MAX_SCALAR; 0 = LEE.  Scalars Z'_k: dZ'/dt = -ubar . grad Z'
+                              - u' . grad Zbar (spatial:2134-2158), filtered, relaxed to 0 in
+                              sponges, Neumann at wall-slip and centerline faces (2-D single
+                              handles; no batches, no adjoint) */
 } lee_config;
+
+#define LEE_MAX_SCALAR 4
 
 /* One BC-file region (src/io/io.cpp:1158-1268) with negative indices
  * resolved: type, normal direction (0 xi, 1 eta, 2 zeta), end (0 = '+' low
@@ -115,8 +124,9 @@ typedef struct {
   int is[3], ie[3];
 } lee_boundary;
 
-/* One block: grid coordinates (3n: x, y, z) and base state (5n: s, u, v,
- * w, p) as read from GRID_FILE and BASESTATE_FILE; nk = 1 in 2-D. */
+/* One block: grid coordinates (3n: x, y, z) and base state (nv*n: s, u, v,
+ * w, p and the scalars' means) as read from GRID_FILE and BASESTATE_FILE;
+ * nk = 1 in 2-D. */
 typedef struct {
   int ni, nj, nk;
   const double *xyz;
@@ -154,7 +164,7 @@ void lee_destroy(lee_handle *h);
 const char *lee_last_error(const lee_handle *h);
 
 /* Restart / IC: io::read_solution (plot3d.cpp:715-833) + compute_dependent_variables.
- * sol = 5n doubles; sets time_step and time_sol like tau[2], tau[3]. */
+ * sol = nv*n doubles; sets time_step and time_sol like tau[2], tau[3]. */
 int lee_set_solution(lee_handle *h, int iblock, const double *sol);
 int lee_get_solution(lee_handle *h, int iblock, double *sol);
 int lee_set_time(lee_handle *h, int time_step, double time_sol, double dt);
@@ -175,7 +185,7 @@ int lee_time_advance(lee_handle *h);
 int lee_solve(lee_handle *h, int nsteps);
 int lee_synchronize(lee_handle *h);
 
-/* Unit entry points on one block (host arrays, 5n doubles).
+/* Unit entry points on one block (host arrays, nv*n doubles).
  * spatial::compute_RHS (spatial:1963-1990) of y at `time`, with the aux
  * density taken from y itself. */
 int lee_compute_rhs(lee_handle *h, int iblock, const double *y, double time, double *rhs);
@@ -184,26 +194,27 @@ int lee_apply_filter(lee_handle *h, int iblock, double *y);
 /* StandardCentral::take_derivative_xyz (spatial:1802-1826) of one field. */
 int lee_take_derivative_xyz(lee_handle *h, int iblock, const double *f, int dir_xyz, double *out);
 /* spatial::update_boundary (spatial:2162-2187): BCs then overset
- * interpolation, in place on all blocks (ys[b] = 5n doubles of block b);
+ * interpolation, in place on all blocks (ys[b] = nv*n doubles of block b);
  * time-dependent boundary data (a harmonic wave) at the handle's time_sol. */
 int lee_update_boundary(lee_handle *h, double *const *ys);
 
 /* lee_update_boundary without any allocation.  No counterpart in the
  * reference; for callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops, harnesses that forbid hipMalloc).
- * ys_dev[b] = 5n doubles of block b in DEVICE memory, updated in place:
+ * ys_dev[b] = nv*n doubles of block b in DEVICE memory, updated in place:
  * BCs, then overset interpolation, enqueued on the handle's stream
  * (lee_synchronize waits; the caller orders its own streams around it).
  * The receiver values are staged in a static device array of
  * LEE_NOALLOC_MAX_RECV * 5 doubles per device (about 5 MiB), held until the
- * process ends; a handle with more overset receivers gets LEE_EINVAL.  All
+ * process ends; a handle whose receivers need more (nrecv * nv doubles)
+ * gets LEE_EINVAL.  All
  * calls on one device share that array: calls that overlap in time on one
  * device race.  No hipMalloc, hipFree or host copy is made. */
 #define LEE_NOALLOC_MAX_RECV 131072
 int lee_update_boundary_noalloc(lee_handle *h, double *const *ys_dev);
 
 /* Init products, for checks: metrics[l][i] (9n), J (n), iblank (n),
- * mean gradients (5*num_dim*n, ivar*num_dim+dir), summed sponge strength (n). */
+ * mean gradients (nv*num_dim*n, ivar*num_dim+dir), summed sponge strength (n). */
 int lee_get_metrics(lee_handle *h, int iblock, double *metrics, double *jac);
 int lee_get_iblank(lee_handle *h, int iblock, int *iblank);
 int lee_get_mean_gradient(lee_handle *h, int iblock, double *grad);

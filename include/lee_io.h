@@ -64,13 +64,17 @@ int lee_deck_settings(const lee_deck *d, lee_run_settings *s);
  *   header  int nblocks; per block ni, nj, nk (function: ni, nj, nk, nvar)
  *   grid    per block x[n], y[n], z[n]
  *   solution per block tau[4] = {0, 1, time_step, time_sol}, then 5 x [n]
+ *            (LEE_P3D_SOLUTION_VARS(nv): nv x [n], the restart file of a
+ *            LEE_SCALAR run, which io::read_solution reads with
+ *            num_vars_sol variables; its solver writes function files)
  *   function per block nvar x [n] */
 #define LEE_P3D_GRID 0
 #define LEE_P3D_SOLUTION 1
 #define LEE_P3D_FUNCTION 2
+#define LEE_P3D_SOLUTION_VARS(nv) (100 + (nv))
 
 /* Block count and, for up to `cap` blocks, dims[4 b + 0..3] = ni, nj, nk,
- * nvar (3 for a grid, 5 for a solution). */
+ * nvar (3 for a grid, 5 for a solution, nv for LEE_P3D_SOLUTION_VARS(nv)). */
 int lee_plot3d_dims(const char *path, int kind, int *nblocks, int *dims, int cap);
 /* Block `iblock` into data (nvar * n doubles); tau (4 doubles) for a
  * solution file, may be NULL otherwise. */
@@ -112,7 +116,9 @@ void lee_overture_close(lee_overture *o);
  * REPORT_TIMESTEP step prints
  * "Time step: ..., time: ..., dt: ..., CFL: ..." (unless quiet), appends the
  * min/max line to out_dir/report/solution/solution and writes
- * out_dir/camille.NNNNNN.q plus camille.nam when the writing cadence says so.
+ * out_dir/camille.NNNNNN.q plus camille.nam when the writing cadence says so
+ * (a LEE_SCALAR run reads a restart file of 5 + NUMBER_SCALAR variables
+ * and writes its solutions as function files, io.cpp:740-761).
  * precision / path select the advance (LEE_FP64 with LEE_PATH_PLAIN writes
  * the reference's bits). */
 int lee_run_deck(const char *deck_path, const char *restart_path, int precision, int path, const char *out_dir,

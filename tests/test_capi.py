@@ -92,6 +92,10 @@ def test_create_rejects_plain_fp32_and_small_blocks():
     assert rc == 1 and "fused path" in err
     rc, err, _ = _create(_cfg(), ni=6, nj=20)
     assert rc == 1 and "too small" in err
+    rc, err, _ = _create(_cfg(num_scalar=5))
+    assert rc == 1 and "NUMBER_SCALAR" in err
+    rc, err, _ = _create(_cfg(num_scalar=1, num_dim=3))
+    assert rc == 1 and "LEE_SCALAR" in err and "2-D" in err
 
 
 def test_create_rejects_bad_boundaries():
@@ -107,6 +111,8 @@ def test_create_rejects_bad_boundaries():
         b.type = t
         rc, err, _ = _create(_cfg(), boundary=b)
         assert rc == 1 and "not implemented yet" in err
+        rc, err, _ = _create(_cfg(num_scalar=1), boundary=b)  # LEE_SCALAR's own message (state.cpp:751-754)
+        assert rc == 1 and "with a scalar is not implemented yet" in err
     b.type = 7
     rc, err, _ = _create(_cfg(), boundary=b)
     assert rc == 1 and "unsupported boundary" in err

@@ -10,6 +10,7 @@ overset_gen).  The GPU test runs whole golden case folders through
 lee_run_deck on the plain path and compares the written solution file
 with the one the reference binary wrote, bit for bit.
 """
+import json
 import os
 
 import numpy as np
@@ -21,6 +22,7 @@ from camille_amd import FP64, PATH_PLAIN, LeeError, parse_bc_line
 from camille_amd import fileio
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+MANIFEST_GOLDEN = json.load(open(os.path.join(HERE, "golden", "manifest.json")))
 
 
 def _write(path, text):
@@ -107,6 +109,20 @@ def test_deck_settings_optimized_schemes(tmp_path):
     assert cfg["path"] == 0  # the fused path takes the optimized schemes too
 
 
+def test_deck_settings_passive_scalars(tmp_path):
+    """PHYSICAL_MODEL = LEE_SCALAR with NUMBER_SCALAR (input.cpp:113-119)
+    sets num_scalar; no scalar, or a case other than CASE_LINEAR_NOZZLE (the
+    model/case check of input.cpp:596-621), is refused with the reference's
+    message."""
+    c = cases.case_scalar()
+    assert fileio.Deck(_deck_for(c, tmp_path)).settings()["cfg"]["num_scalar"] == 2
+    assert fileio.Deck(_deck_for(cases.case_c1(), tmp_path)).settings()["cfg"]["num_scalar"] == 0
+    with pytest.raises(LeeError, match="at least one scalar should be solved for"):
+        fileio.Deck(_deck_for(c, tmp_path, PHYSICAL_MODEL="LEE_SCALAR, NUMBER_SCALAR = 0")).settings()
+    with pytest.raises(LeeError, match="Unknown simulation for the current physical model"):
+        fileio.Deck(_deck_for(c, tmp_path, SIMULATION="CASE_TANNA_TPN49")).settings()
+
+
 @pytest.mark.parametrize("key,value,msg", [
     ("SPATIAL_FILTER", "COMPACT, BLEND = 1.0", "The filter COMPACT is not supported"),
     ("SPATIAL_SCHEME", "COMPACT6", "SPATIAL_SCHEME = COMPACT6"),
@@ -145,6 +161,17 @@ def test_plot3d_round_trips_with_the_fixture_writers(tmp_path):
         assert open(tmp_path / a, "rb").read() == open(tmp_path / b, "rb").read()
     with pytest.raises(LeeError, match="block 2 is not in"):
         fileio.plot3d_read(str(tmp_path / "a.q"), fileio.SOLUTION, 2)
+    # a solution file of 7 variables (the restart file of two passive scalars)
+    sols7 = [rng.standard_normal((7, nk, nj, ni)) for ni, nj, nk in dims]
+    refio.write_solution(str(tmp_path / "c.q"), sols7, 3, 0.5)
+    assert fileio.plot3d_dims(str(tmp_path / "c.q"), fileio.SOLUTION_VARS(7)) == [d + (7,) for d in dims]
+    for b in range(2):
+        q, tau = fileio.plot3d_read(str(tmp_path / "c.q"), fileio.SOLUTION_VARS(7), b)
+        assert np.array_equal(q, sols7[b]) and list(tau) == [0.0, 1.0, 3.0, 0.5]
+    fileio.plot3d_write(str(tmp_path / "d.q"), fileio.SOLUTION_VARS(7), sols7, [0.0, 1.0, 3.0, 0.5])
+    assert open(tmp_path / "c.q", "rb").read() == open(tmp_path / "d.q", "rb").read()
+    with pytest.raises(LeeError, match="bad block"):
+        fileio.plot3d_write(str(tmp_path / "e.q"), fileio.SOLUTION, sols7, [0.0, 1.0, 3.0, 0.5])
 
 
 def test_bc_file_matches_the_python_reader(tmp_path):
@@ -254,3 +281,28 @@ def test_run_deck_writes_the_reference_solution_file(tmp_path, name):
     assert open(os.path.join(d, "camille.nam")).read().split() == \
         ["S'", "UX'", "UR'" if ax else "UY'", "UTHETA'" if ax else "UZ'", "P'"]
     print(f"run_deck {name}: {nb} block(s), {c['nsteps']} steps, bit-identical", flush=True)
+
+
+@pytest.mark.gpu
+def test_run_deck_passive_scalars(tmp_path):
+    """The LEE_SCALAR case folder through lee_run_deck (plain FP64): the
+    restart file holds 7 variables after tau, the solutions come out as
+    function files (io.cpp:740-761) holding the reference's bits, and the
+    name files and report carry the scalars under the reference's names."""
+    d = str(tmp_path)
+    c, _ = _case_folder("scalar", d)
+    g = _golden("scalar")
+    fileio.run_deck(os.path.join(d, "camille.in"), os.path.join(d, "ic.q"), FP64, PATH_PLAIN, d)
+    q = fileio.plot3d_read(os.path.join(d, "camille.000001.q"), fileio.FUNCTION, 0)
+    ref = np.zeros(q.shape[:1] + q.shape[2:])
+    ref[g["vars"]] = g["sol"]
+    assert q.shape[0] == 7 and np.array_equal(q[:, 0], ref)
+    q0 = fileio.plot3d_read(os.path.join(d, "camille.000000.q"), fileio.FUNCTION, 0)
+    assert np.array_equal(q0[:, 0], c["ic"])
+    assert open(os.path.join(d, "camille.nam")).read().split() == ["S'", "UX'", "UY'", "UZ'", "P'", "Z0'", "Z01'"]
+    assert open(os.path.join(d, "mean_camille.nam")).read().split()[5:] == ["Z0bar", "Z01bar"]
+    rep = open(os.path.join(d, "report", "solution", "solution")).read().splitlines()
+    assert len(rep) == 1 and ", Z0': " in rep[0] and ", Z01': " in rep[0]
+    assert refio.time_matches(float(rep[0].split("time:")[1].split(",")[0]), g["tau"],
+                              MANIFEST_GOLDEN["scalar"]["report"])
+    print("run_deck scalar: 7 variables, function-format solution bit-identical", flush=True)
