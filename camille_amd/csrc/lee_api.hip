@@ -138,9 +138,12 @@ int enqueue_fused_step(lee_handle *h) {
       he_bc = harmonic_entropy(h->cfg, t_bc);
     }
     if (h->ktiming) HIPCHK(mark(h));
-    for (int ib = 0; ib < h->nblk; ib++)
-      HIPCHK(h->cfg.num_dim == 3 ? fused_stage3d(h->stream, h, h->b[ib], st, sc)
-                                 : fused_stage(h->stream, h, h->b[ib], st, sc));
+    if (pair_stage_ok(h))
+      HIPCHK(fused_stage_pair(h->stream, h, st, sc));
+    else
+      for (int ib = 0; ib < h->nblk; ib++)
+        HIPCHK(h->cfg.num_dim == 3 ? fused_stage3d(h->stream, h, h->b[ib], st, sc)
+                                   : fused_stage(h->stream, h, h->b[ib], st, sc));
     if (h->ktiming) {
       HIPCHK(mark(h));
       h->evclass.push_back(st == 0 ? 0 : (st == last ? 2 : 1));
@@ -162,7 +165,16 @@ int enqueue_fused_step(lee_handle *h) {
       h->evclass.push_back(4);
     }
   }
-  for (int ib = 0; ib < h->nblk; ib++) {
+  const bool pair_filter = pair_filter_ok(h);
+  if (pair_filter) {
+    if (h->ktiming) HIPCHK(mark(h));
+    HIPCHK(fused_filter_pair(h->stream, h, 1, 0));
+    if (h->ktiming) {
+      HIPCHK(mark(h));
+      h->evclass.push_back(3);
+    }
+  }
+  for (int ib = 0; ib < h->nblk && !pair_filter; ib++) {
     Block &b = h->b[ib];
     if (h->cfg.filter_order > 0) {
       if (h->ktiming) HIPCHK(mark(h));
@@ -664,10 +676,27 @@ int autotune_block3d(lee_handle *h, Block &b) {
 
 int autotune(lee_handle *h) {
   if (is_plain(h)) return LEE_OK;
+  const bool second_free = h->nblk == 2 && !h->b[1].tuned;
   for (int ib = 0; ib < h->nblk; ib++) {
     if (h->b[ib].tuned) continue;
     int rc = h->cfg.num_dim == 3 ? autotune_block3d(h, h->b[ib]) : autotune_block(h, h->b[ib]);
     if (rc) return rc;
+  }
+  // a 2-D pair of blocks with the same kernel instance (an overset
+  // background and body grid) takes block 0's forms for both, so that each
+  // stage and the filter run as one launch (pair_stage_ok, pair_filter_ok);
+  // every form gives the same bits.  Block 1's forms stay as they are when
+  // the caller set them, or when block 0's filter form cannot serve it.
+  if (second_free && h->cfg.num_dim == 2 && !h->batch) {
+    Block &b0 = h->b[0], &b1 = h->b[1];
+    if (b0.has_holes == b1.has_holes && b0.uniform_mean == b1.uniform_mean) {
+      b1.tune_r = b0.tune_r;
+      if (b0.tune_mc > 1) {
+        b1.tune_mc = b0.tune_mc;
+        b1.tune_ml = b0.tune_ml;
+        b1.tune_fgrid = 0;
+      }
+    }
   }
   return LEE_OK;
 }

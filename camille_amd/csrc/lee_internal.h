@@ -325,6 +325,12 @@ hipError_t deriv_xyz3(hipStream_t s, const Block &b, const double *f, double *ou
 hipError_t fused_stage3d(hipStream_t s, lee_handle *h, Block &b, int stage, const StageScalars &sc);
 hipError_t fused_filter3d(hipStream_t s, lee_handle *h, Block &b, int src, int dst, int tmp);
 hipError_t fused_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst);
+// Two-block handles whose blocks share the kernel instance and forms run
+// each stage, and the stacked filter, as one launch over both blocks.
+bool pair_stage_ok(const lee_handle *h);
+bool pair_filter_ok(const lee_handle *h);
+hipError_t fused_stage_pair(hipStream_t s, lee_handle *h, int stage, const StageScalars &sc);
+hipError_t fused_filter_pair(hipStream_t s, lee_handle *h, int src, int dst);
 hipError_t batch_prepare(hipStream_t s, lee_handle *h);
 hipError_t batch_stage(hipStream_t s, lee_handle *h, int stage);
 hipError_t batch_filter(hipStream_t s, lee_handle *h);

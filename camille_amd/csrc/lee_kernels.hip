@@ -450,18 +450,15 @@ __global__ void __launch_bounds__(TPB) k_ov_gather(OvTables<T> t, const int *__r
   for (int v = 0; v < t.nv; v++) buf[(size_t)r * t.nv + v] = acc[v];
 }
 
-// compute_weighted_sum for 5 x 5 stencils, one thread per (receiver,
-// variable): the 25 donor values and coefficients (receiver-minor, so
-// coalesced) are requested together, then summed in the reference's order
-// with the donors outside the block skipped.
+// compute_weighted_sum for 5 x 5 stencils (overset.cpp:886-974), receiver r
+// and variable v: the 25 donor values and coefficients (receiver-minor, so
+// coalesced across receivers) are requested together, then summed in the
+// reference's order with the donors outside the block skipped.
 template <typename T>
-__global__ void __launch_bounds__(TPB) k_ov_gather5(OvTables<T> t, const int *__restrict__ db,
-                                                    const int *__restrict__ dl, const double *__restrict__ coef5,
-                                                    double *__restrict__ buf, int nrecv) {
+__device__ __forceinline__ double ov_sum5(const OvTables<T> &t, const int *__restrict__ db,
+                                          const int *__restrict__ dl, const double *__restrict__ coef5, int nrecv,
+                                          int r, int v) {
 #pragma clang fp contract(off)
-  const int gid = blockIdx.x * TPB + threadIdx.x;
-  if (gid >= t.nv * nrecv) return;
-  const int v = gid / nrecv, r = gid - v * nrecv;
   const int b = db[r];
   const T *y = t.y[b] + (size_t)v * t.nn[b];
   const int ni = t.ni[b], n = t.nn[b], d0 = dl[r];
@@ -481,7 +478,19 @@ __global__ void __launch_bounds__(TPB) k_ov_gather5(OvTables<T> t, const int *__
     const double t1 = a + cc[q] * (double)yv[q];
     a = ok[q] ? t1 : a;
   }
-  buf[(size_t)r * t.nv + v] = a;
+  return a;
+}
+
+// The gather of the two-phase update: one thread per (receiver, variable),
+// every sum into buf before any receiver is written.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_ov_gather5(OvTables<T> t, const int *__restrict__ db,
+                                                    const int *__restrict__ dl, const double *__restrict__ coef5,
+                                                    double *__restrict__ buf, int nrecv) {
+  const int gid = blockIdx.x * TPB + threadIdx.x;
+  if (gid >= t.nv * nrecv) return;
+  const int v = gid / nrecv, r = gid - v * nrecv;
+  buf[(size_t)r * t.nv + v] = ov_sum5(t, db, dl, coef5, nrecv, r, v);
 }
 
 // update_each_grid (overset.cpp:1241-1278); optionally refreshes rho'.
@@ -748,6 +757,22 @@ __global__ void __launch_bounds__(TPB) k_fused_stage(StageArgs<T> a) {
   stage_tile<T, MEANF, HOLES, KIND, R, W>(a, sc, tile, s);
 }
 
+// Two blocks of one handle in one launch (an overset pair such as a
+// background grid and a body grid): block 0's tiles, then block 1's, as one
+// XCD-mapped list, each tile with its block's operands.  Every point runs
+// the single-block arithmetic, so the results are those of two launches;
+// the small block no longer costs a launch of its own.
+template <typename T, bool MEANF, bool HOLES, int KIND, int R>
+__global__ void __launch_bounds__(TPB) k_fused_stage_pair(StageArgs<T> a0, StageArgs<T> a1) {
+  __shared__ StageLdsW<T, R, 4> s;
+  const int total = a0.ntiles + a1.ntiles;
+  const int tile = xcd_tile(blockIdx.x, (total + 7) / 8);
+  if (tile >= total) return;
+  const StepT<T> sc{a0.h, a0.b, a0.a, a0.s1, a0.s2};
+  if (tile < a0.ntiles) stage_tile<T, MEANF, HOLES, KIND, R, 4>(a0, sc, tile, s);
+  else stage_tile<T, MEANF, HOLES, KIND, R, 4>(a1, sc, tile - a0.ntiles, s);
+}
+
 // Batched ensemble: the tiles of all solves form one list; map[2t] is the
 // solve of list entry t and map[2t + 1] its tile within the solve.  A
 // persistent grid walks the
@@ -842,6 +867,17 @@ template <typename T, int ORDER, int NW, int R, bool HOLES>
 __global__ void __launch_bounds__(64 * NW) k_fused_filter_s(FilterArgs<T> a) {
   __shared__ StackLds<T, NW, R> L;
   filter_stack<T, ORDER, NW, R, HOLES>(a, L);
+}
+
+// The stacked filter of two blocks in one launch (see k_fused_stage_pair).
+template <typename T, int ORDER, int NW, int R, bool HOLES>
+__global__ void __launch_bounds__(64 * NW) k_fused_filter_s_pair(FilterArgs<T> a0, FilterArgs<T> a1) {
+  __shared__ StackLds<T, NW, R> L;
+  const int total = a0.ntiles + a1.ntiles;
+  const int t = xcd_tile(blockIdx.x, (total + 7) / 8);
+  if (t >= total) return;
+  if (t < a0.ntiles) stack_item<T, ORDER, NW, R, HOLES>(a0, t, L);
+  else stack_item<T, ORDER, NW, R, HOLES>(a1, t - a0.ntiles, L);
 }
 
 template <typename T>
@@ -1360,6 +1396,73 @@ hipError_t fused_stage(hipStream_t s, lee_handle *h, Block &b, int stage, const 
   return fused_stage_t<double>(s, h, b, stage, sc, b.Y64, b.rho64);
 }
 
+// Whether a 2-D handle's two blocks can share one stage launch and one
+// filter launch: the same kernel instance (holes, mean kind, operator) and
+// the same tile height; the filter also needs both blocks in the same
+// stacked column-march form.
+bool pair_stage_ok(const lee_handle *h) {
+  if (h->nblk != 2 || h->batch || h->cfg.num_dim != 2 || h->cfg.fd_order != 4 || h->cfg.path != LEE_PATH_FUSED)
+    return false;
+  const Block &b0 = h->b[0], &b1 = h->b[1];
+  return b0.has_holes == b1.has_holes && b0.uniform_mean == b1.uniform_mean && b0.tune_r == b1.tune_r;
+}
+
+bool pair_filter_ok(const lee_handle *h) {
+  if (!pair_stage_ok(h) || h->cfg.filter_order == 11 || h->cfg.filter_order == 0) return false;
+  const Block &b0 = h->b[0], &b1 = h->b[1];
+  return b0.tune_mc > 1 && b0.tune_mc == b1.tune_mc && b0.tune_ml == b1.tune_ml && !b0.carry_w && !b1.carry_w;
+}
+
+template <typename T>
+static hipError_t fused_stage_pair_t(hipStream_t s, lee_handle *h, int stage, const StageScalars &sc) {
+  Block &b0 = h->b[0], &b1 = h->b[1];
+  const int kind = stage_kind(h, stage), R = b0.tune_r;
+  T *const *Y0 = sizeof(T) == 8 ? (T *const *)b0.Y64 : (T *const *)b0.Y32;
+  T *const *Y1 = sizeof(T) == 8 ? (T *const *)b1.Y64 : (T *const *)b1.Y32;
+  T *r0 = sizeof(T) == 8 ? (T *)b0.rho64 : (T *)b0.rho32;
+  T *r1 = sizeof(T) == 8 ? (T *)b1.rho64 : (T *)b1.rho32;
+  const StageArgs<T> a0 = make_stage_args<T>(h, b0, stage, Y0, r0, R, sc);
+  const StageArgs<T> a1 = make_stage_args<T>(h, b1, stage, Y1, r1, R, sc);
+  const dim3 grid((a0.ntiles + a1.ntiles + 7) / 8 * 8), block(64, 4);
+  const bool mf = !b0.uniform_mean, ho = b0.has_holes;
+#define LEE_PL(MF, HO, K)                                                                              \
+  do {                                                                                                 \
+    if (R == 1) hipLaunchKernelGGL((k_fused_stage_pair<T, MF, HO, K, 1>), grid, block, 0, s, a0, a1); \
+    else if (R == 2) hipLaunchKernelGGL((k_fused_stage_pair<T, MF, HO, K, 2>), grid, block, 0, s, a0, a1); \
+    else hipLaunchKernelGGL((k_fused_stage_pair<T, MF, HO, K, 4>), grid, block, 0, s, a0, a1);         \
+  } while (0)
+#define LEE_PK(MF, HO)                                     \
+  do {                                                     \
+    if (kind == STAGE_FIRST) LEE_PL(MF, HO, STAGE_FIRST);  \
+    else if (kind == STAGE_MID) LEE_PL(MF, HO, STAGE_MID); \
+    else LEE_PL(MF, HO, STAGE_LAST);                       \
+  } while (0)
+  if (mf && ho) LEE_PK(true, true);
+  else if (mf) LEE_PK(true, false);
+  else if (ho) LEE_PK(false, true);
+  else LEE_PK(false, false);
+#undef LEE_PK
+#undef LEE_PL
+  // the per-block extras of fused_stage_t: a carried w' and the passive scalars
+  for (int ib = 0; ib < 2; ib++) {
+    Block &b = h->b[ib];
+    const StageArgs<T> &a = ib == 0 ? a0 : a1;
+    if (b.carry_w) hipLaunchKernelGGL(k_passive_w<T>, dim3(nblocks(b.n)), dim3(TPB), 0, s, a, kind);
+    if (const int nsc = h->cfg.num_scalar) {
+      const dim3 sg(nblocks((size_t)nsc * b.n));
+      if (kind == STAGE_FIRST) hipLaunchKernelGGL((k_scalar_stage<T, STAGE_FIRST, 4>), sg, dim3(TPB), 0, s, a, nsc);
+      else if (kind == STAGE_MID) hipLaunchKernelGGL((k_scalar_stage<T, STAGE_MID, 4>), sg, dim3(TPB), 0, s, a, nsc);
+      else hipLaunchKernelGGL((k_scalar_stage<T, STAGE_LAST, 4>), sg, dim3(TPB), 0, s, a, nsc);
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t fused_stage_pair(hipStream_t s, lee_handle *h, int stage, const StageScalars &sc) {
+  if (h->cfg.precision == LEE_FP32) return fused_stage_pair_t<float>(s, h, stage, sc);
+  return fused_stage_pair_t<double>(s, h, stage, sc);
+}
+
 // Operands of the filter on block b with tile height TYF (four variables).
 template <typename T>
 static FilterArgs<T> make_filter_args(const lee_handle *h, const Block &b, T *const *Y, int src, int dst, int TYF) {
@@ -1503,6 +1606,53 @@ static hipError_t scalar_filter_t(hipStream_t s, lee_handle *h, Block &b, T *con
 #undef LEE_FLZ
   }
   return hipGetLastError();
+}
+
+// Both blocks of a pair (pair_filter_ok) in one stacked column-march launch.
+template <typename T>
+static hipError_t fused_filter_pair_t(hipStream_t s, lee_handle *h, int src, int dst) {
+  Block &b0 = h->b[0], &b1 = h->b[1];
+  const int NW = b0.tune_mc, L = b0.tune_ml;
+  FilterArgs<T> a[2];
+  for (int ib = 0; ib < 2; ib++) {
+    Block &b = h->b[ib];
+    T *const *Y = sizeof(T) == 8 ? (T *const *)b.Y64 : (T *const *)b.Y32;
+    a[ib] = make_filter_args<T>(h, b, Y, src, dst, b.tune_fty);
+    a[ib].ntx = (b.ni + March::STRIDE - 1) / March::STRIDE;
+    a[ib].nty = (b.nj + NW * L - 1) / (NW * L);
+    a[ib].ntiles = 4 * a[ib].ntx * a[ib].nty;  // stacks of the four active variables
+    a[ib].chunk = (a[ib].ntiles + 7) / 8;
+  }
+  const dim3 grid((a[0].ntiles + a[1].ntiles + 7) / 8 * 8), block(64, NW);
+  const bool ho = b0.has_holes, o8 = h->cfg.filter_order == 8;
+#define LEE_FP(OR, NWV, RR)                                                                                      \
+  do {                                                                                                           \
+    if (ho) hipLaunchKernelGGL((k_fused_filter_s_pair<T, OR, NWV, RR, true>), grid, block, 0, s, a[0], a[1]);   \
+    else hipLaunchKernelGGL((k_fused_filter_s_pair<T, OR, NWV, RR, false>), grid, block, 0, s, a[0], a[1]);     \
+  } while (0)
+#define LEE_FP_O(OR)                    \
+  do {                                  \
+    if (NW == 8) LEE_FP(OR, 8, 8);      \
+    else if (L == 4) LEE_FP(OR, 4, 4);  \
+    else if (L == 8) LEE_FP(OR, 4, 8);  \
+    else LEE_FP(OR, 4, 16);             \
+  } while (0)
+  if (o8) LEE_FP_O(8);
+  else LEE_FP_O(4);
+#undef LEE_FP_O
+#undef LEE_FP
+  hipError_t e = hipGetLastError();
+  for (int ib = 0; ib < 2 && e == hipSuccess && h->cfg.num_scalar; ib++) {
+    Block &b = h->b[ib];
+    T *const *Y = sizeof(T) == 8 ? (T *const *)b.Y64 : (T *const *)b.Y32;
+    e = scalar_filter_t<T>(s, h, b, Y, src, dst);
+  }
+  return e;
+}
+
+hipError_t fused_filter_pair(hipStream_t s, lee_handle *h, int src, int dst) {
+  if (h->cfg.precision == LEE_FP32) return fused_filter_pair_t<float>(s, h, src, dst);
+  return fused_filter_pair_t<double>(s, h, src, dst);
 }
 
 hipError_t fused_filter(hipStream_t s, lee_handle *h, Block &b, int src, int dst) {

@@ -357,8 +357,20 @@ __global__ void __launch_bounds__(TPB) k_filter3d_plane_s(FilterArgs<T> a, size_
 template <typename T, int TY>
 using PlaneRegs = FilterRegs<T, 1, TY>;
 
+// LDS of the single pass: the plane's input and its xi sweep apart (the 2-D
+// tile overlays them), so a plane takes two barriers: after the input is
+// stored and after the xi rows are.  The next plane stores its input once
+// every thread has passed the second barrier, i.e. finished its xi reads,
+// and its xi rows once every thread has passed its first, i.e. finished
+// this plane's eta reads.
+template <typename T, int TY>
+struct alignas(32) PlaneLds {
+  T in[1][TY + 2 * FL_H][FL_TX + 2 * FL_H];
+  T mid[1][TY + 2 * FL_H][FL_TX];
+};
+
 template <typename T, int ORDER, int TY, bool INNER>
-__device__ __forceinline__ void plane_xieta(const FilterArgs<T> &a, int tile, FilterLds<T, 1, TY> &L,
+__device__ __forceinline__ void plane_xieta(const FilterArgs<T> &a, int tile, PlaneLds<T, TY> &L,
                                             const PlaneRegs<T, TY> &R, T (&f)[TY / 4], T (&orig)[TY / 4]) {
   constexpr int NR = TY + 2 * FL_H, KR = NR / 4, NB = ORDER / 2;
   auto &in = L.in;
@@ -409,13 +421,6 @@ __device__ __forceinline__ void plane_xieta(const FilterArgs<T> &a, int tile, Fi
       }
       xo[s][m] = ok ? filter_row<T, ORDER>(&v[m + FL_H], 1, code) : (T)0;
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    const int it = tid + s * TPB;
-    if (NXI % TPB != 0 && it >= NXI) break;
-    const int g = it % NG, r = it / NG;
     *reinterpret_cast<Vec4<T> *>(&mid[0][r][4 * g]) = xo[s];
   }
   __syncthreads();
@@ -436,7 +441,7 @@ __device__ __forceinline__ void plane_xieta(const FilterArgs<T> &a, int tile, Fi
 template <typename T, int ORDER, int TY>
 __global__ void __launch_bounds__(TPB) k_filter3d_fused(FilterArgs<T> a, int nk, int zchunk) {
   constexpr int NP = TY / 4, H = FL_H, NB = ORDER / 2;
-  __shared__ FilterLds<T, 1, TY> L;
+  __shared__ PlaneLds<T, TY> L;
   const int t = xcd_tile(blockIdx.x, a.chunk);
   if (t >= a.ntiles) return;
   const int per_var = a.ntx * a.nty;
@@ -472,7 +477,6 @@ __global__ void __launch_bounds__(TPB) k_filter3d_fused(FilterArgs<T> a, int nk,
       }
       if (inner) plane_xieta<T, ORDER, TY, true>(b, tile, L, R, f, orig);
       else plane_xieta<T, ORDER, TY, false>(b, tile, L, R, f, orig);
-      __syncthreads();  // the next plane's tile overwrites the LDS
       if (kk < klast) R = Rn;
     } else {
 #pragma unroll

@@ -414,3 +414,36 @@ def test_overset_fused_matches_reference(label, prec, name):
           + (f", corner box {max(box):.2e}" if box else ""), flush=True)
     assert max(errs) <= TOL[label]
     s.close()
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("name", ["overset_small", "cylinder_small"])
+def test_two_block_pair_launch_bit_identical(prec, name):
+    """A two-block handle whose blocks share the kernel instance and forms
+    runs each stage and the stacked filter as one launch over both blocks;
+    the autotuner gives block 1 block 0's forms.  The paired launches give
+    the bits of separate launches per block (block 1 pinned to a different
+    stage tile height and the tile filter, which keeps them apart)."""
+    nsteps = 25
+    c, s = _overset_solver(prec, PATH_FUSED, name)
+    s.solve(2)  # the autotuner's forms
+    assert s.tile_shape(1)[0] == s.tile_shape(0)[0]
+    if s.filter_march(0)[0] > 1:
+        assert s.filter_march(1) == s.filter_march(0)
+    s.close()
+    runs = []
+    for paired in (True, False):
+        c, s = _overset_solver(prec, PATH_FUSED, name)
+        s.set_tile_shape(1, 8, ib=0)
+        s.set_filter_march(4, 4, ib=0)
+        if paired:
+            s.set_tile_shape(1, 8, ib=1)
+            s.set_filter_march(4, 4, ib=1)
+        else:
+            s.set_tile_shape(2, 8, ib=1)
+        s.solve(nsteps)
+        runs.append([s.solution(ib) for ib in range(2)])
+        s.close()
+    for ib in range(2):
+        assert np.array_equal(runs[0][ib], runs[1][ib]), (name, prec, ib)
+    print(f"pair launch {name} prec {prec}: bit-identical to separate launches", flush=True)
