@@ -123,6 +123,9 @@ template <typename T, bool MEANF, int KIND>
 __global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
   constexpr int RS = ST_TX + 2 * ST_H, NR = S3_TY + 2 * ST_H;
   __shared__ T s[5][NR][RS];
+  // (tile, chunk) in launch order: an XCD-aware tile-major order (each XCD
+  // a band of tiles through all their chunks) cut the HBM traffic by 4 %
+  // but made the launch 16 % slower at 256^3
   const int tile = blockIdx.x;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * S3_TY;
