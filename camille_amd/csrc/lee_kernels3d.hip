@@ -153,11 +153,15 @@ __global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
 #pragma unroll
       for (int v = 0; v < 5; v++) w[v][4] = ok ? a.yin[v * n + (size_t)kk * plane + col] : (T)0;
     }
-    // plane k with its (i, j) halo: rows ty + 4m, columns tx and 64 + tx
-    T tv[2][2][5];
-#pragma unroll
-    for (int m = 0; m < 2; m++) {
-      const int rj = j0 - ST_H + ty + 4 * m;
+    // plane k with its (i, j) halo.  The centre (region rows 2..5, columns
+    // 2..65) is the window's plane k, already in registers; only the ring
+    // is loaded: region row hr (rows 0, 1 for ty 0, 1; rows 6, 7 for ty 2,
+    // 3) at columns tx and 64 + tx, and for tx < 4 the side column sc (0,
+    // 1, 66, 67) of region row 2 + ty
+    const int hr = ty < 2 ? ty : ty + 4, sc = tx < 2 ? tx : ST_TX + tx;
+    T th[2][5], ts[5];
+    {
+      const int rj = j0 - ST_H + hr;
 #pragma unroll
       for (int half = 0; half < 2; half++) {
         const int c = half == 0 ? tx : ST_TX + tx;
@@ -165,8 +169,13 @@ __global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
         const bool ok = (half == 0 || tx < 2 * ST_H) && rj >= 0 && rj < nj && ri >= 0 && ri < ni;
         const size_t q = (size_t)k * plane + (size_t)(ok ? rj : 0) * ni + (ok ? ri : 0);
 #pragma unroll
-        for (int v = 0; v < 5; v++) tv[m][half][v] = ok ? a.yin[v * n + q] : (T)0;
+        for (int v = 0; v < 5; v++) th[half][v] = ok ? a.yin[v * n + q] : (T)0;
       }
+      const int sj = j0 + ty, si = i0 - ST_H + sc;
+      const bool ok = tx < 2 * ST_H && sj < nj && si >= 0 && si < ni;
+      const size_t q = (size_t)k * plane + (size_t)(ok ? sj : 0) * ni + (ok ? si : 0);
+#pragma unroll
+      for (int v = 0; v < 5; v++) ts[v] = ok ? a.yin[v * n + q] : (T)0;
     }
     // per-point operands (a clamped column for threads outside the block)
     const size_t p = (size_t)k * plane + col;
@@ -190,14 +199,14 @@ __global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
       ynp[v] = KIND == STAGE_MID ? a.yn[v * n + p] : (T)0;
     }
 #pragma unroll
-    for (int m2 = 0; m2 < 2; m2++)
-#pragma unroll
-      for (int half = 0; half < 2; half++) {
-        if (half == 1 && tx >= 2 * ST_H) continue;
-        const int c = half == 0 ? tx : ST_TX + tx;
-#pragma unroll
-        for (int v = 0; v < 5; v++) s[v][ty + 4 * m2][c] = tv[m2][half][v];
+    for (int v = 0; v < 5; v++) {
+      s[v][hr][tx] = th[0][v];
+      s[v][ty + ST_H][tx + ST_H] = w[v][2];
+      if (tx < 2 * ST_H) {
+        s[v][hr][ST_TX + tx] = th[1][v];
+        s[v][ty + ST_H][sc] = ts[v];
       }
+    }
     __syncthreads();
     if (own) {
       const int cx = default_code(gi, ni, 2), cy = default_code(gj, nj, 2), cz = default_code(k, nk, 2);
