@@ -27,7 +27,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-PERSISTENT_GRID = 8 * 256  # workgroups of a persistent launch: 8 per CU on the 256 CUs
 N1D = 1025
 
 
@@ -86,7 +85,7 @@ def c5_leg(steps, warmup, single_steps):
     """Times the batched ensemble (FP32 and FP64 bulk), the FP32 ensemble run
     one handle per solve for comparison, and each solve's FP32 pressure
     error against its FP64 run."""
-    from camille_amd import FP32, FP64, PATH_FUSED, Batch, Solver
+    from camille_amd import FP32, FP64, GRID_RESIDENT, PATH_FUSED, Batch, Solver
     solves = c5_inputs()
     npts = sum(sv["x"].size for sv in solves)
     res = {"solves": len(solves), "points": npts, "steps": steps, "warmup": warmup}
@@ -109,8 +108,9 @@ def c5_leg(steps, warmup, single_steps):
         finals[label] = [b.solution(k)[4] for k in range(len(solves))]
         picked = dict(zip(("graph", "grid_stage", "grid_filter"), b.launch()))
         # the form the workload names: one persistent launch per RK stage
-        # (and filter), the step replayed from a HIP graph
-        b.set_launch(True, PERSISTENT_GRID, PERSISTENT_GRID)
+        # (and filter), one resident wave of workgroups each, the step
+        # replayed from a HIP graph
+        b.set_launch(True, GRID_RESIDENT, GRID_RESIDENT)
         b.solve(warmup)
         b.synchronize()
         t0 = time.perf_counter()

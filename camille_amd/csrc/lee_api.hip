@@ -341,14 +341,13 @@ int batch_autotune(lee_handle *h) {
   if (rc) return rc;
   std::vector<BatchSched> zero(h->nblk * h->cfg.rk_stages, BatchSched{0.0, 0.0, 0.0});
   HIPCHK(hipMemcpy(p.sched, zero.data(), sizeof(BatchSched) * zero.size(), hipMemcpyHostToDevice));
-  const int z2[2] = {0, 0};
-  HIPCHK(hipMemcpy(p.counter, z2, sizeof(z2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.counter, h->bp_zero, sizeof(h->bp_zero), hipMemcpyHostToDevice));
   int dev_cu = 256;
   {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && v > 0) dev_cu = v;
   }
-  const int grids[4] = {0, 4 * dev_cu, 8 * dev_cu, 16 * dev_cu};
+  const int grids[5] = {0, LEE_GRID_RESIDENT, 4 * dev_cu, 8 * dev_cu, 16 * dev_cu};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -1371,7 +1370,8 @@ int create_impl(const lee_config *cfgs, int cfg_stride, const lee_block *blocks,
         rc = LEE_EINVAL;
       }
     if (rc == LEE_OK) {
-      hipError_t e = hipMalloc((void **)&h->bp.counter, 2 * sizeof(int));
+      hipError_t e = hipMalloc((void **)&h->bp.counter, sizeof(h->bp_zero));
+      if (e == hipSuccess) e = hipMemcpy(h->bp.counter, h->bp_zero, sizeof(h->bp_zero), hipMemcpyHostToDevice);
       if (e != hipSuccess) {
         h->err = std::string("HIP error ") + hipGetErrorString(e);
         rc = LEE_EHIP;
@@ -1468,7 +1468,8 @@ int lee_compute_cfl(lee_handle *h, double *cmax) {
 
 int lee_set_batch_launch(lee_handle *h, int use_graph, int grid_stage, int grid_filter) {
   if (!h || !h->batch) return fail(h, LEE_EINVAL, "not a batch");
-  if (grid_stage < 0 || grid_filter < 0) return fail(h, LEE_EINVAL, "grid sizes must be >= 0");
+  if (grid_stage < LEE_GRID_RESIDENT || grid_filter < LEE_GRID_RESIDENT)
+    return fail(h, LEE_EINVAL, "grid sizes must be >= 0 or LEE_GRID_RESIDENT");
   HIPCHK(hipStreamSynchronize(h->stream));
   h->bp.use_graph = use_graph != 0;
   h->bp.grid_stage = grid_stage;

@@ -200,13 +200,15 @@ struct BatchPlan {
   int R = 1, TY = 4;           // stage rows per thread, filter tile height
   int stack = 0;               // filter: rows per wave of four-wave column-march stacks (0: tiles)
   int grid_stage = 0, grid_filter = 0;  // persistent grid sizes (0: one workgroup per tile)
+  int res_stage = 0, res_filter = 0;    // resident workgroups of the prepared stage and filter forms
   void *stage_args = nullptr;  // device StageArgs<T>[stages][nsolve]
   void *filter_args = nullptr; // device FilterArgs<T>[nsolve]
   int *tile_map_stage = nullptr, *tile_map_filter = nullptr;  // (solve, tile) per list entry
   int total_stage = 0, total_filter = 0;
   BatchSched *sched = nullptr;  // device [step][solve][stage]
   size_t sched_steps = 0;
-  int *counter = nullptr;      // device: [0] step index within the schedule, [1] scratch
+  int *counter = nullptr;      // device: [0] step index within the schedule, [1] scratch,
+                               // [32, 288) work queues of the persistent stage launches
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;
 };
@@ -241,7 +243,7 @@ struct lee_handle {
   hipEvent_t bp_host_ev[2] = {nullptr, nullptr};
   bool bp_host_pending[2] = {false, false};
   int bp_host_cur = 0;
-  int bp_zero[2] = {0, 0};
+  int bp_zero[288] = {};       // the batch counters at the start of a solve
   lee::Overset ov;
   lee::OpTable fd{}, flt{};
   lee::WideOp *fdw = nullptr, *flw = nullptr;  // device copies of FDO11P / SFO11P rows (fused path)

@@ -519,26 +519,31 @@ __global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__
 // stencils only, no bounds tests); stage_tile below picks the instance.
 // W: the derivative operator, OA4 (4, compile-time rows, halo 2) or FDO11P
 // (11, rows from a.fdw, halo 6).
-template <typename T, bool MEANF, bool HOLES, int KIND, int R, bool INNER, int W = 4, typename AT>
-__device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, int tile, StageLdsW<T, R, W> &s) {
-  // every fusion is written out (fmx): the interior and edge instances of
-  // this tile, and the batched kernels, round alike
-#pragma clang fp contract(off)
+// The global loads of one stage tile, held in registers: the tile values
+// with their halo (rows ty + 4k, k < KR; columns tx and, for tx < 2H,
+// 64 + tx) and the per-point operands of rows ty + 4k (k < R).
+template <typename T, int R, int W>
+struct StageRegs {
+  static constexpr int H = StageOp<W>::H, KR = (4 * R + 2 * H) / 4;
+  T ta[4][KR], tb[4][KR];
+  double pm[R][4], pj[R];
+  T pacc[R][4], pyn[R][4];
+};
+
+// (1)-(2): the loads of a tile.  INNER: the tile and its halo lie inside the
+// block (no bounds tests); the checked form reads the same values for such
+// a tile.
+template <typename T, int KIND, int R, bool INNER, int W = 4, typename AT>
+__device__ __forceinline__ void stage_load(const AT &a, int tile, StageRegs<T, R, W> &g) {
   constexpr int TY = 4 * R;
-  constexpr int H = StageOp<W>::H, NBC = StageOp<W>::NB;
-  constexpr int KR = (TY + 2 * H) / 4;  // region rows per thread
+  constexpr int H = StageOp<W>::H;
+  constexpr int KR = StageRegs<T, R, W>::KR;
   static_assert(TY + 2 * H == 4 * KR, "tile rows and the halo come in fours");
-  constexpr int RS = ST_TX + 2 * H;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
   const int ni = a.ni, nj = a.nj, n = ni * nj;
   const int VOFF[4] = {IS * n, IUX * n, IUY * n, IP * n};
-
-  // a tile whose halo lies inside the block needs no bounds checks and has
-  // interior stencils only
   constexpr bool inner = INNER;
-  // (1) tile values: rows ty + 4k (k < KR), columns tx and, for tx < 2H, 64 + tx
-  T ta[4][KR], tb[4][KR];
   const bool xtra = tx < 2 * H;
 #pragma unroll
   for (int k = 0; k < KR; k++) {
@@ -549,28 +554,52 @@ __device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, in
     const int qa = gj * ni + gia, qb = gj * ni + gib;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      ta[v][k] = oka ? a.yin[VOFF[v] + qa] : (T)0;
-      tb[v][k] = okb ? a.yin[VOFF[v] + qb] : (T)0;
+      g.ta[v][k] = oka ? a.yin[VOFF[v] + qa] : (T)0;
+      g.tb[v][k] = okb ? a.yin[VOFF[v] + qb] : (T)0;
     }
   }
-  // (2) per-point operands of rows ty + 4k (k < R)
   const int gi = i0 + tx;
-  double pm[R][4], pj[R];
-  T pacc[R][4], pyn[R][4];
 #pragma unroll
   for (int k = 0; k < R; k++) {
     const int gj = j0 + ty + 4 * k;
     const bool ok = inner || (gi < ni && gj < nj);
     const int p = ok ? gj * ni + gi : 0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) pm[k][q] = ok ? a.met[(size_t)q * n + p] : 0.0;
-    pj[k] = ok ? a.jac[p] : 0.0;
+    for (int q = 0; q < 4; q++) g.pm[k][q] = ok ? a.met[(size_t)q * n + p] : 0.0;
+    g.pj[k] = ok ? a.jac[p] : 0.0;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      pacc[k][v] = (KIND != STAGE_FIRST && ok) ? a.acc[VOFF[v] + p] : (T)0;
-      pyn[k][v] = (KIND == STAGE_MID && ok) ? a.yn[VOFF[v] + p] : (T)0;
+      g.pacc[k][v] = (KIND != STAGE_FIRST && ok) ? a.acc[VOFF[v] + p] : (T)0;
+      g.pyn[k][v] = (KIND == STAGE_MID && ok) ? a.yn[VOFF[v] + p] : (T)0;
     }
   }
+}
+
+// (3)-(4): the tile's values through LDS, the derivatives, right-hand side,
+// source, sponges and RK update of its points, from the loaded registers.
+template <typename T, bool MEANF, bool HOLES, int KIND, int R, bool INNER, int W = 4, typename AT>
+__device__ __forceinline__ void stage_finish(const AT &a, const StepT<T> &sc, int tile, const StageRegs<T, R, W> &g,
+                                             StageLdsW<T, R, W> &s) {
+  // every fusion is written out (fmx): the interior and edge instances of
+  // this tile, and the batched kernels, round alike
+#pragma clang fp contract(off)
+  constexpr int TY = 4 * R;
+  constexpr int H = StageOp<W>::H, NBC = StageOp<W>::NB;
+  constexpr int KR = StageRegs<T, R, W>::KR;
+  constexpr int RS = ST_TX + 2 * H;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * TY;
+  const int ni = a.ni, nj = a.nj, n = ni * nj;
+  const int VOFF[4] = {IS * n, IUX * n, IUY * n, IP * n};
+  constexpr bool inner = INNER;
+  const bool xtra = tx < 2 * H;
+  const int gi = i0 + tx;
+  const auto &ta = g.ta;
+  const auto &tb = g.tb;
+  const auto &pm = g.pm;
+  const auto &pj = g.pj;
+  const auto &pacc = g.pacc;
+  const auto &pyn = g.pyn;
   // (3) tile to LDS
 #pragma unroll
   for (int k = 0; k < KR; k++) {
@@ -728,6 +757,13 @@ __device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, in
   }
 }
 
+template <typename T, bool MEANF, bool HOLES, int KIND, int R, bool INNER, int W = 4, typename AT>
+__device__ __forceinline__ void stage_tile_k(const AT &a, const StepT<T> &sc, int tile, StageLdsW<T, R, W> &s) {
+  StageRegs<T, R, W> g;
+  stage_load<T, KIND, R, INNER, W>(a, tile, g);
+  stage_finish<T, MEANF, HOLES, KIND, R, INNER, W>(a, sc, tile, g, s);
+}
+
 template <typename T, bool MEANF, bool HOLES, int KIND, int R, int W = 4, typename AT>
 __device__ __forceinline__ void stage_tile(const AT &a, const StepT<T> &sc, int tile, StageLdsW<T, R, W> &s) {
   constexpr int TY = 4 * R, H = StageOp<W>::H;
@@ -774,13 +810,13 @@ __global__ void __launch_bounds__(TPB) k_fused_stage_pair(StageArgs<T> a0, Stage
 }
 
 // Batched ensemble: the tiles of all solves form one list; map[2t] is the
-// solve of list entry t and map[2t + 1] its tile within the solve.  A
-// persistent grid walks the
-// list with a grid stride; virtual index v maps to list position
-// (v & 7) * chunk + (v >> 3), so with a grid that is a multiple of 8 each
-// XCD works through one contiguous eighth of the list.  Per-solve operands
-// come from `args`, the per-step scalars from `sched`, indexed by the step
-// counter that the step's last kernel advances.
+// solve of list entry t and map[2t + 1] its tile within the solve.  The
+// list is split into eighths of `chunk` entries, one per XCD: workgroup
+// (or virtual index) v maps to list position (v & 7) * chunk + (v >> 3),
+// so with a grid that is a multiple of 8 each XCD works through one
+// contiguous eighth of the list.  Per-solve operands come from `args`, the
+// per-step scalars from `sched`, indexed by the step counter that the
+// step's last kernel advances.
 template <typename T>
 struct BatchStageArgs {
   const StageArgs<T> *args;      // nsolve entries for this stage
@@ -794,21 +830,95 @@ struct BatchStageArgs {
 // The operand tables are read through the constant address space: they do
 // not change during a launch, so their loads are scalar and the pointers
 // loaded from them are known to address global memory (no flat accesses).
+// One workgroup per list entry (the queue argument is unused: this kernel
+// shares its signature with the persistent form below).
 template <typename T, bool MEANF, int KIND, int R>
 __global__ void __launch_bounds__(TPB)
     k_batch_stage(BatchStageArgs<T> ba, const LEE_CONST_AS StageArgs<T> *args, const LEE_CONST_AS int *map,
-                  const LEE_CONST_AS BatchSched *sched, const int *__restrict__ counter) {
+                  const LEE_CONST_AS BatchSched *sched, const int *__restrict__ counter, int *) {
   __shared__ StageLds<T, R> s;
+  const int t = (blockIdx.x & 7) * ba.chunk + (blockIdx.x >> 3);
+  if (t >= ba.total) return;
   // device-scope load: the counter was written by the previous step's filter launch
   const int step = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int v = blockIdx.x; v < 8 * ba.chunk; v += gridDim.x) {
-    const int t = (v & 7) * ba.chunk + (v >> 3);
-    if (t >= ba.total) continue;
+  const int k = map[2 * t], local = map[2 * t + 1];
+  const auto &q = sched[((size_t)step * ba.nsolve + k) * ba.nstage + ba.stage];
+  const StepT<T> sc{(T)q.dt, ba.b, ba.a, q.s1, q.s2};
+  stage_tile<T, MEANF, false, KIND, R>(args[k], sc, local, s);
+}
+
+// Persistent form of the batched stage (a grid smaller than the list).
+// Workgroup b works on eighth b & 7 of the list (the hardware places
+// workgroup b on XCD b mod 8, so each XCD keeps one contiguous eighth),
+// one of the W = grid / 8 workgroups of that eighth.  The first S entries
+// of the eighth (7/8 of them, whole rounds of W) are dealt out statically,
+// entry w + k W to workgroup w; the rest are drawn QCH at a time from an
+// atomic counter, so that the workgroups whose static share held cheap
+// tiles (interior, outside sponges) take more of the remainder and the
+// launch ends with the list instead of with the slowest fixed share.
+// queue[QSTRIDE x] counts the draws from eighth x and queue[QSTRIDE x + 1]
+// the workgroups of eighth x that have finished; the last of those resets
+// both, so every launch starts from zero (each eighth has its own cache
+// line).  A workgroup stops once its entry passes the eighth's end, and
+// every draw advances the counter, so the loop ends.
+constexpr int QCH = 2, QSTRIDE = 32;
+
+template <typename T, bool MEANF, int KIND, int R>
+__global__ void __launch_bounds__(TPB)
+    k_batch_stage_q(BatchStageArgs<T> ba, const LEE_CONST_AS StageArgs<T> *args, const LEE_CONST_AS int *map,
+                    const LEE_CONST_AS BatchSched *sched, const int *__restrict__ counter, int *__restrict__ queue) {
+  __shared__ StageLds<T, R> s;
+  __shared__ int drawn[2];
+  const int step = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int x = blockIdx.x & 7, lo = x * ba.chunk;
+  const int n = min(ba.chunk, ba.total - lo);  // entries of this eighth (may be <= 0)
+  const int W = (int)(gridDim.x >> 3), w = (int)(blockIdx.x >> 3);
+  const int S = n > 0 ? (n * 7 / 8) / W * W : 0;
+  int *const qx = queue + QSTRIDE * x;
+  const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+  // draw d goes through slot d & 1: every thread reads it right after draw
+  // d's barrier, so before draw d + 1's barrier, and the lead thread writes
+  // the slot again only at draw d + 2, after that barrier
+  int nd = 0;
+  auto draw = [&]() {
+    const int slot = nd++ & 1;
+    if (lead) drawn[slot] = __hip_atomic_fetch_add(qx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return S + QCH * drawn[slot];
+  };
+  auto advance = [&](int e) {
+    if (e < S) return e + W < S ? e + W : draw();
+    if ((e - S) % QCH != QCH - 1) return e + 1;
+    return draw();
+  };
+  // the loads take the bounds-checked form, which reads the same values for
+  // an interior tile; the finish takes the tile's own instance
+  auto finish = [&](int e, const StageRegs<T, R, 4> &g) {
+    const int t = lo + e;
     const int k = map[2 * t], local = map[2 * t + 1];
     const auto &q = sched[((size_t)step * ba.nsolve + k) * ba.nstage + ba.stage];
     const StepT<T> sc{(T)q.dt, ba.b, ba.a, q.s1, q.s2};
-    stage_tile<T, MEANF, false, KIND, R>(args[k], sc, local, s);
-    __syncthreads();  // the LDS tile is rewritten by the next iteration
+    const int i0 = (local % args[k].ntx) * ST_TX, j0 = (local / args[k].ntx) * (4 * R);
+    if (i0 >= ST_H && i0 + ST_TX + ST_H <= args[k].ni && j0 >= ST_H && j0 + 4 * R + ST_H <= args[k].nj)
+      stage_finish<T, MEANF, false, KIND, R, true>(args[k], sc, local, g, s);
+    else
+      stage_finish<T, MEANF, false, KIND, R, false>(args[k], sc, local, g, s);
+  };
+  int e = w < S ? w : draw();
+  while (e < n) {
+    const int t = lo + e;
+    StageRegs<T, R, 4> g;
+    stage_load<T, KIND, R, false>(args[map[2 * t]], map[2 * t + 1], g);
+    finish(e, g);
+    __syncthreads();  // the LDS tile is rewritten by the next entry
+    e = advance(e);
+  }
+  if (lead) {
+    const int done = __hip_atomic_fetch_add(qx + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (done == W - 1) {
+      __hip_atomic_store(qx, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(qx + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -1688,6 +1798,9 @@ void batch_release(lee_handle *h) {
 // stage: the right-hand-side form, Y2 -> Y3 and Y1; filter: Y1 -> Y2), so
 // the autotuner can time launches without touching the state in Y0.
 template <typename T>
+static void batch_resident_t(lee_handle *h);
+
+template <typename T>
 static hipError_t batch_prepare_t(hipStream_t s, lee_handle *h) {
   BatchPlan &p = h->bp;
   const int ns = h->nblk, S = h->cfg.rk_stages;
@@ -1733,6 +1846,7 @@ static hipError_t batch_prepare_t(hipStream_t s, lee_handle *h) {
     return e;
   // the host vectors die here: wait for the copies
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  batch_resident_t<T>(h);
   p.ready = true;
   return hipSuccess;
 }
@@ -1743,11 +1857,84 @@ hipError_t batch_prepare(hipStream_t s, lee_handle *h) {
   return batch_prepare_t<double>(s, h);
 }
 
-static unsigned batch_grid(int total, int persistent) {
+// Grid of a batch launch over `total` list entries: one workgroup per entry
+// (setting 0, or a setting at least that large), a persistent grid of the
+// given size rounded up to a multiple of 8, or (LEE_GRID_RESIDENT) the
+// workgroups the device holds at once for that kernel.
+static unsigned batch_grid(int total, int setting, int resident) {
   const int chunk = (total + 7) / 8;
   const int full = 8 * chunk;
-  if (persistent <= 0 || persistent >= full) return (unsigned)full;
-  return (unsigned)((persistent + 7) / 8 * 8);
+  const int g = setting == LEE_GRID_RESIDENT ? resident : setting;
+  if (g <= 0 || g >= full) return (unsigned)full;
+  return (unsigned)((g + 7) / 8 * 8);
+}
+
+template <typename T>
+using BatchStageFn = void (*)(BatchStageArgs<T>, const LEE_CONST_AS StageArgs<T> *, const LEE_CONST_AS int *,
+                              const LEE_CONST_AS BatchSched *, const int *, int *);
+template <typename T>
+using BatchFilterFn = void (*)(BatchFilterArgs<T>, const LEE_CONST_AS FilterArgs<T> *, const LEE_CONST_AS int *);
+
+// queued: the persistent work-queue kernel (a grid smaller than the list)
+template <typename T, bool MF, int K>
+static BatchStageFn<T> batch_stage_fn_r(int R, bool queued) {
+  if (queued) {
+    if (R == 1) return k_batch_stage_q<T, MF, K, 1>;
+    if (R == 2) return k_batch_stage_q<T, MF, K, 2>;
+    return k_batch_stage_q<T, MF, K, 4>;
+  }
+  if (R == 1) return k_batch_stage<T, MF, K, 1>;
+  if (R == 2) return k_batch_stage<T, MF, K, 2>;
+  return k_batch_stage<T, MF, K, 4>;
+}
+
+template <typename T, bool MF>
+static BatchStageFn<T> batch_stage_fn_k(int kind, int R, bool queued) {
+  if (kind == STAGE_FIRST) return batch_stage_fn_r<T, MF, STAGE_FIRST>(R, queued);
+  if (kind == STAGE_MID) return batch_stage_fn_r<T, MF, STAGE_MID>(R, queued);
+  return batch_stage_fn_r<T, MF, STAGE_LAST>(R, queued);
+}
+
+template <typename T>
+static BatchStageFn<T> batch_stage_fn(const lee_handle *h, int kind, bool queued) {
+  const int R = h->bp.R;
+  return h->b[0].uniform_mean ? batch_stage_fn_k<T, false>(kind, R, queued)
+                              : batch_stage_fn_k<T, true>(kind, R, queued);
+}
+
+template <typename T, int OR>
+static BatchFilterFn<T> batch_filter_fn_o(const BatchPlan &p) {
+  if (p.stack == 4) return k_batch_filter_s<T, OR, 4>;
+  if (p.stack == 8) return k_batch_filter_s<T, OR, 8>;
+  if (p.stack) return k_batch_filter_s<T, OR, 16>;
+  if (p.TY == 4) return k_batch_filter<T, OR, 4>;
+  if (p.TY == 8) return k_batch_filter<T, OR, 8>;
+  if (p.TY == 12) return k_batch_filter<T, OR, 12>;
+  return k_batch_filter<T, OR, 16>;
+}
+
+template <typename T>
+static BatchFilterFn<T> batch_filter_fn(const lee_handle *h) {
+  return h->cfg.filter_order == 8 ? batch_filter_fn_o<T, 8>(h->bp) : batch_filter_fn_o<T, 4>(h->bp);
+}
+
+// Workgroups of kernel f resident on the whole device at once (its
+// occupancy per CU, from registers and LDS, times the CUs), a multiple of 8.
+static int resident_workgroups(const void *f, int device) {
+  int per = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, TPB, 0) != hipSuccess || per < 1) per = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 8;
+  return std::max(8, per * cus / 8 * 8);
+}
+
+template <typename T>
+static void batch_resident_t(lee_handle *h) {
+  BatchPlan &p = h->bp;
+  int rs = 1 << 30;
+  for (int kind : {STAGE_FIRST, STAGE_MID, STAGE_LAST})
+    rs = std::min(rs, resident_workgroups(reinterpret_cast<const void *>(batch_stage_fn<T>(h, kind, true)), h->device));
+  p.res_stage = rs;
+  p.res_filter = resident_workgroups(reinterpret_cast<const void *>(batch_filter_fn<T>(h)), h->device);
 }
 
 template <typename T>
@@ -1767,28 +1954,11 @@ static hipError_t batch_stage_t(hipStream_t s, lee_handle *h, int stage) {
   ba.b = (T)h->b_i[stage];
   ba.a = (T)h->a_next[stage];
   const int kind = p.tuning ? STAGE_MID : stage_kind(h, stage);
-  const dim3 grid(batch_grid(p.total_stage, p.grid_stage)), block(64, 4);
-  const bool mf = !h->b[0].uniform_mean;
-  const int R = p.R;
-#define LEE_B(MF, K)                                                                                       \
-  do {                                                                                                     \
-    if (R == 1) hipLaunchKernelGGL((k_batch_stage<T, MF, K, 1>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map), \
-                                   CA(ba.sched), ba.counter);                                                  \
-    else if (R == 2) hipLaunchKernelGGL((k_batch_stage<T, MF, K, 2>), grid, block, 0, s, ba, CA(ba.args),  \
-                                        CA(ba.map), CA(ba.sched), ba.counter);                                     \
-    else hipLaunchKernelGGL((k_batch_stage<T, MF, K, 4>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map),  \
-                            CA(ba.sched), ba.counter);                                                         \
-  } while (0)
-#define LEE_BK(MF)                                  \
-  do {                                              \
-    if (kind == STAGE_FIRST) LEE_B(MF, STAGE_FIRST); \
-    else if (kind == STAGE_MID) LEE_B(MF, STAGE_MID); \
-    else LEE_B(MF, STAGE_LAST);                     \
-  } while (0)
-  if (mf) LEE_BK(true);
-  else LEE_BK(false);
-#undef LEE_BK
-#undef LEE_B
+  const unsigned g = batch_grid(p.total_stage, p.grid_stage, p.res_stage);
+  const bool queued = g < 8u * (unsigned)ba.chunk;
+  const dim3 grid(g), block(64, 4);
+  hipLaunchKernelGGL(batch_stage_fn<T>(h, kind, queued), grid, block, 0, s, ba, CA(ba.args), CA(ba.map), CA(ba.sched),
+                     ba.counter, p.counter + QSTRIDE);
   return hipGetLastError();
 }
 
@@ -1807,35 +1977,8 @@ static hipError_t batch_filter_t(hipStream_t s, lee_handle *h) {
   ba.nsolve = h->nblk;
   ba.total = p.total_filter;
   ba.chunk = (p.total_filter + 7) / 8;
-  const dim3 grid(batch_grid(p.total_filter, p.grid_filter)), block(64, 4);
-  const bool o8 = h->cfg.filter_order == 8;
-  const int TY = p.TY;
-  if (p.stack) {
-#define LEE_BS(OR, RR) hipLaunchKernelGGL((k_batch_filter_s<T, OR, RR>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map))
-#define LEE_BS_O(OR)                        \
-  do {                                      \
-    if (p.stack == 4) LEE_BS(OR, 4);        \
-    else if (p.stack == 8) LEE_BS(OR, 8);   \
-    else LEE_BS(OR, 16);                    \
-  } while (0)
-    if (o8) LEE_BS_O(8);
-    else LEE_BS_O(4);
-#undef LEE_BS_O
-#undef LEE_BS
-    return hipGetLastError();
-  }
-#define LEE_BF(OR)                                                                                   \
-  do {                                                                                               \
-    if (TY == 4) hipLaunchKernelGGL((k_batch_filter<T, OR, 4>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map)); \
-    else if (TY == 8) hipLaunchKernelGGL((k_batch_filter<T, OR, 8>), grid, block, 0, s, ba, CA(ba.args), \
-                                         CA(ba.map));                                                       \
-    else if (TY == 12) hipLaunchKernelGGL((k_batch_filter<T, OR, 12>), grid, block, 0, s, ba, CA(ba.args), \
-                                          CA(ba.map));                                                      \
-    else hipLaunchKernelGGL((k_batch_filter<T, OR, 16>), grid, block, 0, s, ba, CA(ba.args), CA(ba.map));        \
-  } while (0)
-  if (o8) LEE_BF(8);
-  else LEE_BF(4);
-#undef LEE_BF
+  const dim3 grid(batch_grid(p.total_filter, p.grid_filter, p.res_filter)), block(64, 4);
+  hipLaunchKernelGGL(batch_filter_fn<T>(h), grid, block, 0, s, ba, CA(ba.args), CA(ba.map));
   return hipGetLastError();
 }
 

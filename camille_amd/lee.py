@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "liblee_amd.so")
 
 FP64, FP32 = 0, 1
 PATH_FUSED, PATH_PLAIN = 0, 1
+GRID_RESIDENT = -1  # batch launch grid: the workgroups the device holds at once (include/lee.h)
 
 # BC-file type names (src/io/io.cpp:1273-1320) -> src/core/param.h codes
 BC_TYPES = {
@@ -654,6 +655,9 @@ class Batch(Solver):
         return st.value, t.value, dt.value
 
     def set_launch(self, use_graph=True, grid_stage=0, grid_filter=0):
+        """Graph replay on/off and the persistent grids of the stage and
+        filter launches: 0 one workgroup per tile, GRID_RESIDENT one
+        resident wave of workgroups, else that many workgroups."""
         self._check(self.L.lee_set_batch_launch(self.h, 1 if use_graph else 0, int(grid_stage), int(grid_filter)))
 
     def launch(self):

@@ -276,9 +276,12 @@ int lee_get_filter_march(lee_handle *h, int iblock, int *waves, int *rows);
 int lee_batch_create(const lee_config *cfgs, const lee_block *blocks, int nsolve, lee_handle **out);
 int lee_get_solve_time(const lee_handle *h, int isolve, int *time_step, double *time_sol, double *dt);
 /* Launch settings of a batch: graph replay on/off, persistent grid sizes of
- * the stage and filter launches (0 = one workgroup per tile).  Chosen by the
+ * the stage and filter launches (0 = one workgroup per tile;
+ * LEE_GRID_RESIDENT = as many workgroups as the device holds at once for
+ * that kernel, its occupancy per CU times the CUs).  Chosen by the
  * autotuner at the first lee_solve unless set here; every setting gives
  * bit-identical results. */
+#define LEE_GRID_RESIDENT (-1)
 int lee_set_batch_launch(lee_handle *h, int use_graph, int grid_stage, int grid_filter);
 int lee_get_batch_launch(lee_handle *h, int *use_graph, int *grid_stage, int *grid_filter);
 

@@ -13,7 +13,7 @@ import math
 import numpy as np
 import pytest
 
-from camille_amd import FP32, FP64, PATH_FUSED, PATH_PLAIN, Batch, LeeError, Solver, lee
+from camille_amd import FP32, FP64, GRID_RESIDENT, PATH_FUSED, PATH_PLAIN, Batch, LeeError, Solver, lee
 
 GAMMA = 1.4
 
@@ -140,7 +140,8 @@ def test_batch_launch_modes_bit_identical():
     for use_graph, gs, gf, shape, march in [
             (True, 0, 0, (1, 4), None), (False, 0, 0, (1, 4), None), (True, 256, 512, (2, 8), None),
             (True, 1024, 0, (4, 16), None), (False, 8, 8, (1, 8), None), (True, 0, 64, (2, 12), None),
-            (True, 0, 0, (1, 4), (4, 8)), (False, 0, 24, (2, 8), (4, 4)), (True, 0, 512, (1, 4), (4, 16))]:
+            (True, 0, 0, (1, 4), (4, 8)), (False, 0, 24, (2, 8), (4, 4)), (True, 0, 512, (1, 4), (4, 16)),
+            (True, GRID_RESIDENT, GRID_RESIDENT, (2, 8), None), (False, GRID_RESIDENT, GRID_RESIDENT, (4, 4), (4, 8))]:
         b = _batch(solves, FP32)
         b.set_tile_shape(*shape)
         if march:
