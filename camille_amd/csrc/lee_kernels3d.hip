@@ -476,20 +476,22 @@ __global__ void __launch_bounds__(TPB) k_filter3d_fused(FilterArgs<T> a, int nk,
     for (int q = 0; q <= H; q++) yo[p][q] = (T)0;
   }
   const int kfirst = max(k0 - H, 0), klast = min(k1 + H, nk) - 1;
-  PlaneRegs<T, TY> R, Rn;
-  b.src = src + (size_t)kfirst * plane;
-  filter_load<T, false, 1, TY>(b, tile, R);
   const int gi = i0 + threadIdx.x;
-  for (int kk = k0 - H; kk < k1 + H; kk++) {
+  // One plane kk: `cur` holds plane clamp(kk, kfirst, klast) and the loads
+  // of plane clamp(kk + 1, ...) into `nxt` are issued first, whatever the
+  // branch below takes, so that they stay in flight through this plane's
+  // sweeps (a load issued on one path only makes the compiler wait for
+  // every outstanding load before the sweeps).  The loop runs an even
+  // number of planes, alternating the two register sets, so that no copy
+  // between them waits for the loads behind this plane's stores.
+  auto step = [&](int kk, const PlaneRegs<T, TY> &cur, PlaneRegs<T, TY> &nxt) {
     T f[NP], orig[NP];
+    b.src = src + (size_t)min(max(kk + 1, kfirst), klast) * plane;
+    filter_load<T, false, 1, TY>(b, tile, nxt);
+    __builtin_amdgcn_sched_barrier(0);
     if (kk >= kfirst && kk <= klast) {
-      if (kk < klast) {  // the next plane's loads stay in flight through this plane's sweeps
-        b.src = src + (size_t)(kk + 1) * plane;
-        filter_load<T, false, 1, TY>(b, tile, Rn);
-      }
-      if (inner) plane_xieta<T, ORDER, TY, true>(b, tile, L, R, f, orig);
-      else plane_xieta<T, ORDER, TY, false>(b, tile, L, R, f, orig);
-      if (kk < klast) R = Rn;
+      if (inner) plane_xieta<T, ORDER, TY, true>(b, tile, L, cur, f, orig);
+      else plane_xieta<T, ORDER, TY, false>(b, tile, L, cur, f, orig);
     } else {
 #pragma unroll
       for (int p = 0; p < NP; p++) f[p] = orig[p] = (T)0;
@@ -504,7 +506,7 @@ __global__ void __launch_bounds__(TPB) k_filter3d_fused(FilterArgs<T> a, int nk,
       yo[p][H] = orig[p];
     }
     const int k = kk - H;
-    if (k >= k0) {
+    if (k >= k0 && k < k1) {
       const int code = default_code(k, nk, NB);
 #pragma unroll
       for (int p = 0; p < NP; p++) {
@@ -514,6 +516,15 @@ __global__ void __launch_bounds__(TPB) k_filter3d_fused(FilterArgs<T> a, int nk,
         dst[(size_t)k * plane + (size_t)gj * a.ni + gi] = filter_blend(a.rem, yo[p][0], a.blend, fz);
       }
     }
+  };
+  PlaneRegs<T, TY> A, B;
+  b.src = src + (size_t)kfirst * plane;
+  filter_load<T, false, 1, TY>(b, tile, A);
+  // planes k0 - H .. k1 + H - 1, plus one past them when that count is odd
+  const int kend = k0 - H + ((k1 - k0 + 2 * H + 1) & ~1);
+  for (int kk = k0 - H; kk < kend; kk += 2) {
+    step(kk, A, B);
+    step(kk + 1, B, A);
   }
 }
 

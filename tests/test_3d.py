@@ -106,13 +106,15 @@ def test_axis_exchange_symmetry():
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-@pytest.mark.parametrize("dims", [(130, 80, 12), (70, 22, 140)])
+@pytest.mark.parametrize("dims", [(130, 80, 12), (70, 22, 140), (70, 22, 77)])
 def test_3d_plane_filter_forms_bit_identical(prec, dims):
     """Every form of the 3-D filter gives the same bits: the two-pass form
     with its plane pass as tiles or as stacks of 4 waves of 4, 8 or 16 rows,
     and the single pass (tiles of 8 or 16 rows marching through chunks of
-    64 planes).  The 130 x 80 planes have interior tiles and stacks besides
-    edge ones; the 140-plane box spans three chunks of the single pass."""
+    64 planes, two planes per loop trip).  The 130 x 80 planes have
+    interior tiles and stacks besides edge ones; the 140-plane box spans
+    three chunks of the single pass; the 77-plane box ends with a chunk of
+    13 planes, an odd count of planes with their halo."""
     ni, nj, nk = dims
     x, y, z = _box(ni, nj, nk)
     yc, zc = y.mean(), z.mean()
