@@ -28,6 +28,27 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert L.lee_version().startswith(b"camille_amd")
 
 
+def _macros():
+    txt = open(os.path.join(ROOT, "include", "lee.h")).read()
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"#define\s+(LEE_[A-Z0-9_]+)\s+\(?(-?\d+)\)?", txt)}
+
+
+def test_binding_constants_match_the_header():
+    """The ctypes binding's precision, path, launch-grid and BC-type codes
+    are the header's macros."""
+    m = _macros()
+    assert (lee.FP64, lee.FP32) == (m["LEE_FP64"], m["LEE_FP32"])
+    assert (lee.PATH_FUSED, lee.PATH_PLAIN) == (m["LEE_PATH_FUSED"], m["LEE_PATH_PLAIN"])
+    assert lee.GRID_RESIDENT == m["LEE_GRID_RESIDENT"] == camille_amd.GRID_RESIDENT
+    for name, code in lee.BC_TYPES.items():
+        key = "LEE_BC_" + name.upper() if not name.startswith("sponge") else "LEE_" + name.upper()
+        assert m[key] == code, name
+    for name, code in lee.WAVES.items():
+        assert m["LEE_WAVE_" + name.replace("WAVE_", "")] == code, name
+    for name, code in lee.WAVEFORMS.items():
+        assert m["LEE_" + name] == code, name
+
+
 def _cfg(**kw):
     c = lee._Config()
     c.num_dim = 2
