@@ -167,6 +167,36 @@ def test_batch_launch_modes_bit_identical():
 
 
 @pytest.mark.gpu
+def test_batch_persistent_queue_small_lists():
+    """The persistent stage's per-eighth work queue on short tile lists:
+    two small solves give eighths with few or no entries (some workgroups
+    find theirs empty at once), grids of 1 to 4 workgroups per eighth and
+    the resident grid deal out and draw every tile exactly once, on every
+    stage of 9 steps and again on the next solve call (the counters reset
+    themselves): the same bits as one workgroup per tile."""
+    solves = [_solve_inputs(33, 33, 0.3, 0.3 * math.pi, (0.5, 0.5)),
+              _solve_inputs(41, 35, 0.1, 0.45 * math.pi, (0.3, 0.6))]
+    ref = None
+    # 18 tiles of 4 rows (eighths of 3, the last two empty) or 10 of 8 rows
+    # (eighths of 2, the last three empty); grids from the list's length up
+    # launch one workgroup per tile
+    for gs, shape in [(0, (1, 4)), (8, (1, 4)), (16, (1, 4)), (8, (2, 8)), (GRID_RESIDENT, (1, 4))]:
+        b = _batch(solves, FP32)
+        b.set_tile_shape(*shape)
+        b.set_launch(True, gs, 0)
+        b.solve(4)
+        b.solve(5)
+        sols = [b.solution(k) for k in range(len(solves))]
+        b.close()
+        if ref is None:
+            ref = sols
+            assert np.max(np.abs(ref[0][3])) > 0
+        for k in range(len(solves)):
+            assert np.array_equal(sols[k], ref[k]), (gs, shape, k)
+    print("persistent queue on short lists: bit-identical to one workgroup per tile", flush=True)
+
+
+@pytest.mark.gpu
 def test_batch_schedule_chunks():
     """More steps than one schedule chunk (2048): small solves, compared with
     single handles after 2100 steps."""
