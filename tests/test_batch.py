@@ -190,7 +190,7 @@ def test_batch_persistent_queue_small_lists():
         b.close()
         if ref is None:
             ref = sols
-            assert np.max(np.abs(ref[0][3])) > 0
+            assert np.max(np.abs(ref[0][4])) > 0
         for k in range(len(solves)):
             assert np.array_equal(sols[k], ref[k]), (gs, shape, k)
     print("persistent queue on short lists: bit-identical to one workgroup per tile", flush=True)
