@@ -165,6 +165,7 @@ struct AdjArgs {
   double src;      // s1 s2 of this stage
   const int8_t *iblank, *code;  // holes only: iblank, FD codes (xi then eta)
   const int *clean;             // holes only: integral image of points off the interior FD rows
+  const T *rad;                 // axisymmetric runs: the forward's radius field r + 1e-99 (null: none)
 };
 
 template <typename T, int R>
@@ -301,6 +302,13 @@ __global__ void __launch_bounds__(TPB) k_adj_stage(AdjArgs<T> a) {
       lam[2] += -(gsy * ks) - guy * ku - gvy * kv - gpy * kp;
       lam[3] += -(g * gux * kp) - g * gvy * kp;
       mur = -((u0 * gux + v0 * guy) * ku + (u0 * gvx + v0 * gvy) * kv) / rb;
+      if (a.rad) {
+        // axisymmetric term r_p -= (gamma pbar v' + gamma vbar p') / r
+        // (spatial:2119-2126), with the forward's gamma pbar and r fields
+        const T rr = a.rad[p], gpb = (T)(a.gamma * a.mean[IP * n + p]);
+        lam[2] += -(gpb / rr) * kp;
+        lam[3] += -(g * v0 / rr) * kp;
+      }
       if (KIND != STAGE_FIRST) {  // rho' of this stage's input: R(u)
         const double pb = a.mean[IP * n + p];
         lam[0] += -(rb * mur);
@@ -609,6 +617,10 @@ __global__ void __launch_bounds__(TPB) k_adj_bc_face(T *__restrict__ lam, const 
       for (int v = 0; v < NV; v++) neu[v] = true;
       neu[IUX + (a.type - LEE_BC_CENTERLINE_CART_NORM2X)] = false;
       break;
+    case LEE_BC_CENTERLINE_AXISYM:  // u_r = u_y' set to 0 on the axis (bc:498-545)
+      for (int v = 0; v < NV; v++) neu[v] = true;
+      neu[IUY] = false;
+      break;
     default:  // dirichlet_allzero
       break;
   }
@@ -675,6 +687,10 @@ static hipError_t adj_stage_t(hipStream_t s, lee_handle *h, Block &b, const AdjS
   a.iblank = b.iblank;
   a.code = b.code_fd;
   a.clean = b.clean_fd;
+  if (h->cfg.axisym) {  // an axisymmetric block always has its coefficient fields
+    const T *mc = sizeof(T) == 8 ? (const T *)b.mc64 : (const T *)b.mc32;
+    a.rad = mc + (size_t)MC_R * b.n;
+  }
   const dim3 grid(a.chunk * 8), block(64, 4);
   const bool mf = !b.uniform_mean, ho = b.has_holes;
 #define LEE_ADJ(MF, K, HO)                                                                     \

@@ -719,14 +719,13 @@ void advance_clock(lee_handle *h) {
 int adjoint_check(lee_handle *h) {
   if (h->batch) return fail(h, LEE_EINVAL, "the adjoint runs on single handles, not batches");
   if (is_plain(h)) return fail(h, LEE_EINVAL, "the adjoint runs on the fused path");
-  if (h->cfg.axisym) return fail(h, LEE_EINVAL, "the adjoint does not cover the axisymmetric term");
   if (h->cfg.num_scalar > 0) return fail(h, LEE_EINVAL, "the adjoint does not cover passive scalars (LEE_SCALAR)");
   if (h->cfg.fd_order == 11 || h->cfg.filter_order == 11)
     return fail(h, LEE_EINVAL, "the adjoint covers the standard schemes (OA4 derivative, 4th- or 8th-order filter)");
   for (int ib = 0; ib < h->nblk; ib++) {
     for (const auto &f : h->b[ib].faces)
-      if (f.type == LEE_BC_CENTERLINE_AXISYM || f.type == LEE_BC_DIRICHLET_HARMONICWAVE)
-        return fail(h, LEE_EINVAL, "the adjoint does not cover axisymmetric or harmonic-wave boundaries");
+      if (f.type == LEE_BC_DIRICHLET_HARMONICWAVE)
+        return fail(h, LEE_EINVAL, "the adjoint does not cover harmonic-wave boundaries");
     if (h->b[ib].carry_w) return fail(h, LEE_EINVAL, "the adjoint needs w' = 0 in 2-D");
     for (const auto &z : h->b[ib].zones)
       if (z.harmonic) return fail(h, LEE_EINVAL, "the adjoint covers ambient sponges only");
