@@ -621,7 +621,7 @@ __global__ void __launch_bounds__(TPB) k_adj_bc_face(T *__restrict__ lam, const 
       for (int v = 0; v < NV; v++) neu[v] = true;
       neu[IUY] = false;
       break;
-    default:  // dirichlet_allzero
+    default:  // dirichlet_allzero, and dirichlet_harmonicwave (its data do not depend on the state)
       break;
   }
   for (int v = 0; v < NV; v++) lam[(size_t)v * n + p] = (T)keep[v];

@@ -723,12 +723,9 @@ int adjoint_check(lee_handle *h) {
   if (h->cfg.fd_order == 11 || h->cfg.filter_order == 11)
     return fail(h, LEE_EINVAL, "the adjoint covers the standard schemes (OA4 derivative, 4th- or 8th-order filter)");
   for (int ib = 0; ib < h->nblk; ib++) {
-    for (const auto &f : h->b[ib].faces)
-      if (f.type == LEE_BC_DIRICHLET_HARMONICWAVE)
-        return fail(h, LEE_EINVAL, "the adjoint does not cover harmonic-wave boundaries");
+    // harmonic-wave faces and zones are affine in the state: their linear
+    // parts are those of a dirichlet_allzero face and an ambient sponge
     if (h->b[ib].carry_w) return fail(h, LEE_EINVAL, "the adjoint needs w' = 0 in 2-D");
-    for (const auto &z : h->b[ib].zones)
-      if (z.harmonic) return fail(h, LEE_EINVAL, "the adjoint covers ambient sponges only");
   }
   return LEE_OK;
 }

@@ -286,11 +286,11 @@ int lee_set_batch_launch(lee_handle *h, int use_graph, int grid_stage, int grid_
 int lee_get_batch_launch(lee_handle *h, int *use_graph, int *grid_stage, int *grid_filter);
 
 /* Discrete adjoint (no reference counterpart: the reference has no
- * adjoint).  Fused path; 2-D blocks with sponges, sources, holes, overset
- * interpolation, the axisymmetric term and the dirichlet_allzero, Neumann,
- * wall-slip, Cartesian and axisymmetric centerline faces; 3-D blocks with
- * sponges and sources.  Not covered: harmonic-wave boundaries, passive
- * scalars and the 11-point schemes.
+ * adjoint).  Fused path; 2-D blocks with sponges (harmonic-wave zones
+ * included), sources, holes, overset interpolation, the axisymmetric term
+ * and the dirichlet_allzero, harmonic-wave, Neumann, wall-slip, Cartesian
+ * and axisymmetric centerline faces; 3-D blocks with sponges and sources.
+ * Not covered: passive scalars and the 11-point schemes.
  *
  * Source field of a LEE_SOURCE_FIELD block: p-equation source q(x) A sin(omega t). */
 int lee_set_source_field(lee_handle *h, int iblock, const double *q);

@@ -50,12 +50,14 @@ def test_filter_transpose_dot_product(name, prec, tol):
     s.close()
 
 
-@pytest.mark.parametrize("name", ["curv", "shear", "jet", "c1", "axisym"])
+@pytest.mark.parametrize("name", ["curv", "shear", "jet", "c1", "axisym", "roundjet"])
 @pytest.mark.parametrize("prec,tol", [(FP64, 1e-12), (FP32, 1e-5)])
 def test_rhs_transpose_dot_product(name, prec, tol):
     """The axisymmetric case adds the term -(gamma pbar v' + gamma vbar p')
     / (r + 1e-99) of the pressure equation; u_r' = v' is 0 on the axis (the
-    axis face sets it), as in every state the stages see."""
+    axis face sets it), as in every state the stages see.  The round-jet
+    entropy wave's harmonic sponge is affine: L y - L 0 keeps its linear
+    part, -sigma s'."""
     c = cases.ALL[name]()
     if c.get("axisym"):
         # a radial mean velocity as well (0 on the axis), so that both of the
@@ -81,11 +83,12 @@ def test_rhs_transpose_dot_product(name, prec, tol):
     s.close()
 
 
-@pytest.mark.parametrize("name", ["entropy"])
+@pytest.mark.parametrize("name", ["scalar", "opt11"])
 def test_adjoint_rejects_uncovered_terms(name):
-    """The adjoint covers sponges, sources, holes, overset interpolation,
-    the axisymmetric term and the dirichlet_allzero, Neumann, wall-slip,
-    Cartesian and axisymmetric centerline faces; harmonic-wave boundaries
+    """The adjoint covers sponges (harmonic-wave zones included), sources,
+    holes, overset interpolation, the axisymmetric term and the
+    dirichlet_allzero, harmonic-wave, Neumann, wall-slip, Cartesian and
+    axisymmetric centerline faces; passive scalars and the 11-point schemes
     are refused."""
     c = cases.ALL[name]()
     s = Solver.from_case(c)
@@ -94,11 +97,12 @@ def test_adjoint_rejects_uncovered_terms(name):
     s.close()
 
 
-@pytest.mark.parametrize("name", ["wall", "dirichlet", "centerline", "opt11", "axisym"])
+@pytest.mark.parametrize("name", ["wall", "dirichlet", "centerline", "opt11", "axisym", "entropy", "roundjet"])
 def test_boundary_face_transposes_dot_products(name):
-    """<B y, z> = <y, B^T z> for the BC faces of the fixtures (wall-slip and
-    Neumann on curvilinear faces, dirichlet_allzero, Cartesian centerlines,
-    the axis of an axisymmetric run);
+    """<B y - B 0, z> = <y, B^T z> for the BC faces of the fixtures
+    (wall-slip and Neumann on curvilinear faces, dirichlet_allzero,
+    Cartesian centerlines, the axis of an axisymmetric run, the
+    harmonic-wave faces, whose data do not depend on the state);
     the opt11 case runs its faces with the standard operator's closure row
     here (the fused path takes OA4)."""
     c = cases.ALL[name]()
@@ -107,7 +111,7 @@ def test_boundary_face_transposes_dot_products(name):
     s = Solver.from_case(c)
     rng = np.random.default_rng(14)
     y, z = _rand_state(c["x"].shape, rng), _rand_state(c["x"].shape, rng)
-    By = s.update_boundary([y])[0]
+    By = s.update_boundary([y])[0] - s.update_boundary([np.zeros_like(y)])[0]
     Btz = s.update_boundary_adjoint([z])[0]
     err = abs(_dot(By, z) - _dot(y, Btz)) / abs(_dot(By, z))
     print(f"face transposes {name}: rel {err:.2e}", flush=True)
@@ -223,6 +227,45 @@ def test_axisymmetric_gradient_matches_central_differences():
     ady = _dot(gy, ey)
     err_y = abs(fdy - ady) / abs(fdy)
     print(f"axisym gradient: q fd {fd:.12e} adjoint {ad:.12e} rel {err_q:.2e}; "
+          f"y0 fd {fdy:.12e} adjoint {ady:.12e} rel {err_y:.2e}", flush=True)
+    s.close()
+    assert err_q <= 1e-6 and err_y <= 1e-6
+
+
+def test_harmonic_wave_gradient_matches_central_differences():
+    """The round-jet entropy wave fixture (dirichlet_harmonicwave face and
+    harmonic sponge on +xi, a jet-like mean, a wavy grid) driven by a field
+    source as well, observers on a circle: J is quadratic in (q, y0) with
+    the wave's response as an affine part, and the adjoint gradient matches
+    central differences in the source and initial-state directions."""
+    c = cases.ALL["roundjet"]()
+    x, y = c["x"], c["y"]
+    s = Solver([dict(x=x, y=y, mean=c["mean"])], boundaries=c["bc"], precision=FP64, harmonic=c["harmonic"],
+               simulation="SOURCE_FIELD", source_params=(1.0, 0.4 * math.pi))
+    # observers, source and initial pulse next to the wave's face, so that
+    # the harmonic sponge (x > 8.7) and face lie in the adjoint's reach
+    r = np.sqrt((x - 6.0) ** 2 + y ** 2)
+    dx = 0.25
+    obs = [(0, i, j, dx * dx) for j in range(x.shape[0]) for i in range(x.shape[1]) if abs(r[j, i] - 3.0) < 0.6 * dx]
+    assert len(obs) > 20
+    s.set_observer(obs)
+    q = np.exp(-((x - 5.0) ** 2 + (y - 1.0) ** 2) / 2.0)
+    rng = np.random.default_rng(23)
+    y0 = 1e-3 * _rand_state(x.shape, rng) * np.exp(-((x - 7.0) ** 2 + y ** 2) / 8.0)
+    nsteps, window, every = 40, 25, 10
+    J0, gq, gy = _grads(s, q, y0, nsteps, window, every)
+    assert J0 > 0 and np.any(gq != 0) and np.any(gy != 0)
+    e = rng.standard_normal(x.shape) * np.exp(-((x - 5.0) ** 2 + y ** 2) / 20.0)
+    eps = 1e-4
+    fd = (_J(s, q + eps * e, y0, nsteps, window, every) - _J(s, q - eps * e, y0, nsteps, window, every)) / (2 * eps)
+    ad = _dot(gq, e)
+    err_q = abs(fd - ad) / abs(fd)
+    ey = 1e-3 * _rand_state(x.shape, rng)
+    eps = 1e-2
+    fdy = (_J(s, q, y0 + eps * ey, nsteps, window, every) - _J(s, q, y0 - eps * ey, nsteps, window, every)) / (2 * eps)
+    ady = _dot(gy, ey)
+    err_y = abs(fdy - ady) / abs(fdy)
+    print(f"harmonic-wave gradient: q fd {fd:.12e} adjoint {ad:.12e} rel {err_q:.2e}; "
           f"y0 fd {fdy:.12e} adjoint {ady:.12e} rel {err_y:.2e}", flush=True)
     s.close()
     assert err_q <= 1e-6 and err_y <= 1e-6
