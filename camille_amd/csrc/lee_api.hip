@@ -719,7 +719,9 @@ void advance_clock(lee_handle *h) {
 int adjoint_check(lee_handle *h) {
   if (h->batch) return fail(h, LEE_EINVAL, "the adjoint runs on single handles, not batches");
   if (is_plain(h)) return fail(h, LEE_EINVAL, "the adjoint runs on the fused path");
-  if (h->cfg.num_scalar > 0) return fail(h, LEE_EINVAL, "the adjoint does not cover passive scalars (LEE_SCALAR)");
+  // passive scalars (LEE_SCALAR) feed nothing back into s', u', v', p': the
+  // adjoint of a functional of p' vanishes on them, and the adjoint runs on
+  // the five LEE variables alone (checkpoints keep those five)
   if (h->cfg.fd_order == 11 || h->cfg.filter_order == 11)
     return fail(h, LEE_EINVAL, "the adjoint covers the standard schemes (OA4 derivative, 4th- or 8th-order filter)");
   for (int ib = 0; ib < h->nblk; ib++) {

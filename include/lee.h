@@ -287,10 +287,11 @@ int lee_get_batch_launch(lee_handle *h, int *use_graph, int *grid_stage, int *gr
 
 /* Discrete adjoint (no reference counterpart: the reference has no
  * adjoint).  Fused path; 2-D blocks with sponges (harmonic-wave zones
- * included), sources, holes, overset interpolation, the axisymmetric term
- * and the dirichlet_allzero, harmonic-wave, Neumann, wall-slip, Cartesian
- * and axisymmetric centerline faces; 3-D blocks with sponges and sources.
- * Not covered: passive scalars and the 11-point schemes.
+ * included), sources, holes, overset interpolation, the axisymmetric term,
+ * passive scalars (they feed nothing back: the adjoint runs on the five
+ * LEE variables) and the dirichlet_allzero, harmonic-wave, Neumann,
+ * wall-slip, Cartesian and axisymmetric centerline faces; 3-D blocks with
+ * sponges and sources.  Not covered: the 11-point schemes.
  *
  * Source field of a LEE_SOURCE_FIELD block: p-equation source q(x) A sin(omega t). */
 int lee_set_source_field(lee_handle *h, int iblock, const double *q);
