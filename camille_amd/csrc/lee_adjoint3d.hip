@@ -179,7 +179,10 @@ template <typename T, bool MEANF, int KIND>
 __global__ void __launch_bounds__(TPB) k_adj_stage3d(Adj3Args<T> a) {
   constexpr int RS = ST_TX + 2 * ST_H, NR = A3_TY + 2 * ST_H;
   __shared__ T S[10][NR][RS];  // xi fluxes then eta fluxes of the 5 variables
-  const int tile = blockIdx.x;
+  // XCD-banded tiles within each z chunk, as the forward stage
+  const int ntl = a.ntx * a.nty;
+  const int tile = xcd_tile(blockIdx.x, (ntl + 7) / 8);
+  if (tile >= ntl) return;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * A3_TY;
   const int k0 = blockIdx.y * A3_ZC, k1 = min(k0 + A3_ZC, a.nk);
@@ -393,7 +396,7 @@ __global__ void __launch_bounds__(TPB) k_adj_filter3d_plane(AdjF3Args<T> a) {
   constexpr int NR = TY + 2 * FL_H, NC = FL_TX + 2 * FL_H, KR = NR / 4;
   __shared__ T in[5][NR][NC];
   __shared__ T mid[5][TY][NC];
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, (a.ntiles + 7) / 8);  // XCD-banded tiles of each plane
   if (tile >= a.ntiles) return;
   const int k = blockIdx.y;
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -511,7 +514,7 @@ static hipError_t adj_stage3d_t(hipStream_t s, lee_handle *h, Block &b, const Ad
   a.hb = st.hb;
   a.ha = st.ha;
   a.src = st.src;
-  const dim3 grid(a.ntx * a.nty, a.nzc), block(ST_TX, A3_TY);
+  const dim3 grid((a.ntx * a.nty + 7) / 8 * 8, a.nzc), block(ST_TX, A3_TY);
   const bool mf = !b.uniform_mean;
 #define LEE_A3(MF)                                                                                            \
   do {                                                                                                        \
@@ -557,7 +560,7 @@ static hipError_t adj_filter3d_t(hipStream_t s, lee_handle *h, Block &b, int src
   const double blend = h->cfg.filter_order > 0 ? h->cfg.filter_blend : 0.0;
   a.blend = (T)blend;
   a.rem = (T)(1.0 - blend);
-  const dim3 grid(a.ntiles, b.nk), block(64, 4);
+  const dim3 grid((a.ntiles + 7) / 8 * 8, b.nk), block(64, 4);
   if (o8) hipLaunchKernelGGL((k_adj_filter3d_plane<T, 8, TY>), grid, block, 0, s, a);
   else hipLaunchKernelGGL((k_adj_filter3d_plane<T, 4, TY>), grid, block, 0, s, a);
   return hipGetLastError();

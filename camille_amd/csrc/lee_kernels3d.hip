@@ -123,10 +123,12 @@ template <typename T, bool MEANF, int KIND>
 __global__ void __launch_bounds__(TPB) k_stage3d(Stage3Args<T> a) {
   constexpr int RS = ST_TX + 2 * ST_H, NR = S3_TY + 2 * ST_H;
   __shared__ T s[5][NR][RS];
-  // (tile, chunk) in launch order: an XCD-aware tile-major order (each XCD
-  // a band of tiles through all their chunks) cut the HBM traffic by 4 %
-  // but made the launch 16 % slower at 256^3
-  const int tile = blockIdx.x;
+  // within each z chunk, XCD x (workgroups x mod 8 of a grid row that is a
+  // multiple of 8 wide) takes the x-th band of tiles, so that the halo a
+  // tile reads of its neighbours is mostly in the same XCD's L2
+  const int ntl = a.ntx * a.nty;
+  const int tile = xcd_tile(blockIdx.x, (ntl + 7) / 8);
+  if (tile >= ntl) return;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = (tile % a.ntx) * ST_TX, j0 = (tile / a.ntx) * S3_TY;
   const int k0 = blockIdx.y * Z_CHUNK, k1 = min(k0 + Z_CHUNK, a.nk);
@@ -679,7 +681,7 @@ static hipError_t stage3d_t(hipStream_t s, lee_handle *h, Block &b, int stage, c
   a.h = (T)sc.h;
   a.b = (T)sc.b;
   a.a = (T)sc.a;
-  const dim3 grid(a.ntx * a.nty, a.nzc), block(ST_TX, S3_TY);
+  const dim3 grid((a.ntx * a.nty + 7) / 8 * 8, a.nzc), block(ST_TX, S3_TY);
   const bool mf = !b.uniform_mean;
 #define LEE_S3(MF)                                                                                    \
   do {                                                                                                \
