@@ -152,14 +152,7 @@ int enqueue_fused_step(lee_handle *h) {
     bool any_bnd = h->ov.nrecv > 0;
     for (int ib = 0; ib < h->nblk; ib++) any_bnd |= !h->b[ib].faces.empty();
     if (h->ktiming && any_bnd) HIPCHK(mark(h));
-    for (int ib = 0; ib < h->nblk; ib++) {
-      Block &b = h->b[ib];
-      if (b.faces.empty()) continue;
-      const bool rho = st == last && !b.uniform_mean;
-      HIPCHK(apply_bcs(h->stream, h, b, b.Y64[tgt], b.Y32[tgt], f32, rho ? b.rho64 : nullptr,
-                       rho ? b.rho32 : nullptr, he_bc));
-    }
-    HIPCHK(overset_interpolate(h->stream, h, f32, tgt, st == last));
+    if (any_bnd) HIPCHK(boundary_phase_fused(h->stream, h, f32, tgt, st == last, he_bc));
     if (h->ktiming && any_bnd) {
       HIPCHK(mark(h));
       h->evclass.push_back(4);

@@ -312,6 +312,7 @@ double harmonic_gaussian(const lee_config &c, double x, double y, double z);
 hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, float *y32, bool fp32,
                      double *rho64, float *rho32, double he = 0.0);
 hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_index, bool last_stage);
+hipError_t boundary_phase_fused(hipStream_t s, lee_handle *h, bool fp32, int tgt, bool last, double he);
 // Overset interpolation of caller-owned FP64 block arrays ys[b] (device,
 // 5n each), receiver values staged in buf (nrecv * NV doubles, device).
 hipError_t overset_interpolate_arrays(hipStream_t s, lee_handle *h, double *const *ys, double *buf);

@@ -267,16 +267,14 @@ struct BcArgs {
 // bc::apply_BC one face (boundary_condition.cpp:46-209) with bc_neumann
 // (:348-444), bc_wall_slip_kinematic (:448-494), dirichlet_allzero and the
 // Cartesian centerline conditions.  Arithmetic in FP64 for both precisions.
+// Face point q (of the face box, i fastest) of one BC face.
 template <typename T>
-__global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double *__restrict__ met,
-                                                 const double *__restrict__ jac,
-                                                 const int8_t *__restrict__ iblank, BcArgs a,
-                                                 T *__restrict__ rho, const double *__restrict__ mean,
-                                                 const double *__restrict__ rhob, double gi) {
+__device__ __forceinline__ void bc_face_point(T *__restrict__ y, const double *__restrict__ met,
+                                              const double *__restrict__ jac, const int8_t *__restrict__ iblank,
+                                              const BcArgs &a, T *__restrict__ rho, const double *__restrict__ mean,
+                                              const double *__restrict__ rhob, double gi, int q) {
 #pragma clang fp contract(off)
-  const int fw = a.i1 - a.i0 + 1, fh = a.j1 - a.j0 + 1;
-  const int q = blockIdx.x * TPB + threadIdx.x;
-  if (q >= fw * fh) return;
+  const int fw = a.i1 - a.i0 + 1;
   const int i = a.i0 + q % fw, j = a.j0 + q / fw;
   const int n = a.ni * a.nj;
   const int p = j * a.ni + i;
@@ -378,6 +376,17 @@ __global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double
     r *= rhob[p];
     rho[p] = (T)r;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_bc_face(T *__restrict__ y, const double *__restrict__ met,
+                                                 const double *__restrict__ jac,
+                                                 const int8_t *__restrict__ iblank, BcArgs a,
+                                                 T *__restrict__ rho, const double *__restrict__ mean,
+                                                 const double *__restrict__ rhob, double gi) {
+  const int q = blockIdx.x * TPB + threadIdx.x;
+  if (q >= (a.i1 - a.i0 + 1) * (a.j1 - a.j0 + 1)) return;
+  bc_face_point<T>(y, met, jac, iblank, a, rho, mean, rhob, gi, q);
 }
 
 // ---------------------------------------------------------------- overset
@@ -495,12 +504,10 @@ __global__ void __launch_bounds__(TPB) k_ov_gather5(OvTables<T> t, const int *__
 
 // update_each_grid (overset.cpp:1241-1278); optionally refreshes rho'.
 template <typename T>
-__global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__restrict__ rb,
-                                                    const int *__restrict__ rl, const double *__restrict__ buf,
-                                                    int nrecv, int with_rho, double gi) {
+__device__ __forceinline__ void ov_scatter_one(const OvTables<T> &t, const int *__restrict__ rb,
+                                               const int *__restrict__ rl, const double *__restrict__ buf,
+                                               int with_rho, double gi, int r) {
 #pragma clang fp contract(off)
-  const int r = blockIdx.x * TPB + threadIdx.x;
-  if (r >= nrecv) return;
   const int b = rb[r];
   const int n = t.nn[b], p = rl[r];
   if (p < 0 || p >= n) return;
@@ -511,6 +518,15 @@ __global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__
     q *= t.rhob[b][p];
     t.rho[b][p] = (T)q;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_ov_scatter(OvTables<T> t, const int *__restrict__ rb,
+                                                    const int *__restrict__ rl, const double *__restrict__ buf,
+                                                    int nrecv, int with_rho, double gi) {
+  const int r = blockIdx.x * TPB + threadIdx.x;
+  if (r >= nrecv) return;
+  ov_scatter_one<T>(t, rb, rl, buf, with_rho, gi, r);
 }
 
 // ------------------------------------------------------------------ fused
@@ -1284,24 +1300,29 @@ hipError_t plain_iblank_reset(hipStream_t s, const Block &b, double *y, int nv) 
   return hipGetLastError();
 }
 
+static BcArgs make_bc_args(const lee_handle *h, const Block &b, const BcFace &f, double he) {
+  BcArgs a{};
+  a.he = he;
+  a.hg = f.hg;
+  a.type = f.type;
+  a.dir = f.dir;
+  a.end = f.end;
+  a.i0 = f.is[0];
+  a.i1 = f.ie[0];
+  a.j0 = f.is[1];
+  a.j1 = f.ie[1];
+  a.ni = b.ni;
+  a.nj = b.nj;
+  a.sb = h->fd.sb;
+  a.nsc = h->cfg.num_scalar;
+  for (int q = 0; q < 9; q++) a.neu[q] = h->fd.bnd[0][q];
+  return a;
+}
+
 hipError_t apply_bcs(hipStream_t s, lee_handle *h, const Block &b, double *y64, float *y32, bool fp32,
                      double *rho64, float *rho32, double he) {
   for (const auto &f : b.faces) {
-    BcArgs a{};
-    a.he = he;
-    a.hg = f.hg;
-    a.type = f.type;
-    a.dir = f.dir;
-    a.end = f.end;
-    a.i0 = f.is[0];
-    a.i1 = f.ie[0];
-    a.j0 = f.is[1];
-    a.j1 = f.ie[1];
-    a.ni = b.ni;
-    a.nj = b.nj;
-    a.sb = h->fd.sb;
-    a.nsc = h->cfg.num_scalar;
-    for (int q = 0; q < 9; q++) a.neu[q] = h->fd.bnd[0][q];
+    const BcArgs a = make_bc_args(h, b, f, he);
     const size_t cnt = (size_t)(a.i1 - a.i0 + 1) * (a.j1 - a.j0 + 1);
     const int8_t *ib = b.has_holes ? b.iblank : nullptr;
     if (fp32)
@@ -1366,6 +1387,24 @@ hipError_t overset_interpolate(hipStream_t s, lee_handle *h, bool fp32, int buf_
 hipError_t overset_interpolate_arrays(hipStream_t s, lee_handle *h, double *const *ys, double *buf) {
   if (h->ov.nrecv == 0) return hipSuccess;
   return overset_t<double>(s, h, ys, nullptr, false, buf);
+}
+
+// The boundary phase of a fused stage on the stage's target buffers: the
+// BC faces of every block (a launch per face), then the overset
+// interpolation (gather, then scatter).  One cooperative launch with grid
+// barriers in between was measured slower (overset cylinder 1.72e10
+// against 2.18e10 FP32 updates/s over 1000 steps): the runtime's
+// cooperative launch costs more than the launches it replaces.
+hipError_t boundary_phase_fused(hipStream_t s, lee_handle *h, bool fp32, int tgt, bool last, double he) {
+  hipError_t e = hipSuccess;
+  for (int ib = 0; ib < h->nblk; ib++) {
+    Block &b = h->b[ib];
+    if (b.faces.empty()) continue;
+    const bool rho = last && !b.uniform_mean;
+    e = apply_bcs(s, h, b, b.Y64[tgt], b.Y32[tgt], fp32, rho ? b.rho64 : nullptr, rho ? b.rho32 : nullptr, he);
+    if (e != hipSuccess) return e;
+  }
+  return overset_interpolate(s, h, fp32, tgt, last);
 }
 
 template <typename T>
