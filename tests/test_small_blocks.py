@@ -87,3 +87,45 @@ def test_small_block_matches_oracle(shape):
         err = max(_rel(q[v], ref[v]) for v in vars_checked)
         print(f"small block {shape} {label}: {err:.2e}", flush=True)
         assert err <= tol, (label, shape, err)
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_small_blocks_in_a_batch_equal_single_handles(prec):
+    """The seven small blocks and one 70 x 50 block as one batched ensemble
+    (one launch per RK stage over every solve's tiles, so most workgroups
+    hold a tile that straddles a small block's edges): every solve equals
+    its own single handle bit for bit after 12 steps, in the tuner's launch
+    form, the persistent + graph form, direct launches of 16-row tiles and
+    the stacked filter."""
+    from camille_amd import GRID_RESIDENT, Batch
+    shapes = SHAPES + [(70, 50)]
+    solves = []
+    for ni, nj in shapes:
+        x, y, mean, ic = _inputs(ni, nj, seed=ni * 1000 + nj + 1)
+        solves.append(dict(x=x, y=y, mean=mean, ic=ic, mode="FIX_CFL", value=0.5))
+    nsteps = 12
+    singles = []
+    for sv in solves:
+        s = Solver([dict(x=sv["x"], y=sv["y"], mean=sv["mean"])], boundaries=(), precision=prec, path=PATH_FUSED,
+                   mode="FIX_CFL", value=0.5)
+        s.set_solution(sv["ic"])
+        s.solve(nsteps)
+        singles.append((s.solution(), s.clock()))
+        s.close()
+    for form in ("tuned", "persistent", "direct16", "stacked"):
+        b = Batch(solves, precision=prec)
+        for k, sv in enumerate(solves):
+            b.set_solution(sv["ic"], k)
+        if form == "persistent":
+            b.set_launch(True, GRID_RESIDENT, GRID_RESIDENT)
+        elif form == "direct16":
+            b.set_tile_shape(4, 16)
+            b.set_launch(False, 0, 0)
+        elif form == "stacked":
+            b.set_filter_march(4, 8)
+        b.solve(nsteps)
+        for k in range(len(solves)):
+            assert b.clock(k) == singles[k][1], (form, shapes[k])
+            assert np.array_equal(b.solution(k), singles[k][0]), (form, shapes[k])
+        print(f"small blocks batch prec {prec} {form} {b.launch()}: bit-identical to single handles", flush=True)
+        b.close()
