@@ -182,3 +182,41 @@ def test_small_3d_boxes_reproduce_2d(shape):
         assert np.all(np.isfinite(q))
         assert err <= tol
         assert wmax <= tol * np.max(np.abs(ref[4]))
+
+
+def _rand_state(shape, rng):
+    y = rng.standard_normal((5,) + shape)
+    y[3] = 0.0
+    return y
+
+
+@pytest.mark.parametrize("shape", SHAPES + [(9, 9, 9), (12, 9, 70), (131, 9, 10)])
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-12), (FP32, 1e-5)])
+def test_small_block_transposes_dot_products(shape, prec, tol):
+    """The adjoint's transposed right-hand side and filter on small blocks,
+    where the closure rows of opposite faces overlap in the transposes:
+    <L a, k> = <a, L^T k> and <F a, k> = <a, F^T k> for seeded random a, k."""
+    rng = np.random.default_rng(sum(shape))
+    if len(shape) == 2:
+        x, y, mean, _ = _inputs(*shape, seed=7)
+        blk, kw = dict(x=x, y=y, mean=mean), {}
+        a, k = _rand_state(x.shape, rng), _rand_state(x.shape, rng)
+    else:
+        ni, nj, nk = shape
+        x, y, mean, _ = _inputs(ni, nj, seed=7, shift_cells=0.5)
+        x3 = np.repeat(x[None], nk, 0)
+        blk = dict(x=x3, y=np.repeat(y[None], nk, 0), z=0.45 * np.arange(nk)[:, None, None] + 0.0 * x3,
+                   mean=np.repeat(mean[:, None], nk, 1))
+        kw = dict(num_dim=3)
+        a, k = rng.standard_normal((2, 5) + x3.shape)
+    s = Solver([blk], boundaries=(), precision=prec, **kw)
+    t = 0.3
+    La = s.compute_rhs(a, t) - s.compute_rhs(np.zeros_like(a), t)
+    v = [0, 1, 2, 4] if len(shape) == 2 else [0, 1, 2, 3, 4]
+    lhs, rhs = float(np.sum(La[v] * k[v])), float(np.sum(a * s.compute_rhs_adjoint(k)))
+    e1 = abs(lhs - rhs) / abs(lhs)
+    lhs2, rhs2 = float(np.sum(s.apply_filter(a) * k)), float(np.sum(a * s.apply_filter_adjoint(k)))
+    e2 = abs(lhs2 - rhs2) / abs(lhs2)
+    s.close()
+    print(f"small block {shape} prec {prec} dot products: rhs {e1:.2e}, filter {e2:.2e}", flush=True)
+    assert e1 <= tol and e2 <= tol
