@@ -220,3 +220,48 @@ def test_small_block_transposes_dot_products(shape, prec, tol):
     s.close()
     print(f"small block {shape} prec {prec} dot products: rhs {e1:.2e}, filter {e2:.2e}", flush=True)
     assert e1 <= tol and e2 <= tol
+
+
+BC_FACES = [(0, "wall_slip_kinematic", "+eta", "0", "-0", "0", "0", "0", "0"),
+            (0, "wall_slip_kinematic", "-eta", "0", "-0", "-0", "-0", "0", "0"),
+            (0, "neumann", "+xi", "0", "0", "0", "-0", "0", "0"),
+            (0, "dirichlet_allzero", "-xi", "-0", "-0", "0", "-0", "0", "0")]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_small_block_with_boundary_faces_matches_oracle(shape):
+    """The small blocks with a BC on every face (wall-slip on both eta
+    faces, Neumann on the first xi face, Dirichlet-zero on the last;
+    boundary_condition.cpp:98-112, 348-494), so faces of different types
+    meet at corners a few points apart and the face kernels cover whole
+    sides of blocks narrower than their launches. With the faces the
+    oracle stays bounded (its response to a 1e-7 relative perturbation of
+    the initial state stays below 4e-7 over 30 steps), so every path is
+    held after 30 steps: plain bit for bit, fused FP64 to 1e-12, FP32 to
+    1e-5 (p')."""
+    from oracle.pyoracle import Oracle
+    ni, nj = shape
+    x, y, mean, ic = _inputs(ni, nj, seed=ni * 1000 + nj + 3)
+    nsteps = 30
+    o = Oracle([dict(x=x, y=y, mean=mean, ic=ic)], SCHEME, BC_FACES, None, "FIX_CFL", 0.5)
+    o.run(nsteps)
+    ref = o.solution(0)[:, 0]
+    assert np.all(np.isfinite(ref)) and np.max(np.abs(ref[4])) > 1e-5
+    for label, prec, path, tol in (("plain", FP64, PATH_PLAIN, 0.0), ("fused64", FP64, PATH_FUSED, 1e-12),
+                                   ("fused32", FP32, PATH_FUSED, 1e-5)):
+        print(f"small block {shape} with faces {label}", flush=True)
+        s = Solver([dict(x=x, y=y, mean=mean)], boundaries=BC_FACES, precision=prec, path=path, mode="FIX_CFL",
+                   value=0.5)
+        s.set_solution(ic)
+        s.solve(nsteps)
+        q = s.solution()
+        step, time, dt = s.clock()
+        s.close()
+        assert step == nsteps and time == o.time and dt == o.dt, (label, shape)
+        if tol == 0.0:
+            assert np.array_equal(q, ref), (label, shape)
+            continue
+        vars_checked = (4,) if prec == FP32 else (0, 1, 2, 4)
+        err = max(_rel(q[v], ref[v]) for v in vars_checked)
+        print(f"small block {shape} with faces {label}: {err:.2e}", flush=True)
+        assert err <= tol, (label, shape, err)
