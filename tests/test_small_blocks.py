@@ -6,11 +6,11 @@ keeps one interior row: four boundary rows each side) up to thin strips of
 4-64 rows), filter strip (56 outputs plus a 4-point halo each side) and
 stack the kernels use, so every workgroup straddles the block's edges and
 the one-sided FD closures and boundary filter rows of opposite faces meet
-in the middle. No BC faces or sponges: every face takes the scheme's
-default closures (spatial_discretization.cpp:1078-1336 for the FD rows,
-348-492 and 2191-2246 for the filter), on a curvilinear grid with a
-uniform mean flow at an angle, a pressure pulse and seeded noise at every
-point.
+in the middle. The first test has no BC faces or sponges: every face
+takes the scheme's default closures (spatial_discretization.cpp:1078-1336
+for the FD rows, 348-492 and 2191-2246 for the filter), on a curvilinear
+grid with a uniform mean flow at an angle, a pressure pulse and seeded
+noise at every point.
 
 On blocks this small the closures of opposite faces amplify small
 perturbations: the oracle's own response to a 1e-7 relative perturbation
@@ -18,10 +18,12 @@ of the initial state reaches 3e-5 (relative) after 30 steps on the
 9 x 14 block, 3e-6 after 8. FP32 bulk arithmetic perturbs the state at
 that level on every step, so it is held to its bar after 8 steps.
 
-Bars (relative to max |.| of each field): after 30 steps the plain path
-equal to the oracle bit for bit and the fused FP64 path to 1e-12
-(rho', u', v', p'); after 8 steps FP32 bulk to 1e-5 (p'). The clock
-(step, time, dt) equals the oracle's.
+Bars of the first test (relative to max |.| of each field): after 30
+steps the plain path equal to the oracle bit for bit and the fused FP64
+path to 1e-12 (rho', u', v', p'); after 8 steps FP32 bulk to 1e-5 (p').
+The clock (step, time, dt) equals the oracle's. The other tests put the same
+blocks in a batch, extrude them to 3-D, check the adjoint's transposes on
+them and give them a boundary condition on every face.
 """
 import numpy as np
 import pytest
