@@ -267,3 +267,47 @@ def test_small_block_with_boundary_faces_matches_oracle(shape):
         err = max(_rel(q[v], ref[v]) for v in vars_checked)
         print(f"small block {shape} with faces {label}: {err:.2e}", flush=True)
         assert err <= tol, (label, shape, err)
+
+
+@pytest.mark.parametrize("shape", [(11, 11), (14, 11), (11, 17), (131, 11), (11, 131), (66, 13)])
+def test_small_block_optimized_schemes_match_oracle(shape):
+    """FDO11P and SFO11P (strength 0.6; spatial_discretization.cpp:228-344,
+    348-492) on small blocks with a BC on every face, from 11 points a
+    side (the narrowest with an interior row: five closure rows each
+    side). Without the faces these blocks grow fast under the 11-point
+    schemes (the oracle's max |p'| reaches 5 from 1e-3 in 30 steps), with
+    them they stay bounded; the oracle's response to a 1e-7 relative
+    perturbation of the initial state is up to 3.3e-6 after 30 steps and
+    1.1e-6 after 8. Plain bit for bit and fused FP64 to 1e-12 after 30
+    steps; FP32 to 1e-5 (p') after 8."""
+    from oracle.pyoracle import Oracle
+    ni, nj = shape
+    x, y, mean, ic = _inputs(ni, nj, seed=ni * 1000 + nj + 4)
+    scheme = dict(SCHEME, fd_order=11, filter_order=11, filter_strength=0.6)
+    o = Oracle([dict(x=x, y=y, mean=mean, ic=ic)], scheme, BC_FACES, None, "FIX_CFL", 0.5)
+    refs = {}
+    o.run(8)
+    refs[8] = (o.solution(0)[:, 0], o.time, o.dt)
+    o.run(22)
+    refs[30] = (o.solution(0)[:, 0], o.time, o.dt)
+    assert np.all(np.isfinite(refs[30][0])) and np.max(np.abs(refs[30][0][4])) > 1e-5
+    for label, prec, path, nsteps, tol in (("plain", FP64, PATH_PLAIN, 30, 0.0),
+                                           ("fused64", FP64, PATH_FUSED, 30, 1e-12),
+                                           ("fused32", FP32, PATH_FUSED, 8, 1e-5)):
+        print(f"small block {shape} 11-point {label}", flush=True)
+        ref, t_ref, dt_ref = refs[nsteps]
+        s = Solver([dict(x=x, y=y, mean=mean)], boundaries=BC_FACES, precision=prec, path=path, mode="FIX_CFL",
+                   value=0.5, fd_order=11, filter_order=11, filter_strength=0.6)
+        s.set_solution(ic)
+        s.solve(nsteps)
+        q = s.solution()
+        step, time, dt = s.clock()
+        s.close()
+        assert step == nsteps and time == t_ref and dt == dt_ref, (label, shape)
+        if tol == 0.0:
+            assert np.array_equal(q, ref), (label, shape)
+            continue
+        vars_checked = (4,) if prec == FP32 else (0, 1, 2, 4)
+        err = max(_rel(q[v], ref[v]) for v in vars_checked)
+        print(f"small block {shape} 11-point {label}: {err:.2e}", flush=True)
+        assert err <= tol, (label, shape, err)
